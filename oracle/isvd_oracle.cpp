@@ -1,0 +1,740 @@
+// Oracle for the iSVD hot path: a plain FP64 CPU restatement of MCNLA 0.7.2's iSVD stages.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// may load this library, and only as the checker.  Nothing in the mcnla_amd product links it.
+//
+// Conventions (as in the reference): every dense matrix is row-major with an explicit leading
+// dimension unless a parameter says "col-major".  A sketch collection Qs/Y is one row-major
+// m x (N*l) buffer whose sketch i occupies columns [i*l, (i+1)*l)
+// (include/mcnla/core/matrix/collection/dense/dense_matrix_collection_col_block.hpp:329-345).
+//
+// Each exported function names the reference lines it restates.  Parity pins (checked in tests/):
+//   orc_syev, orc_gesvd_tall, orc_inverse  -> closed-form / invariant checks and every stage pin
+//   orc_gramian_orthogonalize              -> data/yi.mtx -> data/qi.mtx projectors at 1e-8
+//   orc_kn_*                               -> data/qi.mtx -> data/qb_kn.mtx, 81 iterations, 1e-8
+//   orc_wy_*                               -> data/qi.mtx -> data/qb_wy.mtx, 21 iterations, 1e-8
+//   orc_svd_former / orc_gramian_former    -> data/a.mtx + data/q.mtx -> data/u.mtx projectors
+//   orc_omega_ref                          -> moments + determinism only ("parity unpinned" for
+//                                             the bit stream: no reference fixture stores Omega)
+//   orc_sketch                             -> closed form (numpy A @ Omega)
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+using idx = int64_t;
+
+// C[M x N] = alpha * op(A) * op(B) + beta * C, all row-major.
+// op(A) is M x K; tA means A is stored K x M.  op(B) is K x N; tB means B is stored N x K.
+void mm(bool tA, bool tB, idx M, idx N, idx K, double alpha, const double* A, idx lda,
+        const double* B, idx ldb, double beta, double* C, idx ldc) {
+  std::vector<double> acc(N);
+  for (idx i = 0; i < M; ++i) {
+    std::fill(acc.begin(), acc.end(), 0.0);
+    for (idx k = 0; k < K; ++k) {
+      const double a = tA ? A[k * lda + i] : A[i * lda + k];
+      if (a == 0.0) continue;
+      if (!tB) {
+        const double* b = B + k * ldb;
+        for (idx j = 0; j < N; ++j) acc[j] += a * b[j];
+      } else {
+        for (idx j = 0; j < N; ++j) acc[j] += a * B[j * ldb + k];
+      }
+    }
+    double* c = C + i * ldc;
+    for (idx j = 0; j < N; ++j) c[j] = alpha * acc[j] + (beta == 0.0 ? 0.0 : beta * c[j]);
+  }
+}
+
+double dot(idx n, const double* x, const double* y) {
+  double s = 0.0;
+  for (idx i = 0; i < n; ++i) s += x[i] * y[i];
+  return s;
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major, destroyed).
+// Eigenvalues ascending (the order LAPACK dsyev returns), eigenvectors in the COLUMNS of V.
+// Each eigenvector's sign is fixed so that its largest-magnitude entry is positive.
+void jacobi_eig(idx n, double* a, double* w, double* V) {
+  for (idx i = 0; i < n; ++i)
+    for (idx j = 0; j < n; ++j) V[i * n + j] = (i == j) ? 1.0 : 0.0;
+  double fro = 0.0;
+  for (idx i = 0; i < n * n; ++i) fro += a[i] * a[i];
+  fro = std::sqrt(fro);
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (idx p = 0; p < n; ++p)
+      for (idx q = p + 1; q < n; ++q) off += a[p * n + q] * a[p * n + q];
+    if (std::sqrt(2.0 * off) <= 1e-17 * fro || off == 0.0) break;
+    for (idx p = 0; p < n - 1; ++p) {
+      for (idx q = p + 1; q < n; ++q) {
+        const double apq = a[p * n + q];
+        if (apq == 0.0) continue;
+        const double app = a[p * n + p], aqq = a[q * n + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(1.0 + theta * theta));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+        for (idx k = 0; k < n; ++k) {  // columns p, q
+          const double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - s * akq;
+          a[k * n + q] = s * akp + c * akq;
+        }
+        for (idx k = 0; k < n; ++k) {  // rows p, q
+          const double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - s * aqk;
+          a[q * n + k] = s * apk + c * aqk;
+        }
+        a[p * n + q] = a[q * n + p] = 0.0;
+        for (idx k = 0; k < n; ++k) {
+          const double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * vkq;
+          V[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+    }
+  }
+  // sort ascending
+  std::vector<idx> ord(n);
+  for (idx i = 0; i < n; ++i) ord[i] = i;
+  std::vector<double> d(n);
+  for (idx i = 0; i < n; ++i) d[i] = a[i * n + i];
+  std::stable_sort(ord.begin(), ord.end(), [&](idx x, idx y) { return d[x] < d[y]; });
+  std::vector<double> Vs(n * n);
+  for (idx j = 0; j < n; ++j) {
+    w[j] = d[ord[j]];
+    idx big = 0;
+    for (idx i = 0; i < n; ++i)
+      if (std::fabs(V[i * n + ord[j]]) > std::fabs(V[big * n + ord[j]])) big = i;
+    const double sg = V[big * n + ord[j]] < 0 ? -1.0 : 1.0;
+    for (idx i = 0; i < n; ++i) Vs[i * n + j] = sg * V[i * n + ord[j]];
+  }
+  std::memcpy(V, Vs.data(), sizeof(double) * n * n);
+}
+
+// One-sided (Hestenes) Jacobi SVD of a tall row-major X (r x c, r >= c, destroyed).
+// On return: s (c, descending), Ux (r x c, left vectors, row-major), Vx (c x c, right vectors in
+// columns).  Sign fixed so the largest-magnitude entry of each right vector is positive.
+void onesided_svd(idx r, idx c, double* X, double* s, double* Ux, double* Vx) {
+  std::vector<double> V(c * c, 0.0);
+  for (idx i = 0; i < c; ++i) V[i * c + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    bool rotated = false;
+    for (idx p = 0; p < c - 1; ++p) {
+      for (idx q = p + 1; q < c; ++q) {
+        double al = 0, be = 0, ga = 0;
+        for (idx i = 0; i < r; ++i) {
+          const double xp = X[i * c + p], xq = X[i * c + q];
+          al += xp * xp; be += xq * xq; ga += xp * xq;
+        }
+        if (ga == 0.0 || std::fabs(ga) <= 1e-16 * std::sqrt(al * be)) continue;
+        rotated = true;
+        const double zeta = (be - al) / (2.0 * ga);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        for (idx i = 0; i < r; ++i) {
+          const double xp = X[i * c + p], xq = X[i * c + q];
+          X[i * c + p] = cs * xp - sn * xq;
+          X[i * c + q] = sn * xp + cs * xq;
+        }
+        for (idx i = 0; i < c; ++i) {
+          const double vp = V[i * c + p], vq = V[i * c + q];
+          V[i * c + p] = cs * vp - sn * vq;
+          V[i * c + q] = sn * vp + cs * vq;
+        }
+      }
+    }
+    if (!rotated) break;
+  }
+  std::vector<double> nrm(c);
+  for (idx j = 0; j < c; ++j) {
+    double t = 0;
+    for (idx i = 0; i < r; ++i) t += X[i * c + j] * X[i * c + j];
+    nrm[j] = std::sqrt(t);
+  }
+  std::vector<idx> ord(c);
+  for (idx i = 0; i < c; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](idx x, idx y) { return nrm[x] > nrm[y]; });
+  for (idx j = 0; j < c; ++j) {
+    const idx o = ord[j];
+    idx big = 0;
+    for (idx i = 0; i < c; ++i)
+      if (std::fabs(V[i * c + o]) > std::fabs(V[big * c + o])) big = i;
+    const double sg = V[big * c + o] < 0 ? -1.0 : 1.0;
+    s[j] = nrm[o];
+    for (idx i = 0; i < c; ++i) Vx[i * c + j] = sg * V[i * c + o];
+    const double inv = nrm[o] > 0 ? sg / nrm[o] : 0.0;
+    for (idx i = 0; i < r; ++i) Ux[i * c + j] = X[i * c + o] * inv;
+  }
+}
+
+// Inverse by Gauss-Jordan elimination with partial pivoting (the result LAPACK dgetrf+dgetri
+// compute, include/mcnla/core/la/dense/driver/getrfi.hpp:117-124).  Row-major n x n, in place.
+int inverse(idx n, double* a) {
+  std::vector<double> b(n * 2 * n, 0.0);
+  for (idx i = 0; i < n; ++i) {
+    for (idx j = 0; j < n; ++j) b[i * 2 * n + j] = a[i * n + j];
+    b[i * 2 * n + n + i] = 1.0;
+  }
+  for (idx col = 0; col < n; ++col) {
+    idx piv = col;
+    for (idx i = col + 1; i < n; ++i)
+      if (std::fabs(b[i * 2 * n + col]) > std::fabs(b[piv * 2 * n + col])) piv = i;
+    if (b[piv * 2 * n + col] == 0.0) return 1;
+    if (piv != col)
+      for (idx j = 0; j < 2 * n; ++j) std::swap(b[piv * 2 * n + j], b[col * 2 * n + j]);
+    const double inv = 1.0 / b[col * 2 * n + col];
+    for (idx j = 0; j < 2 * n; ++j) b[col * 2 * n + j] *= inv;
+    for (idx i = 0; i < n; ++i) {
+      if (i == col) continue;
+      const double f = b[i * 2 * n + col];
+      if (f == 0.0) continue;
+      for (idx j = 0; j < 2 * n; ++j) b[i * 2 * n + j] -= f * b[col * 2 * n + j];
+    }
+  }
+  for (idx i = 0; i < n; ++i)
+    for (idx j = 0; j < n; ++j) a[i * n + j] = b[i * 2 * n + n + j];
+  return 0;
+}
+
+// The KN l x l step shared by every KN form (row_block_kolmogorov_nagumo_integrator.hpp:146-201):
+// given Bc (Nl x l) and Bgc (Nl x l) compute Dc, Z = Dgc - Dc^2, eig, s = sqrt(1/2 + sqrt(1/4 - l)),
+// C = V diag(s) V', Cinv = V diag(1/s) V', F = C - Dc * Cinv.  Returns ||s - 1||_2.
+double kn_small_step(idx Nl, idx l, double N, const double* Bc, const double* Bgc, double* Dc,
+                     double* F, double* Cinv) {
+  std::vector<double> Z(l * l), V(l * l), w(l), C(l * l);
+  mm(true, false, l, l, Nl, 1.0 / N, Bc, l, Bc, l, 0.0, Dc, l);   // Dc = Bc' Bc / N
+  mm(true, false, l, l, Nl, 1.0 / N, Bc, l, Bgc, l, 0.0, Z.data(), l);  // Dgc = Bc' Bgc / N
+  mm(false, false, l, l, l, -1.0, Dc, l, Dc, l, 1.0, Z.data(), l);   // Z = Dgc - Dc^2
+  for (idx i = 0; i < l; ++i)  // symmetrize (the reference's rk/syev read one triangle)
+    for (idx j = 0; j < i; ++j) Z[i * l + j] = Z[j * l + i];
+  jacobi_eig(l, Z.data(), w.data(), V.data());
+  double res = 0.0;
+  for (idx i = 0; i < l; ++i) {
+    w[i] = std::sqrt(0.5 + std::sqrt(0.25 - w[i]));
+    res += (w[i] - 1.0) * (w[i] - 1.0);
+  }
+  for (idx i = 0; i < l; ++i)
+    for (idx j = 0; j < l; ++j) {
+      double c = 0, ci = 0;
+      for (idx e = 0; e < l; ++e) {
+        const double vv = V[i * l + e] * V[j * l + e];
+        c += vv * w[e];
+        ci += vv / w[e];
+      }
+      C[i * l + j] = c;
+      Cinv[i * l + j] = ci;
+    }
+  // F = C - Dc * Cinv
+  std::memcpy(F, C.data(), sizeof(double) * l * l);
+  mm(false, false, l, l, l, -1.0, Dc, l, Cinv, l, 1.0, F, l);
+  return std::sqrt(res);
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_version() { return 1; }
+
+// ------------------------------------------------------------------------------------------------
+// Small dense kernels (exported so tests can pin them against closed forms / numpy).
+
+// Symmetric eigen-decomposition; w ascending, eigenvectors in the columns of V.
+void orc_syev(int64_t n, double* a, double* w, double* V) { jacobi_eig(n, a, w, V); }
+
+// SVD of a tall r x c row-major X: s descending, U (r x c), V (c x c, columns).
+void orc_gesvd_tall(int64_t r, int64_t c, double* X, double* s, double* U, double* V) {
+  onesided_svd(r, c, X, s, U, V);
+}
+
+int orc_inverse(int64_t n, double* a) { return inverse(n, a); }
+
+// ------------------------------------------------------------------------------------------------
+// Omega, the reference's seeded Gaussian recipe without MKL:
+//   include/mcnla/core/random/streams/streams.hpp:118-150 (seed chains),
+//   include/mcnla/core/random/routine/gaussian.hpp:39-56 (thread chunking),
+//   include/mcnla/core/random/routine/gaussian_detail_nomkl.hh:31-41 (std::normal_distribution<>).
+// col_block != 0: Streams(seed, root, comm) -> seed_seq{seed} makes one seed per rank, this rank's
+//   seed seeds a second seed_seq that makes one seed per OpenMP thread
+//   (col_block_gaussian_projection_sketcher.hpp:84).
+// col_block == 0: Streams(seed) -> seed_seq{seed} makes one seed per OpenMP thread
+//   (row_block_gaussian_projection_sketcher.hpp:87-89).
+// Each thread i runs std::mt19937 (32-bit index_t build) over chunk i of len/threads values, the
+// last chunk taking the remainder.
+void orc_omega_ref(uint32_t seed, int col_block, int nprocs, int rank, int threads, int64_t len,
+                   double* out) {
+  uint32_t s0 = seed;
+  if (col_block) {
+    std::seed_seq seq{seed};
+    std::vector<uint32_t> per_rank(nprocs);
+    seq.generate(per_rank.begin(), per_rank.end());
+    s0 = per_rank[rank];
+  }
+  std::seed_seq seq2{s0};
+  std::vector<uint32_t> per_thread(threads);
+  seq2.generate(per_thread.begin(), per_thread.end());
+  for (int t = 0; t < threads; ++t) {
+    int64_t chunk = len / threads;
+    const int64_t start = chunk * t;
+    if (t == threads - 1) chunk = len - start;
+    std::mt19937 gen(per_thread[t]);
+    std::normal_distribution<> dis(0.0, 1.0);
+    for (int64_t i = 0; i < chunk; ++i) out[start + i] = dis(gen);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Sketch Y = A * Omega (col_block_gaussian_projection_sketcher.hpp:99,
+// row_block_gaussian_projection_sketcher.hpp:104).  A is m x n, col-major if a_colmajor else
+// row-major, with leading dimension lda.  Omega row-major n x Nl.  Y row-major m x Nl.
+void orc_sketch(int64_t m, int64_t n, int64_t Nl, const double* A, int64_t lda, int a_colmajor,
+                const double* Omega, double* Y) {
+  if (a_colmajor) {
+    std::vector<double> acc(Nl);
+    for (int64_t i = 0; i < m; ++i) {
+      std::fill(acc.begin(), acc.end(), 0.0);
+      for (int64_t j = 0; j < n; ++j) {
+        const double a = A[i + j * lda];
+        const double* o = Omega + j * Nl;
+        for (int64_t c = 0; c < Nl; ++c) acc[c] += a * o[c];
+      }
+      std::memcpy(Y + i * Nl, acc.data(), sizeof(double) * Nl);
+    }
+  } else {
+    mm(false, false, m, Nl, n, 1.0, A, lda, Omega, Nl, 0.0, Y, Nl);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Gramian orthogonalizer (row_block_gramian_orthogonalizer.hpp:79-101, gramian_orthogonalizer.hpp:
+// 83-101): W_i = Y_i' Y_i; eig(W_i) = V_i S_i V_i' (gesvd of the symmetric W_i, singular values
+// descending); Q_i = Y_i V_i S_i^{-1/2}.  In place on the m x (N*l) collection.
+// Returns the number of sketches whose Gram matrix had a non-positive eigenvalue.
+int orc_gramian_orthogonalize(int64_t m, int64_t l, int64_t N, double* Y) {
+  const int64_t Nl = N * l;
+  std::vector<double> W(l * l), V(l * l), w(l), T(l * l), row(l);
+  int bad = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    double* Yi = Y + i * l;
+    mm(true, false, l, l, m, 1.0, Yi, Nl, Yi, Nl, 0.0, W.data(), l);
+    jacobi_eig(l, W.data(), w.data(), V.data());
+    // descending order, T = V diag(1/sqrt(s))
+    for (int64_t j = 0; j < l; ++j) {
+      const int64_t src = l - 1 - j;
+      if (!(w[src] > 0)) ++bad;
+      const double f = 1.0 / std::sqrt(w[src]);
+      for (int64_t r = 0; r < l; ++r) T[r * l + j] = V[r * l + src] * f;
+    }
+    for (int64_t r = 0; r < m; ++r) {
+      double* y = Yi + r * Nl;
+      for (int64_t j = 0; j < l; ++j) {
+        double acc = 0;
+        for (int64_t e = 0; e < l; ++e) acc += y[e] * T[e * l + j];
+        row[j] = acc;
+      }
+      std::memcpy(y, row.data(), sizeof(double) * l);
+    }
+  }
+  return bad;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Row-block Kolmogorov-Nagumo integrator (row_block_kolmogorov_nagumo_integrator.hpp:104-205).
+// Qs: m x (N*l); Qbar: m x l output.  Returns the iteration count (iterations executed).
+int orc_kn_rowblock(int64_t m, int64_t l, int64_t N, const double* Qs, int maxiter, double tol,
+                    double* Qbar) {
+  const int64_t Nl = N * l;
+  std::vector<double> Qc(m * l), Qp(m * l), G(m * l), Bc(Nl * l), Bp(Nl * l), Bg(Nl * l);
+  std::vector<double> Dc(l * l), F(l * l), Cinv(l * l);
+  for (int64_t r = 0; r < m; ++r)  // Qc := Q0
+    std::memcpy(&Qc[r * l], Qs + r * Nl, sizeof(double) * l);
+  mm(true, false, Nl, l, m, 1.0, Qs, Nl, Qc.data(), l, 0.0, Bc.data(), l);  // Bc := Qs' Qc
+  int it = 0;
+  bool conv = false;
+  for (it = 0; it < maxiter && !conv; ++it) {
+    mm(false, false, m, l, Nl, 1.0 / N, Qs, Nl, Bc.data(), l, 0.0, G.data(), l);   // Gc
+    mm(true, false, Nl, l, m, 1.0, Qs, Nl, G.data(), l, 0.0, Bg.data(), l);       // Bgc
+    const double res = kn_small_step(Nl, l, (double)N, Bc.data(), Bg.data(), Dc.data(), F.data(),
+                                     Cinv.data());
+    mm(false, false, m, l, l, 1.0, Qc.data(), l, F.data(), l, 0.0, Qp.data(), l);  // Q+ = Qc F
+    mm(false, false, m, l, l, 1.0, G.data(), l, Cinv.data(), l, 1.0, Qp.data(), l);  // + Gc Cinv
+    mm(false, false, Nl, l, l, 1.0, Bc.data(), l, F.data(), l, 0.0, Bp.data(), l);
+    mm(false, false, Nl, l, l, 1.0, Bg.data(), l, Cinv.data(), l, 1.0, Bp.data(), l);
+    std::swap(Qc, Qp);
+    std::swap(Bc, Bp);
+    conv = !(res >= tol);
+  }
+  std::memcpy(Qbar, Qc.data(), sizeof(double) * m * l);
+  return it;
+}
+
+// Kolmogorov-Nagumo integrator, X form (kolmogorov_nagumo_integrator.hpp:103-182):
+// B = Qs' Qc; D = B'B; X = Qs B/N - Qc D/N; Z = X'X; eig; C, Cinv; Qc = Qc C + X Cinv.
+int orc_kn_xform(int64_t m, int64_t l, int64_t N, const double* Qs, int maxiter, double tol,
+                 double* Qbar) {
+  const int64_t Nl = N * l;
+  std::vector<double> Qc(m * l), Qt(m * l), X(m * l), B(Nl * l), D(l * l), Z(l * l), V(l * l),
+      w(l), C(l * l), Ci(l * l);
+  for (int64_t r = 0; r < m; ++r) std::memcpy(&Qc[r * l], Qs + r * Nl, sizeof(double) * l);
+  int it = 0;
+  bool conv = false;
+  for (it = 0; it < maxiter && !conv; ++it) {
+    mm(true, false, Nl, l, m, 1.0, Qs, Nl, Qc.data(), l, 0.0, B.data(), l);
+    mm(true, false, l, l, Nl, 1.0, B.data(), l, B.data(), l, 0.0, D.data(), l);
+    mm(false, false, m, l, Nl, 1.0 / N, Qs, Nl, B.data(), l, 0.0, X.data(), l);
+    mm(false, false, m, l, l, -1.0 / N, Qc.data(), l, D.data(), l, 1.0, X.data(), l);
+    mm(true, false, l, l, m, 1.0, X.data(), l, X.data(), l, 0.0, Z.data(), l);
+    for (int64_t i = 0; i < l; ++i)
+      for (int64_t j = 0; j < i; ++j) Z[i * l + j] = Z[j * l + i];
+    jacobi_eig(l, Z.data(), w.data(), V.data());
+    double res = 0;
+    for (int64_t i = 0; i < l; ++i) {
+      w[i] = std::sqrt(0.5 + std::sqrt(0.25 - w[i]));
+      res += (w[i] - 1) * (w[i] - 1);
+    }
+    for (int64_t i = 0; i < l; ++i)
+      for (int64_t j = 0; j < l; ++j) {
+        double c = 0, ci = 0;
+        for (int64_t e = 0; e < l; ++e) {
+          const double vv = V[i * l + e] * V[j * l + e];
+          c += vv * w[e];
+          ci += vv / w[e];
+        }
+        C[i * l + j] = c;
+        Ci[i * l + j] = ci;
+      }
+    Qt = Qc;
+    mm(false, false, m, l, l, 1.0, Qt.data(), l, C.data(), l, 0.0, Qc.data(), l);
+    mm(false, false, m, l, l, 1.0, X.data(), l, Ci.data(), l, 1.0, Qc.data(), l);
+    conv = !(std::sqrt(res) >= tol);
+  }
+  std::memcpy(Qbar, Qc.data(), sizeof(double) * m * l);
+  return it;
+}
+
+// Gramian KN (row_block_gramian_kolmogorov_nagumo_integrator.hpp:106-226): Bs = Qs'Qs once, then
+// Bgc = Bs Bc / N; F~+ = F~c Fc; E~+ = E~c Fc + Bc Cinv / N; B+ = Bc Fc + Bgc Cinv;
+// Qbar = Q0 F~ + Qs E~.
+int orc_kn_gramian(int64_t m, int64_t l, int64_t N, const double* Qs, int maxiter, double tol,
+                   double* Qbar) {
+  const int64_t Nl = N * l;
+  std::vector<double> Bs(Nl * Nl), Bc(Nl * l), Bp(Nl * l), Bg(Nl * l), Dc(l * l), F(l * l),
+      Ci(l * l), FF(l * l, 0.0), FFp(l * l), EE(Nl * l, 0.0), EEp(Nl * l);
+  mm(true, false, Nl, Nl, m, 1.0, Qs, Nl, Qs, Nl, 0.0, Bs.data(), Nl);
+  for (int64_t r = 0; r < Nl; ++r) std::memcpy(&Bc[r * l], &Bs[r * Nl], sizeof(double) * l);
+  for (int64_t i = 0; i < l; ++i) FF[i * l + i] = 1.0;
+  int it = 0;
+  bool conv = false;
+  for (it = 0; it < maxiter && !conv; ++it) {
+    mm(false, false, Nl, l, Nl, 1.0 / N, Bs.data(), Nl, Bc.data(), l, 0.0, Bg.data(), l);
+    const double res = kn_small_step(Nl, l, (double)N, Bc.data(), Bg.data(), Dc.data(), F.data(),
+                                     Ci.data());
+    mm(false, false, Nl, l, l, 1.0 / N, Bc.data(), l, Ci.data(), l, 0.0, EEp.data(), l);
+    mm(false, false, l, l, l, 1.0, FF.data(), l, F.data(), l, 0.0, FFp.data(), l);
+    mm(false, false, Nl, l, l, 1.0, EE.data(), l, F.data(), l, 1.0, EEp.data(), l);
+    mm(false, false, Nl, l, l, 1.0, Bc.data(), l, F.data(), l, 0.0, Bp.data(), l);
+    mm(false, false, Nl, l, l, 1.0, Bg.data(), l, Ci.data(), l, 1.0, Bp.data(), l);
+    std::swap(Bc, Bp);
+    std::swap(FF, FFp);
+    std::swap(EE, EEp);
+    conv = !(res >= tol);
+  }
+  for (int64_t r = 0; r < m; ++r) {  // Qbar = Q0 F~ + Qs E~
+    const double* q = Qs + r * Nl;
+    for (int64_t j = 0; j < l; ++j) {
+      double acc = 0;
+      for (int64_t e = 0; e < l; ++e) acc += q[e] * FF[e * l + j];
+      for (int64_t e = 0; e < Nl; ++e) acc += q[e] * EE[e * l + j];
+      Qbar[r * l + j] = acc;
+    }
+  }
+  return it;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Wen-Yin line-search pieces shared by both WY forms (row_block_wen_yin_integrator.hh:75-94).
+struct WyConst {
+  double tau0 = 1.0, taumax = 1e14, taumin = 1e-14, beta = 0.5, sigma = 1e-4, eta = 0.85;
+  int taumaxiter = 20;
+};
+
+// Builds C = [Dc/2 - I/tau, I/2; -Dgc/2, -Dc/2 - I/tau] (2l x 2l), inverts it, then
+// Fc = I + invC22*Dc - invC21 (stored in Fc) and Fgc = invC12*Dc - invC11 (stored in Fgc)
+// (row_block_wen_yin_integrator.hpp:178-194).
+void wy_trial(int64_t l, double tau, const double* Dc, const double* Dgc, double* Fc, double* Fgc) {
+  const int64_t n2 = 2 * l;
+  std::vector<double> C(n2 * n2, 0.0);
+  for (int64_t i = 0; i < l; ++i)
+    for (int64_t j = 0; j < l; ++j) {
+      C[i * n2 + j] = 0.5 * Dc[i * l + j];
+      C[(l + i) * n2 + j] = -0.5 * Dgc[i * l + j];
+      C[(l + i) * n2 + l + j] = -0.5 * Dc[i * l + j];
+    }
+  for (int64_t i = 0; i < l; ++i) {
+    C[i * n2 + i] -= 1.0 / tau;
+    C[i * n2 + l + i] += 0.5;
+    C[(l + i) * n2 + l + i] -= 1.0 / tau;
+  }
+  inverse(n2, C.data());
+  // C#1 := C#2 * Dc - C#1 ; then C21 += I
+  for (int64_t i = 0; i < n2; ++i)
+    for (int64_t j = 0; j < l; ++j) {
+      double acc = 0;
+      for (int64_t e = 0; e < l; ++e) acc += C[i * n2 + l + e] * Dc[e * l + j];
+      const double v = acc - C[i * n2 + j];
+      if (i < l) Fgc[i * l + j] = v;
+      else Fc[(i - l) * l + j] = v + ((i - l) == j ? 1.0 : 0.0);
+    }
+}
+
+// Row-block Wen-Yin integrator (row_block_wen_yin_integrator.hpp:105-278).
+int orc_wy_rowblock(int64_t m, int64_t l, int64_t N, const double* Qs, int maxiter, double tol,
+                    double* Qbar) {
+  const WyConst K;
+  const int64_t Nl = N * l;
+  const double one_n = 1.0 / N, one_2n = 0.5 / N;
+  std::vector<double> Q[2] = {std::vector<double>(m * l), std::vector<double>(m * l)};
+  std::vector<double> X[2] = {std::vector<double>(m * l), std::vector<double>(m * l)};
+  std::vector<double> B[2] = {std::vector<double>(Nl * l), std::vector<double>(Nl * l)};
+  std::vector<double> G(m * l), Bg(Nl * l), Dc(l * l), Dgc(l * l), Fc(l * l), Fgc(l * l);
+  for (int64_t r = 0; r < m; ++r) std::memcpy(&Q[0][r * l], Qs + r * Nl, sizeof(double) * l);
+  mm(true, false, Nl, l, m, 1.0, Qs, Nl, Q[0].data(), l, 0.0, B[0].data(), l);
+  mm(true, false, l, l, Nl, one_n, B[0].data(), l, B[0].data(), l, 0.0, Dc.data(), l);
+  mm(false, false, m, l, Nl, one_n, Qs, Nl, B[0].data(), l, 0.0, G.data(), l);
+  mm(true, false, Nl, l, m, 1.0, Qs, Nl, G.data(), l, 0.0, Bg.data(), l);
+  mm(true, false, l, l, Nl, one_n, B[0].data(), l, Bg.data(), l, 0.0, Dgc.data(), l);
+  X[0] = G;
+  mm(false, false, m, l, l, -1.0, Q[0].data(), l, Dc.data(), l, 1.0, X[0].data(), l);
+  double taug = K.tau0, zeta = 1.0, phi = one_2n * dot(Nl * l, B[0].data(), B[0].data());
+  double mu = 0;
+  for (int64_t i = 0; i < l; ++i) mu += std::fabs(Dgc[i * l + i]);
+  mu -= dot(l * l, Dc.data(), Dc.data());
+
+  int it = 0;
+  int odd = 0;
+  for (it = 0; it < maxiter; ++it) {
+    const int c = odd, p = !odd;
+    odd = !odd;
+    double tau = taug, phit = phi;
+    for (int ti = 0; ti < K.taumaxiter; ++ti, tau *= K.beta) {
+      wy_trial(l, tau, Dc.data(), Dgc.data(), Fc.data(), Fgc.data());
+      mm(false, false, Nl, l, l, 1.0, B[c].data(), l, Fc.data(), l, 0.0, B[p].data(), l);
+      mm(false, false, Nl, l, l, 1.0, Bg.data(), l, Fgc.data(), l, 1.0, B[p].data(), l);
+      phit = one_2n * dot(Nl * l, B[p].data(), B[p].data());
+      if (phit >= phi + tau * K.sigma * mu) break;
+    }
+    phi = (K.eta * zeta * phi + phit) / (K.eta * zeta + 1);
+    zeta = K.eta * zeta + 1;
+    mm(false, false, m, l, l, 1.0, Q[c].data(), l, Fc.data(), l, 0.0, Q[p].data(), l);
+    mm(false, false, m, l, l, 1.0, G.data(), l, Fgc.data(), l, 1.0, Q[p].data(), l);
+    mm(true, false, l, l, Nl, one_n, B[p].data(), l, B[p].data(), l, 0.0, Dc.data(), l);
+    mm(false, false, m, l, Nl, one_n, Qs, Nl, B[p].data(), l, 0.0, G.data(), l);
+    mm(true, false, Nl, l, m, 1.0, Qs, Nl, G.data(), l, 0.0, Bg.data(), l);
+    mm(true, false, l, l, Nl, one_n, B[p].data(), l, Bg.data(), l, 0.0, Dgc.data(), l);
+    mu = 0;
+    for (int64_t i = 0; i < l; ++i) mu += std::fabs(Dgc[i * l + i]);
+    mu -= dot(l * l, Dc.data(), Dc.data());
+    if (mu < tol * tol) {
+      ++it;
+      break;
+    }
+    // X+ = G+ - Q+ D+ ; dQ = Qc - Q+ ; dX = Xc - X+
+    X[p] = G;
+    mm(false, false, m, l, l, -1.0, Q[p].data(), l, Dc.data(), l, 1.0, X[p].data(), l);
+    for (int64_t i = 0; i < m * l; ++i) {
+      Q[c][i] -= Q[p][i];
+      X[c][i] -= X[p][i];
+    }
+    double t1, t2;
+    if (odd) {
+      t1 = dot(m * l, Q[c].data(), Q[c].data());
+      t2 = dot(m * l, Q[c].data(), X[c].data());
+    } else {
+      t1 = dot(m * l, Q[c].data(), X[c].data());
+      t2 = dot(m * l, X[c].data(), X[c].data());
+    }
+    taug = std::fabs(t1 / t2);
+    if (taug < K.taumin) taug = K.taumin;
+    if (taug > K.taumax) taug = K.taumax;
+  }
+  std::memcpy(Qbar, Q[odd].data(), sizeof(double) * m * l);
+  return it;
+}
+
+// Gramian Wen-Yin integrator (row_block_gramian_wen_yin_integrator.hpp:114-328).
+int orc_wy_gramian(int64_t m, int64_t l, int64_t N, const double* Qs, int maxiter, double tol,
+                   double* Qbar) {
+  const WyConst K;
+  const int64_t Nl = N * l;
+  const double one_n = 1.0 / N, one_2n = 0.5 / N;
+  std::vector<double> Bs(Nl * Nl);
+  std::vector<double> B[2] = {std::vector<double>(Nl * l), std::vector<double>(Nl * l)};
+  std::vector<double> D[2] = {std::vector<double>(l * l), std::vector<double>(l * l)};
+  std::vector<double> FF[2] = {std::vector<double>(l * l, 0.0), std::vector<double>(l * l)};
+  std::vector<double> EE[2] = {std::vector<double>(Nl * l, 0.0), std::vector<double>(Nl * l)};
+  std::vector<double> Bg(Nl * l), Dgc(l * l), Fc(l * l), Fgc(l * l), U1(Nl * l), U2(Nl * l),
+      P1(l * l), P2(l * l), BtU1(l * l), BtU2(l * l), BsU(Nl * l);
+  mm(true, false, Nl, Nl, m, 1.0, Qs, Nl, Qs, Nl, 0.0, Bs.data(), Nl);
+  for (int64_t r = 0; r < Nl; ++r) std::memcpy(&B[0][r * l], &Bs[r * Nl], sizeof(double) * l);
+  mm(false, false, Nl, l, Nl, one_n, Bs.data(), Nl, B[0].data(), l, 0.0, Bg.data(), l);
+  mm(true, false, l, l, Nl, one_n, B[0].data(), l, B[0].data(), l, 0.0, D[0].data(), l);
+  mm(true, false, l, l, Nl, one_n, B[0].data(), l, Bg.data(), l, 0.0, Dgc.data(), l);
+  for (int64_t i = 0; i < l; ++i) FF[0][i * l + i] = 1.0;
+  double taug = K.tau0, zeta = 1.0, phi = one_2n * dot(Nl * l, B[0].data(), B[0].data());
+  double mu = 0;
+  for (int64_t i = 0; i < l; ++i) mu += std::fabs(Dgc[i * l + i]);
+  mu -= dot(l * l, D[0].data(), D[0].data());
+
+  int it = 0;
+  int odd = 0;
+  for (it = 0; it < maxiter; ++it) {
+    const int c = odd, p = !odd;
+    odd = !odd;
+    double tau = taug, phit = phi;
+    for (int ti = 0; ti < K.taumaxiter; ++ti, tau *= K.beta) {
+      wy_trial(l, tau, D[c].data(), Dgc.data(), Fc.data(), Fgc.data());
+      mm(false, false, Nl, l, l, 1.0, B[c].data(), l, Fc.data(), l, 0.0, B[p].data(), l);
+      mm(false, false, Nl, l, l, 1.0, Bg.data(), l, Fgc.data(), l, 1.0, B[p].data(), l);
+      phit = one_2n * dot(Nl * l, B[p].data(), B[p].data());
+      if (phit >= phi + tau * K.sigma * mu) break;
+    }
+    phi = (K.eta * zeta * phi + phit) / (K.eta * zeta + 1);
+    zeta = K.eta * zeta + 1;
+    mm(false, false, Nl, l, l, one_n, B[c].data(), l, Fgc.data(), l, 0.0, U1.data(), l);  // Ec
+    mm(false, false, l, l, l, 1.0, FF[c].data(), l, Fc.data(), l, 0.0, FF[p].data(), l);
+    EE[p] = U1;
+    mm(false, false, Nl, l, l, 1.0, EE[c].data(), l, Fc.data(), l, 1.0, EE[p].data(), l);
+    mm(false, false, Nl, l, Nl, one_n, Bs.data(), Nl, B[p].data(), l, 0.0, Bg.data(), l);
+    mm(true, false, l, l, Nl, one_n, B[p].data(), l, B[p].data(), l, 0.0, D[p].data(), l);
+    mm(true, false, l, l, Nl, one_n, B[p].data(), l, Bg.data(), l, 0.0, Dgc.data(), l);
+    mu = 0;
+    for (int64_t i = 0; i < l; ++i) mu += std::fabs(Dgc[i * l + i]);
+    mu -= dot(l * l, D[p].data(), D[p].data());
+    if (mu < tol * tol) {
+      ++it;
+      break;
+    }
+    // Phi2 = Dc - Fc D+ ; Phi1 = Fc - I ; Upsilon2 = (B+ - Bc)/N - Ec D+
+    P2 = D[c];
+    mm(false, false, l, l, l, -1.0, Fc.data(), l, D[p].data(), l, 1.0, P2.data(), l);
+    P1 = Fc;
+    for (int64_t i = 0; i < l; ++i) P1[i * l + i] -= 1.0;
+    for (int64_t i = 0; i < Nl * l; ++i) U2[i] = B[p][i] - B[c][i];
+    mm(false, false, Nl, l, l, -1.0, U1.data(), l, D[p].data(), l, one_n, U2.data(), l);
+    mm(true, false, l, l, Nl, 1.0, B[c].data(), l, U1.data(), l, 0.0, BtU1.data(), l);
+    mm(true, false, l, l, Nl, 1.0, B[c].data(), l, U2.data(), l, 0.0, BtU2.data(), l);
+    double t1, t2;
+    if (odd) {
+      mm(false, false, Nl, l, Nl, 1.0, Bs.data(), Nl, U1.data(), l, 0.0, BsU.data(), l);
+      t1 = dot(l * l, P1.data(), P1.data()) + 2 * dot(l * l, P1.data(), BtU1.data()) +
+           dot(Nl * l, U1.data(), BsU.data());
+      t2 = dot(l * l, P1.data(), P2.data()) + dot(l * l, P1.data(), BtU2.data()) +
+           dot(l * l, P2.data(), BtU1.data()) + dot(Nl * l, U2.data(), BsU.data());
+    } else {
+      mm(false, false, Nl, l, Nl, 1.0, Bs.data(), Nl, U2.data(), l, 0.0, BsU.data(), l);
+      t1 = dot(l * l, P1.data(), P2.data()) + dot(l * l, P1.data(), BtU2.data()) +
+           dot(l * l, P2.data(), BtU1.data()) + dot(Nl * l, U1.data(), BsU.data());
+      t2 = dot(l * l, P2.data(), P2.data()) + 2 * dot(l * l, P2.data(), BtU2.data()) +
+           dot(Nl * l, U2.data(), BsU.data());
+    }
+    taug = std::fabs(t1 / t2);
+    if (taug < K.taumin) taug = K.taumin;
+    if (taug > K.taumax) taug = K.taumax;
+  }
+  const std::vector<double>& Ff = FF[odd];
+  const std::vector<double>& Ef = EE[odd];
+  for (int64_t r = 0; r < m; ++r) {
+    const double* q = Qs + r * Nl;
+    for (int64_t j = 0; j < l; ++j) {
+      double acc = 0;
+      for (int64_t e = 0; e < l; ++e) acc += q[e] * Ff[e * l + j];
+      for (int64_t e = 0; e < Nl; ++e) acc += q[e] * Ef[e * l + j];
+      Qbar[r * l + j] = acc;
+    }
+  }
+  return it;
+}
+
+// ------------------------------------------------------------------------------------------------
+// SvdFormer (svd_former.hpp:86-116; svd_former.hh:93 gesvd<'S','O'> col-major):
+// W = Qbar' A (l x n); svd(W) = Uw S Vw'; U = Qbar Uw[:, :k] (col-major m x k); S = s[:k];
+// V = Vw[:, :k] (row-major n x k).  A is col-major if a_colmajor else row-major (lda).
+void orc_svd_former(int64_t m, int64_t n, int64_t l, int64_t k, const double* A, int64_t lda,
+                    int a_colmajor, const double* Qbar, double* S, double* U_colmajor,
+                    double* V_rowmajor) {
+  // X = W' = A' Qbar (n x l), then W = Vx S Ux'  =>  Uw = Vx, Vw = Ux.
+  std::vector<double> X(n * l, 0.0), s(l), Ux(n * l), Vx(l * l);
+  for (int64_t r = 0; r < m; ++r) {
+    const double* q = Qbar + r * l;
+    for (int64_t j = 0; j < n; ++j) {
+      const double a = a_colmajor ? A[r + j * lda] : A[r * lda + j];
+      double* x = &X[j * l];
+      for (int64_t c = 0; c < l; ++c) x[c] += a * q[c];
+    }
+  }
+  onesided_svd(n, l, X.data(), s.data(), Ux.data(), Vx.data());
+  for (int64_t i = 0; i < k; ++i) S[i] = s[i];
+  for (int64_t r = 0; r < m; ++r)
+    for (int64_t j = 0; j < k; ++j) {
+      double acc = 0;
+      for (int64_t e = 0; e < l; ++e) acc += Qbar[r * l + e] * Vx[e * l + j];
+      U_colmajor[r + j * m] = acc;
+    }
+  if (V_rowmajor)
+    for (int64_t r = 0; r < n; ++r)
+      for (int64_t j = 0; j < k; ++j) V_rowmajor[r * k + j] = Ux[r * l + j];
+}
+
+// Gramian formers (col_block_gramian_former.hpp:96-128, row_block_gramian_former.hpp:108-143):
+// Z = A' Qbar (n x l); W = Z'Z; eig(W) descending; s = sqrt(eig); U = Qbar W[:, :k] (row-major
+// m x k); V = Z W[:, :k] / s (row-major n x k).
+void orc_gramian_former(int64_t m, int64_t n, int64_t l, int64_t k, const double* A, int64_t lda,
+                        int a_colmajor, const double* Qbar, double* S, double* U_rowmajor,
+                        double* V_rowmajor) {
+  std::vector<double> Z(n * l, 0.0), W(l * l), w(l), V(l * l);
+  for (int64_t r = 0; r < m; ++r) {
+    const double* q = Qbar + r * l;
+    for (int64_t j = 0; j < n; ++j) {
+      const double a = a_colmajor ? A[r + j * lda] : A[r * lda + j];
+      double* z = &Z[j * l];
+      for (int64_t c = 0; c < l; ++c) z[c] += a * q[c];
+    }
+  }
+  mm(true, false, l, l, n, 1.0, Z.data(), l, Z.data(), l, 0.0, W.data(), l);
+  jacobi_eig(l, W.data(), w.data(), V.data());
+  std::vector<double> Wk(l * k);
+  for (int64_t j = 0; j < k; ++j) {
+    const int64_t src = l - 1 - j;
+    S[j] = std::sqrt(w[src]);
+    for (int64_t e = 0; e < l; ++e) Wk[e * k + j] = V[e * l + src];
+  }
+  mm(false, false, m, k, l, 1.0, Qbar, l, Wk.data(), k, 0.0, U_rowmajor, k);
+  if (V_rowmajor) {
+    mm(false, false, n, k, l, 1.0, Z.data(), l, Wk.data(), k, 0.0, V_rowmajor, k);
+    for (int64_t r = 0; r < n; ++r)
+      for (int64_t j = 0; j < k; ++j) V_rowmajor[r * k + j] /= S[j];
+  }
+}
+
+// The drivers' error check (src/isvd_dense_cb.cpp:308-332): ||A - U S V'||_F / ||A||_F with
+// U col-major m x k, V row-major n x k, A col-major if a_colmajor.
+double orc_frerr(int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, int a_colmajor,
+                 const double* S, const double* U_colmajor, const double* V_rowmajor) {
+  double num = 0, den = 0;
+  for (int64_t r = 0; r < m; ++r)
+    for (int64_t j = 0; j < n; ++j) {
+      const double a = a_colmajor ? A[r + j * lda] : A[r * lda + j];
+      double usv = 0;
+      for (int64_t e = 0; e < k; ++e) usv += U_colmajor[r + e * m] * S[e] * V_rowmajor[j * k + e];
+      num += (a - usv) * (a - usv);
+      den += a * a;
+    }
+  return std::sqrt(num) / std::sqrt(den);
+}
+
+}  // extern "C"
