@@ -1,0 +1,229 @@
+/* C ABI of the MI355X-native iSVD (integrated randomized SVD) library, libmcnla_amd.so.
+ *
+ * Everything here is plain C: opaque handles, plain pointers and sizes.  Matrix buffers passed to
+ * the *_run / solve entry points are DEVICE pointers (hipMalloc'd memory, or anything the HIP
+ * runtime accepts as device memory); `stream` is a hipStream_t passed as void* (NULL = the null
+ * stream).  Every call returns an mcnla_status; mcnla_last_error() gives the message of the last
+ * failure on the calling thread.
+ *
+ * Layouts follow the reference (emfomy/mcnla 0.7.2):
+ *   - A: m x n, column-major (isvd_dense_cb, src/isvd_dense_cb.cpp:166-176) or row-major
+ *     (isvd_dense_rb, src/isvd_dense_rb.cpp:166), leading dimension lda;
+ *   - the sketch collection Y / Qs: ONE row-major m x (N*l) buffer, sketch i in columns
+ *     [i*l, (i+1)*l) (include/mcnla/core/matrix/collection/dense/dense_matrix_collection_col_block.hpp:329-345);
+ *   - Qbar: row-major m x l;  S: k values;  U: m x k (column-major for the SVD former,
+ *     svd_former.hh:108, row-major for the Gramian formers);  V: row-major n x k.
+ *
+ * Each stage handle is the counterpart of one reference stage object: *_create = constructor +
+ * initialize() (allocates the stage workspace, include/mcnla/isvd/core/stage_wrapper.hpp:37-46),
+ * *_run = operator() (stage_wrapper.hpp:52-61), mcnla_stage_time = time()/times()
+ * (stage_wrapper.hpp:115-164).
+ */
+#ifndef MCNLA_AMD_ISVD_CAPI_H_
+#define MCNLA_AMD_ISVD_CAPI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------------
+ * Status codes.  The reference is noexcept and only asserts (include/mcnla/def.hpp:71-107);
+ * these codes surface the same conditions as errors instead. */
+typedef enum {
+  MCNLA_OK = 0,
+  MCNLA_ERR_INVALID = 1,       /* bad shape/argument, e.g. k+p > min(m,n) */
+  MCNLA_ERR_NOMEM = 2,         /* device allocation failed or the HBM budget is exceeded */
+  MCNLA_ERR_HIP = 3,           /* a HIP runtime call failed */
+  MCNLA_ERR_NOT_CONVERGED = 4, /* integrator hit maxiter; results are still returned */
+  MCNLA_ERR_NUMERIC = 5,       /* NaN/Inf, or a Gram matrix that is not positive definite */
+  MCNLA_ERR_STATE = 6,         /* call out of order (e.g. getter before run) */
+  MCNLA_ERR_NO_DEVICE = 7
+} mcnla_status;
+
+typedef enum { MCNLA_F64 = 0, MCNLA_F32 = 1 } mcnla_dtype;
+typedef enum { MCNLA_COL_MAJOR = 0, MCNLA_ROW_MAJOR = 1 } mcnla_order;
+
+/* Arithmetic of the sketch GEMM (and of the collection it writes). */
+typedef enum {
+  MCNLA_PREC_F64 = 0,        /* everything FP64 (FP64 MFMA) */
+  MCNLA_PREC_MIXED_BF16 = 1, /* sketch: BF16 MFMA, FP32 accumulate, FP32 Y; Gram/eig/KN/SVD FP64 */
+  MCNLA_PREC_MIXED_BF16X3 = 2/* as MIXED_BF16 with a 3-product hi/lo BF16 split of A and Omega */
+} mcnla_precision;
+
+/* isvd::Parameters (include/mcnla/isvd/core/parameters.hh:31-119): m, n, k, p, N. */
+typedef struct {
+  int64_t nrow;
+  int64_t ncol;
+  int64_t rank;       /* k */
+  int64_t over_rank;  /* p */
+  int64_t num_sketch; /* N */
+} mcnla_params;
+
+/* ---------------------------------------------------------------------------------------------
+ * Library / device plumbing. */
+const char* mcnla_last_error(void);
+const char* mcnla_version(void);
+mcnla_status mcnla_device_count(int* count);
+mcnla_status mcnla_set_device(int device);
+mcnla_status mcnla_malloc(void** ptr, size_t bytes);
+mcnla_status mcnla_free(void* ptr);
+mcnla_status mcnla_memcpy_htod(void* dst, const void* src, size_t bytes);
+mcnla_status mcnla_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+mcnla_status mcnla_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+mcnla_status mcnla_memset(void* ptr, int value, size_t bytes);
+mcnla_status mcnla_stream_create(void** stream);
+mcnla_status mcnla_stream_destroy(void* stream);
+mcnla_status mcnla_stream_synchronize(void* stream);
+mcnla_status mcnla_device_synchronize(void);
+mcnla_status mcnla_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ---------------------------------------------------------------------------------------------
+ * Stage handles. */
+typedef struct mcnla_stage_s* mcnla_stage;
+
+/* Sketchers.  Counterparts of ColBlockGaussianProjectionSketcher
+ * (include/mcnla/isvd/sketcher/col_block_gaussian_projection_sketcher.hpp:49-102),
+ * RowBlockGaussianProjectionSketcher (row_block_gaussian_projection_sketcher.hpp:51-114) and
+ * GaussianProjectionSketcher with exponent 0 (gaussian_projection_sketcher.hpp:66-103). */
+typedef enum {
+  MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN = 0,
+  MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN = 1,
+  MCNLA_SKETCHER_GAUSSIAN = 2
+} mcnla_sketcher_kind;
+
+/* Omega generation.  PHILOX: counter-based N(0,1) generated on the device (Philox4x32-10 +
+ * Box-Muller; element e of Omega depends only on (seed, e)).  REFERENCE: the reference's own
+ * stream (std::seed_seq -> std::mt19937 -> std::normal_distribution<>, split into `threads`
+ * chunks; include/mcnla/core/random/streams/streams.hpp:118-150, routine/gaussian.hpp:39-56),
+ * generated on the host and uploaded, as the reference's GPU sketcher does
+ * (col_block_gaussian_projection_sketcher_gpu.hpp:110). */
+typedef enum { MCNLA_OMEGA_PHILOX = 0, MCNLA_OMEGA_REFERENCE = 1 } mcnla_omega_mode;
+
+mcnla_status mcnla_sketcher_create(const mcnla_params* params, int kind, int precision,
+                                   mcnla_stage* out);
+/* setSeed (col_block_gaussian_projection_sketcher.hh:76).  `threads` is the OpenMP thread count
+ * of the reference run whose stream should be reproduced (REFERENCE mode only). */
+mcnla_status mcnla_sketcher_set_seed(mcnla_stage s, uint64_t seed, int omega_mode, int threads);
+/* Y = A * Omega.  A: m x n (a_dtype, a_order, lda).  Y: row-major m x (N*l), FP64 for
+ * MCNLA_PREC_F64, FP32 for the mixed precisions. */
+mcnla_status mcnla_sketcher_run(mcnla_stage s, const void* A, int64_t lda, int a_dtype,
+                                int a_order, void* Y, void* stream);
+/* Device pointer to the n x (N*l) row-major FP64 Omega of the last run (for checking). */
+mcnla_status mcnla_sketcher_omega(mcnla_stage s, const double** omega);
+/* The REFERENCE-mode stream on the host (no device involved): `len` N(0,1) values for `seed`,
+ * with the col-block (kind COL_BLOCK_GAUSSIAN / GAUSSIAN) or row-block seed chain and `threads`
+ * chunks (streams.hpp:118-150, gaussian.hpp:39-56).  Used by the drivers and for checking. */
+mcnla_status mcnla_omega_reference(uint64_t seed, int sketcher_kind, int threads, int64_t len,
+                                   double* out);
+
+/* Orthogonalizers: GramianOrthogonalizer (gramian_orthogonalizer.hpp:62-104) and
+ * RowBlockGramianOrthogonalizer (row_block_gramian_orthogonalizer.hpp:63-105).
+ * In place: Q_i = Y_i V_i S_i^{-1/2} with eig(Y_i' Y_i) = V_i S_i V_i'. */
+typedef enum { MCNLA_ORTH_GRAMIAN = 0, MCNLA_ORTH_ROW_BLOCK_GRAMIAN = 1 } mcnla_orth_kind;
+mcnla_status mcnla_orthogonalizer_create(const mcnla_params* params, int kind, int precision,
+                                         mcnla_stage* out);
+mcnla_status mcnla_orthogonalizer_run(mcnla_stage s, void* Qs, void* stream);
+
+/* Integrators: KolmogorovNagumoIntegrator (kolmogorov_nagumo_integrator.hpp:77-185),
+ * RowBlockKolmogorovNagumoIntegrator (row_block_kolmogorov_nagumo_integrator.hpp:80-208),
+ * RowBlockGramianKolmogorovNagumoIntegrator (row_block_gramian_kolmogorov_nagumo_integrator.hpp:80-229),
+ * RowBlockWenYinIntegrator (row_block_wen_yin_integrator.hpp:80-281),
+ * RowBlockGramianWenYinIntegrator (row_block_gramian_wen_yin_integrator.hpp:87-331).
+ * Defaults of the reference: maxiter 256; tol 1e-4 (KN) / 1e-3 (WY). */
+typedef enum {
+  MCNLA_INT_KN = 0,
+  MCNLA_INT_ROW_BLOCK_KN = 1,
+  MCNLA_INT_ROW_BLOCK_GRAMIAN_KN = 2,
+  MCNLA_INT_ROW_BLOCK_WY = 3,
+  MCNLA_INT_ROW_BLOCK_GRAMIAN_WY = 4
+} mcnla_integrator_kind;
+mcnla_status mcnla_integrator_create(const mcnla_params* params, int kind, int precision,
+                                     int64_t maxiter, double tol, mcnla_stage* out);
+/* Qs: row-major m x (N*l) (FP64, or FP32 for mixed precisions); Qbar: row-major m x l FP64. */
+mcnla_status mcnla_integrator_run(mcnla_stage s, const void* Qs, double* Qbar, void* stream);
+/* iteration() (row_block_kolmogorov_nagumo_integrator.hh:121): iterations executed. */
+mcnla_status mcnla_integrator_iteration(mcnla_stage s, int64_t* iteration);
+
+/* Formers: SvdFormer (svd_former.hpp:71-116), GramianFormer (gramian_former.hpp),
+ * ColBlockGramianFormer (col_block_gramian_former.hpp:74-131) and RowBlockGramianFormer
+ * (row_block_gramian_former.hpp:79-146).  jobv != 0 also forms V. */
+typedef enum {
+  MCNLA_FORM_SVD = 0,
+  MCNLA_FORM_GRAMIAN = 1,
+  MCNLA_FORM_COL_BLOCK_GRAMIAN = 2,
+  MCNLA_FORM_ROW_BLOCK_GRAMIAN = 3
+} mcnla_former_kind;
+mcnla_status mcnla_former_create(const mcnla_params* params, int kind, int jobv, int precision,
+                                 mcnla_stage* out);
+mcnla_status mcnla_former_run(mcnla_stage s, const void* A, int64_t lda, int a_dtype,
+                              int a_order, const double* Qbar, void* stream);
+/* vectorS() / matrixU() / matrixV(): device pointers owned by the stage.  U is column-major
+ * m x k for MCNLA_FORM_SVD and row-major m x k otherwise; V is row-major n x k. */
+mcnla_status mcnla_former_results(mcnla_stage s, const double** S, const double** U,
+                                  const double** V);
+
+/* Common to every stage. */
+mcnla_status mcnla_stage_destroy(mcnla_stage s);
+/* time() and the per-part times of the last run, in seconds (stage_wrapper.hpp:115-164).
+ * Writes up to max_parts part times; *nparts receives the number of parts. */
+mcnla_status mcnla_stage_time(mcnla_stage s, double* total, double* parts, int max_parts,
+                              int* nparts);
+const char* mcnla_stage_name(mcnla_stage s);
+const char* mcnla_stage_part_names(mcnla_stage s);
+
+/* ---------------------------------------------------------------------------------------------
+ * Solver: sketch -> orthogonalize -> integrate -> form as one plan, with A resident on the
+ * device and every intermediate (Y, Gram, integrator state, Qbar) kept in HBM.  This is the
+ * isvd::Solver facade over the stage plugins; it never materializes Qs: the Gram matrix of Y
+ * carries the orthogonalizer and the integrator, and Qbar is formed by one pass over Y. */
+typedef struct mcnla_solver_s* mcnla_solver;
+
+typedef struct {
+  int sketcher;        /* mcnla_sketcher_kind */
+  int orthogonalizer;  /* mcnla_orth_kind */
+  int integrator;      /* mcnla_integrator_kind */
+  int former;          /* mcnla_former_kind */
+  int precision;       /* mcnla_precision */
+  int jobv;            /* form V too */
+  int64_t maxiter;
+  double tol;
+  uint64_t seed;
+  int omega_mode;      /* mcnla_omega_mode */
+  int omega_threads;   /* REFERENCE mode: OpenMP thread count of the stream to reproduce */
+} mcnla_solver_config;
+
+/* Fills `cfg` with the defaults of the reference's isvd_dense_cb driver as used on this path:
+ * col-block Gaussian sketcher, Gramian orthogonalizer, row-block KN (maxiter 256, tol 1e-4),
+ * SVD former, FP64, Philox Omega, seed 0. */
+void mcnla_solver_config_default(mcnla_solver_config* cfg);
+mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_config* cfg,
+                                 mcnla_solver* out);
+mcnla_status mcnla_solver_destroy(mcnla_solver s);
+/* Device bytes the solver holds (workspaces, not counting A). */
+mcnla_status mcnla_solver_workspace_bytes(mcnla_solver s, size_t* bytes);
+mcnla_status mcnla_solver_set_seed(mcnla_solver s, uint64_t seed);
+mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
+                                int a_order, void* stream);
+mcnla_status mcnla_solver_results(mcnla_solver s, const double** S, const double** U,
+                                  const double** V, const double** Qbar);
+mcnla_status mcnla_solver_iteration(mcnla_solver s, int64_t* iteration);
+/* Per-stage device times of the last solve (seconds): [sketch, orthogonalize, integrate, form]
+ * plus, in kernel_ms, the duration of the sketch GEMM launch alone. */
+mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* sketch_kernel_s);
+
+/* ---------------------------------------------------------------------------------------------
+ * Synthetic test matrices (the reference demo's construction, demo/demo.cpp:250-293):
+ * A = U diag(sigma) V' + noise * G, with U (m x r, row-major) and V (n x r, row-major) device
+ * buffers, G ~ N(0,1) from Philox(seed).  A is written with the given dtype/order (lda). */
+mcnla_status mcnla_synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
+                                 const double* sigma, double noise, uint64_t seed, void* A,
+                                 int64_t lda, int a_dtype, int a_order, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCNLA_AMD_ISVD_CAPI_H_ */

@@ -1,0 +1,27 @@
+"""mcnla_amd: MI355X-native integrated randomized SVD (iSVD) behind MCNLA's stage-plugin API.
+
+The compute lives in libmcnla_amd.so (hand-written HIP kernels for gfx950 behind the C ABI in
+include/mcnla_amd/isvd_capi.h).  This package is the Python host mirror of the reference's
+``mcnla::isvd`` classes; see :mod:`mcnla_amd.isvd`.
+"""
+from . import _lib
+from ._lib import McnlaError, lib
+from .device import DeviceArray, device_count, mem_info, synchronize
+from .isvd import (
+    ColBlockGaussianProjectionSketcher, ColBlockGramianFormer, CollectionFromPartialSumToRowBlockConverter,
+    CollectionFromRowBlockConverter, CollectionToRowBlockConverter, DummyConverter,
+    GaussianProjectionSketcher, GramianFormer, GramianOrthogonalizer, KolmogorovNagumoIntegrator,
+    MatrixFromColBlockToAllConverter, MatrixFromRowBlockConverter, MatrixFromRowBlockToAllConverter,
+    MatrixToRowBlockConverter, Parameters, RowBlockGaussianProjectionSketcher,
+    RowBlockGramianFormer, RowBlockGramianKolmogorovNagumoIntegrator, RowBlockGramianOrthogonalizer,
+    RowBlockGramianWenYinIntegrator, RowBlockKolmogorovNagumoIntegrator, RowBlockWenYinIntegrator,
+    Solver, SvdFormer, synth_lowrank,
+)
+
+PREC_F64 = _lib.PREC_F64
+PREC_MIXED_BF16 = _lib.PREC_MIXED_BF16
+PREC_MIXED_BF16X3 = _lib.PREC_MIXED_BF16X3
+OMEGA_PHILOX = _lib.OMEGA_PHILOX
+OMEGA_REFERENCE = _lib.OMEGA_REFERENCE
+
+__version__ = "0.1.0"

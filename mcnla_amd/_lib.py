@@ -1,0 +1,135 @@
+"""ctypes binding of libmcnla_amd.so (the C ABI declared in include/mcnla_amd/isvd_capi.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C mcnla_amd/csrc``).  There
+is no fallback: if the shared library is missing, importing any compute entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmcnla_amd.so")
+
+# mcnla_status
+OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_NOT_CONVERGED, ERR_NUMERIC, ERR_STATE, ERR_NO_DEVICE = range(8)
+# mcnla_dtype / mcnla_order
+F64, F32 = 0, 1
+COL_MAJOR, ROW_MAJOR = 0, 1
+# mcnla_precision
+PREC_F64, PREC_MIXED_BF16, PREC_MIXED_BF16X3 = 0, 1, 2
+# kinds
+SKETCHER_COL_BLOCK_GAUSSIAN, SKETCHER_ROW_BLOCK_GAUSSIAN, SKETCHER_GAUSSIAN = 0, 1, 2
+OMEGA_PHILOX, OMEGA_REFERENCE = 0, 1
+ORTH_GRAMIAN, ORTH_ROW_BLOCK_GRAMIAN = 0, 1
+INT_KN, INT_ROW_BLOCK_KN, INT_ROW_BLOCK_GRAMIAN_KN, INT_ROW_BLOCK_WY, INT_ROW_BLOCK_GRAMIAN_WY = range(5)
+FORM_SVD, FORM_GRAMIAN, FORM_COL_BLOCK_GRAMIAN, FORM_ROW_BLOCK_GRAMIAN = range(4)
+
+STATUS_NAMES = {
+    OK: "OK", ERR_INVALID: "INVALID", ERR_NOMEM: "NOMEM", ERR_HIP: "HIP",
+    ERR_NOT_CONVERGED: "NOT_CONVERGED", ERR_NUMERIC: "NUMERIC", ERR_STATE: "STATE",
+    ERR_NO_DEVICE: "NO_DEVICE",
+}
+
+
+class McnlaError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), message))
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("nrow", ctypes.c_int64), ("ncol", ctypes.c_int64), ("rank", ctypes.c_int64),
+                ("over_rank", ctypes.c_int64), ("num_sketch", ctypes.c_int64)]
+
+
+class SolverConfig(ctypes.Structure):
+    _fields_ = [("sketcher", ctypes.c_int), ("orthogonalizer", ctypes.c_int),
+                ("integrator", ctypes.c_int), ("former", ctypes.c_int),
+                ("precision", ctypes.c_int), ("jobv", ctypes.c_int),
+                ("maxiter", ctypes.c_int64), ("tol", ctypes.c_double),
+                ("seed", ctypes.c_uint64), ("omega_mode", ctypes.c_int),
+                ("omega_threads", ctypes.c_int)]
+
+
+# Every exported symbol and its signature; tests check the library exports exactly these.
+_vp = ctypes.c_void_p
+_pp = ctypes.POINTER(ctypes.c_void_p)
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+_dp = ctypes.POINTER(ctypes.c_double)
+_st = ctypes.c_int  # mcnla_status
+
+SIGNATURES = {
+    "mcnla_last_error": (ctypes.c_char_p, []),
+    "mcnla_version": (ctypes.c_char_p, []),
+    "mcnla_device_count": (_st, [ctypes.POINTER(ctypes.c_int)]),
+    "mcnla_set_device": (_st, [_i]),
+    "mcnla_malloc": (_st, [_pp, _sz]),
+    "mcnla_free": (_st, [_vp]),
+    "mcnla_memcpy_htod": (_st, [_vp, _vp, _sz]),
+    "mcnla_memcpy_dtoh": (_st, [_vp, _vp, _sz]),
+    "mcnla_memcpy_dtod": (_st, [_vp, _vp, _sz, _vp]),
+    "mcnla_memset": (_st, [_vp, _i, _sz]),
+    "mcnla_stream_create": (_st, [_pp]),
+    "mcnla_stream_destroy": (_st, [_vp]),
+    "mcnla_stream_synchronize": (_st, [_vp]),
+    "mcnla_device_synchronize": (_st, []),
+    "mcnla_mem_info": (_st, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "mcnla_sketcher_create": (_st, [ctypes.POINTER(Params), _i, _i, _pp]),
+    "mcnla_sketcher_set_seed": (_st, [_vp, _u64, _i, _i]),
+    "mcnla_sketcher_run": (_st, [_vp, _vp, _i64, _i, _i, _vp, _vp]),
+    "mcnla_sketcher_omega": (_st, [_vp, _pp]),
+    "mcnla_omega_reference": (_st, [_u64, _i, _i, _i64, _dp]),
+    "mcnla_orthogonalizer_create": (_st, [ctypes.POINTER(Params), _i, _i, _pp]),
+    "mcnla_orthogonalizer_run": (_st, [_vp, _vp, _vp]),
+    "mcnla_integrator_create": (_st, [ctypes.POINTER(Params), _i, _i, _i64, _d, _pp]),
+    "mcnla_integrator_run": (_st, [_vp, _vp, _vp, _vp]),
+    "mcnla_integrator_iteration": (_st, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "mcnla_former_create": (_st, [ctypes.POINTER(Params), _i, _i, _i, _pp]),
+    "mcnla_former_run": (_st, [_vp, _vp, _i64, _i, _i, _vp, _vp]),
+    "mcnla_former_results": (_st, [_vp, _pp, _pp, _pp]),
+    "mcnla_stage_destroy": (_st, [_vp]),
+    "mcnla_stage_time": (_st, [_vp, _dp, _dp, _i, ctypes.POINTER(ctypes.c_int)]),
+    "mcnla_stage_name": (ctypes.c_char_p, [_vp]),
+    "mcnla_stage_part_names": (ctypes.c_char_p, [_vp]),
+    "mcnla_solver_config_default": (None, [ctypes.POINTER(SolverConfig)]),
+    "mcnla_solver_create": (_st, [ctypes.POINTER(Params), ctypes.POINTER(SolverConfig), _pp]),
+    "mcnla_solver_destroy": (_st, [_vp]),
+    "mcnla_solver_workspace_bytes": (_st, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "mcnla_solver_set_seed": (_st, [_vp, _u64]),
+    "mcnla_solver_solve": (_st, [_vp, _vp, _i64, _i, _i, _vp]),
+    "mcnla_solver_results": (_st, [_vp, _pp, _pp, _pp, _pp]),
+    "mcnla_solver_iteration": (_st, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "mcnla_solver_stage_times": (_st, [_vp, _dp, _dp]),
+    "mcnla_synth_lowrank": (_st, [_i64, _i64, _i64, _vp, _vp, _vp, _d, _u64, _vp, _i64, _i, _i,
+                                  _vp]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Loads libmcnla_amd.so once; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libmcnla_amd.so is not built (expected at %s); run __graft_entry__.build() or "
+                "`make -C mcnla_amd/csrc`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status, allow=()):
+    """Raises McnlaError for a non-OK status not listed in `allow`; returns the status."""
+    if status != OK and status not in allow:
+        msg = lib().mcnla_last_error()
+        raise McnlaError(status, msg.decode() if msg else "")
+    return status

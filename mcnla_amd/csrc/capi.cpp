@@ -1,0 +1,1113 @@
+// C ABI of libmcnla_amd: stage handles (the reference's iSVD stage plugins) and the fused solver.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mcnla {
+
+void omega_reference(uint32_t seed, bool col_block, int threads, int64_t len, double* out);
+
+namespace {
+thread_local std::string g_error;
+}
+void set_error(const std::string& msg) { g_error = msg; }
+const char* get_error() { return g_error.c_str(); }
+
+namespace {
+
+template <class F>
+mcnla_status guard(F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return MCNLA_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return MCNLA_ERR_INVALID;
+  } catch (...) {
+    set_error("unknown error");
+    return MCNLA_ERR_INVALID;
+  }
+}
+
+// Device allocation owned by a stage (grows, never shrinks).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    if (p) MCNLA_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    if (n == 0) return;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      throw Error{MCNLA_ERR_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed"};
+    }
+    bytes = n;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// hipEvent timing of a stage: time() and per-part times (stage_wrapper.hpp:170-192).
+struct Timer {
+  std::vector<hipEvent_t> ev;
+  int used = 0;
+  ~Timer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  void start(hipStream_t st) {
+    used = 0;
+    mark(st);
+  }
+  void mark(hipStream_t st) {
+    if (used == static_cast<int>(ev.size())) {
+      hipEvent_t e;
+      MCNLA_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+    MCNLA_HIP(hipEventRecord(ev[used], st));
+    ++used;
+  }
+  std::vector<double> parts() {
+    std::vector<double> out;
+    if (used < 2) return out;
+    MCNLA_HIP(hipEventSynchronize(ev[used - 1]));
+    for (int i = 1; i < used; ++i) {
+      float ms = 0.f;
+      MCNLA_HIP(hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
+      out.push_back(ms * 1e-3);
+    }
+    return out;
+  }
+};
+
+void check_params(const mcnla_params* p) {
+  MCNLA_REQUIRE(p != nullptr, MCNLA_ERR_INVALID, "params is null");
+  MCNLA_REQUIRE(p->nrow >= 1 && p->ncol >= 1, MCNLA_ERR_INVALID, "matrix size must be positive");
+  MCNLA_REQUIRE(p->rank >= 1 && p->over_rank >= 0 && p->num_sketch >= 1, MCNLA_ERR_INVALID,
+                "need rank >= 1, over_rank >= 0, num_sketch >= 1");
+  const int64_t l = p->rank + p->over_rank;
+  MCNLA_REQUIRE(l <= p->nrow && l <= p->ncol, MCNLA_ERR_INVALID,
+                "rank + over_rank must not exceed min(nrow, ncol)");
+  MCNLA_REQUIRE(l <= kMaxL, MCNLA_ERR_INVALID, "rank + over_rank must be at most 64");
+}
+
+bool is_mixed(int precision) {
+  return precision == MCNLA_PREC_MIXED_BF16 || precision == MCNLA_PREC_MIXED_BF16X3;
+}
+
+void check_precision(int precision) {
+  MCNLA_REQUIRE(precision == MCNLA_PREC_F64 || is_mixed(precision), MCNLA_ERR_INVALID,
+                "unknown precision");
+}
+
+void check_a(const mcnla_params& p, const void* A, int64_t lda, int dtype, int order) {
+  MCNLA_REQUIRE(A != nullptr, MCNLA_ERR_INVALID, "A is null");
+  MCNLA_REQUIRE(dtype == MCNLA_F64 || dtype == MCNLA_F32, MCNLA_ERR_INVALID, "A dtype");
+  MCNLA_REQUIRE(order == MCNLA_COL_MAJOR || order == MCNLA_ROW_MAJOR, MCNLA_ERR_INVALID, "A order");
+  MCNLA_REQUIRE(lda >= (order == MCNLA_COL_MAJOR ? p.nrow : p.ncol), MCNLA_ERR_INVALID,
+                "lda smaller than the leading dimension of A");
+}
+
+GemmShape tall_shape(int64_t ncols) {
+  return ncols <= 32 ? GemmShape::kTallN32 : GemmShape::kTallN64;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Shared pipeline pieces.
+
+// Y = A * Omega with Omega generated into `omega` (n x Nl FP64).
+struct SketchCore {
+  DevBuf omega, ws;
+  std::vector<double> host;
+  hipEvent_t gemm_beg = nullptr, gemm_end = nullptr;
+  ~SketchCore() {
+    if (gemm_beg) (void)hipEventDestroy(gemm_beg);
+    if (gemm_end) (void)hipEventDestroy(gemm_end);
+  }
+  void generate(const mcnla_params& p, uint64_t seed, int mode, int threads, bool col_block,
+                hipStream_t st) {
+    const int64_t nl = (p.rank + p.over_rank) * p.num_sketch;
+    const int64_t count = p.ncol * nl;
+    omega.ensure(count * sizeof(double));
+    if (mode == MCNLA_OMEGA_REFERENCE) {
+      host.resize(count);
+      omega_reference(static_cast<uint32_t>(seed), col_block, threads, count, host.data());
+      MCNLA_HIP(hipMemcpyAsync(omega.p, host.data(), count * sizeof(double), hipMemcpyHostToDevice, st));
+      MCNLA_HIP(hipStreamSynchronize(st));  // `host` is reused by the next call
+    } else {
+      omega_philox(omega.as<double>(), count, seed, st);
+    }
+  }
+  void project(const mcnla_params& p, int precision, const void* A, int64_t lda, int dtype,
+               int order, void* Y, hipStream_t st) {
+    const int64_t nl = (p.rank + p.over_rank) * p.num_sketch;
+    if (!gemm_beg) {
+      MCNLA_HIP(hipEventCreate(&gemm_beg));
+      MCNLA_HIP(hipEventCreate(&gemm_end));
+    }
+    if (precision == MCNLA_PREC_F64) {
+      GemmF64Args g;
+      g.M = p.nrow;
+      g.N = nl;
+      g.K = p.ncol;
+      g.A = A;
+      g.lda = lda;
+      g.a_mcontig = (order == MCNLA_COL_MAJOR);
+      g.a_f32 = (dtype == MCNLA_F32);
+      g.B = omega.p;
+      g.ldb = nl;
+      g.b_ncontig = true;
+      g.C = static_cast<double*>(Y);
+      g.ldc = nl;
+      g.shape = GemmShape::kSquare;
+      MCNLA_HIP(hipEventRecord(gemm_beg, st));
+      gemm_f64(g, st);
+      MCNLA_HIP(hipEventRecord(gemm_end, st));
+    } else {
+      SketchBf16Args s;
+      s.m = p.nrow;
+      s.n = p.ncol;
+      s.nl = nl;
+      s.A = A;
+      s.lda = lda;
+      s.a_colmajor = (order == MCNLA_COL_MAJOR);
+      s.a_f32 = (dtype == MCNLA_F32);
+      s.omega = omega.as<double>();
+      s.Y = static_cast<float*>(Y);
+      s.ldy = nl;
+      s.terms = (precision == MCNLA_PREC_MIXED_BF16X3) ? 3 : 1;
+      ws.ensure(sketch_bf16_workspace(s));
+      sketch_bf16(s, ws.p, ws.bytes, st, gemm_beg, gemm_end);
+    }
+  }
+  double gemm_seconds() {
+    if (!gemm_beg) return 0.0;
+    MCNLA_HIP(hipEventSynchronize(gemm_end));
+    float ms = 0.f;
+    MCNLA_HIP(hipEventElapsedTime(&ms, gemm_beg, gemm_end));
+    return ms * 1e-3;
+  }
+};
+
+// G = Y' * Y (Nl x Nl, FP64, full symmetric), Y row-major m x Nl.
+struct GramCore {
+  DevBuf partial;
+  void run(const void* Y, bool y_f32, int64_t m, int64_t nl, double* G, hipStream_t st) {
+    const int64_t t = ceil_div(nl, 128);
+    GemmF64Args g;
+    g.M = nl;
+    g.N = nl;
+    g.K = m;
+    g.A = Y;
+    g.lda = nl;
+    g.a_mcontig = true;
+    g.a_f32 = y_f32;
+    g.B = Y;
+    g.ldb = nl;
+    g.b_ncontig = true;
+    g.b_f32 = y_f32;
+    g.C = G;
+    g.ldc = nl;
+    g.lower_only = true;
+    g.shape = GemmShape::kSquare;
+    g.splits = gemm_f64_splits_for(t * (t + 1) / 2, m, 2048);
+    g.kchunk = ceil_div(m, g.splits);
+    g.kchunk = ceil_div(g.kchunk, 16) * 16;
+    g.splits = ceil_div(m, g.kchunk);
+    partial.ensure(gemm_f64_partial_elems(nl, nl, g.splits) * sizeof(double));
+    g.partial = partial.as<double>();
+    gemm_f64(g, st);
+  }
+};
+
+// Z = A' Qbar (n x l), Wg = Z' Z, eig, and the outputs of either former.
+struct FormCore {
+  DevBuf Z, partial, Wg, eigw, eigv, Wk, S, U, V, Vtmp;
+  void run(const mcnla_params& p, bool svd, bool jobv, const void* A, int64_t lda, int dtype,
+           int order, const double* Qbar, hipStream_t st) {
+    const int64_t m = p.nrow, n = p.ncol, k = p.rank;
+    const int l = static_cast<int>(p.rank + p.over_rank);
+    Z.ensure(n * l * sizeof(double));
+    Wg.ensure(l * l * sizeof(double));
+    eigw.ensure(l * sizeof(double));
+    eigv.ensure(l * l * sizeof(double));
+    Wk.ensure(l * k * sizeof(double));
+    S.ensure(k * sizeof(double));
+    U.ensure(m * k * sizeof(double));
+    V.ensure(n * k * sizeof(double));
+    // Z = A' Qbar: element (j, r) of A' is A(r, j).
+    GemmF64Args z;
+    z.M = n;
+    z.N = l;
+    z.K = m;
+    z.A = A;
+    z.lda = lda;
+    z.a_mcontig = (order == MCNLA_ROW_MAJOR);
+    z.a_f32 = (dtype == MCNLA_F32);
+    z.B = Qbar;
+    z.ldb = l;
+    z.C = Z.as<double>();
+    z.ldc = l;
+    z.shape = tall_shape(l);
+    z.splits = gemm_f64_splits_for(ceil_div(n, 128), m, 2048);
+    z.kchunk = ceil_div(ceil_div(m, z.splits), 16) * 16;
+    z.splits = ceil_div(m, z.kchunk);
+    size_t need = gemm_f64_partial_elems(n, l, z.splits);
+    // Wg = Z' Z
+    GemmF64Args w;
+    w.M = l;
+    w.N = l;
+    w.K = n;
+    w.A = Z.p;
+    w.lda = l;
+    w.a_mcontig = true;
+    w.B = Z.p;
+    w.ldb = l;
+    w.C = Wg.as<double>();
+    w.ldc = l;
+    w.shape = GemmShape::kSmall;
+    w.splits = gemm_f64_splits_for(1, n, 64);
+    w.kchunk = ceil_div(ceil_div(n, w.splits), 16) * 16;
+    w.splits = ceil_div(n, w.kchunk);
+    need = std::max(need, gemm_f64_partial_elems(l, l, w.splits));
+    partial.ensure(need * sizeof(double));
+    z.partial = partial.as<double>();
+    w.partial = partial.as<double>();
+    gemm_f64(z, st);
+    gemm_f64(w, st);
+    former_small(Wg.as<double>(), l, static_cast<int>(k), eigw.as<double>(), eigv.as<double>(),
+                 Wk.as<double>(), S.as<double>(), st);
+    // V = Z Wk (n x k); the SVD former takes sigma_j = ||Z w_j||.
+    GemmF64Args v;
+    v.M = n;
+    v.N = k;
+    v.K = l;
+    v.A = Z.p;
+    v.lda = l;
+    v.a_mcontig = false;
+    v.B = Wk.p;
+    v.ldb = k;
+    v.C = V.as<double>();
+    v.ldc = k;
+    v.shape = tall_shape(k);
+    if (svd || jobv) gemm_f64(v, st);
+    if (svd) column_norms(V.as<double>(), n, static_cast<int>(k), S.as<double>(), st);
+    if (jobv) scale_columns(V.as<double>(), n, static_cast<int>(k), S.as<double>(), true, st);
+    GemmF64Args u;
+    if (svd) {
+      // U column-major m x k == row-major k x m = Wk' Qbar'.
+      u.M = k;
+      u.N = m;
+      u.K = l;
+      u.A = Wk.p;
+      u.lda = k;
+      u.a_mcontig = true;
+      u.B = Qbar;
+      u.ldb = l;
+      u.b_ncontig = false;
+      u.C = U.as<double>();
+      u.ldc = m;
+      u.shape = (k <= 32) ? GemmShape::kWideM32 : GemmShape::kSquare;
+    } else {
+      u.M = m;
+      u.N = k;
+      u.K = l;
+      u.A = Qbar;
+      u.lda = l;
+      u.a_mcontig = false;
+      u.B = Wk.p;
+      u.ldb = k;
+      u.C = U.as<double>();
+      u.ldc = k;
+      u.shape = tall_shape(k);
+    }
+    gemm_f64(u, st);
+  }
+};
+
+}  // namespace
+}  // namespace mcnla
+
+// ================================================================================================
+// Stage objects.
+struct mcnla_stage_s {
+  enum Type { kSketcher, kOrthogonalizer, kIntegrator, kFormer } type;
+  int kind = 0;
+  int precision = 0;
+  mcnla_params prm{};
+  std::string name, part_names;
+  mcnla::Timer timer;
+  bool computed = false;
+  explicit mcnla_stage_s(Type t) : type(t) {}
+  virtual ~mcnla_stage_s() = default;
+  int64_t l() const { return prm.rank + prm.over_rank; }
+  int64_t nl() const { return l() * prm.num_sketch; }
+};
+
+namespace mcnla {
+namespace {
+
+struct Sketcher : mcnla_stage_s {
+  uint64_t seed = 0;
+  int mode = MCNLA_OMEGA_PHILOX;
+  int threads = 1;
+  SketchCore core;
+  Sketcher() : mcnla_stage_s(kSketcher) {}
+};
+
+struct Orthogonalizer : mcnla_stage_s {
+  DevBuf W, w, V, T, partial, bad;
+  Orthogonalizer() : mcnla_stage_s(kOrthogonalizer) {}
+};
+
+struct Integrator : mcnla_stage_s {
+  int64_t maxiter = 256;
+  double tol = 1e-4;
+  int64_t iteration = 0;
+  DevBuf Bs, ws, coef;
+  GramCore gram;
+  Integrator() : mcnla_stage_s(kIntegrator) {}
+};
+
+struct Former : mcnla_stage_s {
+  bool jobv = true;
+  FormCore core;
+  Former() : mcnla_stage_s(kFormer) {}
+};
+
+const char* sketcher_name(int kind) {
+  switch (kind) {
+    case MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN: return "Gaussian Projection Sketcher (Column-Block Version)";
+    case MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN: return "Gaussian Projection Sketcher (Row-Block Version)";
+    default: return "Gaussian Projection Sketcher";
+  }
+}
+const char* integrator_name(int kind) {
+  switch (kind) {
+    case MCNLA_INT_KN: return "Kolmogorov-Nagumo-Type Integrator";
+    case MCNLA_INT_ROW_BLOCK_KN: return "Kolmogorov-Nagumo-Type Integrator (Row-Block Version)";
+    case MCNLA_INT_ROW_BLOCK_GRAMIAN_KN:
+      return "Kolmogorov-Nagumo-Type Integrator (Row-Block Gramian Version)";
+    case MCNLA_INT_ROW_BLOCK_WY: return "Wen-Yin Line Search Integrator (Row-Block Version)";
+    default: return "Wen-Yin Line Search Integrator (Row-Block Gramian Version)";
+  }
+}
+const char* former_name(int kind) {
+  switch (kind) {
+    case MCNLA_FORM_SVD: return "SVD Former";
+    case MCNLA_FORM_GRAMIAN: return "Gramian Former";
+    case MCNLA_FORM_COL_BLOCK_GRAMIAN: return "Gramian Former (Column-Block Version)";
+    default: return "Gramian Former (Row-Block Version)";
+  }
+}
+
+bool is_wen_yin(int kind) {
+  return kind == MCNLA_INT_ROW_BLOCK_WY || kind == MCNLA_INT_ROW_BLOCK_GRAMIAN_WY;
+}
+
+hipStream_t S(void* stream) { return static_cast<hipStream_t>(stream); }
+
+// Orthogonalize the collection in place: Q_i = Y_i V_i S_i^{-1/2}.
+int orthogonalize_inplace(Orthogonalizer& o, void* Qs, hipStream_t st) {
+  const auto& p = o.prm;
+  const int l = static_cast<int>(o.l());
+  const int64_t N = p.num_sketch, m = p.nrow, nl = o.nl();
+  const bool f32 = is_mixed(o.precision);
+  o.W.ensure(N * l * l * sizeof(double));
+  o.w.ensure(N * l * sizeof(double));
+  o.V.ensure(N * l * l * sizeof(double));
+  o.T.ensure(N * l * l * sizeof(double));
+  o.bad.ensure(sizeof(int));
+  GemmF64Args g;
+  g.M = l;
+  g.N = l;
+  g.K = m;
+  g.A = Qs;
+  g.lda = nl;
+  g.a_mcontig = true;
+  g.a_f32 = f32;
+  g.B = Qs;
+  g.ldb = nl;
+  g.b_f32 = f32;
+  g.C = o.W.as<double>();
+  g.ldc = l;
+  g.shape = (l <= 32) ? GemmShape::kSmall : GemmShape::kSquare;
+  g.batch = N;
+  g.stride_a = l;
+  g.stride_b = l;
+  g.stride_c = static_cast<int64_t>(l) * l;
+  g.splits = gemm_f64_splits_for(N, m, 2048);
+  g.kchunk = ceil_div(ceil_div(m, g.splits), 16) * 16;
+  g.splits = ceil_div(m, g.kchunk);
+  o.partial.ensure(gemm_f64_partial_elems(l, l, g.splits, N) * sizeof(double));
+  g.partial = o.partial.as<double>();
+  gemm_f64(g, st);
+  MCNLA_HIP(hipMemsetAsync(o.bad.p, 0, sizeof(int), st));
+  batched_syev(o.W.as<double>(), o.w.as<double>(), o.V.as<double>(), N, l, st);
+  orth_transforms(o.w.as<double>(), o.V.as<double>(), o.T.as<double>(), N, l, o.bad.as<int>(), st);
+  apply_blockdiag(Qs, f32, m, N, l, o.T.as<double>(), st);
+  int bad = 0;
+  MCNLA_HIP(hipMemcpyAsync(&bad, o.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  MCNLA_HIP(hipStreamSynchronize(st));
+  return bad;
+}
+
+// Runs a Gram-based integrator over an orthonormal collection Qs; Qbar = Qs * coef.
+mcnla_status integrate(Integrator& it, const void* Qs, double* Qbar, hipStream_t st) {
+  const auto& p = it.prm;
+  const int l = static_cast<int>(it.l());
+  const int64_t N = p.num_sketch, m = p.nrow, nl = it.nl();
+  const bool f32 = is_mixed(it.precision);
+  it.Bs.ensure(nl * nl * sizeof(double));
+  it.ws.ensure(integrator_workspace_bytes(N, l));
+  it.coef.ensure(nl * l * sizeof(double));
+  it.timer.start(st);
+  it.gram.run(Qs, f32, m, nl, it.Bs.as<double>(), st);
+  GramIntegrator gi;
+  gi.N = N;
+  gi.l = l;
+  gi.wen_yin = is_wen_yin(it.kind);
+  gi.maxiter = it.maxiter;
+  gi.tol = it.tol;
+  gi.Bs = it.Bs.as<double>();
+  gi.ws = it.ws.as<char>();
+  gi.ws_bytes = it.ws.bytes;
+  it.timer.mark(st);
+  bool conv = false, ok = true;
+  it.iteration = integrate_gram(gi, it.coef.as<double>(), st, &conv, &ok);
+  it.timer.mark(st);
+  GemmF64Args q;
+  q.M = m;
+  q.N = l;
+  q.K = nl;
+  q.A = Qs;
+  q.lda = nl;
+  q.a_mcontig = false;
+  q.a_f32 = f32;
+  q.B = it.coef.p;
+  q.ldb = l;
+  q.C = Qbar;
+  q.ldc = l;
+  q.shape = tall_shape(l);
+  gemm_f64(q, st);
+  it.timer.mark(st);
+  it.computed = true;
+  if (!ok) {
+    set_error("non-finite value in the integrator's l x l step");
+    return MCNLA_ERR_NUMERIC;
+  }
+  if (!conv && it.maxiter > 0) {
+    set_error("integrator reached maxiter without meeting the tolerance");
+    return MCNLA_ERR_NOT_CONVERGED;
+  }
+  return MCNLA_OK;
+}
+
+}  // namespace
+}  // namespace mcnla
+
+// ================================================================================================
+// Solver.
+struct mcnla_solver_s {
+  mcnla_params prm{};
+  mcnla_solver_config cfg{};
+  mcnla::SketchCore sketch;
+  mcnla::GramCore gram;
+  mcnla::FormCore form;
+  mcnla::DevBuf Y, G, W, w, V, T, Bs, tmp, ws, coef, M2, Qbar, bad;
+  mcnla::Timer timer;
+  int64_t iteration = 0;
+  bool computed = false;
+  bool converged = false;
+};
+
+using namespace mcnla;
+
+extern "C" {
+
+const char* mcnla_last_error(void) { return get_error(); }
+const char* mcnla_version(void) { return "mcnla_amd 0.1.0 (gfx950)"; }
+
+mcnla_status mcnla_device_count(int* count) {
+  return guard([&] {
+    MCNLA_REQUIRE(count, MCNLA_ERR_INVALID, "count is null");
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess || *count == 0) {
+      *count = 0;
+      (void)hipGetLastError();
+      throw Error{MCNLA_ERR_NO_DEVICE, "no HIP device"};
+    }
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_set_device(int device) {
+  return guard([&] {
+    MCNLA_HIP(hipSetDevice(device));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_malloc(void** ptr, size_t bytes) {
+  return guard([&] {
+    MCNLA_REQUIRE(ptr, MCNLA_ERR_INVALID, "ptr is null");
+    *ptr = nullptr;
+    if (bytes == 0) return MCNLA_OK;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      *ptr = nullptr;
+      throw Error{MCNLA_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+    }
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_free(void* ptr) {
+  return guard([&] {
+    if (ptr) MCNLA_HIP(hipFree(ptr));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_memcpy_htod(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    if (bytes) MCNLA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    if (bytes) MCNLA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+  return guard([&] {
+    if (bytes) MCNLA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream)));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_memset(void* ptr, int value, size_t bytes) {
+  return guard([&] {
+    if (bytes) MCNLA_HIP(hipMemset(ptr, value, bytes));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_stream_create(void** stream) {
+  return guard([&] {
+    hipStream_t s;
+    MCNLA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_stream_destroy(void* stream) {
+  return guard([&] {
+    if (stream) MCNLA_HIP(hipStreamDestroy(S(stream)));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_stream_synchronize(void* stream) {
+  return guard([&] {
+    MCNLA_HIP(hipStreamSynchronize(S(stream)));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_device_synchronize(void) {
+  return guard([&] {
+    MCNLA_HIP(hipDeviceSynchronize());
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  return guard([&] {
+    MCNLA_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return MCNLA_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------
+mcnla_status mcnla_sketcher_create(const mcnla_params* params, int kind, int precision,
+                                   mcnla_stage* out) {
+  return guard([&] {
+    check_params(params);
+    check_precision(precision);
+    MCNLA_REQUIRE(out, MCNLA_ERR_INVALID, "out is null");
+    MCNLA_REQUIRE(kind >= MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN && kind <= MCNLA_SKETCHER_GAUSSIAN,
+                  MCNLA_ERR_INVALID, "unknown sketcher kind");
+    auto s = std::make_unique<Sketcher>();
+    s->kind = kind;
+    s->precision = precision;
+    s->prm = *params;
+    s->name = sketcher_name(kind);
+    s->part_names = "random generating / projection";
+    s->core.omega.ensure(params->ncol * s->nl() * sizeof(double));
+    *out = s.release();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_sketcher_set_seed(mcnla_stage s, uint64_t seed, int omega_mode, int threads) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kSketcher, MCNLA_ERR_INVALID, "not a sketcher");
+    MCNLA_REQUIRE(omega_mode == MCNLA_OMEGA_PHILOX || omega_mode == MCNLA_OMEGA_REFERENCE,
+                  MCNLA_ERR_INVALID, "unknown omega mode");
+    auto* k = static_cast<Sketcher*>(s);
+    k->seed = seed;
+    k->mode = omega_mode;
+    k->threads = threads < 1 ? 1 : threads;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_sketcher_run(mcnla_stage s, const void* A, int64_t lda, int a_dtype,
+                                int a_order, void* Y, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kSketcher, MCNLA_ERR_INVALID, "not a sketcher");
+    auto* k = static_cast<Sketcher*>(s);
+    check_a(k->prm, A, lda, a_dtype, a_order);
+    MCNLA_REQUIRE(Y, MCNLA_ERR_INVALID, "Y is null");
+    hipStream_t st = S(stream);
+    k->timer.start(st);
+    k->core.generate(k->prm, k->seed, k->mode, k->threads,
+                     k->kind != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN, st);
+    k->timer.mark(st);
+    k->core.project(k->prm, k->precision, A, lda, a_dtype, a_order, Y, st);
+    k->timer.mark(st);
+    k->computed = true;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_sketcher_omega(mcnla_stage s, const double** omega) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kSketcher, MCNLA_ERR_INVALID, "not a sketcher");
+    MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "sketcher has not run");
+    *omega = static_cast<Sketcher*>(s)->core.omega.as<double>();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_omega_reference(uint64_t seed, int sketcher_kind, int threads, int64_t len,
+                                   double* out) {
+  return guard([&] {
+    MCNLA_REQUIRE(out || len == 0, MCNLA_ERR_INVALID, "out is null");
+    MCNLA_REQUIRE(len >= 0, MCNLA_ERR_INVALID, "negative length");
+    MCNLA_REQUIRE(threads >= 1 && threads <= 1024, MCNLA_ERR_INVALID, "threads out of range");
+    omega_reference(static_cast<uint32_t>(seed), sketcher_kind != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN,
+                    threads, len, out);
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_orthogonalizer_create(const mcnla_params* params, int kind, int precision,
+                                         mcnla_stage* out) {
+  return guard([&] {
+    check_params(params);
+    check_precision(precision);
+    MCNLA_REQUIRE(out, MCNLA_ERR_INVALID, "out is null");
+    MCNLA_REQUIRE(kind == MCNLA_ORTH_GRAMIAN || kind == MCNLA_ORTH_ROW_BLOCK_GRAMIAN,
+                  MCNLA_ERR_INVALID, "unknown orthogonalizer kind");
+    auto o = std::make_unique<Orthogonalizer>();
+    o->kind = kind;
+    o->precision = precision;
+    o->prm = *params;
+    o->name = kind == MCNLA_ORTH_GRAMIAN ? "Gramian Orthogonalizer"
+                                         : "Gramian Orthogonalizer (Row-Block Version)";
+    o->part_names = "computing";
+    *out = o.release();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_orthogonalizer_run(mcnla_stage s, void* Qs, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kOrthogonalizer, MCNLA_ERR_INVALID,
+                  "not an orthogonalizer");
+    MCNLA_REQUIRE(Qs, MCNLA_ERR_INVALID, "Qs is null");
+    auto* o = static_cast<Orthogonalizer*>(s);
+    hipStream_t st = S(stream);
+    o->timer.start(st);
+    const int bad = orthogonalize_inplace(*o, Qs, st);
+    o->timer.mark(st);
+    o->computed = true;
+    if (bad) {
+      set_error(std::to_string(bad) + " sketch Gram matrices are not positive definite");
+      return MCNLA_ERR_NUMERIC;
+    }
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_integrator_create(const mcnla_params* params, int kind, int precision,
+                                     int64_t maxiter, double tol, mcnla_stage* out) {
+  return guard([&] {
+    check_params(params);
+    check_precision(precision);
+    MCNLA_REQUIRE(out, MCNLA_ERR_INVALID, "out is null");
+    MCNLA_REQUIRE(kind >= MCNLA_INT_KN && kind <= MCNLA_INT_ROW_BLOCK_GRAMIAN_WY, MCNLA_ERR_INVALID,
+                  "unknown integrator kind");
+    MCNLA_REQUIRE(maxiter >= 0 && maxiter < (1LL << 30), MCNLA_ERR_INVALID, "bad maxiter");
+    MCNLA_REQUIRE(tol >= 0.0, MCNLA_ERR_INVALID, "tolerance must be non-negative");
+    auto it = std::make_unique<Integrator>();
+    it->kind = kind;
+    it->precision = precision;
+    it->prm = *params;
+    it->maxiter = maxiter;
+    it->tol = tol;
+    it->name = integrator_name(kind);
+    it->part_names = "initializing / iterating / forming";
+    *out = it.release();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_integrator_run(mcnla_stage s, const void* Qs, double* Qbar, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kIntegrator, MCNLA_ERR_INVALID,
+                  "not an integrator");
+    MCNLA_REQUIRE(Qs && Qbar, MCNLA_ERR_INVALID, "null buffer");
+    return integrate(*static_cast<Integrator*>(s), Qs, Qbar, S(stream));
+  });
+}
+
+mcnla_status mcnla_integrator_iteration(mcnla_stage s, int64_t* iteration) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kIntegrator, MCNLA_ERR_INVALID,
+                  "not an integrator");
+    *iteration = static_cast<Integrator*>(s)->iteration;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_former_create(const mcnla_params* params, int kind, int jobv, int precision,
+                                 mcnla_stage* out) {
+  return guard([&] {
+    check_params(params);
+    check_precision(precision);
+    MCNLA_REQUIRE(out, MCNLA_ERR_INVALID, "out is null");
+    MCNLA_REQUIRE(kind >= MCNLA_FORM_SVD && kind <= MCNLA_FORM_ROW_BLOCK_GRAMIAN, MCNLA_ERR_INVALID,
+                  "unknown former kind");
+    auto f = std::make_unique<Former>();
+    f->kind = kind;
+    f->precision = precision;
+    f->prm = *params;
+    f->jobv = jobv != 0;
+    f->name = former_name(kind);
+    f->part_names = "projection / decomposition / forming";
+    *out = f.release();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_former_run(mcnla_stage s, const void* A, int64_t lda, int a_dtype,
+                              int a_order, const double* Qbar, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kFormer, MCNLA_ERR_INVALID, "not a former");
+    auto* f = static_cast<Former*>(s);
+    check_a(f->prm, A, lda, a_dtype, a_order);
+    MCNLA_REQUIRE(Qbar, MCNLA_ERR_INVALID, "Qbar is null");
+    hipStream_t st = S(stream);
+    f->timer.start(st);
+    f->core.run(f->prm, f->kind == MCNLA_FORM_SVD, f->jobv, A, lda, a_dtype, a_order, Qbar, st);
+    f->timer.mark(st);
+    f->computed = true;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_former_results(mcnla_stage s, const double** Sv, const double** U,
+                                  const double** V) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kFormer, MCNLA_ERR_INVALID, "not a former");
+    auto* f = static_cast<Former*>(s);
+    MCNLA_REQUIRE(f->computed, MCNLA_ERR_STATE, "former has not run");
+    if (Sv) *Sv = f->core.S.as<double>();
+    if (U) *U = f->core.U.as<double>();
+    if (V) *V = f->jobv ? f->core.V.as<double>() : nullptr;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_stage_destroy(mcnla_stage s) {
+  return guard([&] {
+    delete s;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_stage_time(mcnla_stage s, double* total, double* parts, int max_parts,
+                              int* nparts) {
+  return guard([&] {
+    MCNLA_REQUIRE(s, MCNLA_ERR_INVALID, "stage is null");
+    MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "stage has not run");
+    const auto v = s->timer.parts();
+    double t = 0.0;
+    for (size_t i = 0; i < v.size(); ++i) {
+      t += v[i];
+      if (parts && static_cast<int>(i) < max_parts) parts[i] = v[i];
+    }
+    if (total) *total = t;
+    if (nparts) *nparts = static_cast<int>(v.size());
+    return MCNLA_OK;
+  });
+}
+
+const char* mcnla_stage_name(mcnla_stage s) { return s ? s->name.c_str() : ""; }
+const char* mcnla_stage_part_names(mcnla_stage s) { return s ? s->part_names.c_str() : ""; }
+
+// ---------------------------------------------------------------------------------------------
+void mcnla_solver_config_default(mcnla_solver_config* cfg) {
+  if (!cfg) return;
+  cfg->sketcher = MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN;
+  cfg->orthogonalizer = MCNLA_ORTH_ROW_BLOCK_GRAMIAN;
+  cfg->integrator = MCNLA_INT_ROW_BLOCK_KN;
+  cfg->former = MCNLA_FORM_SVD;
+  cfg->precision = MCNLA_PREC_F64;
+  cfg->jobv = 1;
+  cfg->maxiter = 256;
+  cfg->tol = 1e-4;
+  cfg->seed = 0;
+  cfg->omega_mode = MCNLA_OMEGA_PHILOX;
+  cfg->omega_threads = 1;
+}
+
+mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_config* cfg,
+                                 mcnla_solver* out) {
+  return guard([&] {
+    check_params(params);
+    MCNLA_REQUIRE(cfg && out, MCNLA_ERR_INVALID, "null argument");
+    check_precision(cfg->precision);
+    MCNLA_REQUIRE(cfg->integrator >= MCNLA_INT_KN && cfg->integrator <= MCNLA_INT_ROW_BLOCK_GRAMIAN_WY,
+                  MCNLA_ERR_INVALID, "unknown integrator kind");
+    MCNLA_REQUIRE(cfg->former >= MCNLA_FORM_SVD && cfg->former <= MCNLA_FORM_ROW_BLOCK_GRAMIAN,
+                  MCNLA_ERR_INVALID, "unknown former kind");
+    MCNLA_REQUIRE(cfg->omega_mode == MCNLA_OMEGA_PHILOX || cfg->omega_mode == MCNLA_OMEGA_REFERENCE,
+                  MCNLA_ERR_INVALID, "unknown omega mode");
+    MCNLA_REQUIRE(cfg->maxiter >= 0 && cfg->tol >= 0.0, MCNLA_ERR_INVALID, "bad maxiter/tol");
+    auto s = std::make_unique<mcnla_solver_s>();
+    s->prm = *params;
+    s->cfg = *cfg;
+    const int64_t l = params->rank + params->over_rank, N = params->num_sketch, nl = l * N;
+    const int64_t m = params->nrow;
+    const size_t yelem = is_mixed(cfg->precision) ? sizeof(float) : sizeof(double);
+    // Allocate the resident workspaces up front (the stages' initialize()).
+    s->Y.ensure(m * nl * yelem);
+    s->G.ensure(nl * nl * sizeof(double));
+    s->Bs.ensure(nl * nl * sizeof(double));
+    s->tmp.ensure(nl * nl * sizeof(double));
+    s->W.ensure(N * l * l * sizeof(double));
+    s->w.ensure(N * l * sizeof(double));
+    s->V.ensure(N * l * l * sizeof(double));
+    s->T.ensure(N * l * l * sizeof(double));
+    s->ws.ensure(integrator_workspace_bytes(N, static_cast<int>(l)));
+    s->coef.ensure(nl * l * sizeof(double));
+    s->M2.ensure(nl * l * sizeof(double));
+    s->Qbar.ensure(m * l * sizeof(double));
+    s->bad.ensure(sizeof(int));
+    s->sketch.omega.ensure(params->ncol * nl * sizeof(double));
+    *out = s.release();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_destroy(mcnla_solver s) {
+  return guard([&] {
+    delete s;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_workspace_bytes(mcnla_solver s, size_t* bytes) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && bytes, MCNLA_ERR_INVALID, "null argument");
+    size_t b = 0;
+    for (const DevBuf* d : {&s->Y, &s->G, &s->W, &s->w, &s->V, &s->T, &s->Bs, &s->tmp, &s->ws,
+                            &s->coef, &s->M2, &s->Qbar, &s->bad, &s->sketch.omega, &s->sketch.ws,
+                            &s->gram.partial, &s->form.Z, &s->form.partial, &s->form.U,
+                            &s->form.V})
+      b += d->bytes;
+    *bytes = b;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_set_seed(mcnla_solver s, uint64_t seed) {
+  return guard([&] {
+    MCNLA_REQUIRE(s, MCNLA_ERR_INVALID, "solver is null");
+    s->cfg.seed = seed;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
+                                int a_order, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(s, MCNLA_ERR_INVALID, "solver is null");
+    check_a(s->prm, A, lda, a_dtype, a_order);
+    const auto& p = s->prm;
+    const auto& c = s->cfg;
+    const int l = static_cast<int>(p.rank + p.over_rank);
+    const int64_t N = p.num_sketch, nl = l * N, m = p.nrow;
+    const bool mixed = is_mixed(c.precision);
+    hipStream_t st = S(stream);
+    s->timer.start(st);
+    // 1. sketch: Y = A * Omega
+    s->sketch.generate(p, c.seed, c.omega_mode, c.omega_threads,
+                       c.sketcher != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN, st);
+    s->sketch.project(p, c.precision, A, lda, a_dtype, a_order, s->Y.p, st);
+    s->timer.mark(st);
+    // 2. orthogonalize through the Gram matrix: G = Y'Y, T_i from eig(G_ii), Bs = T' G T
+    s->gram.run(s->Y.p, mixed, m, nl, s->G.as<double>(), st);
+    gram_diag_blocks(s->G.as<double>(), nl, s->W.as<double>(), N, l, st);
+    MCNLA_HIP(hipMemsetAsync(s->bad.p, 0, sizeof(int), st));
+    batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), N, l, st);
+    orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T.as<double>(), N, l, s->bad.as<int>(),
+                    st);
+    congruence_blockdiag(s->G.as<double>(), s->T.as<double>(), s->Bs.as<double>(),
+                         s->tmp.as<double>(), N, l, st);
+    s->timer.mark(st);
+    // 3. integrate on Bs, then Qbar = Y * (T * coef) in one pass over Y
+    GramIntegrator gi;
+    gi.N = N;
+    gi.l = l;
+    gi.wen_yin = is_wen_yin(c.integrator);
+    gi.maxiter = c.maxiter;
+    gi.tol = c.tol;
+    gi.Bs = s->Bs.as<double>();
+    gi.ws = s->ws.as<char>();
+    gi.ws_bytes = s->ws.bytes;
+    bool conv = false, ok = true;
+    s->iteration = integrate_gram(gi, s->coef.as<double>(), st, &conv, &ok);
+    s->converged = conv;
+    GemmF64Args tm;  // M2_i = T_i * coef_i  (batched l x l)
+    tm.M = l;
+    tm.N = l;
+    tm.K = l;
+    tm.A = s->T.p;
+    tm.lda = l;
+    tm.a_mcontig = false;
+    tm.B = s->coef.p;
+    tm.ldb = l;
+    tm.C = s->M2.as<double>();
+    tm.ldc = l;
+    tm.shape = GemmShape::kSmall;
+    tm.batch = N;
+    tm.stride_a = static_cast<int64_t>(l) * l;
+    tm.stride_b = static_cast<int64_t>(l) * l;
+    tm.stride_c = static_cast<int64_t>(l) * l;
+    gemm_f64(tm, st);
+    GemmF64Args q;
+    q.M = m;
+    q.N = l;
+    q.K = nl;
+    q.A = s->Y.p;
+    q.lda = nl;
+    q.a_mcontig = false;
+    q.a_f32 = mixed;
+    q.B = s->M2.p;
+    q.ldb = l;
+    q.C = s->Qbar.as<double>();
+    q.ldc = l;
+    q.shape = tall_shape(l);
+    gemm_f64(q, st);
+    s->timer.mark(st);
+    // 4. form
+    s->form.run(p, c.former == MCNLA_FORM_SVD, c.jobv != 0, A, lda, a_dtype, a_order,
+                s->Qbar.as<double>(), st);
+    s->timer.mark(st);
+    s->computed = true;
+    int bad = 0;
+    MCNLA_HIP(hipMemcpyAsync(&bad, s->bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    MCNLA_HIP(hipStreamSynchronize(st));
+    if (bad || !ok) {
+      set_error(bad ? "a sketch Gram matrix is not positive definite"
+                    : "non-finite value in the integrator's l x l step");
+      return MCNLA_ERR_NUMERIC;
+    }
+    if (!conv && c.maxiter > 0) {
+      set_error("integrator reached maxiter without meeting the tolerance");
+      return MCNLA_ERR_NOT_CONVERGED;
+    }
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_results(mcnla_solver s, const double** Sv, const double** U,
+                                  const double** V, const double** Qbar) {
+  return guard([&] {
+    MCNLA_REQUIRE(s, MCNLA_ERR_INVALID, "solver is null");
+    MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "solver has not run");
+    if (Sv) *Sv = s->form.S.as<double>();
+    if (U) *U = s->form.U.as<double>();
+    if (V) *V = s->cfg.jobv ? s->form.V.as<double>() : nullptr;
+    if (Qbar) *Qbar = s->Qbar.as<double>();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_iteration(mcnla_solver s, int64_t* iteration) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && iteration, MCNLA_ERR_INVALID, "null argument");
+    *iteration = s->iteration;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* sketch_kernel_s) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && times, MCNLA_ERR_INVALID, "null argument");
+    MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "solver has not run");
+    const auto v = s->timer.parts();
+    for (int i = 0; i < 4; ++i) times[i] = i < static_cast<int>(v.size()) ? v[i] : 0.0;
+    if (sketch_kernel_s) *sketch_kernel_s = s->sketch.gemm_seconds();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
+                                 const double* sigma, double noise, uint64_t seed, void* A,
+                                 int64_t lda, int a_dtype, int a_order, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(m >= 1 && n >= 1 && r >= 0 && A, MCNLA_ERR_INVALID, "bad synth arguments");
+    MCNLA_REQUIRE(r == 0 || (U && V && sigma), MCNLA_ERR_INVALID, "null factors");
+    MCNLA_REQUIRE(lda >= (a_order == MCNLA_COL_MAJOR ? m : n), MCNLA_ERR_INVALID, "lda too small");
+    synth_lowrank(m, n, r, U, V, sigma, noise, seed, A, lda, a_dtype == MCNLA_F32,
+                  a_order == MCNLA_COL_MAJOR, S(stream));
+    return MCNLA_OK;
+  });
+}
+
+}  // extern "C"

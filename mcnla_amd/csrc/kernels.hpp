@@ -1,0 +1,124 @@
+// Host-side launchers of the HIP kernels of libmcnla_amd.
+#pragma once
+
+#include "common.hpp"
+
+namespace mcnla {
+
+// ------------------------------------------------------------------------------------------------
+// FP64 MFMA GEMM (gemm_f64.hip).
+enum class GemmShape { kSquare, kTallN32, kTallN64, kWideM32, kSmall };
+
+struct GemmF64Args {
+  int64_t M = 0, N = 0, K = 0;
+  const void* A = nullptr;  // element (i, k) at A[i + k*lda] if a_mcontig else A[i*lda + k]
+  int64_t lda = 0;
+  bool a_mcontig = true;
+  bool a_f32 = false;
+  const void* B = nullptr;  // element (k, j) at B[k*ldb + j] if b_ncontig else B[k + j*ldb]
+  int64_t ldb = 0;
+  bool b_ncontig = true;
+  bool b_f32 = false;
+  double* C = nullptr;      // row-major, ldc; may be null when only partials are wanted
+  int64_t ldc = 0;
+  double alpha = 1.0, beta = 0.0;
+  int64_t splits = 1;       // split-K factor (requires `partial`)
+  int64_t kchunk = 0;       // K range per split (rounded up to the K tile); 0 = K / splits
+  double* partial = nullptr;  // splits * M * N scratch for split-K
+  bool lower_only = false;  // Gram: compute tiles with tm >= tn only, mirror on reduction
+  GemmShape shape = GemmShape::kSquare;
+  int64_t batch = 1;        // independent problems; operand b starts at A + b*stride_a, ...
+  int64_t stride_a = 0, stride_b = 0, stride_c = 0;
+};
+
+void gemm_f64(GemmF64Args p, hipStream_t st);
+size_t gemm_f64_partial_elems(int64_t M, int64_t N, int64_t splits, int64_t batch = 1);
+// Split-K factor giving about target_blocks workgroups for `tiles` output tiles.
+int64_t gemm_f64_splits_for(int64_t tiles, int64_t K, int64_t target_blocks);
+
+// ------------------------------------------------------------------------------------------------
+// Mixed-precision sketch (gemm_bf16.hip): Y (FP32, row-major m x nl) = A * Omega with A and
+// Omega rounded to BF16 (one term) or split hi+lo (three products), FP32 accumulation.
+struct SketchBf16Args {
+  int64_t m = 0, n = 0, nl = 0;
+  const void* A = nullptr;  // FP32 or FP64 source
+  int64_t lda = 0;
+  bool a_colmajor = true;
+  bool a_f32 = true;
+  const double* omega = nullptr;  // n x nl row-major FP64
+  float* Y = nullptr;
+  int64_t ldy = 0;
+  int terms = 1;  // 1 or 3
+};
+// Packs A and Omega, then runs the GEMM; the optional events bracket the GEMM launch alone.
+void sketch_bf16(const SketchBf16Args& a, void* workspace, size_t workspace_bytes, hipStream_t st,
+                 hipEvent_t gemm_begin = nullptr, hipEvent_t gemm_end = nullptr);
+size_t sketch_bf16_workspace(const SketchBf16Args& a);
+
+// ------------------------------------------------------------------------------------------------
+// Random numbers and synthetic data (rng.hip).
+void omega_philox(double* out, int64_t count, uint64_t seed, hipStream_t st);
+void synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
+                   const double* sigma, double noise, uint64_t seed, void* A, int64_t lda,
+                   bool a_f32, bool a_colmajor, hipStream_t st);
+
+// ------------------------------------------------------------------------------------------------
+// Small dense kernels (small.hip).  All FP64, l <= kMaxL.
+constexpr int kMaxL = 64;
+
+// Batched symmetric eigen-decomposition (parallel cyclic Jacobi, one workgroup per matrix).
+// In: W[b] (n x n row-major, symmetric, stride n*n).  Out: w[b] (n, DESCENDING), V[b] (n x n,
+// eigenvectors in columns, each with its largest-magnitude entry positive).
+void batched_syev(const double* W, double* w, double* V, int64_t batch, int n, hipStream_t st);
+
+// Orthogonalizer transforms T_i = V_i diag(1/sqrt(w_i)) for every sketch, written into the
+// block-diagonal Nl x l stack Tst (row-major; rows i*l..(i+1)*l hold T_i).  Flags non-positive
+// eigenvalues in *bad (atomic add).
+void orth_transforms(const double* w, const double* V, double* Tst, int64_t N, int l, int* bad,
+                     hipStream_t st);
+
+// Copies the N diagonal l x l blocks of an Nl x Nl matrix G into W (N x l x l).
+void gram_diag_blocks(const double* G, int64_t ldg, double* W, int64_t N, int l, hipStream_t st);
+
+// Bs = T' G T with T = blockdiag(T_0..T_{N-1}) (Tst as above); Bs and G are Nl x Nl row-major.
+// tmp: Nl x Nl scratch.
+void congruence_blockdiag(const double* G, const double* Tst, double* Bs, double* tmp, int64_t N,
+                          int l, hipStream_t st);
+
+// In-place per-sketch apply of the collection: Q_i = Y_i T_i  (Y row-major m x Nl, FP64 or FP32).
+void apply_blockdiag(void* Y, bool y_f32, int64_t m, int64_t N, int l, const double* Tst,
+                     hipStream_t st);
+
+// ------------------------------------------------------------------------------------------------
+// Integrator state machine on the Gram matrix Bs (integrate.hip).
+struct IntegratorState;  // device-side scalars (flags, counters, tau, phi, ...)
+
+struct GramIntegrator {
+  int64_t N = 0;
+  int l = 0;
+  bool wen_yin = false;
+  int64_t maxiter = 256;
+  double tol = 1e-4;
+  // device buffers (owned by the caller, sized by integrator_workspace_bytes)
+  double* Bs = nullptr;   // Nl x Nl
+  char* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+size_t integrator_workspace_bytes(int64_t N, int l);
+// Runs the whole iteration on the device.  On return (stream-ordered) the Nl x l coefficient
+// matrix M with Qbar = Qs * M is in `coef` (Qs = the orthonormal collection whose Gram is Bs).
+// Returns the iteration count (this call synchronizes the stream once per chunk of iterations).
+int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bool* converged,
+                       bool* numeric_ok);
+
+// ------------------------------------------------------------------------------------------------
+// Former helpers (former.hip).
+// Given Z = A' Qbar (n x l row-major) and Wg = Z'Z (l x l): eig, S (k), the l x k rotation
+// Wk (row-major), and, if V != null, V = Z Wk diag(1/S) (n x k row-major).
+// svd_sigma: compute S as ||Z w_j|| instead of sqrt(eig) (the SVD former's singular values).
+void former_small(const double* Wg, int l, int k, double* eigw, double* eigv, double* Wk,
+                  double* S, hipStream_t st);
+void scale_columns(double* X, int64_t rows, int k, const double* S, bool inverse, hipStream_t st);
+void column_norms(const double* X, int64_t rows, int k, double* out, hipStream_t st);
+
+}  // namespace mcnla

@@ -1,0 +1,116 @@
+// Counter-based Gaussian random numbers (Philox4x32-10, Salmon et al., SC'11, + Box-Muller) and
+// the synthetic low-rank test-matrix generator.
+//
+// Element e of a stream depends only on (seed, e): pair p = e / 2 uses counter (p, 0, 0, 0) and
+// key (seed_lo, seed_hi); its four 32-bit outputs make two 53-bit uniforms u1 in (0,1],
+// u2 in [0,1), and z = sqrt(-2 ln u1) * (cos, sin)(2 pi u2) gives elements 2p and 2p+1.  The
+// result is independent of the launch shape, so every precision and every shape of a solve sees
+// the same Omega.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mcnla {
+
+namespace {
+
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 ctr, uint32_t k0, uint32_t k1) {
+  constexpr uint32_t kM0 = 0xD2511F53u, kM1 = 0xCD9E8D57u;
+  constexpr uint32_t kW0 = 0x9E3779B9u, kW1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(kM0, ctr.x), lo0 = kM0 * ctr.x;
+    const uint32_t hi1 = __umulhi(kM1, ctr.z), lo1 = kM1 * ctr.z;
+    ctr = U4{hi1 ^ ctr.y ^ k0, lo1, hi0 ^ ctr.w ^ k1, lo0};
+    k0 += kW0;
+    k1 += kW1;
+  }
+  return ctr;
+}
+
+__device__ __forceinline__ void gaussian_pair(uint64_t seed, uint64_t pair, double& z0,
+                                              double& z1) {
+  const U4 r = philox4x32_10(U4{static_cast<uint32_t>(pair), static_cast<uint32_t>(pair >> 32), 0u, 0u},
+                             static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+  const uint64_t a = (static_cast<uint64_t>(r.x) << 21) ^ (r.y >> 11);  // 53 bits
+  const uint64_t b = (static_cast<uint64_t>(r.z) << 21) ^ (r.w >> 11);
+  const double u1 = (static_cast<double>(a & ((1ull << 53) - 1)) + 1.0) * 0x1.0p-53;  // (0, 1]
+  const double u2 = static_cast<double>(b & ((1ull << 53) - 1)) * 0x1.0p-53;          // [0, 1)
+  const double rad = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  z0 = rad * c;
+  z1 = rad * s;
+}
+
+__global__ void omega_philox_kernel(double* __restrict__ out, int64_t count, uint64_t seed) {
+  const int64_t npairs = (count + 1) / 2;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npairs;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    double z0, z1;
+    gaussian_pair(seed, static_cast<uint64_t>(p), z0, z1);
+    out[2 * p] = z0;
+    if (2 * p + 1 < count) out[2 * p + 1] = z1;
+  }
+}
+
+// A(i, j) = sum_t U(i, t) sigma(t) V(j, t) + noise * g(i + j * m): the noise of element (i, j)
+// is element i + j*m of the Philox stream, so the same A results for both storage orders.
+// Threads walk the storage order so that writes are coalesced.
+template <typename T>
+__global__ void synth_kernel(int64_t m, int64_t n, int64_t r, const double* __restrict__ U,
+                             const double* __restrict__ V, const double* __restrict__ sigma,
+                             double noise, uint64_t seed, T* __restrict__ A, int64_t lda,
+                             bool colmajor) {
+  const int64_t total = m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = colmajor ? e % m : e / n;
+    const int64_t j = colmajor ? e / m : e % n;
+    double acc = 0.0;
+    for (int64_t t = 0; t < r; ++t) acc += U[i * r + t] * sigma[t] * V[j * r + t];
+    if (noise != 0.0) {
+      const int64_t canon = i + j * m;
+      double z0, z1;
+      gaussian_pair(seed, static_cast<uint64_t>(canon >> 1), z0, z1);
+      acc += noise * ((canon & 1) ? z1 : z0);
+    }
+    A[colmajor ? (i + j * lda) : (i * lda + j)] = static_cast<T>(acc);
+  }
+}
+
+}  // namespace
+
+void omega_philox(double* out, int64_t count, uint64_t seed, hipStream_t st) {
+  if (count <= 0) return;
+  const int threads = 256;
+  int64_t blocks = ceil_div((count + 1) / 2, threads);
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(omega_philox_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0,
+                     st, out, count, seed);
+  MCNLA_LAUNCHED();
+}
+
+void synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
+                   const double* sigma, double noise, uint64_t seed, void* A, int64_t lda,
+                   bool a_f32, bool a_colmajor, hipStream_t st) {
+  if (m <= 0 || n <= 0) return;
+  const int threads = 256;
+  int64_t blocks = ceil_div(m * n, threads);
+  if (blocks > 65536) blocks = 65536;
+  if (a_f32) {
+    hipLaunchKernelGGL(synth_kernel<float>, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0,
+                       st, m, n, r, U, V, sigma, noise, seed, static_cast<float*>(A), lda,
+                       a_colmajor);
+  } else {
+    hipLaunchKernelGGL(synth_kernel<double>, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0,
+                       st, m, n, r, U, V, sigma, noise, seed, static_cast<double*>(A), lda,
+                       a_colmajor);
+  }
+  MCNLA_LAUNCHED();
+}
+
+}  // namespace mcnla

@@ -1,0 +1,122 @@
+"""The C-ABI boundary on the CPU: the library loads, exports every function include/*.h declares
+(and nothing undeclared under the mcnla_ prefix), and rejects bad arguments before touching a
+device.  No compute runs here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import mcnla_amd as mc
+from mcnla_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include", "mcnla_amd")
+    for f in os.listdir(inc):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(mcnla_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def exported_functions():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if re.search(r" T mcnla_", ln)}
+
+
+def test_library_loads():
+    assert mc.lib() is not None
+    assert b"gfx950" in mc.lib().mcnla_version()
+
+
+def test_every_declared_symbol_is_exported():
+    decl = declared_functions()
+    assert len(decl) > 30
+    exp = exported_functions()
+    assert decl <= exp, sorted(decl - exp)
+    assert exp <= decl, sorted(exp - decl)  # nothing exported that the header does not declare
+    assert decl == set(_lib.SIGNATURES), sorted(decl ^ set(_lib.SIGNATURES))
+
+
+def test_header_is_plain_c():
+    # the boundary must compile as C (no torch/HIP/C++ types in signatures)
+    hdr = os.path.join(ROOT, "include", "mcnla_amd", "isvd_capi.h")
+    r = subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-x", "c", hdr], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def _params(m=100, n=80, k=5, p=5, N=4):
+    return _lib.Params(m, n, k, p, N)
+
+
+@pytest.mark.parametrize("prm,msg", [
+    (_params(k=60, p=10), "rank + over_rank"),
+    (_params(m=10, k=8, p=4), "rank + over_rank"),
+    (_params(N=0), "num_sketch"),
+    (_params(k=0), "rank"),
+    (_params(m=0), "matrix size"),
+])
+def test_invalid_parameters_rejected(prm, msg):
+    h = ctypes.c_void_p()
+    L = mc.lib()
+    for st in (L.mcnla_sketcher_create(ctypes.byref(prm), 0, 0, ctypes.byref(h)),
+               L.mcnla_orthogonalizer_create(ctypes.byref(prm), 0, 0, ctypes.byref(h)),
+               L.mcnla_integrator_create(ctypes.byref(prm), 1, 0, 256, 1e-4, ctypes.byref(h)),
+               L.mcnla_former_create(ctypes.byref(prm), 0, 1, 0, ctypes.byref(h))):
+        assert st == _lib.ERR_INVALID
+        assert msg in L.mcnla_last_error().decode()
+
+
+def test_invalid_kinds_rejected():
+    L = mc.lib()
+    h = ctypes.c_void_p()
+    prm = _params()
+    assert L.mcnla_integrator_create(ctypes.byref(prm), 9, 0, 256, 1e-4, ctypes.byref(h)) == _lib.ERR_INVALID
+    assert L.mcnla_integrator_create(ctypes.byref(prm), 1, 0, -1, 1e-4, ctypes.byref(h)) == _lib.ERR_INVALID
+    assert L.mcnla_integrator_create(ctypes.byref(prm), 1, 0, 10, -1.0, ctypes.byref(h)) == _lib.ERR_INVALID
+    assert L.mcnla_former_create(ctypes.byref(prm), 7, 1, 0, ctypes.byref(h)) == _lib.ERR_INVALID
+    assert L.mcnla_sketcher_create(ctypes.byref(prm), 0, 9, ctypes.byref(h)) == _lib.ERR_INVALID
+    cfg = _lib.SolverConfig()
+    L.mcnla_solver_config_default(ctypes.byref(cfg))
+    assert cfg.integrator == _lib.INT_ROW_BLOCK_KN and cfg.maxiter == 256 and cfg.tol == 1e-4
+    cfg.omega_mode = 5
+    assert L.mcnla_solver_create(ctypes.byref(prm), ctypes.byref(cfg), ctypes.byref(h)) == _lib.ERR_INVALID
+
+
+def test_reference_omega_matches_oracle():
+    import oracle
+    L = mc.lib()
+    for kind, cb in ((_lib.SKETCHER_COL_BLOCK_GAUSSIAN, True), (_lib.SKETCHER_ROW_BLOCK_GAUSSIAN, False),
+                     (_lib.SKETCHER_GAUSSIAN, True)):
+        for threads in (1, 3, 8):
+            out = np.empty(1234)
+            assert L.mcnla_omega_reference(2024, kind, threads, out.size,
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+            ref = oracle.omega_ref(2024, 1, 1234, col_block=cb, threads=threads).ravel()
+            assert np.array_equal(out, ref)
+
+
+def test_python_mirror_names():
+    # the reference's stage class names are all available (src/isvd_dense_cb.cpp:20-34 macros)
+    for name in ("ColBlockGaussianProjectionSketcher", "RowBlockGaussianProjectionSketcher",
+                 "GaussianProjectionSketcher", "GramianOrthogonalizer",
+                 "RowBlockGramianOrthogonalizer", "KolmogorovNagumoIntegrator",
+                 "RowBlockKolmogorovNagumoIntegrator", "RowBlockGramianKolmogorovNagumoIntegrator",
+                 "RowBlockWenYinIntegrator", "RowBlockGramianWenYinIntegrator", "SvdFormer",
+                 "GramianFormer", "ColBlockGramianFormer", "RowBlockGramianFormer",
+                 "CollectionFromPartialSumToRowBlockConverter", "MatrixFromRowBlockToAllConverter",
+                 "MatrixFromColBlockToAllConverter", "MatrixFromRowBlockConverter", "Parameters"):
+        assert hasattr(mc, name), name
+    P = mc.Parameters().setSize(2000, 1000).setRank(10).setOverRank(10).setNumSketch(4)
+    assert P.dimSketch() == 20 and P.dimSketchTotal() == 80 and P.nrowRank() == 2000

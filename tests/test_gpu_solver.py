@@ -1,0 +1,167 @@
+"""End-to-end GPU parity of the fused Solver (sketch -> orthogonalize -> integrate -> form) on
+the CPU-runnable workload (A 2000 x 1000 FP64 = U diag(0.95^i + 1e-6) V', k=10, p=10, N=4, KN tol
+1e-4, SVD former), with the reference's Omega stream, against the oracle pipeline; and the mixed
+precision paths against the FP64 path on the same Omega (target: sigma rel err <= 1e-5)."""
+import sys
+
+import numpy as np
+import pytest
+
+import mcnla_amd as mc
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def proj(Q):
+    return Q @ Q.T
+
+
+M, N_COL, K, P_OS, NSK, SEED = 2000, 1000, 10, 10, 4, 2
+
+
+@pytest.fixture(scope="module")
+def problem():
+    rng = np.random.default_rng(1)
+    U, _ = np.linalg.qr(rng.standard_normal((M, N_COL)))
+    V, _ = np.linalg.qr(rng.standard_normal((N_COL, N_COL)))
+    sig = 0.95 ** np.arange(N_COL) + 1e-6
+    A = (U * sig) @ V.T
+    l = K + P_OS
+    om = oracle.omega_ref(SEED, N_COL, NSK * l, col_block=True)
+    Y = oracle.sketch(A, om)
+    Q, _ = oracle.gramian_orthogonalize(Y, l, NSK)
+    out = {"A": A, "sig": sig, "Q": Q}
+    for name, fn in (("kn", oracle.kn_rowblock), ("wy", oracle.wy_rowblock)):
+        tol = 1e-4 if name == "kn" else 1e-3
+        Qb, it = fn(Q, l, NSK, 256, tol)
+        S, Uo, Vo = oracle.svd_former(A, Qb, K)
+        out[name] = (Qb, it, S, Uo, Vo)
+    return out
+
+
+def params():
+    return mc.Parameters().setSize(M, N_COL).setRank(K).setOverRank(P_OS).setNumSketch(NSK)
+
+
+@pytest.mark.parametrize("integ,key", [(mc.RowBlockKolmogorovNagumoIntegrator, "kn"),
+                                       (mc.KolmogorovNagumoIntegrator, "kn"),
+                                       (mc.RowBlockWenYinIntegrator, "wy"),
+                                       (mc.RowBlockGramianWenYinIntegrator, "wy")])
+def test_solver_f64_matches_oracle(gpu, problem, integ, key):
+    note("solver f64 %s" % integ.__name__)
+    A = problem["A"]
+    Qb_o, it_o, S_o, U_o, V_o = problem[key]
+    s = mc.Solver(params(), integrator=integ, seed=SEED, omega_mode=mc.OMEGA_REFERENCE)
+    s.solve(mc.DeviceArray.from_numpy(np.asfortranarray(A)))
+    assert s.converged
+    assert abs(s.iteration() - it_o) <= 1
+    S, U, V = s.vectorS(), s.matrixU(), s.matrixV()
+    assert np.max(np.abs(S - S_o) / S_o) < 1e-9
+    assert np.abs(proj(U) - proj(U_o)).max() < 1e-8
+    assert np.abs(proj(V) - proj(V_o)).max() < 1e-8
+    # subspace distance ||Qb Qb' - Qb_ref Qb_ref'||_2
+    Qb = s.matrixQbar()
+    assert np.linalg.norm(proj(Qb) - proj(Qb_o), 2) < 1e-7
+    assert np.allclose(Qb.T @ Qb, np.eye(K + P_OS), atol=1e-12)
+    err = oracle.frerr(A, S, U, V)
+    assert abs(err - oracle.frerr(A, S_o, U_o, V_o)) < 1e-9
+    times, gemm = s.stage_times()
+    assert all(t > 0 for t in times) and 0 < gemm <= times[0]
+
+
+def test_stage_pipeline_matches_solver(gpu, problem):
+    """The four stage objects called like src/isvd_dense_cb.cpp:194-230 give the solver's
+    result."""
+    note("stage pipeline c1")
+    A = problem["A"]
+    Qb_o, it_o, S_o, U_o, V_o = problem["kn"]
+    P = params()
+    sk = mc.ColBlockGaussianProjectionSketcher(P, SEED, omega_mode=mc.OMEGA_REFERENCE)
+    orth = mc.RowBlockGramianOrthogonalizer(P)
+    integ = mc.RowBlockKolmogorovNagumoIntegrator(P)
+    form = mc.SvdFormer(P, True)
+    so = mc.CollectionFromPartialSumToRowBlockConverter(P)
+    iff = mc.MatrixFromRowBlockToAllConverter(P)
+    for st in (sk, orth, integ, form, so, iff):
+        st.initialize()
+    dA = mc.DeviceArray.from_numpy(np.asfortranarray(A))
+    Qjp, Qbarj = P.createCollectionQjp(), P.createMatrixQbarj()
+    sk(dA, Qjp)
+    so(Qjp, Qjp)
+    orth(Qjp)
+    integ(Qjp, Qbarj)
+    iff(Qbarj, Qbarj)
+    form(dA, Qbarj)
+    assert abs(integ.iteration() - it_o) <= 1
+    assert np.max(np.abs(form.vectorS() - S_o) / S_o) < 1e-9
+    assert np.abs(proj(form.matrixU()) - proj(U_o)).max() < 1e-8
+    assert np.abs(proj(Qbarj.numpy()) - proj(Qb_o)).max() < 1e-8
+    for st in (sk, orth, integ, form):
+        assert st.time() > 0
+
+
+def test_solver_rowmajor_and_gramian_former(gpu, problem):
+    note("solver row-major A + row-block Gramian former")
+    A = problem["A"]
+    l = K + P_OS
+    om = oracle.omega_ref(SEED, N_COL, NSK * l, col_block=False)
+    Q, _ = oracle.gramian_orthogonalize(oracle.sketch(A, om, a_colmajor=False), l, NSK)
+    Qb_o, it_o = oracle.kn_rowblock(Q, l, NSK, 256, 1e-4)
+    S_o, U_o, V_o = oracle.gramian_former(A, Qb_o, K, a_colmajor=False)
+    s = mc.Solver(params(), sketcher=mc.RowBlockGaussianProjectionSketcher,
+                  former=mc.RowBlockGramianFormer, seed=SEED, omega_mode=mc.OMEGA_REFERENCE)
+    s.solve(mc.DeviceArray.from_numpy(np.ascontiguousarray(A)))
+    assert abs(s.iteration() - it_o) <= 1
+    assert np.max(np.abs(s.vectorS() - S_o) / S_o) < 1e-9
+    assert np.abs(proj(s.matrixU()) - proj(U_o)).max() < 1e-8
+
+
+@pytest.mark.parametrize("prec,bound", [(mc.PREC_MIXED_BF16, 1e-5), (mc.PREC_MIXED_BF16X3, 1e-7)])
+def test_solver_mixed_vs_f64(gpu, problem, prec, bound):
+    """Mixed precision on the same Omega (Philox, seed 5) as the FP64 solve: sigma relative
+    error within `bound`, iteration count within +-1, subspace distance stated."""
+    note("solver mixed precision %d" % prec)
+    A = problem["A"]
+    P = params()
+    ref = mc.Solver(P, seed=5)
+    ref.solve(mc.DeviceArray.from_numpy(np.asfortranarray(A)))
+    mix = mc.Solver(P, seed=5, precision=prec)
+    mix.solve(mc.DeviceArray.from_numpy(np.asfortranarray(A.astype(np.float32))))
+    S64, Smx = ref.vectorS(), mix.vectorS()
+    rel = np.max(np.abs(Smx - S64) / S64)
+    dist = np.linalg.norm(proj(mix.matrixQbar()) - proj(ref.matrixQbar()), 2)
+    note("  sigma rel err %.3e, subspace distance %.3e, iterations %d vs %d"
+         % (rel, dist, mix.iteration(), ref.iteration()))
+    assert rel <= bound
+    assert abs(mix.iteration() - ref.iteration()) <= 1
+
+
+def test_solver_reuse_and_seed(gpu, problem):
+    note("solver reuse")
+    A = problem["A"]
+    s = mc.Solver(params(), seed=11)
+    dA = mc.DeviceArray.from_numpy(np.asfortranarray(A))
+    s.solve(dA)
+    S1 = s.vectorS()
+    s.solve(dA)
+    assert np.array_equal(S1, s.vectorS())  # deterministic
+    s.setSeed(12).solve(dA)
+    S2 = s.vectorS()
+    assert not np.array_equal(S1, S2)
+    assert np.max(np.abs(S1 - S2) / S1) < 0.1
+
+
+def test_invalid_solve_arguments(gpu):
+    note("solver invalid arguments")
+    P = params()
+    s = mc.Solver(P)
+    bad = mc.DeviceArray((M - 1, N_COL), np.float64, "F")
+    with pytest.raises(mc.McnlaError) as e:
+        s.solve(bad)
+    assert e.value.status == mc._lib.ERR_INVALID

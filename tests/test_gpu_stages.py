@@ -1,0 +1,212 @@
+"""GPU parity of each stage, through the C ABI (libmcnla_amd), against the reference's golden
+fixtures and the CPU oracle.  Tolerances are those of the reference's own tests (1e-8, element-wise
+for the integrators, on projectors for the orthogonalizer and formers) unless stated."""
+import sys
+
+import numpy as np
+import pytest
+
+import mcnla_amd as mc
+import mtx
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def proj(Q):
+    return Q @ Q.T
+
+
+def params(m, n, k, p, N):
+    return mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+
+
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(101, 1001, 11, 0, 12), (2000, 1000, 10, 10, 4), (257, 129, 3, 8, 3)])
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_sketch_f64(gpu, shape, order):
+    m, n, k, p, N = shape
+    note("sketch f64 %s order=%s" % (shape, order))
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((m, n))
+    P = params(m, n, k, p, N)
+    sk = mc.ColBlockGaussianProjectionSketcher(P, seed=5)
+    sk.initialize()
+    dA = mc.DeviceArray.from_numpy(np.asfortranarray(A) if order == "F" else np.ascontiguousarray(A))
+    Y = P.createCollectionQ()
+    sk(dA, Y)
+    om = sk.omega()
+    assert om.shape == (n, (k + p) * N)
+    ref = A @ om
+    assert np.abs(Y.numpy() - ref).max() < 1e-12 * np.abs(ref).max()
+    # Philox Omega: N(0,1) moments and determinism per seed
+    assert abs(om.mean()) < 0.05 and abs(om.std() - 1) < 0.05
+    sk(dA, Y)
+    assert np.array_equal(sk.omega(), om)
+
+
+@pytest.mark.parametrize("kind,cb", [(mc.ColBlockGaussianProjectionSketcher, True),
+                                     (mc.RowBlockGaussianProjectionSketcher, False)])
+def test_sketch_reference_omega(gpu, kind, cb):
+    m, n, k, p, N = 300, 200, 5, 5, 3
+    note("sketch reference omega %s" % kind.__name__)
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((m, n))
+    P = params(m, n, k, p, N)
+    sk = kind(P, seed=77, omega_mode=mc.OMEGA_REFERENCE, omega_threads=4)
+    Y = P.createCollectionQ()
+    order = "F" if cb else "C"
+    sk(mc.DeviceArray.from_numpy(np.asarray(A, order=order)), Y)
+    om_ref = oracle.omega_ref(77, n, N * (k + p), col_block=cb, threads=4)
+    assert np.array_equal(sk.omega(), om_ref)  # bit-exact stream
+    assert np.abs(Y.numpy() - oracle.sketch(A, om_ref, a_colmajor=cb)).max() < 1e-11
+
+
+def bf16_round(x):
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_sketch_mixed(gpu, order):
+    m, n, k, p, N = 333, 517, 10, 6, 5
+    note("sketch mixed order=%s" % order)
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((m, n)).astype(np.float32)
+    P = params(m, n, k, p, N)
+    dA = mc.DeviceArray.from_numpy(np.asarray(A, order=order))
+    out = {}
+    for prec in (mc.PREC_MIXED_BF16, mc.PREC_MIXED_BF16X3):
+        sk = mc.ColBlockGaussianProjectionSketcher(P, seed=9, precision=prec)
+        Y = P.createCollectionQ(np.float32)
+        sk(dA, Y)
+        om = sk.omega()
+        out[prec] = (Y.numpy().astype(np.float64), om)
+    Y1, om = out[mc.PREC_MIXED_BF16]
+    exact = A.astype(np.float64) @ om
+    scale = np.abs(exact).max()
+    # one term: exactly the product of the BF16-rounded operands, up to FP32 accumulation
+    pinned = bf16_round(A) @ bf16_round(om)
+    assert np.abs(Y1 - pinned).max() < 2e-5 * scale
+    assert np.linalg.norm(Y1 - exact) / np.linalg.norm(exact) < 6e-3
+    Y3, om3 = out[mc.PREC_MIXED_BF16X3]
+    assert np.array_equal(om3, om)
+    assert np.linalg.norm(Y3 - exact) / np.linalg.norm(exact) < 5e-5
+
+
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", [mc.GramianOrthogonalizer, mc.RowBlockGramianOrthogonalizer])
+def test_orthogonalizer_fixture(gpu, kind):
+    # check/isvd/orthogonalizer/gramian_orthogonalizer.cpp:48-72
+    note("orthogonalizer fixture %s" % kind.__name__)
+    yi, qi = mtx.load("yi.mtx"), mtx.load("qi.mtx")
+    P = params(101, 11, 11, 0, 12)
+    o = kind(P)
+    o.initialize()
+    Q = mc.DeviceArray.from_numpy(yi)
+    o(Q)
+    q = Q.numpy()
+    for i in range(12):
+        b = slice(11 * i, 11 * (i + 1))
+        assert np.abs(proj(q[:, b]) - proj(qi[:, b])).max() < 1e-8
+    assert o.time() > 0
+
+
+INTEGRATORS = [
+    (mc.KolmogorovNagumoIntegrator, "qb_kn.mtx", 1e-4, 81),
+    (mc.RowBlockKolmogorovNagumoIntegrator, "qb_kn.mtx", 1e-4, 81),
+    (mc.RowBlockGramianKolmogorovNagumoIntegrator, "qb_kn.mtx", 1e-4, 81),
+    (mc.RowBlockWenYinIntegrator, "qb_wy.mtx", 1e-3, 21),
+    (mc.RowBlockGramianWenYinIntegrator, "qb_wy.mtx", 1e-3, 21),
+]
+
+
+@pytest.mark.parametrize("kind,fixture,tol,iters", INTEGRATORS)
+def test_integrator_fixture(gpu, kind, fixture, tol, iters):
+    # check/isvd/integrator/*.cpp: iteration() == 81 (KN) / 21 (WY), Qbar within 1e-8
+    note("integrator fixture %s" % kind.__name__)
+    qi, qb = mtx.load("qi.mtx"), mtx.load(fixture)
+    P = params(101, 11, 11, 0, 12)
+    it = kind(P, 256, tol)
+    it.initialize()
+    Qs = mc.DeviceArray.from_numpy(qi)
+    Qbar = P.createMatrixQbar()
+    it(Qs, Qbar)
+    assert it.iteration() == iters
+    assert it.converged
+    assert np.abs(Qbar.numpy() - qb).max() < 1e-8
+
+
+def test_integrator_not_converged_returns_result(gpu):
+    note("integrator maxiter exhaustion")
+    qi = mtx.load("qi.mtx")
+    P = params(101, 11, 11, 0, 12)
+    it = mc.RowBlockKolmogorovNagumoIntegrator(P, 5, 1e-4)
+    Qbar = P.createMatrixQbar()
+    it(mc.DeviceArray.from_numpy(qi), Qbar)
+    assert it.iteration() == 5 and it.converged is False
+    Qo, ito = oracle.kn_rowblock(qi, 11, 12, 5, 1e-4)
+    assert ito == 5
+    assert np.abs(Qbar.numpy() - Qo).max() < 1e-10
+
+
+FORMERS = [(mc.SvdFormer, "F"), (mc.GramianFormer, "F"), (mc.ColBlockGramianFormer, "F"),
+           (mc.RowBlockGramianFormer, "C"), (mc.SvdFormer, "C")]
+
+
+@pytest.mark.parametrize("kind,order", FORMERS)
+def test_former_fixture(gpu, kind, order):
+    # check/isvd/former/*.cpp: U projectors within 1e-8 (k = 5, p = 6)
+    note("former fixture %s order=%s" % (kind.__name__, order))
+    a, q, u = mtx.load("a.mtx"), mtx.load("q.mtx"), mtx.load("u.mtx")
+    k = 5
+    P = params(101, 1001, k, 6, 1)
+    f = kind(P, True)
+    f.initialize()
+    f(mc.DeviceArray.from_numpy(np.asarray(a, order=order)), mc.DeviceArray.from_numpy(q))
+    U, S, V = f.matrixU(), f.vectorS(), f.matrixV()
+    assert np.abs(proj(U) - proj(u[:, :k])).max() < 1e-8
+    S_ref, U_ref, V_ref = oracle.svd_former(a, q, k)
+    assert np.allclose(S, S_ref, rtol=1e-12)
+    assert np.abs(proj(V) - proj(V_ref)).max() < 1e-8
+    assert np.allclose((q.T @ a) @ V, (q.T @ U) * S, atol=1e-9 * S[0])
+
+
+# ------------------------------------------------------------------------------------------------
+def test_small_eigensolver_and_l64(gpu):
+    """l = 64 (the largest supported): a pipeline through the stage API on a small problem,
+    checked against the oracle stage by stage."""
+    m, n, k, p, N = 700, 300, 50, 14, 2
+    note("stage pipeline l=64")
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((m, n)) * (0.98 ** np.arange(n))
+    P = params(m, n, k, p, N)
+    sk = mc.RowBlockGaussianProjectionSketcher(P, seed=3, omega_mode=mc.OMEGA_REFERENCE)
+    orth = mc.RowBlockGramianOrthogonalizer(P)
+    integ = mc.RowBlockKolmogorovNagumoIntegrator(P, 300, 1e-4)
+    form = mc.RowBlockGramianFormer(P, True)
+    dA = mc.DeviceArray.from_numpy(np.ascontiguousarray(A))
+    Y = P.createCollectionQ()
+    sk(dA, Y)
+    orth(Y)
+    Qbar = P.createMatrixQbar()
+    integ(Y, Qbar)
+    form(dA, Qbar)
+    l = k + p
+    om = oracle.omega_ref(3, n, N * l, col_block=False)
+    Yo = oracle.sketch(A, om, a_colmajor=False)
+    Qo, _ = oracle.gramian_orthogonalize(Yo, l, N)
+    for i in range(N):
+        b = slice(l * i, l * (i + 1))
+        assert np.abs(proj(Y.numpy()[:, b]) - proj(Qo[:, b])).max() < 1e-8
+    Qbo, ito = oracle.kn_rowblock(Qo, l, N, 300, 1e-4)
+    assert abs(integ.iteration() - ito) <= 1
+    So, Uo, Vo = oracle.gramian_former(A, Qbo, k, a_colmajor=False)
+    assert np.allclose(form.vectorS(), So, rtol=1e-8)
+    assert np.abs(proj(form.matrixU()) - proj(Uo)).max() < 1e-6
