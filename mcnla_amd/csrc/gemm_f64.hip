@@ -90,6 +90,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
   __shared__ double As[2][BK * Cfg::kLdA];
   __shared__ double Bs[2][BK * Cfg::kLdB];
 
+  if (p.gate != nullptr && *p.gate < p.gate_t) return;
   // ---- tile schedule: logical id = (batch, split, tile), tile fastest -------------------------
   const int64_t tiles_m = ceil_div(p.M, BM), tiles_n = ceil_div(p.N, BN);
   const int64_t ntiles = p.lower_only ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
@@ -202,7 +203,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
 __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t splits,
                                      int64_t batch, int64_t M, int64_t N, double alpha,
                                      double beta, double* __restrict__ C, int64_t ldc,
-                                     int64_t stride_c, int sym_tile) {
+                                     int64_t stride_c, int sym_tile, const int* gate, int gate_t) {
+  if (gate != nullptr && *gate < gate_t) return;
   const int64_t per = M * N;
   const int64_t total = per * batch;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
@@ -298,7 +300,7 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0,
                        st, p.partial, p.splits, p.batch, p.M, p.N, p.alpha, p.beta, p.C, p.ldc,
-                       p.stride_c, sym);
+                       p.stride_c, sym, p.gate, p.gate_t);
     MCNLA_LAUNCHED();
   }
 }

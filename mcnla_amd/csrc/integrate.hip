@@ -22,6 +22,8 @@ namespace mcnla {
 struct IntegratorState {
   int converged_at;  // iteration index that met the stopping test, INT_MAX while running
   int nan_seen;      // a non-finite value appeared in the l x l step
+  int have_warm;     // the KN step holds eigenvectors of the previous iteration's Z
+  int pad;
   double res;        // last residual (KN: ||s - 1||_2, WY: mu)
   double taug, zeta, phi, mu;  // Wen-Yin scalars
 };
@@ -29,127 +31,131 @@ struct IntegratorState {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRows = 16;  // Nl rows per workgroup in the Bs products
-constexpr int kKc = 32;    // k chunk of the Bs products
+constexpr int kRows = 16;   // Nl rows per workgroup in the WY step-size pass
+constexpr int kPRows = 64;  // Nl rows per workgroup in the partial dot products
 
 __device__ __forceinline__ bool skip(const IntegratorState* st, int t) {
   return st->converged_at < t;
 }
 
-// Bg = Bs * B * invN for rows [blk*kRows, ...), plus per-block partial products
-// P[blk][0] = B_blk' B_blk, P[blk][1] = B_blk' Bg_blk, P[blk][2] = Bg_blk' Bg_blk (if nparts 3).
-__global__ __launch_bounds__(kThreads) void bs_product_kernel(int t, const IntegratorState* st,
-                                                             const double* __restrict__ Bs,
-                                                             int64_t nl, int l, double invN,
-                                                             const double* __restrict__ B,
-                                                             double* __restrict__ Bg,
-                                                             double* __restrict__ P, int nparts) {
+// Per-block partial products over kPRows rows of the Nl x l matrices B and Bg:
+// P[blk][0] = B_blk' B_blk, P[blk][1] = B_blk' Bg_blk, P[blk][2] = Bg_blk' Bg_blk (nparts 3).
+__global__ __launch_bounds__(kThreads) void gram_partials_kernel(int t, const IntegratorState* st,
+                                                                int64_t nl, int l,
+                                                                const double* __restrict__ B,
+                                                                const double* __restrict__ Bg,
+                                                                double* __restrict__ P,
+                                                                int nparts) {
   if (skip(st, t)) return;
-  __shared__ double sS[kRows][kKc + 1];
-  __shared__ double sB[kKc][kMaxL];
-  __shared__ double sBr[kRows][kMaxL];
-  __shared__ double sGr[kRows][kMaxL];
-  const int64_t r0 = blockIdx.x * (int64_t)kRows;
-  const int lw = l <= 16 ? 16 : (l <= 32 ? 32 : 64);
-  const int groups = kThreads / lw;             // row groups
-  const int rpt = kRows / groups;               // rows per thread (1, 2 or 4)
-  const int c = threadIdx.x % lw, g = threadIdx.x / lw;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t e0 = 0; e0 < nl; e0 += kKc) {
-    for (int q = threadIdx.x; q < kRows * kKc; q += kThreads) {
-      const int rr = q / kKc, ee = q % kKc;
-      const int64_t gr = r0 + rr, ge = e0 + ee;
-      sS[rr][ee] = (gr < nl && ge < nl) ? Bs[gr * nl + ge] : 0.0;
-    }
-    for (int q = threadIdx.x; q < kKc * l; q += kThreads) {
-      const int ee = q / l, cc = q % l;
-      const int64_t ge = e0 + ee;
-      sB[ee][cc] = (ge < nl) ? B[ge * l + cc] : 0.0;
-    }
-    __syncthreads();
-    if (c < l) {
-#pragma unroll 8
-      for (int ee = 0; ee < kKc; ++ee) {
-        const double b = sB[ee][c];
-        for (int i = 0; i < rpt; ++i) acc[i] += sS[g * rpt + i][ee] * b;
-      }
-    }
-    __syncthreads();
-  }
-  if (c < l) {
-    for (int i = 0; i < rpt; ++i) {
-      const int rr = g * rpt + i;
-      const int64_t gr = r0 + rr;
-      const double v = acc[i] * invN;
-      sGr[rr][c] = (gr < nl) ? v : 0.0;
-      sBr[rr][c] = (gr < nl) ? B[gr * l + c] : 0.0;
-      if (gr < nl) Bg[gr * l + c] = v;
-    }
+  __shared__ double sB[kPRows][kMaxL + 1];
+  __shared__ double sG[kPRows][kMaxL + 1];
+  const int64_t r0 = blockIdx.x * (int64_t)kPRows;
+  for (int e = threadIdx.x; e < kPRows * l; e += kThreads) {
+    const int rr = e / l, j = e % l;
+    const int64_t r = r0 + rr;
+    sB[rr][j] = (r < nl) ? B[r * l + j] : 0.0;
+    sG[rr][j] = (r < nl) ? Bg[r * l + j] : 0.0;
   }
   __syncthreads();
   double* Pb = P + blockIdx.x * (int64_t)nparts * l * l;
   for (int q = threadIdx.x; q < nparts * l * l; q += kThreads) {
     const int part = q / (l * l), i = (q / l) % l, j = q % l;
-    double s = 0.0;
-    for (int rr = 0; rr < kRows; ++rr) {
-      const double x = (part == 2) ? sGr[rr][i] : sBr[rr][i];
-      const double y = (part == 0) ? sBr[rr][j] : sGr[rr][j];
-      s += x * y;
+    double s0 = 0.0, s1 = 0.0;
+    for (int rr = 0; rr < kPRows; rr += 2) {
+      const double x0 = (part == 2) ? sG[rr][i] : sB[rr][i];
+      const double y0 = (part == 0) ? sB[rr][j] : sG[rr][j];
+      const double x1 = (part == 2) ? sG[rr + 1][i] : sB[rr + 1][i];
+      const double y1 = (part == 0) ? sB[rr + 1][j] : sG[rr + 1][j];
+      s0 += x0 * y0;
+      s1 += x1 * y1;
     }
-    Pb[q] = s;
+    Pb[q] = s0 + s1;
   }
 }
 
-// out[q] = scale * sum_blk P[blk][q] for q < count (fixed order: deterministic).
+// out[q] = scale * sum_blk P[blk][q] for q < count, in a fixed order (deterministic); four
+// independent accumulators keep several loads in flight.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ P, int64_t nblk,
+                                               int64_t count, int64_t q) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t b = 0;
+  for (; b + 4 <= nblk; b += 4) {
+    s0 += P[b * count + q];
+    s1 += P[(b + 1) * count + q];
+    s2 += P[(b + 2) * count + q];
+    s3 += P[(b + 3) * count + q];
+  }
+  for (; b < nblk; ++b) s0 += P[b * count + q];
+  return (s0 + s1) + (s2 + s3);
+}
+
 __global__ void reduce_partials_kernel(int t, const IntegratorState* st,
                                        const double* __restrict__ P, int64_t nblk, int count,
                                        double scale, double* __restrict__ out) {
   if (skip(st, t)) return;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= count) return;
-  double s = 0.0;
-  for (int64_t b = 0; b < nblk; ++b) s += P[b * count + q];
-  out[q] = scale * s;
+  out[q] = scale * sum_partials(P, nblk, count, q);
 }
 
-// The KN l x l step (row_block_kolmogorov_nagumo_integrator.hpp:155-201): Z = Dgc - Dc^2,
+// The KN l x l step (row_block_kolmogorov_nagumo_integrator.hpp:146-201): Dc = Bc'Bc/N and
+// Dgc = Bc'Bgc/N from the per-block partials P (nblk blocks of [Bc'Bc, Bc'Bgc]), Z = Dgc - Dc^2,
 // eig(Z), s = sqrt(1/2 + sqrt(1/4 - lambda)), C = V diag(s) V', Cinv = V diag(1/s) V',
-// F = C - Dc Cinv, convergence when !(||s - 1||_2 >= tol).  D = [Dc; Dgc] (2 x l x l).
+// F = C - Dc Cinv, convergence when !(||s - 1||_2 >= tol).  The eigen-decomposition starts
+// from the previous iteration's eigenvectors Vw (Z moves little between iterations), which
+// leaves C and Cinv unchanged but needs one or two Jacobi sweeps instead of six to ten.
 __global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorState* st, int l,
-                                                          double tol,
-                                                          const double* __restrict__ D,
+                                                          double invN, double tol,
+                                                          const double* __restrict__ P,
+                                                          int64_t nblk, double* __restrict__ Vw,
                                                           double* __restrict__ F,
                                                           double* __restrict__ Cinv) {
   if (skip(st, t)) return;
   __shared__ double a[kMaxL * (kMaxL + 1)];
   __shared__ double v[kMaxL * (kMaxL + 1)];
   __shared__ double dc[kMaxL * (kMaxL + 1)];
+  __shared__ double z[kMaxL * (kMaxL + 1)];
   __shared__ double s[kMaxL];
   __shared__ double cs[2 * kMaxL];
   __shared__ int pq[2 * kMaxL];
   __shared__ double red[2 * (kThreads / kWave)];
   const int ld = lds_ld(l);
-  const double* Dc = D;
-  const double* Dgc = D + l * l;
-  for (int e = threadIdx.x; e < l * l; e += kThreads) dc[(e / l) * ld + e % l] = Dc[e];
+  const int ll = l * l;
+  const bool warm = st->have_warm != 0;
+  // Dc into dc, Dgc into a
+  for (int e = threadIdx.x; e < 2 * ll; e += kThreads) {
+    const double x = invN * sum_partials(P, nblk, 2 * ll, e);
+    const int q = e % ll;
+    (e < ll ? dc : a)[(q / l) * ld + q % l] = x;
+  }
+  if (warm)
+    for (int e = threadIdx.x; e < ll; e += kThreads) v[(e / l) * ld + e % l] = Vw[e];
   __syncthreads();
-  for (int e = threadIdx.x; e < l * l; e += kThreads) {
+  // Z = sym(Dgc) - Dc Dc' (exactly symmetric as computed)
+  for (int e = threadIdx.x; e < ll; e += kThreads) {
     const int i = e / l, j = e % l;
-    double z = 0.5 * (Dgc[i * l + j] + Dgc[j * l + i]);
-    for (int q = 0; q < l; ++q) z -= dc[i * ld + q] * dc[q * ld + j];
-    a[i * ld + j] = z;
+    double x = 0.5 * (a[i * ld + j] + a[j * ld + i]);
+    for (int q = 0; q < l; ++q) x -= dc[i * ld + q] * dc[j * ld + q];
+    z[i * ld + j] = x;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < l * l; e += kThreads) {  // exact symmetry for the eigensolver
-    const int i = e / l, j = e % l;
-    if (i > j) a[i * ld + j] = a[j * ld + i];
+  if (warm) {  // z <- W' Z W
+    lds_matmul_nn(l, z, v, a);
+    __syncthreads();
+    lds_matmul_tn(l, v, a, z);
+    __syncthreads();
+    for (int e = threadIdx.x; e < ll; e += kThreads) {  // restore exact symmetry
+      const int i = e / l, j = e % l;
+      if (i > j) z[i * ld + j] = z[j * ld + i];
+    }
+    __syncthreads();
   }
+  lds_jacobi(l, z, v, !warm, cs, pq, red);
+  for (int i = threadIdx.x; i < l; i += kThreads) s[i] = sqrt(0.5 + sqrt(0.25 - z[i * ld + i]));
+  for (int e = threadIdx.x; e < ll; e += kThreads) Vw[e] = v[(e / l) * ld + e % l];
   __syncthreads();
-  lds_jacobi(l, a, v, true, cs, pq, red);
-  for (int i = threadIdx.x; i < l; i += kThreads) s[i] = sqrt(0.5 + sqrt(0.25 - a[i * ld + i]));
-  __syncthreads();
-  // Cinv into `a` (no longer needed), then F = C - Dc Cinv with C recomputed on the fly.
-  for (int e = threadIdx.x; e < l * l; e += kThreads) {
+  // Cinv into `a`, then F = C - Dc Cinv with C recomputed on the fly.
+  for (int e = threadIdx.x; e < ll; e += kThreads) {
     const int i = e / l, j = e % l;
     double ci = 0.0;
     for (int q = 0; q < l; ++q) ci += v[i * ld + q] * v[j * ld + q] / s[q];
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorStat
     Cinv[e] = ci;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < l * l; e += kThreads) {
+  for (int e = threadIdx.x; e < ll; e += kThreads) {
     const int i = e / l, j = e % l;
     double c = 0.0;
     for (int q = 0; q < l; ++q) c += v[i * ld + q] * v[j * ld + q] * s[q];
@@ -174,6 +180,7 @@ __global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorStat
     }
     r = sqrt(r);
     st->res = r;
+    st->have_warm = 1;
     if (bad) st->nan_seen = 1;
     if (!(r >= tol)) st->converged_at = t;
   }
@@ -443,18 +450,21 @@ __global__ void wy_check_kernel(int t, IntegratorState* st, int l, double tol,
 
 // Nl-row pass for the step-size terms (row_block_gramian_wen_yin_integrator.hpp:260-312):
 // U2 = (B+ - Bc)/N - U1 D+, BsU = Bs U1 (odd) or Bs U2 = (Bg+ - Bgc) - BsU1 D+ (even);
-// per-block partials [Bc'U1, Bc'U2 (l x l each), U1.BsU, U2.BsU].
+// per-block partials [Bc'U1, Bc'U2, Bc'Mc (l x l each), U1.BsU, U2.BsU], where
+// Mc = E~c + [F~c; 0] are the coefficients of Qc = Qs Mc, so Bc'Mc = Qc'Qc.
 __global__ __launch_bounds__(kThreads) void wy_terms_kernel(
     int t, const IntegratorState* st, int64_t nl, int l, double invN, int odd,
     const double* __restrict__ Bc, const double* __restrict__ Bp, const double* __restrict__ Bgc,
     const double* __restrict__ Bgp, const double* __restrict__ U1,
-    const double* __restrict__ BsU1, const double* __restrict__ Dp, double* __restrict__ P) {
+    const double* __restrict__ BsU1, const double* __restrict__ Dp,
+    const double* __restrict__ Ec, const double* __restrict__ FFc, double* __restrict__ P) {
   if (skip(st, t) || st->converged_at == t) return;
   __shared__ double sD[kMaxL * kMaxL];
   __shared__ double sBc[kRows][kMaxL];
   __shared__ double sU1[kRows][kMaxL];
   __shared__ double sU2[kRows][kMaxL];
   __shared__ double sBsU[kRows][kMaxL];
+  __shared__ double sM[kRows][kMaxL];
   __shared__ double red[2][kThreads / kWave];
   for (int e = threadIdx.x; e < l * l; e += kThreads) sD[e] = Dp[e];
   __syncthreads();
@@ -462,11 +472,12 @@ __global__ __launch_bounds__(kThreads) void wy_terms_kernel(
   for (int e = threadIdx.x; e < kRows * l; e += kThreads) {
     const int rr = e / l, j = e % l;
     const int64_t r = r0 + rr;
-    double bc = 0, u1 = 0, u2 = 0, bsu = 0;
+    double bc = 0, u1 = 0, u2 = 0, bsu = 0, mc = 0;
     if (r < nl) {
       const int64_t o = r * l;
       bc = Bc[o + j];
       u1 = U1[o + j];
+      mc = Ec[o + j] + (r < l ? FFc[o + j] : 0.0);
       double u1d = 0.0, bsu1d = 0.0;
       for (int q = 0; q < l; ++q) {
         u1d += U1[o + q] * sD[q * l + j];
@@ -479,14 +490,18 @@ __global__ __launch_bounds__(kThreads) void wy_terms_kernel(
     sU1[rr][j] = u1;
     sU2[rr][j] = u2;
     sBsU[rr][j] = bsu;
+    sM[rr][j] = mc;
   }
   __syncthreads();
-  const int count = 2 * l * l + 2;
+  const int count = 3 * l * l + 2;
   double* Pb = P + blockIdx.x * (int64_t)count;
-  for (int q = threadIdx.x; q < 2 * l * l; q += kThreads) {
+  for (int q = threadIdx.x; q < 3 * l * l; q += kThreads) {
     const int part = q / (l * l), i = (q / l) % l, j = q % l;
     double s = 0.0;
-    for (int rr = 0; rr < kRows; ++rr) s += sBc[rr][i] * (part == 0 ? sU1[rr][j] : sU2[rr][j]);
+    for (int rr = 0; rr < kRows; ++rr) {
+      const double y = part == 0 ? sU1[rr][j] : (part == 1 ? sU2[rr][j] : sM[rr][j]);
+      s += sBc[rr][i] * y;
+    }
     Pb[q] = s;
   }
   double d1 = 0.0, d2 = 0.0;
@@ -508,45 +523,60 @@ __global__ __launch_bounds__(kThreads) void wy_terms_kernel(
       s1 += red[0][w];
       s2 += red[1][w];
     }
-    Pb[2 * l * l] = s1;
-    Pb[2 * l * l + 1] = s2;
+    Pb[3 * l * l] = s1;
+    Pb[3 * l * l + 1] = s2;
   }
 }
 
-// t1, t2 and the Barzilai-Borwein step (row_block_gramian_wen_yin_integrator.hpp:260-316).
-// T = [Bc'U1; Bc'U2; U1.BsU; U2.BsU] (reduced, unscaled).
+// t1, t2 and the Barzilai-Borwein step (row_block_gramian_wen_yin_integrator.hpp:260-316), with
+// T = [Bc'U1; Bc'U2; Qc'Qc; U1.BsU; U2.BsU] (reduced, unscaled).  The quadratic terms in Phi use
+// the actual Qc'Qc where the reference's Gramian form assumes the identity: ||dQ||^2 =
+// ||Qc Phi1 + Qs U1||^2 then equals the row-block form's explicit sum of squares, and the step
+// size no longer amplifies the drift of Qc'Qc from I (with the identity, a 1e-15 perturbation of
+// the input moves the result by ~1e-5 at the CPU-runnable workload).
 __global__ void wy_tau_kernel(int t, IntegratorState* st, int l, int odd,
                               const double* __restrict__ Dc, const double* __restrict__ Dp,
                               const double* __restrict__ Fc, const double* __restrict__ T) {
   if (skip(st, t) || st->converged_at == t) return;
-  __shared__ double red[6][kThreads / kWave];
+  __shared__ double sP1[kMaxL * kMaxL];
+  __shared__ double sP2[kMaxL * kMaxL];
+  __shared__ double red7[7][kThreads / kWave];
   const double* BtU1 = T;
   const double* BtU2 = T + l * l;
-  const double u1bsu = T[2 * l * l], u2bsu = T[2 * l * l + 1];
+  const double* QQ = T + 2 * l * l;
+  const double u1bsu = T[3 * l * l], u2bsu = T[3 * l * l + 1];
   // Phi1 = Fc - I, Phi2 = Dc - Fc D+
-  double p11 = 0, p1bu1 = 0, p12 = 0, p1bu2 = 0, p2bu1 = 0, p22 = 0;
-  double p2bu2 = 0;
   for (int e = threadIdx.x; e < l * l; e += blockDim.x) {
     const int i = e / l, j = e % l;
-    const double ph1 = Fc[e] - (i == j ? 1.0 : 0.0);
     double fd = 0.0;
     for (int q = 0; q < l; ++q) fd += Fc[i * l + q] * Dp[q * l + j];
-    const double ph2 = Dc[e] - fd;
-    p11 += ph1 * ph1;
+    sP1[e] = Fc[e] - (i == j ? 1.0 : 0.0);
+    sP2[e] = Dc[e] - fd;
+  }
+  __syncthreads();
+  // <X, QQ Y> = sum_ij X_ij (QQ Y)_ij
+  double p11 = 0, p1bu1 = 0, p12 = 0, p1bu2 = 0, p2bu1 = 0, p22 = 0, p2bu2 = 0;
+  for (int e = threadIdx.x; e < l * l; e += blockDim.x) {
+    const int i = e / l, j = e % l;
+    double q1 = 0.0, q2 = 0.0;
+    for (int q = 0; q < l; ++q) {
+      q1 += QQ[i * l + q] * sP1[q * l + j];
+      q2 += QQ[i * l + q] * sP2[q * l + j];
+    }
+    const double ph1 = sP1[e], ph2 = sP2[e];
+    p11 += ph1 * q1;
+    p12 += ph1 * q2;
+    p22 += ph2 * q2;
     p1bu1 += ph1 * BtU1[e];
-    p12 += ph1 * ph2;
     p1bu2 += ph1 * BtU2[e];
     p2bu1 += ph2 * BtU1[e];
-    p22 += ph2 * ph2;
     p2bu2 += ph2 * BtU2[e];
   }
   double vals[7] = {p11, p1bu1, p12, p1bu2, p2bu1, p22, p2bu2};
-  __shared__ double red7[7][kThreads / kWave];
   for (int k = 0; k < 7; ++k) {
     const double v = wave_sum(vals[k]);
     if (threadIdx.x % kWave == 0) red7[k][threadIdx.x / kWave] = v;
   }
-  (void)red;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -571,6 +601,7 @@ __global__ void wy_tau_kernel(int t, IntegratorState* st, int l, int odd,
 __global__ void init_state_kernel(IntegratorState* st) {
   st->converged_at = INT_MAX;
   st->nan_seen = 0;
+  st->have_warm = 0;
   st->res = 0.0;
   st->taug = WyConst::tau0;
   st->zeta = 1.0;
@@ -618,7 +649,6 @@ __global__ void coef_kernel(const double* __restrict__ E, const double* __restri
   }
 }
 
-unsigned grid_rows(int64_t nl) { return static_cast<unsigned>(ceil_div(nl, kRows)); }
 unsigned grid_elems(int64_t total) {
   int64_t b = ceil_div(total, kThreads);
   if (b > 4096) b = 4096;
@@ -629,7 +659,9 @@ unsigned grid_elems(int64_t total) {
 struct Layout {
   int64_t nl;
   int l;
-  int64_t nblk;
+  int64_t nblk_p;  // blocks of gram_partials_kernel
+  int64_t nblk_t;  // blocks of wy_terms_kernel
+  int64_t splits;  // split-K factor of the Bs products
   double* B[2];
   double* Bg[2];
   double* E[2];
@@ -637,10 +669,12 @@ struct Layout {
   double* D3[2];   // [Dc; Dgc; Hgg] per parity
   double* F;       // KN F / WY Fc
   double* C;       // KN Cinv / WY Fgc
+  double* Vw;      // KN warm-start eigenvectors
   double* U1;
   double* BsU1;
-  double* P;       // partials, nblk * max(3 l^2, 2 l^2 + 2)
-  double* T;       // reduced WY terms, 2 l^2 + 2
+  double* P;       // partial dot products
+  double* Pbg;     // split-K partials of the Bs products
+  double* T;       // reduced WY terms, 3 l^2 + 2
   IntegratorState* st;
   size_t bytes;
 };
@@ -649,7 +683,15 @@ Layout layout(char* base, int64_t N, int l) {
   Layout L{};
   L.nl = N * l;
   L.l = l;
-  L.nblk = ceil_div(L.nl, kRows);
+  L.nblk_p = ceil_div(L.nl, kPRows);
+  L.nblk_t = ceil_div(L.nl, kRows);
+  {  // about 256 workgroups, at least 128 k per split
+    const int64_t tiles = ceil_div(L.nl, 128);
+    int64_t sp = ceil_div(256, tiles);
+    const int64_t cap = ceil_div(L.nl, 128);
+    L.splits = sp < cap ? sp : cap;
+    if (L.splits < 1) L.splits = 1;
+  }
   const size_t nll = static_cast<size_t>(L.nl) * l, ll = static_cast<size_t>(l) * l;
   size_t off = 0;
   auto take = [&](size_t elems) {
@@ -666,10 +708,14 @@ Layout layout(char* base, int64_t N, int l) {
   }
   L.F = take(ll);
   L.C = take(ll);
+  L.Vw = take(ll);
   L.U1 = take(nll);
   L.BsU1 = take(nll);
-  L.P = take(static_cast<size_t>(L.nblk) * (3 * ll + 2));
-  L.T = take(2 * ll + 2);
+  const size_t pp = static_cast<size_t>(L.nblk_p) * 3 * ll;
+  const size_t pt = static_cast<size_t>(L.nblk_t) * (3 * ll + 2);
+  L.P = take(pp > pt ? pp : pt);
+  L.Pbg = take(static_cast<size_t>(L.splits + 1) * nll);
+  L.T = take(3 * ll + 2);
   L.st = reinterpret_cast<IntegratorState*>(base ? base + off : nullptr);
   off += 256;
   L.bytes = off;
@@ -690,6 +736,31 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
   const double invN = 1.0 / static_cast<double>(g.N);
   const int ll = l * l;
 
+  // Bout = Bs * Bin / N on the MFMA GEMM (split-K, deterministic reduction), gated on t.
+  // Bs is exactly symmetric, so its rows serve as the M-contiguous operand.
+  auto bs_product = [&](const double* Bin, double* Bout, int t) {
+    GemmF64Args p;
+    p.M = nl;
+    p.N = l;
+    p.K = nl;
+    p.A = g.Bs;
+    p.lda = nl;
+    p.a_mcontig = true;
+    p.B = Bin;
+    p.ldb = l;
+    p.C = Bout;
+    p.ldc = l;
+    p.alpha = invN;
+    p.shape = l <= 32 ? GemmShape::kTallN32 : GemmShape::kTallN64;
+    p.splits = L.splits;
+    p.kchunk = ceil_div(ceil_div(nl, L.splits), 16) * 16;
+    p.partial = L.Pbg;
+    p.gate = &L.st->converged_at;
+    p.gate_t = t;
+    gemm_f64(p, st);
+  };
+  const unsigned gp = static_cast<unsigned>(L.nblk_p);
+
   hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(1), 0, st, L.st);
   MCNLA_LAUNCHED();
   hipLaunchKernelGGL(init_coef_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, g.Bs, nl, l,
@@ -703,11 +774,12 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(wy_lds)));
     // Bg0 = Bs B0 / N and [Dc; Dgc; Hgg] of parity 0.
-    hipLaunchKernelGGL(bs_product_kernel, dim3(grid_rows(nl)), dim3(kThreads), 0, st, -1, L.st,
-                       g.Bs, nl, l, invN, L.B[0], L.Bg[0], L.P, 3);
+    bs_product(L.B[0], L.Bg[0], -1);
+    hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, -1, L.st, nl, l,
+                       L.B[0], L.Bg[0], L.P, 3);
     MCNLA_LAUNCHED();
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll, kThreads)), dim3(kThreads), 0,
-                       st, -1, L.st, L.P, L.nblk, 3 * ll, invN, L.D3[0]);
+                       st, -1, L.st, L.P, L.nblk_p, 3 * ll, invN, L.D3[0]);
     MCNLA_LAUNCHED();
     hipLaunchKernelGGL(wy_init_scalars_kernel, dim3(1), dim3(64), 0, st, L.st, l, L.D3[0]);
     MCNLA_LAUNCHED();
@@ -722,14 +794,12 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
       const int c = static_cast<int>(t & 1), p = c ^ 1;
       const int ti = static_cast<int>(t);
       if (!g.wen_yin) {
-        hipLaunchKernelGGL(bs_product_kernel, dim3(grid_rows(nl)), dim3(kThreads), 0, st, ti, L.st,
-                           g.Bs, nl, l, invN, L.B[c], L.Bg[0], L.P, 2);
+        bs_product(L.B[c], L.Bg[0], ti);
+        hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, ti, L.st, nl, l,
+                           L.B[c], L.Bg[0], L.P, 2);
         MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(2 * ll, kThreads)), dim3(kThreads),
-                           0, st, ti, L.st, L.P, L.nblk, 2 * ll, invN, L.D3[0]);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(kn_step_kernel, dim3(1), dim3(kThreads), 0, st, ti, L.st, l, g.tol,
-                           L.D3[0], L.F, L.C);
+        hipLaunchKernelGGL(kn_step_kernel, dim3(1), dim3(kThreads), 0, st, ti, L.st, l, invN, g.tol,
+                           L.P, L.nblk_p, L.Vw, L.F, L.C);
         MCNLA_LAUNCHED();
         hipLaunchKernelGGL(kn_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, ti,
                            L.st, nl, l, invN, L.B[c], L.Bg[0], L.B[p], L.E[c], L.E[p], L.FF[c],
@@ -744,24 +814,23 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
                            L.st, nl, l, invN, L.B[c], L.Bg[c], L.B[p], L.U1, L.BsU1, L.E[c],
                            L.E[p], L.FF[c], L.FF[p], L.F, L.C);
         MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(bs_product_kernel, dim3(grid_rows(nl)), dim3(kThreads), 0, st, ti, L.st,
-                           g.Bs, nl, l, invN, L.B[p], L.Bg[p], L.P, 3);
+        bs_product(L.B[p], L.Bg[p], ti);
+        hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, ti, L.st, nl, l,
+                           L.B[p], L.Bg[p], L.P, 3);
         MCNLA_LAUNCHED();
         hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll, kThreads)), dim3(kThreads),
-                           0, st, ti, L.st, L.P, L.nblk, 3 * ll, invN, L.D3[p]);
+                           0, st, ti, L.st, L.P, L.nblk_p, 3 * ll, invN, L.D3[p]);
         MCNLA_LAUNCHED();
         hipLaunchKernelGGL(wy_check_kernel, dim3(1), dim3(kThreads), 0, st, ti, L.st, l, g.tol,
                            L.D3[p]);
         MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(wy_terms_kernel, dim3(grid_rows(nl)), dim3(kThreads), 0, st, ti, L.st,
-                           nl, l, invN, odd, L.B[c], L.B[p], L.Bg[c], L.Bg[p], L.U1, L.BsU1,
-                           L.D3[p], L.P);
+        hipLaunchKernelGGL(wy_terms_kernel, dim3(static_cast<unsigned>(L.nblk_t)), dim3(kThreads), 0,
+                           st, ti, L.st, nl, l, invN, odd, L.B[c], L.B[p], L.Bg[c], L.Bg[p], L.U1,
+                           L.BsU1, L.D3[p], L.E[c], L.FF[c], L.P);
         MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(2 * ll + 2, kThreads)),
-                           dim3(kThreads), 0, st, ti, L.st, L.P, L.nblk, 2 * ll + 2, 1.0, L.T);
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll + 2, kThreads)),
+                           dim3(kThreads), 0, st, ti, L.st, L.P, L.nblk_t, 3 * ll + 2, 1.0, L.T);
         MCNLA_LAUNCHED();
-        // wy_tau reads the reduced terms; the reduce kernel above is gated like wy_terms via
-        // converged_at (it sees converged_at == t and still runs, harmlessly).
         hipLaunchKernelGGL(wy_tau_kernel, dim3(1), dim3(kThreads), 0, st, ti, L.st, l, odd, L.D3[c],
                            L.D3[p], L.F, L.T);
         MCNLA_LAUNCHED();

@@ -29,6 +29,10 @@ struct GemmF64Args {
   GemmShape shape = GemmShape::kSquare;
   int64_t batch = 1;        // independent problems; operand b starts at A + b*stride_a, ...
   int64_t stride_a = 0, stride_b = 0, stride_c = 0;
+  // Optional device-side gate: when *gate < gate_t the launch does nothing (used by
+  // iteration loops that are enqueued ahead of their convergence test).
+  const int* gate = nullptr;
+  int gate_t = 0;
 };
 
 void gemm_f64(GemmF64Args p, hipStream_t st);

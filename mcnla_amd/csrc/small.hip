@@ -76,13 +76,16 @@ __global__ void right_blockdiag_kernel(const double* __restrict__ G, const doubl
   }
 }
 
-// Bs = blockdiag(T)' * H: Bs[r][c] = sum_e T_blk(r)[e][r % l] * H[blk(r)*l + e][c]
+// Bs = blockdiag(T)' * H: Bs[r][c] = sum_e T_blk(r)[e][r % l] * H[blk(r)*l + e][c].  Bs is
+// symmetric in exact arithmetic; the entry with r >= c is computed and written to both halves so
+// that it is exactly symmetric (its rows can then stand in for its columns).
 __global__ void left_blockdiag_t_kernel(const double* __restrict__ H, const double* __restrict__ T,
                                         double* __restrict__ Bs, int64_t nl, int l) {
   const int64_t total = nl * nl;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = e / nl, c = e % nl;
+    if (r < c) continue;
     const int64_t blk = r / l;
     const int rr = r % l;
     const double* h = H + blk * l * nl + c;
@@ -90,6 +93,7 @@ __global__ void left_blockdiag_t_kernel(const double* __restrict__ H, const doub
     double acc = 0.0;
     for (int q = 0; q < l; ++q) acc += t[q * l] * h[q * nl];
     Bs[e] = acc;
+    Bs[c * nl + r] = acc;
   }
 }
 

@@ -12,44 +12,27 @@ __host__ __device__ constexpr int lds_ld(int n) { return n + 1; }
 
 // Parallel cyclic Jacobi eigen-decomposition of the symmetric n x n matrix `a` (LDS, ld = n+1).
 // On return the diagonal of `a` holds the eigenvalues and the columns of `v` (LDS, ld = n+1) the
-// eigenvectors.  If init_v is true `v` starts as the identity, otherwise `a` must already be
-// expressed in the basis held by `v` (warm start).  Pairs follow the round-robin tournament
-// ordering: n-1 rounds of n/2 disjoint rotations per sweep.  `scratch` needs 3*64 doubles and
-// 2*64 ints of LDS.  All threads of the workgroup must call this.
-__device__ inline void lds_jacobi(int n, double* a, double* v, bool init_v, double* cs,
-                                  int* pq, double* red) {
+// eigenvectors.  If init_v is true `v` starts as the identity, otherwise `v` holds a basis W
+// and `a` must already be W' A W (warm start; on return v = W * rotations).  Pairs follow the
+// round-robin tournament ordering: n-1 rounds of n/2 disjoint rotations per sweep.  A pair is
+// rotated only while |a_pq| > 1e-15 sqrt(|a_pp a_qq|) (the threshold of one-sided/two-sided
+// Jacobi with high relative accuracy, Demmel & Veselic 1992), and the annihilated entry is set
+// to zero; the iteration stops after a sweep without rotations.  `cs` needs 2*64 doubles, `pq`
+// 2*64 ints, `red` 8 doubles of LDS.  All threads of the workgroup must call this.
+__device__ inline int lds_jacobi(int n, double* a, double* v, bool init_v, double* cs, int* pq,
+                                 double* red) {
   const int ld = lds_ld(n);
   const int tid = threadIdx.x, nt = blockDim.x;
   if (init_v)
     for (int e = tid; e < n * n; e += nt) v[(e / n) * ld + e % n] = (e / n == e % n) ? 1.0 : 0.0;
   const int np = (n + 1) & ~1;  // players, padded to even (index n is a bye when n is odd)
   const int npairs = np / 2;
+  int* rotated = reinterpret_cast<int*>(red);
+  int sweeps = 0;
   __syncthreads();
-  for (int sweep = 0; sweep < 40; ++sweep) {
-    // convergence test: off-diagonal Frobenius norm vs full norm
-    double off = 0.0, tot = 0.0;
-    for (int e = tid; e < n * n; e += nt) {
-      const int i = e / n, j = e % n;
-      const double x = a[i * ld + j];
-      tot += x * x;
-      if (i != j) off += x * x;
-    }
-    off = wave_sum(off);
-    tot = wave_sum(tot);
-    if (tid % kWave == 0) {
-      red[2 * (tid / kWave)] = off;
-      red[2 * (tid / kWave) + 1] = tot;
-    }
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    if (tid == 0) *rotated = 0;
     __syncthreads();
-    double offs = 0.0, tots = 0.0;
-    for (int w = 0; w < nt / kWave; ++w) {
-      offs += red[2 * w];
-      tots += red[2 * w + 1];
-    }
-    __syncthreads();
-    // stop once the off-diagonal mass is below (3e-16 * ||a||_F)^2: further rotations only
-    // shuffle rounding noise
-    if (!(offs > 1e-31 * tots) || offs == 0.0) break;
     for (int round = 0; round < np - 1; ++round) {
       // rotation parameters of this round's pairs
       for (int k = tid; k < npairs; k += nt) {
@@ -60,12 +43,13 @@ __device__ inline void lds_jacobi(int n, double* a, double* v, bool init_v, doub
         double c = 1.0, s = 0.0;
         if (q < n) {
           const double apq = a[p * ld + q];
-          if (apq != 0.0) {
-            const double app = a[p * ld + p], aqq = a[q * ld + q];
+          const double app = a[p * ld + p], aqq = a[q * ld + q];
+          if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && apq != 0.0) {
             const double theta = (aqq - app) / (2.0 * apq);
             const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
             c = 1.0 / sqrt(1.0 + t * t);
             s = t * c;
+            *rotated = 1;
           }
         } else {
           p = q = -1;
@@ -101,7 +85,43 @@ __device__ inline void lds_jacobi(int n, double* a, double* v, bool init_v, doub
         v[i * ld + q] = s * vp + c * vq;
       }
       __syncthreads();
+      // the rotation annihilates a_pq analytically; drop its rounding residue
+      for (int k = tid; k < npairs; k += nt) {
+        const int p = pq[2 * k], q = pq[2 * k + 1];
+        if (p >= 0 && cs[2 * k + 1] != 0.0) {
+          a[p * ld + q] = 0.0;
+          a[q * ld + p] = 0.0;
+        }
+      }
+      __syncthreads();
     }
+    ++sweeps;
+    const int any = *rotated;
+    __syncthreads();
+    if (!any) break;
+  }
+  return sweeps;
+}
+
+// c = a' * b for n x n LDS matrices (ld = n+1); c must not alias a or b.
+__device__ inline void lds_matmul_tn(int n, const double* a, const double* b, double* c) {
+  const int ld = lds_ld(n);
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e % n;
+    double s = 0.0;
+    for (int q = 0; q < n; ++q) s += a[q * ld + i] * b[q * ld + j];
+    c[i * ld + j] = s;
+  }
+}
+
+// c = a * b for n x n LDS matrices (ld = n+1); c must not alias a or b.
+__device__ inline void lds_matmul_nn(int n, const double* a, const double* b, double* c) {
+  const int ld = lds_ld(n);
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e % n;
+    double s = 0.0;
+    for (int q = 0; q < n; ++q) s += a[i * ld + q] * b[q * ld + j];
+    c[i * ld + j] = s;
   }
 }
 
