@@ -161,8 +161,10 @@ struct SketchCore {
       omega_philox(omega.as<double>(), count, seed, st);
     }
   }
+  // y_f64: a mixed-precision sketch writes its FP32-accumulated Y as FP64 (the fused solver's
+  // Gram and basis products then read FP64); otherwise mixed Y is FP32.
   void project(const mcnla_params& p, int precision, const void* A, int64_t lda, int dtype,
-               int order, void* Y, hipStream_t st) {
+               int order, void* Y, hipStream_t st, bool y_f64 = false) {
     const int64_t nl = (p.rank + p.over_rank) * p.num_sketch;
     if (!gemm_beg) {
       MCNLA_HIP(hipEventCreate(&gemm_beg));
@@ -196,7 +198,10 @@ struct SketchCore {
       s.a_colmajor = (order == MCNLA_COL_MAJOR);
       s.a_f32 = (dtype == MCNLA_F32);
       s.omega = omega.as<double>();
-      s.Y = static_cast<float*>(Y);
+      if (y_f64)
+        s.Yd = static_cast<double*>(Y);
+      else
+        s.Y = static_cast<float*>(Y);
       s.ldy = nl;
       s.terms = (precision == MCNLA_PREC_MIXED_BF16X3) ? 3 : 1;
       ws.ensure(sketch_bf16_workspace(s));
@@ -922,9 +927,10 @@ mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_
     s->cfg = *cfg;
     const int64_t l = params->rank + params->over_rank, N = params->num_sketch, nl = l * N;
     const int64_t m = params->nrow;
-    const size_t yelem = is_mixed(cfg->precision) ? sizeof(float) : sizeof(double);
-    // Allocate the resident workspaces up front (the stages' initialize()).
-    s->Y.ensure(m * nl * yelem);
+    // Allocate the resident workspaces up front (the stages' initialize()).  Y is FP64 in every
+    // precision: the mixed sketch accumulates in FP32 and stores FP64, so that the FP64 Gram and
+    // basis products stream it at full rate.
+    s->Y.ensure(m * nl * sizeof(double));
     s->G.ensure(nl * nl * sizeof(double));
     s->Bs.ensure(nl * nl * sizeof(double));
     s->tmp.ensure(nl * nl * sizeof(double));
@@ -981,16 +987,15 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     const auto& c = s->cfg;
     const int l = static_cast<int>(p.rank + p.over_rank);
     const int64_t N = p.num_sketch, nl = l * N, m = p.nrow;
-    const bool mixed = is_mixed(c.precision);
     hipStream_t st = S(stream);
     s->timer.start(st);
     // 1. sketch: Y = A * Omega
     s->sketch.generate(p, c.seed, c.omega_mode, c.omega_threads,
                        c.sketcher != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN, st);
-    s->sketch.project(p, c.precision, A, lda, a_dtype, a_order, s->Y.p, st);
+    s->sketch.project(p, c.precision, A, lda, a_dtype, a_order, s->Y.p, st, true);
     s->timer.mark(st);
     // 2. orthogonalize through the Gram matrix: G = Y'Y, T_i from eig(G_ii), Bs = T' G T
-    s->gram.run(s->Y.p, mixed, m, nl, s->G.as<double>(), st);
+    s->gram.run(s->Y.p, false, m, nl, s->G.as<double>(), st);
     gram_diag_blocks(s->G.as<double>(), nl, s->W.as<double>(), N, l, st);
     MCNLA_HIP(hipMemsetAsync(s->bad.p, 0, sizeof(int), st));
     batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), N, l, st);
@@ -1036,7 +1041,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     q.A = s->Y.p;
     q.lda = nl;
     q.a_mcontig = false;
-    q.a_f32 = mixed;
+    q.a_f32 = false;
     q.B = s->M2.p;
     q.ldb = l;
     q.C = s->Qbar.as<double>();

@@ -99,11 +99,11 @@ __device__ __forceinline__ void tile_lstore_impl(uint16_t (*sa)[PANEL], uint16_t
 }
 
 
-template <int TERMS>
+template <int TERMS, typename TY>
 __global__ __launch_bounds__(kThreads) void sketch_bf16_kernel(
     const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Alo,
     const uint16_t* __restrict__ Whi, const uint16_t* __restrict__ Wlo, int64_t kpad, int64_t m,
-    int64_t nl, float* __restrict__ Y, int64_t ldy) {
+    int64_t nl, TY* __restrict__ Y, int64_t ldy) {
   constexpr int BK = bk_for(TERMS);
   constexpr int NOPS = TERMS == 1 ? 1 : 2;   // hi (+ lo) panels per operand
   constexpr int LDK = BK + 8;                // bf16 per LDS row (16-byte pad)
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void sketch_bf16_kernel(
       for (int r = 0; r < 4; ++r) {
         const int64_t gi = m0 + wm0 + i * 16 + 4 * (lane / 16) + r;
         const int64_t gj = n0 + wn0 + j * 16 + lane % 16;
-        if (gi < m && gj < nl) Y[gi * ldy + gj] = acc[i][j][r];
+        if (gi < m && gj < nl) Y[gi * ldy + gj] = static_cast<TY>(acc[i][j][r]);
       }
 }
 
@@ -244,12 +244,19 @@ void sketch_bf16(const SketchBf16Args& a, void* workspace, size_t workspace_byte
   launch_pack(a.omega, a.nl, true, a.nl, a.n, d.np, d.kp, Whi, Wlo, st);
   const int64_t nblocks = ceil_div(a.m, kBM) * ceil_div(a.nl, kBN);
   if (gemm_begin) MCNLA_HIP(hipEventRecord(gemm_begin, st));
-  if (a.terms == 1)
-    hipLaunchKernelGGL(sketch_bf16_kernel<1>, dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), 0,
-                       st, Ahi, Alo, Whi, Wlo, d.kp, a.m, a.nl, a.Y, a.ldy);
+  const dim3 grid(static_cast<unsigned>(nblocks));
+  if (a.terms == 1 && a.Yd)
+    hipLaunchKernelGGL((sketch_bf16_kernel<1, double>), grid, dim3(kThreads), 0, st, Ahi, Alo, Whi,
+                       Wlo, d.kp, a.m, a.nl, a.Yd, a.ldy);
+  else if (a.terms == 1)
+    hipLaunchKernelGGL((sketch_bf16_kernel<1, float>), grid, dim3(kThreads), 0, st, Ahi, Alo, Whi,
+                       Wlo, d.kp, a.m, a.nl, a.Y, a.ldy);
+  else if (a.Yd)
+    hipLaunchKernelGGL((sketch_bf16_kernel<3, double>), grid, dim3(kThreads), 0, st, Ahi, Alo, Whi,
+                       Wlo, d.kp, a.m, a.nl, a.Yd, a.ldy);
   else
-    hipLaunchKernelGGL(sketch_bf16_kernel<3>, dim3(static_cast<unsigned>(nblocks)), dim3(kThreads), 0,
-                       st, Ahi, Alo, Whi, Wlo, d.kp, a.m, a.nl, a.Y, a.ldy);
+    hipLaunchKernelGGL((sketch_bf16_kernel<3, float>), grid, dim3(kThreads), 0, st, Ahi, Alo, Whi,
+                       Wlo, d.kp, a.m, a.nl, a.Y, a.ldy);
   MCNLA_LAUNCHED();
   if (gemm_end) MCNLA_HIP(hipEventRecord(gemm_end, st));
 }

@@ -22,8 +22,7 @@ namespace mcnla {
 struct IntegratorState {
   int converged_at;  // iteration index that met the stopping test, INT_MAX while running
   int nan_seen;      // a non-finite value appeared in the l x l step
-  int have_warm;     // the KN step holds eigenvectors of the previous iteration's Z
-  int pad;
+  int pad[2];
   double res;        // last residual (KN: ||s - 1||_2, WY: mu)
   double taug, zeta, phi, mu;  // Wen-Yin scalars
 };
@@ -101,61 +100,41 @@ __global__ void reduce_partials_kernel(int t, const IntegratorState* st,
 // The KN l x l step (row_block_kolmogorov_nagumo_integrator.hpp:146-201): Dc = Bc'Bc/N and
 // Dgc = Bc'Bgc/N from the per-block partials P (nblk blocks of [Bc'Bc, Bc'Bgc]), Z = Dgc - Dc^2,
 // eig(Z), s = sqrt(1/2 + sqrt(1/4 - lambda)), C = V diag(s) V', Cinv = V diag(1/s) V',
-// F = C - Dc Cinv, convergence when !(||s - 1||_2 >= tol).  The eigen-decomposition starts
-// from the previous iteration's eigenvectors Vw (Z moves little between iterations), which
-// leaves C and Cinv unchanged but needs one or two Jacobi sweeps instead of six to ten.
-__global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorState* st, int l,
-                                                          double invN, double tol,
-                                                          const double* __restrict__ P,
-                                                          int64_t nblk, double* __restrict__ Vw,
-                                                          double* __restrict__ F,
-                                                          double* __restrict__ Cinv) {
-  if (skip(st, t)) return;
-  __shared__ double a[kMaxL * (kMaxL + 1)];
-  __shared__ double v[kMaxL * (kMaxL + 1)];
-  __shared__ double dc[kMaxL * (kMaxL + 1)];
-  __shared__ double z[kMaxL * (kMaxL + 1)];
+// F = C - Dc Cinv, convergence when !(||s - 1||_2 >= tol).
+//
+// C and Cinv are matrix functions of Z, so they are computed here without the eigenvectors:
+// X = (I - 4Z)^{1/2} and then C = ((I + X)/2)^{1/2}, Cinv = C^{-1} by two coupled Newton-Schulz
+// iterations on MFMA tiles in LDS (both converge quadratically because Z is positive
+// semidefinite with lambda <= 1/4), and ||s - 1||_2 = ||C - I||_F.  Should an iteration not
+// converge (lambda at or above 1/4, non-finite input) the step falls back to a Jacobi
+// eigen-decomposition and the formulas above, which also reproduce the reference's NaN there.
+__device__ void kn_step_eigen(int t, IntegratorState* st, int l, double invN, double tol,
+                              const double* __restrict__ P, int64_t nblk, double* __restrict__ F,
+                              double* __restrict__ Cinv, double* dc, double* a, double* v,
+                              double* z) {
   __shared__ double s[kMaxL];
   __shared__ double cs[2 * kMaxL];
   __shared__ int pq[2 * kMaxL];
-  __shared__ double red[2 * (kThreads / kWave)];
+  __shared__ double red[kThreads / kWave];
   const int ld = lds_ld(l);
   const int ll = l * l;
-  const bool warm = st->have_warm != 0;
-  // Dc into dc, Dgc into a
-  for (int e = threadIdx.x; e < 2 * ll; e += kThreads) {
+  for (int e = threadIdx.x; e < 2 * ll; e += kThreads) {  // Dc into dc, Dgc into a
     const double x = invN * sum_partials(P, nblk, 2 * ll, e);
     const int q = e % ll;
     (e < ll ? dc : a)[(q / l) * ld + q % l] = x;
   }
-  if (warm)
-    for (int e = threadIdx.x; e < ll; e += kThreads) v[(e / l) * ld + e % l] = Vw[e];
   __syncthreads();
-  // Z = sym(Dgc) - Dc Dc' (exactly symmetric as computed)
-  for (int e = threadIdx.x; e < ll; e += kThreads) {
+  for (int e = threadIdx.x; e < ll; e += kThreads) {  // Z = sym(Dgc) - Dc Dc'
     const int i = e / l, j = e % l;
     double x = 0.5 * (a[i * ld + j] + a[j * ld + i]);
     for (int q = 0; q < l; ++q) x -= dc[i * ld + q] * dc[j * ld + q];
     z[i * ld + j] = x;
   }
   __syncthreads();
-  if (warm) {  // z <- W' Z W
-    lds_matmul_nn(l, z, v, a);
-    __syncthreads();
-    lds_matmul_tn(l, v, a, z);
-    __syncthreads();
-    for (int e = threadIdx.x; e < ll; e += kThreads) {  // restore exact symmetry
-      const int i = e / l, j = e % l;
-      if (i > j) z[i * ld + j] = z[j * ld + i];
-    }
-    __syncthreads();
-  }
-  lds_jacobi(l, z, v, !warm, cs, pq, red);
+  lds_jacobi(l, z, v, true, cs, pq, red);
   for (int i = threadIdx.x; i < l; i += kThreads) s[i] = sqrt(0.5 + sqrt(0.25 - z[i * ld + i]));
-  for (int e = threadIdx.x; e < ll; e += kThreads) Vw[e] = v[(e / l) * ld + e % l];
   __syncthreads();
-  // Cinv into `a`, then F = C - Dc Cinv with C recomputed on the fly.
-  for (int e = threadIdx.x; e < ll; e += kThreads) {
+  for (int e = threadIdx.x; e < ll; e += kThreads) {  // Cinv into `a`
     const int i = e / l, j = e % l;
     double ci = 0.0;
     for (int q = 0; q < l; ++q) ci += v[i * ld + q] * v[j * ld + q] / s[q];
@@ -163,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorStat
     Cinv[e] = ci;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ll; e += kThreads) {
+  for (int e = threadIdx.x; e < ll; e += kThreads) {  // F = C - Dc Cinv, C on the fly
     const int i = e / l, j = e % l;
     double c = 0.0;
     for (int q = 0; q < l; ++q) c += v[i * ld + q] * v[j * ld + q] * s[q];
@@ -180,8 +159,85 @@ __global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorStat
     }
     r = sqrt(r);
     st->res = r;
-    st->have_warm = 1;
     if (bad) st->nan_seen = 1;
+    if (!(r >= tol)) st->converged_at = t;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorState* st, int l,
+                                                          double invN, double tol,
+                                                          const double* __restrict__ P,
+                                                          int64_t nblk, double* __restrict__ F,
+                                                          double* __restrict__ Cinv) {
+  if (skip(st, t)) return;
+  __shared__ double sDc[kMaxL * (kMaxL + 1)];
+  __shared__ double sY[kMaxL * (kMaxL + 1)];
+  __shared__ double sW[kMaxL * (kMaxL + 1)];
+  __shared__ double sT[kMaxL * (kMaxL + 1)];
+  __shared__ double red[kThreads / kWave];
+  const int np = pad16(l), ld = np + 1, ll = l * l;
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
+  const int nt1 = np / 16, ntiles = nt1 * nt1;
+  // Dc into sDc, Dgc into sT, both zero-padded to np x np
+  for (int e = tid; e < np * np; e += kThreads) {
+    const int i = e / np, j = e % np;
+    const bool in = i < l && j < l;
+    sDc[i * ld + j] = in ? invN * sum_partials(P, nblk, 2 * ll, i * l + j) : 0.0;
+    sT[i * ld + j] = in ? invN * sum_partials(P, nblk, 2 * ll, ll + i * l + j) : 0.0;
+  }
+  __syncthreads();
+  // Y = I - 4Z with Z = sym(Dgc) - Dc Dc'
+  for (int tile = wave; tile < ntiles; tile += kThreads / kWave) {
+    const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;
+    const d4 acc = lds_mfma_tile<false, true>(np, ld, sDc, sDc, i0, j0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + (lane >> 4) + 4 * r, j = j0 + (lane & 15);
+      const double zz = 0.5 * (sT[i * ld + j] + sT[j * ld + i]) - acc[r];
+      sY[i * ld + j] = ((i == j) ? 1.0 : 0.0) - 4.0 * zz;
+    }
+  }
+  __syncthreads();
+  int ok = lds_ns_sqrt(np, ld, sY, sW, sT, red, 60);
+  if (ok > 0) {
+    for (int e = tid; e < np * np; e += kThreads) {
+      const int i = e / np, j = e % np;
+      sY[i * ld + j] = 0.5 * (((i == j) ? 1.0 : 0.0) + sY[i * ld + j]);
+    }
+    __syncthreads();
+    ok = lds_ns_sqrt(np, ld, sY, sW, sT, red, 60);
+  }
+  if (ok <= 0) {  // rare: eigen-decomposition instead (uses its own layout of the buffers)
+    kn_step_eigen(t, st, l, invN, tol, P, nblk, F, Cinv, sDc, sT, sW, sY);
+    return;
+  }
+  // C = sY, Cinv = sW; F = C - Dc Cinv; ||C - I||_F
+  double rs = 0.0;
+  for (int tile = wave; tile < ntiles; tile += kThreads / kWave) {
+    const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;
+    const d4 acc = lds_mfma_tile<false, false>(np, ld, sDc, sW, i0, j0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + (lane >> 4) + 4 * r, j = j0 + (lane & 15);
+      if (i < l && j < l) {
+        const double c = sY[i * ld + j];
+        F[i * l + j] = c - acc[r];
+        Cinv[i * l + j] = sW[i * ld + j];
+        const double d = c - ((i == j) ? 1.0 : 0.0);
+        rs += d * d;
+      }
+    }
+  }
+  rs = wave_sum(rs);
+  __syncthreads();  // red is reused
+  if (lane == 0) red[wave] = rs;
+  __syncthreads();
+  if (tid == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < kThreads / kWave; ++w) tot += red[w];
+    const double r = sqrt(tot);
+    st->res = r;
+    if (!isfinite(r)) st->nan_seen = 1;
     if (!(r >= tol)) st->converged_at = t;
   }
 }
@@ -601,7 +657,6 @@ __global__ void wy_tau_kernel(int t, IntegratorState* st, int l, int odd,
 __global__ void init_state_kernel(IntegratorState* st) {
   st->converged_at = INT_MAX;
   st->nan_seen = 0;
-  st->have_warm = 0;
   st->res = 0.0;
   st->taug = WyConst::tau0;
   st->zeta = 1.0;
@@ -639,6 +694,57 @@ __global__ void wy_init_scalars_kernel(IntegratorState* st, int l, const double*
   st->mu = mu;
 }
 
+// Final re-orthonormalization of the basis Qbar = Qs M: Gm = M' Bs M = N * sum_blk P[blk][1]
+// (P from gram_partials_kernel over M and Bs M / N), W = Gm^{-1/2} by the Newton-Schulz square
+// root; Qbar W is the orthonormal basis nearest to Qbar (Loewdin).  The iterations keep Qbar
+// orthonormal only to the rounding they accumulate (as the reference's own updates do); this
+// step removes it.  W = I when Gm is not positive definite.
+__global__ __launch_bounds__(kThreads) void polish_kernel(int l, double N,
+                                                         const double* __restrict__ P,
+                                                         int64_t nblk, double* __restrict__ Wout) {
+  __shared__ double sY[kMaxL * (kMaxL + 1)];
+  __shared__ double sW[kMaxL * (kMaxL + 1)];
+  __shared__ double sT[kMaxL * (kMaxL + 1)];
+  __shared__ double red[kThreads / kWave];
+  const int np = pad16(l), ld = np + 1, ll = l * l;
+  for (int e = threadIdx.x; e < np * np; e += kThreads) {
+    const int i = e / np, j = e % np;
+    sT[i * ld + j] = (i < l && j < l) ? N * sum_partials(P, nblk, 2 * ll, ll + i * l + j)
+                                      : ((i == j) ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < np * np; e += kThreads) {
+    const int i = e / np, j = e % np;
+    sY[i * ld + j] = 0.5 * (sT[i * ld + j] + sT[j * ld + i]);
+  }
+  __syncthreads();
+  const int ok = lds_ns_sqrt(np, ld, sY, sW, sT, red, 60);
+  for (int e = threadIdx.x; e < ll; e += kThreads) {
+    const int i = e / l, j = e % l;
+    Wout[e] = ok > 0 ? sW[i * ld + j] : ((i == j) ? 1.0 : 0.0);
+  }
+}
+
+// out = X W for X Nl x l and W l x l (row-major).
+__global__ __launch_bounds__(kThreads) void right_mul_kernel(int64_t nl, int l,
+                                                            const double* __restrict__ X,
+                                                            const double* __restrict__ W,
+                                                            double* __restrict__ out) {
+  __shared__ double sW[kMaxL * kMaxL];
+  for (int e = threadIdx.x; e < l * l; e += kThreads) sW[e] = W[e];
+  __syncthreads();
+  const int64_t total = nl * l;
+  for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kThreads) {
+    const int64_t r = e / l;
+    const int j = e % l;
+    const double* x = X + r * l;
+    double acc = 0.0;
+    for (int q = 0; q < l; ++q) acc += x[q] * sW[q * l + j];
+    out[e] = acc;
+  }
+}
+
 // M = E + [FF; 0]  (Nl x l).
 __global__ void coef_kernel(const double* __restrict__ E, const double* __restrict__ FF,
                             int64_t nl, int l, double* __restrict__ M) {
@@ -669,7 +775,6 @@ struct Layout {
   double* D3[2];   // [Dc; Dgc; Hgg] per parity
   double* F;       // KN F / WY Fc
   double* C;       // KN Cinv / WY Fgc
-  double* Vw;      // KN warm-start eigenvectors
   double* U1;
   double* BsU1;
   double* P;       // partial dot products
@@ -708,7 +813,6 @@ Layout layout(char* base, int64_t N, int l) {
   }
   L.F = take(ll);
   L.C = take(ll);
-  L.Vw = take(ll);
   L.U1 = take(nll);
   L.BsU1 = take(nll);
   const size_t pp = static_cast<size_t>(L.nblk_p) * 3 * ll;
@@ -799,7 +903,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
                            L.B[c], L.Bg[0], L.P, 2);
         MCNLA_LAUNCHED();
         hipLaunchKernelGGL(kn_step_kernel, dim3(1), dim3(kThreads), 0, st, ti, L.st, l, invN, g.tol,
-                           L.P, L.nblk_p, L.Vw, L.F, L.C);
+                           L.P, L.nblk_p, L.F, L.C);
         MCNLA_LAUNCHED();
         hipLaunchKernelGGL(kn_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, ti,
                            L.st, nl, l, invN, L.B[c], L.Bg[0], L.B[p], L.E[c], L.E[p], L.FF[c],
@@ -844,7 +948,17 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
   const int64_t iters = (host_conv != INT_MAX) ? static_cast<int64_t>(host_conv) + 1 : g.maxiter;
   const int fin = static_cast<int>(iters & 1);
   hipLaunchKernelGGL(coef_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, L.E[fin],
-                     L.FF[fin], nl, l, coef);
+                     L.FF[fin], nl, l, L.U1);
+  MCNLA_LAUNCHED();
+  bs_product(L.U1, L.BsU1, -1);
+  hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, -1, L.st, nl, l, L.U1,
+                     L.BsU1, L.P, 2);
+  MCNLA_LAUNCHED();
+  hipLaunchKernelGGL(polish_kernel, dim3(1), dim3(kThreads), 0, st, l, static_cast<double>(g.N),
+                     L.P, L.nblk_p, L.F);
+  MCNLA_LAUNCHED();
+  hipLaunchKernelGGL(right_mul_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, nl, l, L.U1,
+                     L.F, coef);
   MCNLA_LAUNCHED();
   int nan_seen = 0;
   MCNLA_HIP(hipMemcpyAsync(&nan_seen, &L.st->nan_seen, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -51,6 +51,7 @@ struct SketchBf16Args {
   bool a_f32 = true;
   const double* omega = nullptr;  // n x nl row-major FP64
   float* Y = nullptr;
+  double* Yd = nullptr;  // when set, Y is written in FP64 here instead (values are the same)
   int64_t ldy = 0;
   int terms = 1;  // 1 or 3
 };

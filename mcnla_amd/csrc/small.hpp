@@ -125,6 +125,102 @@ __device__ inline void lds_matmul_nn(int n, const double* a, const double* b, do
   }
 }
 
+// ---- MFMA on LDS-resident square matrices ------------------------------------------------------
+// np x np matrices (np a multiple of 16, at most 64) kept row-major in LDS with leading dimension
+// ld.  Matrices of order l < np are embedded as diag(X, I) so that products, square roots and
+// inverses leave the padding block equal to the identity.
+
+__host__ __device__ constexpr int pad16(int n) { return (n + 15) & ~15; }
+
+// One 16 x 16 tile of op(A) op(B) on v_mfma_f64_16x16x4f64 (the whole wave must call it).  Slot r
+// of lane L holds element (i0 + L/16 + 4r, j0 + L%16) of the product.
+template <bool TA, bool TB>
+__device__ __forceinline__ d4 lds_mfma_tile(int np, int ld, const double* A, const double* B,
+                                            int i0, int j0) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int r = lane & 15, kq = lane >> 4;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < np; k0 += 4) {
+    const int k = k0 + kq;
+    const double a = TA ? A[k * ld + i0 + r] : A[(i0 + r) * ld + k];
+    const double b = TB ? B[(j0 + r) * ld + k] : B[k * ld + j0 + r];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// max that keeps a NaN once one was seen
+__device__ __forceinline__ double nan_max(double a, double b) {
+  if (a != a || b != b) return a + b;
+  return a > b ? a : b;
+}
+
+// Coupled Newton-Schulz iteration for the principal square root (Higham, "Functions of
+// Matrices", SIAM 2008, eq. (6.35)): Y0 = M, W0 = I, T = (3I - W Y)/2, Y <- Y T, W <- T W.  Y
+// converges quadratically to M^{1/2} and W to M^{-1/2} when ||I - M|| < 1.  On entry Y holds the
+// symmetric M; W is set to I here and T is scratch (all LDS, np x np, leading dimension ld).  The
+// update is done in place: every wave keeps its output tiles in registers across a barrier.
+// Stops once max|T - I| < 1e-8 (the following update leaves an error of about 3 (1e-8)^2).
+// Returns the number of iterations, or -1 when that did not happen within maxit (M not positive
+// definite, too ill-conditioned, or not finite).  `red` needs one double per wave.  Every thread
+// of the workgroup (at most 4 waves) must call this.
+__device__ inline int lds_ns_sqrt(int np, int ld, double* Y, double* W, double* T, double* red,
+                                  int maxit) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int wave = tid / kWave, nw = nt / kWave, lane = tid & (kWave - 1);
+  const int nt1 = np / 16, ntiles = nt1 * nt1;
+  for (int e = tid; e < np * np; e += nt) W[(e / np) * ld + e % np] = (e / np == e % np) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int it = 1; it <= maxit; ++it) {
+    double dmax = 0.0;
+    for (int tile = wave; tile < ntiles; tile += nw) {
+      const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;
+      const d4 acc = lds_mfma_tile<false, false>(np, ld, W, Y, i0, j0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + (lane >> 4) + 4 * r, j = j0 + (lane & 15);
+        const double id = (i == j) ? 1.0 : 0.0;
+        const double x = 1.5 * id - 0.5 * acc[r];
+        T[i * ld + j] = x;
+        dmax = nan_max(dmax, fabs(x - id));
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) dmax = nan_max(dmax, __shfl_xor(dmax, off, kWave));
+    if (lane == 0) red[wave] = dmax;
+    __syncthreads();
+    double d = 0.0;
+    for (int w = 0; w < nw; ++w) d = nan_max(d, red[w]);
+    d4 ay[4], aw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int tile = wave + q * nw;
+      if (tile < ntiles) {
+        const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;
+        ay[q] = lds_mfma_tile<false, false>(np, ld, Y, T, i0, j0);
+        aw[q] = lds_mfma_tile<false, false>(np, ld, T, W, i0, j0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int tile = wave + q * nw;
+      if (tile < ntiles) {
+        const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + (lane >> 4) + 4 * r, j = j0 + (lane & 15);
+          Y[i * ld + j] = ay[q][r];
+          W[i * ld + j] = aw[q][r];
+        }
+      }
+    }
+    __syncthreads();
+    if (d < 1e-8) return it;
+    if (!(d == d) || d > 1e6) return -1;
+  }
+  return -1;
+}
+
 // Sorts the eigenpairs left by lds_jacobi in DESCENDING order of eigenvalue and fixes each
 // eigenvector's sign so its largest-magnitude entry is positive.  Writes w (n) and the sorted,
 // sign-fixed vectors to vout (global, row-major n x n, vectors in columns).  `ord` needs n ints.
