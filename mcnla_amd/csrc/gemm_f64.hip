@@ -36,57 +36,100 @@ struct TileCfg {
   static_assert(BK % 4 == 0, "BK");
 };
 
-// Loads one operand tile (rows = BK k-values, cols = MN values) from global memory into
-// registers.  MN_CONTIG: element (k, mn) lives at base[mn + k*ld]; otherwise at base[mn*ld + k].
-template <typename TS, bool MN_CONTIG, int MN, int BK, int NT, int ELEMS>
+// Thread -> element map of an operand tile (BK k-values x MN values).  Elements go in groups of
+// VEC that are consecutive along the contiguous index: group g = e * NT + tid of a thread holds
+// (k, mn..mn+VEC-1) when MN_CONTIG, else (k..k+VEC-1, mn).  Loads and LDS stores use the same map.
+template <bool MN_CONTIG, int MN, int BK, int VEC>
+__device__ __forceinline__ void group_origin(int g, int& kk, int& mm) {
+  if (MN_CONTIG) {
+    kk = g / (MN / VEC);
+    mm = (g % (MN / VEC)) * VEC;
+  } else {
+    mm = g / (BK / VEC);
+    kk = (g % (BK / VEC)) * VEC;
+  }
+}
+
+template <typename TS, int VEC>
+struct VecOf;
+template <>
+struct VecOf<double, 1> { typedef double type; };
+template <>
+struct VecOf<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <>
+struct VecOf<float, 1> { typedef float type; };
+template <>
+struct VecOf<float, 2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <>
+struct VecOf<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+
+// Loads one operand tile into registers.  MN_CONTIG: element (k, mn) lives at base[mn + k*ld];
+// otherwise at base[mn*ld + k].  A full tile with VEC > 1 is read with one 8/16-byte load per
+// group (the launcher guarantees the alignment); partial tiles are read element by element with
+// zero fill.
+template <typename TS, bool MN_CONTIG, int MN, int BK, int NT, int ELEMS, int VEC>
 __device__ __forceinline__ void load_tile(double (&r)[ELEMS], const TS* __restrict__ base,
                                           int64_t ld, int64_t mn0, int64_t k0, int64_t mn_lim,
                                           int64_t k_lim, bool full) {
+  static_assert(ELEMS % VEC == 0, "vector groups");
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int e = 0; e < ELEMS; ++e) {
-    const int idx = e * NT + tid;
+  for (int e = 0; e < ELEMS / VEC; ++e) {
     int kk, mm;
-    if (MN_CONTIG) {
-      kk = idx / MN;
-      mm = idx % MN;
-    } else {
-      mm = idx / BK;
-      kk = idx % BK;
-    }
+    group_origin<MN_CONTIG, MN, BK, VEC>(e * NT + tid, kk, mm);
     const int64_t gm = mn0 + mm, gk = k0 + kk;
-    const int64_t off = MN_CONTIG ? (gm + gk * ld) : (gm * ld + gk);
-    if (full || (gm < mn_lim && gk < k_lim)) {
-      r[e] = static_cast<double>(base[off]);
-    } else {
-      r[e] = 0.0;
+    bool done = false;
+    if constexpr (VEC > 1) {
+      if (full) {
+        typedef typename VecOf<TS, VEC>::type V;
+        const int64_t off = MN_CONTIG ? (gm + gk * ld) : (gm * ld + gk);
+        const V v = *reinterpret_cast<const V*>(base + off);
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) r[e * VEC + q] = static_cast<double>(v[q]);
+        done = true;
+      }
+    }
+    if (!done) {
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) {
+        const int64_t qm = MN_CONTIG ? gm + q : gm, qk = MN_CONTIG ? gk : gk + q;
+        r[e * VEC + q] = (full || (qm < mn_lim && qk < k_lim))
+                             ? static_cast<double>(base[MN_CONTIG ? (qm + qk * ld) : (qm * ld + qk)])
+                             : 0.0;
+      }
     }
   }
 }
 
-template <bool MN_CONTIG, int MN, int BK, int NT, int ELEMS, int LD>
+template <bool MN_CONTIG, int MN, int BK, int NT, int ELEMS, int LD, int VEC>
 __device__ __forceinline__ void store_tile(double* __restrict__ lds, const double (&r)[ELEMS]) {
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int e = 0; e < ELEMS; ++e) {
-    const int idx = e * NT + tid;
+  for (int e = 0; e < ELEMS / VEC; ++e) {
     int kk, mm;
-    if (MN_CONTIG) {
-      kk = idx / MN;
-      mm = idx % MN;
-    } else {
-      mm = idx / BK;
-      kk = idx % BK;
+    group_origin<MN_CONTIG, MN, BK, VEC>(e * NT + tid, kk, mm);
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) {
+      if (MN_CONTIG)
+        lds[kk * LD + mm + q] = r[e * VEC + q];
+      else
+        lds[(kk + q) * LD + mm] = r[e * VEC + q];
     }
-    lds[kk * LD + mm] = r[e];
   }
 }
 
-template <typename TA, typename TB, bool A_MC, bool B_NC, int BM, int BN, int BK, int WM, int WN>
+constexpr int vec_for(int elem_bytes, int elems, bool ok) {
+  return !ok ? 1 : ((16 / elem_bytes) <= elems ? 16 / elem_bytes : elems);
+}
+
+template <typename TA, typename TB, bool A_MC, bool B_NC, int BM, int BN, int BK, int WM, int WN,
+          bool VECOK>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
     GemmF64Args p) {
   using Cfg = TileCfg<BM, BN, BK, WM, WN>;
   constexpr int NT = Cfg::kThreads;
+  constexpr int VA = vec_for(sizeof(TA), Cfg::kElemsA, VECOK);
+  constexpr int VB = vec_for(sizeof(TB), Cfg::kElemsB, VECOK);
   __shared__ double As[2][BK * Cfg::kLdA];
   __shared__ double Bs[2][BK * Cfg::kLdB];
 
@@ -134,10 +177,10 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
     const int64_t ntk = ceil_div(kend - kbeg, BK);
     {
       const bool full = full_mn && (kbeg + BK <= kend);
-      load_tile<TA, A_MC, BM, BK, NT, Cfg::kElemsA>(ra, A, p.lda, m0, kbeg, p.M, kend, full);
-      load_tile<TB, B_NC, BN, BK, NT, Cfg::kElemsB>(rb, B, p.ldb, n0, kbeg, p.N, kend, full);
-      store_tile<A_MC, BM, BK, NT, Cfg::kElemsA, Cfg::kLdA>(As[0], ra);
-      store_tile<B_NC, BN, BK, NT, Cfg::kElemsB, Cfg::kLdB>(Bs[0], rb);
+      load_tile<TA, A_MC, BM, BK, NT, Cfg::kElemsA, VA>(ra, A, p.lda, m0, kbeg, p.M, kend, full);
+      load_tile<TB, B_NC, BN, BK, NT, Cfg::kElemsB, VB>(rb, B, p.ldb, n0, kbeg, p.N, kend, full);
+      store_tile<A_MC, BM, BK, NT, Cfg::kElemsA, Cfg::kLdA, VA>(As[0], ra);
+      store_tile<B_NC, BN, BK, NT, Cfg::kElemsB, Cfg::kLdB, VB>(Bs[0], rb);
     }
     __syncthreads();
     for (int64_t kt = 0; kt < ntk; ++kt) {
@@ -146,8 +189,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
       if (more) {
         const int64_t k0 = kbeg + (kt + 1) * BK;
         const bool full = full_mn && (k0 + BK <= kend);
-        load_tile<TA, A_MC, BM, BK, NT, Cfg::kElemsA>(ra, A, p.lda, m0, k0, p.M, kend, full);
-        load_tile<TB, B_NC, BN, BK, NT, Cfg::kElemsB>(rb, B, p.ldb, n0, k0, p.N, kend, full);
+        load_tile<TA, A_MC, BM, BK, NT, Cfg::kElemsA, VA>(ra, A, p.lda, m0, k0, p.M, kend, full);
+        load_tile<TB, B_NC, BN, BK, NT, Cfg::kElemsB, VB>(rb, B, p.ldb, n0, k0, p.N, kend, full);
       }
       const double* __restrict__ as = As[cur];
       const double* __restrict__ bs = Bs[cur];
@@ -166,8 +209,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
       }
       if (more) {
-        store_tile<A_MC, BM, BK, NT, Cfg::kElemsA, Cfg::kLdA>(As[cur ^ 1], ra);
-        store_tile<B_NC, BN, BK, NT, Cfg::kElemsB, Cfg::kLdB>(Bs[cur ^ 1], rb);
+        store_tile<A_MC, BM, BK, NT, Cfg::kElemsA, Cfg::kLdA, VA>(As[cur ^ 1], ra);
+        store_tile<B_NC, BN, BK, NT, Cfg::kElemsB, Cfg::kLdB, VB>(Bs[cur ^ 1], rb);
       }
       __syncthreads();
     }
@@ -182,6 +225,157 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
   for (int i = 0; i < Cfg::kFragM; ++i)
 #pragma unroll
     for (int j = 0; j < Cfg::kFragN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = m0 + wm0 + i * 16 + (lane / 16) + 4 * r;
+        const int64_t gj = n0 + wn0 + j * 16 + lane % 16;
+        if (gi < p.M && gj < p.N) {
+          const double v = acc[i][j][r];
+          if (p.partial) {
+            C[gi * ldc + gj] = v;
+          } else {
+            double* c = C + gi * ldc + gj;
+            *c = (p.beta == 0.0) ? p.alpha * v : p.alpha * v + p.beta * *c;
+          }
+        }
+      }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Fast path for FP64 operands that are both contiguous along M / N (the sketch of a column-major
+// A, and Gram matrices Y'Y of a row-major Y): tiles go from HBM straight into LDS with
+// global_load_lds_dwordx4 (no staging registers), through a three-stage ring.
+//
+// Each wave instruction moves one 1 KiB row segment (128 doubles) of a tile.  Rows beyond M / N
+// read from the tile's first row (always valid) and only feed output rows that are never stored;
+// k-rows beyond the split's range read its first k-row and are zeroed in the A fragments, so a
+// tile never reads outside the operands.  Stage buffers are distinct LDS arrays addressed with
+// static indices (the loop is unrolled by three), which lets the compiler see that the in-flight
+// DMA of one stage never aliases the fragment reads of another; completion is tracked by hand
+// with s_waitcnt vmcnt and a bare s_barrier.
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int VM>
+__device__ __forceinline__ void wait_vmcnt() {
+  // gfx9 s_waitcnt fields: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt high bits [15:14]
+  __builtin_amdgcn_s_waitcnt((VM & 0xF) | (0x7 << 4) | (0xF << 8) | (((VM >> 4) & 3) << 14));
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kernel(GemmF64Args p) {
+  constexpr int NW = (BM / WM) * (BN / WN);
+  constexpr int LDA = BM + 4, LDB = BN + 4;
+  constexpr int RA = BM / 128, RB = BN / 128;          // wave instructions per tile row
+  constexpr int LA = BK * RA / NW, LB = BK * RB / NW;  // per wave and stage
+  constexpr int L = LA + LB;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  static_assert(BM % 128 == 0 && BN % 128 == 0 && (BK * RA) % NW == 0 && (BK * RB) % NW == 0,
+                "DMA tile shape");
+  __shared__ __attribute__((aligned(16))) double A0[BK * LDA];
+  __shared__ __attribute__((aligned(16))) double A1[BK * LDA];
+  __shared__ __attribute__((aligned(16))) double A2[BK * LDA];
+  __shared__ __attribute__((aligned(16))) double B0[BK * LDB];
+  __shared__ __attribute__((aligned(16))) double B1[BK * LDB];
+  __shared__ __attribute__((aligned(16))) double B2[BK * LDB];
+
+  if (p.gate != nullptr && *p.gate < p.gate_t) return;
+  const int64_t tiles_m = ceil_div(p.M, BM), tiles_n = ceil_div(p.N, BN);
+  const int64_t ntiles = p.lower_only ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
+  const int64_t nblocks = ntiles * p.splits * p.batch;
+  const int64_t logical = xcd_swizzle(blockIdx.x, nblocks);
+  const int64_t bat = logical / (ntiles * p.splits);
+  const int64_t split = (logical / ntiles) % p.splits;
+  const int64_t t = logical % ntiles;
+  int64_t tm, tn;
+  if (p.lower_only) {  // square tiles, lower triangle (BM == BN)
+    tm = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(t) + 1.0) - 1.0) * 0.5);
+    while (tm * (tm + 1) / 2 > t) --tm;
+    while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
+    tn = t - tm * (tm + 1) / 2;
+  } else {
+    tm = t / tiles_n;
+    tn = t % tiles_n;
+  }
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const int64_t kbeg = split * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int wm0 = (wave / (BN / WN)) * WM, wn0 = (wave % (BN / WN)) * WN;
+  const double* __restrict__ A = static_cast<const double*>(p.A) + bat * p.stride_a;
+  const double* __restrict__ B = static_cast<const double*>(p.B) + bat * p.stride_b;
+
+  d4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int64_t ntk = kbeg < kend ? ceil_div(kend - kbeg, BK) : 0;
+  auto issue = [&](int64_t kt, double* Ad, double* Bd) {
+    const int64_t k0 = kbeg + kt * BK;
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const int idx = q * NW + wave;
+      const int kk = idx / RA, ch = idx % RA;
+      const int64_t gk = (k0 + kk < kend) ? k0 + kk : kbeg;
+      int64_t gm = m0 + ch * 128 + lane * 2;
+      if (gm >= p.M) gm = m0;
+      __builtin_amdgcn_global_load_lds(A + gm + gk * p.lda, (lds_void*)&Ad[kk * LDA + ch * 128], 16,
+                                       0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+      const int idx = q * NW + wave;
+      const int kk = idx / RB, ch = idx % RB;
+      const int64_t gk = (k0 + kk < kend) ? k0 + kk : kbeg;
+      int64_t gn = n0 + ch * 128 + lane * 2;
+      if (gn >= p.N) gn = n0;
+      __builtin_amdgcn_global_load_lds(B + gn + gk * p.ldb, (lds_void*)&Bd[kk * LDB + ch * 128], 16,
+                                       0, 0);
+    }
+  };
+  auto step = [&](int64_t kt, const double* as, const double* bs, double* An, double* Bn) {
+    asm volatile("" ::: "memory");
+    if (kt + 1 < ntk) wait_vmcnt<L>();  // this stage landed; the next may still be in flight
+    else wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();       // ... for every wave; and stage kt-1 is fully read
+    asm volatile("" ::: "memory");
+    if (kt + 2 < ntk) issue(kt + 2, An, Bn);
+    const int64_t left = kend - (kbeg + kt * BK);
+    const int kvalid = left < BK ? static_cast<int>(left) : BK;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int krow = kk + lane / 16;
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const double v = as[krow * LDA + wm0 + i * 16 + lane % 16];
+        af[i] = krow < kvalid ? v : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = bs[krow * LDB + wn0 + j * 16 + lane % 16];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  };
+  if (ntk > 0) issue(0, A0, B0);
+  if (ntk > 1) issue(1, A1, B1);
+  for (int64_t kt = 0; kt < ntk; kt += 3) {
+    step(kt, A0, B0, A2, B2);
+    if (kt + 1 < ntk) step(kt + 1, A1, B1, A0, B0);
+    if (kt + 2 < ntk) step(kt + 2, A2, B2, A1, B1);
+  }
+
+  double* __restrict__ C = p.partial ? p.partial + (bat * p.splits + split) * p.M * p.N
+                                     : p.C + bat * p.stride_c;
+  const int64_t ldc = p.partial ? p.N : p.ldc;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gi = m0 + wm0 + i * 16 + (lane / 16) + 4 * r;
@@ -231,8 +425,19 @@ void launch_cfg(const GemmF64Args& p, hipStream_t st) {
   const int64_t ntiles = p.lower_only ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
   const int64_t nblocks = ntiles * p.splits * p.batch;
   MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
-  hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, A_MC, B_NC, BM, BN, BK, WM, WN>),
-                     dim3(static_cast<unsigned>(nblocks)), dim3(Cfg::kThreads), 0, st, p);
+  // 16-byte loads need 16-byte aligned operands whose leading dimensions and batch strides are
+  // multiples of the vector length.
+  auto aligned = [](const void* ptr, int64_t ld, int64_t stride, int64_t vec) {
+    return (reinterpret_cast<uintptr_t>(ptr) % 16 == 0) && ld % vec == 0 && stride % vec == 0;
+  };
+  const bool vec = aligned(p.A, p.lda, p.stride_a, 16 / sizeof(TA)) &&
+                   aligned(p.B, p.ldb, p.stride_b, 16 / sizeof(TB));
+  if (vec)
+    hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, A_MC, B_NC, BM, BN, BK, WM, WN, true>),
+                       dim3(static_cast<unsigned>(nblocks)), dim3(Cfg::kThreads), 0, st, p);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, A_MC, B_NC, BM, BN, BK, WM, WN, false>),
+                       dim3(static_cast<unsigned>(nblocks)), dim3(Cfg::kThreads), 0, st, p);
   MCNLA_LAUNCHED();
 }
 
@@ -285,7 +490,29 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
   MCNLA_REQUIRE(!p.lower_only || p.M == p.N, MCNLA_ERR_INVALID, "lower_only needs M == N");
   MCNLA_REQUIRE(!p.lower_only || p.shape == GemmShape::kSquare || p.shape == GemmShape::kSmall,
                 MCNLA_ERR_INVALID, "lower_only needs square tiles");
-  if (p.a_f32) {
+  auto even16 = [](const void* ptr, int64_t ld, int64_t stride) {
+    return reinterpret_cast<uintptr_t>(ptr) % 16 == 0 && ld % 2 == 0 && stride % 2 == 0;
+  };
+  const bool dma = !p.a_f32 && !p.b_f32 && p.a_mcontig && p.b_ncontig &&
+                   p.shape == GemmShape::kSquare && p.M >= 128 && p.N >= 128 &&
+                   even16(p.A, p.lda, p.stride_a) &&
+                   even16(p.B, p.ldb, p.stride_b);
+  if (dma) {
+    // 256 x 128 tiles of 8 waves (64 x 64 each); lower-triangle schedules need square tiles:
+    // 128 x 128 of 8 waves (32 x 64 each).
+    const int64_t tiles = p.lower_only
+                              ? ceil_div(p.M, 128) * (ceil_div(p.M, 128) + 1) / 2
+                              : ceil_div(p.M, 256) * ceil_div(p.N, 128);
+    const int64_t nblocks = tiles * p.splits * p.batch;
+    MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
+    if (p.lower_only)
+      hipLaunchKernelGGL((gemm_f64_dma_kernel<128, 128, 16, 32, 64>),
+                         dim3(static_cast<unsigned>(nblocks)), dim3(512), 0, st, p);
+    else
+      hipLaunchKernelGGL((gemm_f64_dma_kernel<256, 128, 16, 64, 64>),
+                         dim3(static_cast<unsigned>(nblocks)), dim3(512), 0, st, p);
+    MCNLA_LAUNCHED();
+  } else if (p.a_f32) {
     if (p.b_f32) launch_types<float, float>(p, st);
     else launch_types<float, double>(p, st);
   } else {

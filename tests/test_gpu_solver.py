@@ -182,6 +182,30 @@ def test_solver_mixed_vs_f64_headline_spectrum(gpu, prec, bound):
     assert np.max(np.abs(S64 - sig[:20]) / sig[:20]) < 1e-2
 
 
+def test_solver_ragged_tiles(gpu):
+    """Sizes that leave partial tiles on every edge of the sketch and Gram products (m=998, n=333,
+    l=17, N=10, so Nl=170 spans one full and one partial 128-tile, K=333 and K=998 are not
+    multiples of the K tile) against the oracle pipeline on the reference Omega."""
+    m, n, k, p, N, seed = 998, 333, 13, 4, 10, 9
+    note("solver ragged tiles %dx%d l=%d N=%d" % (m, n, k + p, N))
+    rng = np.random.default_rng(5)
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (U * (0.9 ** np.arange(n) + 1e-6)) @ V.T
+    l = k + p
+    om = oracle.omega_ref(seed, n, N * l, col_block=True)
+    Q, _ = oracle.gramian_orthogonalize(oracle.sketch(A, om), l, N)
+    Qb_o, it_o = oracle.kn_rowblock(Q, l, N, 256, 1e-4)
+    S_o, U_o, V_o = oracle.svd_former(A, Qb_o, k)
+    P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+    s = mc.Solver(P, seed=seed, omega_mode=mc.OMEGA_REFERENCE)
+    s.solve(mc.DeviceArray.from_numpy(np.asfortranarray(A)))
+    assert abs(s.iteration() - it_o) <= 1
+    assert np.max(np.abs(s.vectorS() - S_o) / S_o) < 1e-9
+    assert np.abs(proj(s.matrixU()) - proj(U_o)).max() < 1e-8
+    assert np.linalg.norm(proj(s.matrixQbar()) - proj(Qb_o), 2) < 1e-7
+
+
 def test_solver_reuse_and_seed(gpu, problem):
     note("solver reuse")
     A = problem["A"]
