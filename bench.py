@@ -34,19 +34,31 @@ def make_factors(m, n, r, seed):
 
 
 def timed_solves(mc, solver, A, steps, warmup):
+    """Times `steps` solves (after `warmup`); also returns, per solve, the HIP-event durations of
+    the sketch GEMM and Gram GEMM launches and of the four stages."""
     for _ in range(warmup):
         solver.solve(A)
     mc.synchronize()
-    gemm, stages = [], []
+    kern, stages = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         solver.solve(A)
-        times, g = solver.stage_times()
-        gemm.append(g)
+        times, _ = solver.stage_times()
+        kern.append(solver.kernel_times())
         stages.append(times)
     mc.synchronize()
     dt = time.perf_counter() - t0
-    return dt, np.array(gemm), np.array(stages)
+    return dt, np.array(kern), np.array(stages)
+
+
+def load_traffic():
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
+    plain copy of known size), committed under profiles/; None when absent."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
 
 
 def cpu_baseline(m, n, k, p, N, iters, budget_rows):
@@ -121,12 +133,14 @@ def main():
     smx = mc.Solver(P, seed=2, precision=prec)
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else max(3, args.steps // 2)
-    dt64, g64, st64 = timed_solves(mc, s64, A64, fp64_steps, 1)
-    log("bench: fp64 %.2f ms/solve (sketch gemm %.2f ms), iterations %d"
-        % (dt64 / fp64_steps * 1e3, g64.mean() * 1e3, s64.iteration()))
-    dtmx, gmx, stmx = timed_solves(mc, smx, A32, args.steps, args.warmup)
-    log("bench: mixed %.2f ms/solve (sketch gemm %.2f ms), iterations %d"
-        % (dtmx / args.steps * 1e3, gmx.mean() * 1e3, smx.iteration()))
+    dt64, k64, st64 = timed_solves(mc, s64, A64, fp64_steps, 1)
+    g64 = k64[:, 0]
+    log("bench: fp64 %.2f ms/solve (sketch gemm %.2f ms, gram gemm %.2f ms), iterations %d"
+        % (dt64 / fp64_steps * 1e3, g64.mean() * 1e3, k64[:, 1].mean() * 1e3, s64.iteration()))
+    dtmx, kmx, stmx = timed_solves(mc, smx, A32, args.steps, args.warmup)
+    gmx, grmx = kmx[:, 0], kmx[:, 1]
+    log("bench: mixed %.2f ms/solve (sketch gemm %.2f ms, gram gemm %.2f ms), iterations %d"
+        % (dtmx / args.steps * 1e3, gmx.mean() * 1e3, grmx.mean() * 1e3, smx.iteration()))
     S64, Smx = s64.vectorS(), smx.vectorS()
     rel = float(np.max(np.abs(Smx - S64) / S64))
     Q64, Qmx = s64.matrixQbar(), smx.matrixQbar()
@@ -134,10 +148,31 @@ def main():
     c = np.linalg.svd(Q64.T @ Qmx, compute_uv=False)
     dist = float(np.sqrt(max(0.0, 1.0 - c.min() ** 2)))
 
-    flops_bf16 = 2.0 * m * n * N * l * (1 if args.mixed == "bf16" else 3)
-    flops_f64 = 2.0 * m * n * N * l
+    nl = N * l
+    flops_bf16 = 2.0 * m * n * nl * (1 if args.mixed == "bf16" else 3)
+    flops_f64 = 2.0 * m * n * nl
+    flops_gram = 1.0 * m * nl * (nl + 1)  # lower triangle with diagonal, 2 flops per product
     ach_mx = flops_bf16 / gmx.mean()
     ach_64 = flops_f64 / g64.mean()
+    traffic = load_traffic()
+    kernels = {
+        "sketch_gemm": {"kernel": "sketch_blk_kernel (Y = A*Omega, BF16 MFMA, FP32 accumulate)"
+                        if args.mixed == "bf16" else
+                        "sketch_bf16_kernel<3> (Y = A*Omega, BF16x3 MFMA, FP32 accumulate)",
+                        "ms": round(gmx.mean() * 1e3, 3), "bound": "mfma",
+                        "achieved": round(ach_mx / 1e12, 2), "peak": MI355X_PEAK["bf16"] / 1e12,
+                        "unit": "TFLOP/s", "frac": round(ach_mx / MI355X_PEAK["bf16"], 4),
+                        "traffic": traffic.get("sketch_blk_kernel"),
+                        "algorithmic": "2*m*n*N*l = %.4e FLOP per launch" % flops_bf16},
+        "gram_gemm": {"kernel": "gemm_f64_dma_kernel (G = Y'Y, FP64 MFMA, lower-triangle tiles)",
+                      "ms": round(grmx.mean() * 1e3, 3), "bound": "mfma",
+                      "achieved": round(flops_gram / grmx.mean() / 1e12, 2),
+                      "peak": MI355X_PEAK["f64"] / 1e12, "unit": "TFLOP/s",
+                      "frac": round(flops_gram / grmx.mean() / MI355X_PEAK["f64"], 4),
+                      "traffic": traffic.get("gemm_f64_dma_kernel_gram"),
+                      "algorithmic": "m*Nl*(Nl+1) = %.4e FLOP per launch" % flops_gram},
+    }
+    dominant = max(kernels, key=lambda key: kernels[key]["ms"])
     value = args.steps / dtmx
     fp64_value = fp64_steps / dt64
     out = {
@@ -162,18 +197,16 @@ def main():
                  "iterations": s64.iteration(),
                  "stage_ms": [round(x * 1e3, 3) for x in st64.mean(axis=0)],
                  "sketch_gemm_ms": round(g64.mean() * 1e3, 3),
+                 "gram_gemm_ms": round(k64[:, 1].mean() * 1e3, 3),
                  "sketch_gemm_tflops": round(ach_64 / 1e12, 2),
                  "sketch_gemm_frac_of_f64_peak": round(ach_64 / MI355X_PEAK["f64"], 3)},
         "mixed": {"iterations": smx.iteration(),
                   "stage_ms": [round(x * 1e3, 3) for x in stmx.mean(axis=0)],
                   "sigma_rel_err_vs_fp64": rel, "subspace_dist_vs_fp64": dist,
-                  "speedup_vs_fp64": round(value / fp64_value, 2)},
+                  "speedup_vs_fp64": round(value / fp64_value, 2),
+                  "kernels": kernels},
         "stage_order": ["sketch", "orthogonalize", "integrate", "form"],
-        "roofline": {"kernel": "sketch_bf16_kernel (Y = A*Omega, BF16 MFMA)", "bound": "mfma",
-                     "achieved": round(ach_mx / 1e12, 2), "peak": MI355X_PEAK["bf16"] / 1e12,
-                     "unit": "TFLOP/s", "frac": round(ach_mx / MI355X_PEAK["bf16"], 4),
-                     "traffic": None,
-                     "algorithmic": "2*m*n*N*l = %.3e FLOP per launch" % flops_bf16},
+        "roofline": {k: v for k, v in kernels[dominant].items() if k != "ms"},
     }
     if not args.no_cpu:
         log("bench: cpu baseline (oracle, %d-row sample)" % args.cpu_rows)

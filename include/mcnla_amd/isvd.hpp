@@ -428,6 +428,25 @@ template <typename _Val> using RowBlockGramianKolmogorovNagumoIntegrator =
 template <typename _Val> using RowBlockWenYinIntegrator = IntegratorImpl<MCNLA_INT_ROW_BLOCK_WY, _Val>;
 template <typename _Val> using RowBlockGramianWenYinIntegrator =
     IntegratorImpl<MCNLA_INT_ROW_BLOCK_GRAMIAN_WY, _Val>;
+// The combining integrators do not iterate (maxIteration/tolerance are unused, iteration() is 0);
+// the extrinsic mean's (Qj, Q, Qbar) call form is accepted with Qj = Q on one rank
+// (row_block_extrinsic_mean_integrator.hh:109-111).
+template <typename _Val> using RowBlockReductionIntegrator =
+    IntegratorImpl<MCNLA_INT_ROW_BLOCK_REDUCTION, _Val>;
+template <typename _Val>
+class RowBlockExtrinsicMeanIntegrator
+    : public IntegratorImpl<MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN, _Val> {
+  using Base = IntegratorImpl<MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN, _Val>;
+
+ public:
+  using Base::Base;
+  using Base::operator();
+  void operator()(const DenseMatrixCollectionColBlockRowMajor<_Val>& qj,
+                  const DenseMatrixCollectionColBlockRowMajor<_Val>& /*q*/,
+                  DenseMatrixRowMajor<_Val>& qbar) {
+    Base::operator()(qj, qbar);
+  }
+};
 
 // ---------------------------------------------------------------------------------------------
 // Formers (include/mcnla/isvd/former/*.hh).  The SVD former returns U column-major
@@ -456,7 +475,7 @@ class FormerImpl : public StageBase {
     u_.reconstruct(m, k);
     check(mcnla_memcpy_dtoh(s_.valPtr(), S, k * sizeof(double)), "S");
     check(mcnla_memcpy_dtoh(u_.valPtr(), U, m * k * sizeof(double)), "U");
-    if (_jobv) {
+    if (_jobv && V != nullptr) {
       v_.reconstruct(n, k);
       check(mcnla_memcpy_dtoh(v_.valPtr(), V, n * k * sizeof(double)), "V");
     }
@@ -481,6 +500,10 @@ template <typename _Val, bool _jobv = true> using ColBlockGramianFormer =
     FormerImpl<MCNLA_FORM_COL_BLOCK_GRAMIAN, _Val, _jobv>;
 template <typename _Val, bool _jobv = true> using RowBlockGramianFormer =
     FormerImpl<MCNLA_FORM_ROW_BLOCK_GRAMIAN, _Val, _jobv>;
+// Square symmetric A: vectorS() = the k largest Ritz values (ascending), matrixUj() = Qbar W;
+// V is not formed (row_block_symmetric_former.hpp:98-131).
+template <typename _Val, bool _jobv = true> using RowBlockSymmetricFormer =
+    FormerImpl<MCNLA_FORM_ROW_BLOCK_SYMMETRIC, _Val, _jobv>;
 
 // ---------------------------------------------------------------------------------------------
 // Converters (include/mcnla/isvd/converter/*.hh): with every row and every sketch on one GPU

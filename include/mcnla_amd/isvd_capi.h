@@ -131,14 +131,19 @@ mcnla_status mcnla_orthogonalizer_run(mcnla_stage s, void* Qs, void* stream);
  * RowBlockKolmogorovNagumoIntegrator (row_block_kolmogorov_nagumo_integrator.hpp:80-208),
  * RowBlockGramianKolmogorovNagumoIntegrator (row_block_gramian_kolmogorov_nagumo_integrator.hpp:80-229),
  * RowBlockWenYinIntegrator (row_block_wen_yin_integrator.hpp:80-281),
- * RowBlockGramianWenYinIntegrator (row_block_gramian_wen_yin_integrator.hpp:87-331).
+ * RowBlockGramianWenYinIntegrator (row_block_gramian_wen_yin_integrator.hpp:87-331),
+ * RowBlockReductionIntegrator (row_block_reduction_integrator.hpp:70-121) and
+ * RowBlockExtrinsicMeanIntegrator (row_block_extrinsic_mean_integrator.hpp:90-163); the last two
+ * do not iterate (maxiter and tol are ignored, iteration() is 0).
  * Defaults of the reference: maxiter 256; tol 1e-4 (KN) / 1e-3 (WY). */
 typedef enum {
   MCNLA_INT_KN = 0,
   MCNLA_INT_ROW_BLOCK_KN = 1,
   MCNLA_INT_ROW_BLOCK_GRAMIAN_KN = 2,
   MCNLA_INT_ROW_BLOCK_WY = 3,
-  MCNLA_INT_ROW_BLOCK_GRAMIAN_WY = 4
+  MCNLA_INT_ROW_BLOCK_GRAMIAN_WY = 4,
+  MCNLA_INT_ROW_BLOCK_REDUCTION = 5,
+  MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN = 6
 } mcnla_integrator_kind;
 mcnla_status mcnla_integrator_create(const mcnla_params* params, int kind, int precision,
                                      int64_t maxiter, double tol, mcnla_stage* out);
@@ -148,13 +153,16 @@ mcnla_status mcnla_integrator_run(mcnla_stage s, const void* Qs, double* Qbar, v
 mcnla_status mcnla_integrator_iteration(mcnla_stage s, int64_t* iteration);
 
 /* Formers: SvdFormer (svd_former.hpp:71-116), GramianFormer (gramian_former.hpp),
- * ColBlockGramianFormer (col_block_gramian_former.hpp:74-131) and RowBlockGramianFormer
- * (row_block_gramian_former.hpp:79-146).  jobv != 0 also forms V. */
+ * ColBlockGramianFormer (col_block_gramian_former.hpp:74-131), RowBlockGramianFormer
+ * (row_block_gramian_former.hpp:79-146) and RowBlockSymmetricFormer
+ * (row_block_symmetric_former.hpp:98-131, square symmetric A: S holds the k algebraically
+ * largest Ritz values in ascending order, U = Qbar W; V is not formed).  jobv != 0 also forms V. */
 typedef enum {
   MCNLA_FORM_SVD = 0,
   MCNLA_FORM_GRAMIAN = 1,
   MCNLA_FORM_COL_BLOCK_GRAMIAN = 2,
-  MCNLA_FORM_ROW_BLOCK_GRAMIAN = 3
+  MCNLA_FORM_ROW_BLOCK_GRAMIAN = 3,
+  MCNLA_FORM_ROW_BLOCK_SYMMETRIC = 4
 } mcnla_former_kind;
 mcnla_status mcnla_former_create(const mcnla_params* params, int kind, int jobv, int precision,
                                  mcnla_stage* out);
@@ -211,8 +219,13 @@ mcnla_status mcnla_solver_results(mcnla_solver s, const double** S, const double
                                   const double** V, const double** Qbar);
 mcnla_status mcnla_solver_iteration(mcnla_solver s, int64_t* iteration);
 /* Per-stage device times of the last solve (seconds): [sketch, orthogonalize, integrate, form]
- * plus, in kernel_ms, the duration of the sketch GEMM launch alone. */
+ * plus, in sketch_kernel_s, the duration of the sketch GEMM launch alone. */
 mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* sketch_kernel_s);
+/* Durations (seconds, HIP events on the solve's stream) of single kernel launches of the last
+ * solve: times[0] = sketch GEMM, times[1] = Gram GEMM G = Y'Y (without its split-K reduction).
+ * No reference counterpart (the reference times whole stages only, stage_wrapper.hpp:120-140);
+ * this is for roofline accounting. */
+mcnla_status mcnla_solver_kernel_times(mcnla_solver s, double times[2]);
 
 /* ---------------------------------------------------------------------------------------------
  * Synthetic test matrices (the reference demo's construction, demo/demo.cpp:250-293):

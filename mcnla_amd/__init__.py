@@ -13,9 +13,10 @@ from .isvd import (
     GaussianProjectionSketcher, GramianFormer, GramianOrthogonalizer, KolmogorovNagumoIntegrator,
     MatrixFromColBlockToAllConverter, MatrixFromRowBlockConverter, MatrixFromRowBlockToAllConverter,
     MatrixToRowBlockConverter, Parameters, RowBlockGaussianProjectionSketcher,
-    RowBlockGramianFormer, RowBlockGramianKolmogorovNagumoIntegrator, RowBlockGramianOrthogonalizer,
-    RowBlockGramianWenYinIntegrator, RowBlockKolmogorovNagumoIntegrator, RowBlockWenYinIntegrator,
-    Solver, SvdFormer, synth_lowrank,
+    RowBlockExtrinsicMeanIntegrator, RowBlockGramianFormer, RowBlockGramianKolmogorovNagumoIntegrator,
+    RowBlockGramianOrthogonalizer, RowBlockGramianWenYinIntegrator,
+    RowBlockKolmogorovNagumoIntegrator, RowBlockReductionIntegrator, RowBlockSymmetricFormer,
+    RowBlockWenYinIntegrator, Solver, SvdFormer, synth_lowrank,
 )
 
 PREC_F64 = _lib.PREC_F64

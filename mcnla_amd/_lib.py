@@ -20,8 +20,10 @@ PREC_F64, PREC_MIXED_BF16, PREC_MIXED_BF16X3 = 0, 1, 2
 SKETCHER_COL_BLOCK_GAUSSIAN, SKETCHER_ROW_BLOCK_GAUSSIAN, SKETCHER_GAUSSIAN = 0, 1, 2
 OMEGA_PHILOX, OMEGA_REFERENCE = 0, 1
 ORTH_GRAMIAN, ORTH_ROW_BLOCK_GRAMIAN = 0, 1
-INT_KN, INT_ROW_BLOCK_KN, INT_ROW_BLOCK_GRAMIAN_KN, INT_ROW_BLOCK_WY, INT_ROW_BLOCK_GRAMIAN_WY = range(5)
-FORM_SVD, FORM_GRAMIAN, FORM_COL_BLOCK_GRAMIAN, FORM_ROW_BLOCK_GRAMIAN = range(4)
+(INT_KN, INT_ROW_BLOCK_KN, INT_ROW_BLOCK_GRAMIAN_KN, INT_ROW_BLOCK_WY, INT_ROW_BLOCK_GRAMIAN_WY,
+ INT_ROW_BLOCK_REDUCTION, INT_ROW_BLOCK_EXTRINSIC_MEAN) = range(7)
+(FORM_SVD, FORM_GRAMIAN, FORM_COL_BLOCK_GRAMIAN, FORM_ROW_BLOCK_GRAMIAN,
+ FORM_ROW_BLOCK_SYMMETRIC) = range(5)
 
 STATUS_NAMES = {
     OK: "OK", ERR_INVALID: "INVALID", ERR_NOMEM: "NOMEM", ERR_HIP: "HIP",
@@ -103,6 +105,7 @@ SIGNATURES = {
     "mcnla_solver_results": (_st, [_vp, _pp, _pp, _pp, _pp]),
     "mcnla_solver_iteration": (_st, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "mcnla_solver_stage_times": (_st, [_vp, _dp, _dp]),
+    "mcnla_solver_kernel_times": (_st, [_vp, _dp]),
     "mcnla_synth_lowrank": (_st, [_i64, _i64, _i64, _vp, _vp, _vp, _d, _u64, _vp, _i64, _i, _i,
                                   _vp]),
 }

@@ -220,9 +220,30 @@ struct SketchCore {
 // G = Y' * Y (Nl x Nl, FP64, full symmetric), Y row-major m x Nl.
 struct GramCore {
   DevBuf partial;
+  hipEvent_t beg = nullptr, end = nullptr;
+  ~GramCore() {
+    if (beg) (void)hipEventDestroy(beg);
+    if (end) (void)hipEventDestroy(end);
+  }
+  double gemm_seconds() {
+    if (!beg) return 0.0;
+    MCNLA_HIP(hipEventSynchronize(end));
+    float ms = 0.f;
+    MCNLA_HIP(hipEventElapsedTime(&ms, beg, end));
+    return ms * 1e-3;
+  }
   void run(const void* Y, bool y_f32, int64_t m, int64_t nl, double* G, hipStream_t st) {
-    const int64_t t = ceil_div(nl, 128);
+    if (!beg) {
+      MCNLA_HIP(hipEventCreate(&beg));
+      MCNLA_HIP(hipEventCreate(&end));
+    }
+    // Lower-triangle tiles: 256 x 128 on the FP64 fast path (tm (tm+1) tiles), else 128 x 128.
+    // Split K so that the grid fills whole rounds of about 2304 workgroups (9 per CU).
+    const int64_t tm = ceil_div(nl, 256), t = ceil_div(nl, 128);
+    const int64_t tiles = (!y_f32 && nl >= 128) ? tm * (tm + 1) : t * (t + 1) / 2;
     GemmF64Args g;
+    g.ev_begin = beg;
+    g.ev_end = end;
     g.M = nl;
     g.N = nl;
     g.K = m;
@@ -238,7 +259,7 @@ struct GramCore {
     g.ldc = nl;
     g.lower_only = true;
     g.shape = GemmShape::kSquare;
-    g.splits = gemm_f64_splits_for(t * (t + 1) / 2, m, 2048);
+    g.splits = gemm_f64_splits_for(tiles, m, 2304);
     g.kchunk = ceil_div(m, g.splits);
     g.kchunk = ceil_div(g.kchunk, 16) * 16;
     g.splits = ceil_div(m, g.kchunk);
@@ -251,8 +272,13 @@ struct GramCore {
 // Z = A' Qbar (n x l), Wg = Z' Z, eig, and the outputs of either former.
 struct FormCore {
   DevBuf Z, partial, Wg, eigw, eigv, Wk, S, U, V, Vtmp;
-  void run(const mcnla_params& p, bool svd, bool jobv, const void* A, int64_t lda, int dtype,
+  // kind: MCNLA_FORM_*.  The symmetric former (square symmetric A) uses Wg = Z'Qbar = Qbar'A Qbar
+  // in place of Z'Z and forms no V.
+  void run(const mcnla_params& p, int kind, bool jobv, const void* A, int64_t lda, int dtype,
            int order, const double* Qbar, hipStream_t st) {
+    const bool svd = kind == MCNLA_FORM_SVD;
+    const bool sym = kind == MCNLA_FORM_ROW_BLOCK_SYMMETRIC;
+    if (sym) jobv = false;
     const int64_t m = p.nrow, n = p.ncol, k = p.rank;
     const int l = static_cast<int>(p.rank + p.over_rank);
     Z.ensure(n * l * sizeof(double));
@@ -281,7 +307,7 @@ struct FormCore {
     z.kchunk = ceil_div(ceil_div(m, z.splits), 16) * 16;
     z.splits = ceil_div(m, z.kchunk);
     size_t need = gemm_f64_partial_elems(n, l, z.splits);
-    // Wg = Z' Z
+    // Wg = Z' Z (symmetric former: Z' Qbar, Qbar having n = m rows)
     GemmF64Args w;
     w.M = l;
     w.N = l;
@@ -289,7 +315,7 @@ struct FormCore {
     w.A = Z.p;
     w.lda = l;
     w.a_mcontig = true;
-    w.B = Z.p;
+    w.B = sym ? static_cast<const void*>(Qbar) : Z.p;
     w.ldb = l;
     w.C = Wg.as<double>();
     w.ldc = l;
@@ -304,7 +330,7 @@ struct FormCore {
     gemm_f64(z, st);
     gemm_f64(w, st);
     former_small(Wg.as<double>(), l, static_cast<int>(k), eigw.as<double>(), eigv.as<double>(),
-                 Wk.as<double>(), S.as<double>(), st);
+                 Wk.as<double>(), S.as<double>(), st, sym);
     // V = Z Wk (n x k); the SVD former takes sigma_j = ||Z w_j||.
     GemmF64Args v;
     v.M = n;
@@ -417,6 +443,9 @@ const char* integrator_name(int kind) {
     case MCNLA_INT_ROW_BLOCK_GRAMIAN_KN:
       return "Kolmogorov-Nagumo-Type Integrator (Row-Block Gramian Version)";
     case MCNLA_INT_ROW_BLOCK_WY: return "Wen-Yin Line Search Integrator (Row-Block Version)";
+    case MCNLA_INT_ROW_BLOCK_REDUCTION: return "Reduction Integrator (Row-Block Version)";
+    case MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN:
+      return "Extrinsic Mean Integrator (Row-Block Version)";
     default: return "Wen-Yin Line Search Integrator (Row-Block Gramian Version)";
   }
 }
@@ -425,12 +454,48 @@ const char* former_name(int kind) {
     case MCNLA_FORM_SVD: return "SVD Former";
     case MCNLA_FORM_GRAMIAN: return "Gramian Former";
     case MCNLA_FORM_COL_BLOCK_GRAMIAN: return "Gramian Former (Column-Block Version)";
+    case MCNLA_FORM_ROW_BLOCK_SYMMETRIC: return "Symmetric Former (Row-Block Version)";
     default: return "Gramian Former (Row-Block Version)";
   }
 }
 
 bool is_wen_yin(int kind) {
   return kind == MCNLA_INT_ROW_BLOCK_WY || kind == MCNLA_INT_ROW_BLOCK_GRAMIAN_WY;
+}
+
+bool is_combining(int kind) {
+  return kind == MCNLA_INT_ROW_BLOCK_REDUCTION || kind == MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN;
+}
+
+size_t integration_workspace(int kind, int64_t N, int l) {
+  return is_combining(kind) ? combine_workspace_bytes(kind == MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN, N, l)
+                            : integrator_workspace_bytes(N, l);
+}
+
+// Runs integrator `kind` on the Gram matrix Bs (Nl x Nl) of an orthonormal collection; writes
+// the Nl x l coefficients of Qbar.  Returns the iteration count (0 for the combining kinds).
+int64_t run_integration(int kind, const double* Bs, int64_t N, int l, int64_t maxiter, double tol,
+                        double* coef, DevBuf& ws, hipStream_t st, bool* conv, bool* ok) {
+  ws.ensure(integration_workspace(kind, N, l));
+  if (is_combining(kind)) {
+    if (kind == MCNLA_INT_ROW_BLOCK_REDUCTION)
+      integrate_reduction(Bs, N, l, coef, ws.as<char>(), ws.bytes, st);
+    else
+      integrate_extrinsic(Bs, N, l, coef, ws.as<char>(), ws.bytes, st);
+    *conv = true;
+    *ok = true;
+    return 0;
+  }
+  GramIntegrator gi;
+  gi.N = N;
+  gi.l = l;
+  gi.wen_yin = is_wen_yin(kind);
+  gi.maxiter = maxiter;
+  gi.tol = tol;
+  gi.Bs = const_cast<double*>(Bs);
+  gi.ws = ws.as<char>();
+  gi.ws_bytes = ws.bytes;
+  return integrate_gram(gi, coef, st, conv, ok);
 }
 
 hipStream_t S(void* stream) { return static_cast<hipStream_t>(stream); }
@@ -487,22 +552,13 @@ mcnla_status integrate(Integrator& it, const void* Qs, double* Qbar, hipStream_t
   const int64_t N = p.num_sketch, m = p.nrow, nl = it.nl();
   const bool f32 = is_mixed(it.precision);
   it.Bs.ensure(nl * nl * sizeof(double));
-  it.ws.ensure(integrator_workspace_bytes(N, l));
   it.coef.ensure(nl * l * sizeof(double));
   it.timer.start(st);
   it.gram.run(Qs, f32, m, nl, it.Bs.as<double>(), st);
-  GramIntegrator gi;
-  gi.N = N;
-  gi.l = l;
-  gi.wen_yin = is_wen_yin(it.kind);
-  gi.maxiter = it.maxiter;
-  gi.tol = it.tol;
-  gi.Bs = it.Bs.as<double>();
-  gi.ws = it.ws.as<char>();
-  gi.ws_bytes = it.ws.bytes;
   it.timer.mark(st);
   bool conv = false, ok = true;
-  it.iteration = integrate_gram(gi, it.coef.as<double>(), st, &conv, &ok);
+  it.iteration = run_integration(it.kind, it.Bs.as<double>(), N, l, it.maxiter, it.tol,
+                                 it.coef.as<double>(), it.ws, st, &conv, &ok);
   it.timer.mark(st);
   GemmF64Args q;
   q.M = m;
@@ -782,7 +838,7 @@ mcnla_status mcnla_integrator_create(const mcnla_params* params, int kind, int p
     check_params(params);
     check_precision(precision);
     MCNLA_REQUIRE(out, MCNLA_ERR_INVALID, "out is null");
-    MCNLA_REQUIRE(kind >= MCNLA_INT_KN && kind <= MCNLA_INT_ROW_BLOCK_GRAMIAN_WY, MCNLA_ERR_INVALID,
+    MCNLA_REQUIRE(kind >= MCNLA_INT_KN && kind <= MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN, MCNLA_ERR_INVALID,
                   "unknown integrator kind");
     MCNLA_REQUIRE(maxiter >= 0 && maxiter < (1LL << 30), MCNLA_ERR_INVALID, "bad maxiter");
     MCNLA_REQUIRE(tol >= 0.0, MCNLA_ERR_INVALID, "tolerance must be non-negative");
@@ -823,7 +879,7 @@ mcnla_status mcnla_former_create(const mcnla_params* params, int kind, int jobv,
     check_params(params);
     check_precision(precision);
     MCNLA_REQUIRE(out, MCNLA_ERR_INVALID, "out is null");
-    MCNLA_REQUIRE(kind >= MCNLA_FORM_SVD && kind <= MCNLA_FORM_ROW_BLOCK_GRAMIAN, MCNLA_ERR_INVALID,
+    MCNLA_REQUIRE(kind >= MCNLA_FORM_SVD && kind <= MCNLA_FORM_ROW_BLOCK_SYMMETRIC, MCNLA_ERR_INVALID,
                   "unknown former kind");
     auto f = std::make_unique<Former>();
     f->kind = kind;
@@ -844,9 +900,11 @@ mcnla_status mcnla_former_run(mcnla_stage s, const void* A, int64_t lda, int a_d
     auto* f = static_cast<Former*>(s);
     check_a(f->prm, A, lda, a_dtype, a_order);
     MCNLA_REQUIRE(Qbar, MCNLA_ERR_INVALID, "Qbar is null");
+    MCNLA_REQUIRE(f->kind != MCNLA_FORM_ROW_BLOCK_SYMMETRIC || f->prm.nrow == f->prm.ncol,
+                  MCNLA_ERR_INVALID, "the symmetric former needs a square A");
     hipStream_t st = S(stream);
     f->timer.start(st);
-    f->core.run(f->prm, f->kind == MCNLA_FORM_SVD, f->jobv, A, lda, a_dtype, a_order, Qbar, st);
+    f->core.run(f->prm, f->kind, f->jobv, A, lda, a_dtype, a_order, Qbar, st);
     f->timer.mark(st);
     f->computed = true;
     return MCNLA_OK;
@@ -861,7 +919,8 @@ mcnla_status mcnla_former_results(mcnla_stage s, const double** Sv, const double
     MCNLA_REQUIRE(f->computed, MCNLA_ERR_STATE, "former has not run");
     if (Sv) *Sv = f->core.S.as<double>();
     if (U) *U = f->core.U.as<double>();
-    if (V) *V = f->jobv ? f->core.V.as<double>() : nullptr;
+    if (V)
+      *V = (f->jobv && f->kind != MCNLA_FORM_ROW_BLOCK_SYMMETRIC) ? f->core.V.as<double>() : nullptr;
     return MCNLA_OK;
   });
 }
@@ -915,10 +974,12 @@ mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_
     check_params(params);
     MCNLA_REQUIRE(cfg && out, MCNLA_ERR_INVALID, "null argument");
     check_precision(cfg->precision);
-    MCNLA_REQUIRE(cfg->integrator >= MCNLA_INT_KN && cfg->integrator <= MCNLA_INT_ROW_BLOCK_GRAMIAN_WY,
+    MCNLA_REQUIRE(cfg->integrator >= MCNLA_INT_KN && cfg->integrator <= MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN,
                   MCNLA_ERR_INVALID, "unknown integrator kind");
-    MCNLA_REQUIRE(cfg->former >= MCNLA_FORM_SVD && cfg->former <= MCNLA_FORM_ROW_BLOCK_GRAMIAN,
+    MCNLA_REQUIRE(cfg->former >= MCNLA_FORM_SVD && cfg->former <= MCNLA_FORM_ROW_BLOCK_SYMMETRIC,
                   MCNLA_ERR_INVALID, "unknown former kind");
+    MCNLA_REQUIRE(cfg->former != MCNLA_FORM_ROW_BLOCK_SYMMETRIC || params->nrow == params->ncol,
+                  MCNLA_ERR_INVALID, "the symmetric former needs a square A");
     MCNLA_REQUIRE(cfg->omega_mode == MCNLA_OMEGA_PHILOX || cfg->omega_mode == MCNLA_OMEGA_REFERENCE,
                   MCNLA_ERR_INVALID, "unknown omega mode");
     MCNLA_REQUIRE(cfg->maxiter >= 0 && cfg->tol >= 0.0, MCNLA_ERR_INVALID, "bad maxiter/tol");
@@ -938,7 +999,7 @@ mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_
     s->w.ensure(N * l * sizeof(double));
     s->V.ensure(N * l * l * sizeof(double));
     s->T.ensure(N * l * l * sizeof(double));
-    s->ws.ensure(integrator_workspace_bytes(N, static_cast<int>(l)));
+    s->ws.ensure(integration_workspace(cfg->integrator, N, static_cast<int>(l)));
     s->coef.ensure(nl * l * sizeof(double));
     s->M2.ensure(nl * l * sizeof(double));
     s->Qbar.ensure(m * l * sizeof(double));
@@ -1005,17 +1066,9 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
                          s->tmp.as<double>(), N, l, st);
     s->timer.mark(st);
     // 3. integrate on Bs, then Qbar = Y * (T * coef) in one pass over Y
-    GramIntegrator gi;
-    gi.N = N;
-    gi.l = l;
-    gi.wen_yin = is_wen_yin(c.integrator);
-    gi.maxiter = c.maxiter;
-    gi.tol = c.tol;
-    gi.Bs = s->Bs.as<double>();
-    gi.ws = s->ws.as<char>();
-    gi.ws_bytes = s->ws.bytes;
     bool conv = false, ok = true;
-    s->iteration = integrate_gram(gi, s->coef.as<double>(), st, &conv, &ok);
+    s->iteration = run_integration(c.integrator, s->Bs.as<double>(), N, l, c.maxiter, c.tol,
+                                   s->coef.as<double>(), s->ws, st, &conv, &ok);
     s->converged = conv;
     GemmF64Args tm;  // M2_i = T_i * coef_i  (batched l x l)
     tm.M = l;
@@ -1050,7 +1103,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     gemm_f64(q, st);
     s->timer.mark(st);
     // 4. form
-    s->form.run(p, c.former == MCNLA_FORM_SVD, c.jobv != 0, A, lda, a_dtype, a_order,
+    s->form.run(p, c.former, c.jobv != 0, A, lda, a_dtype, a_order,
                 s->Qbar.as<double>(), st);
     s->timer.mark(st);
     s->computed = true;
@@ -1077,7 +1130,9 @@ mcnla_status mcnla_solver_results(mcnla_solver s, const double** Sv, const doubl
     MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "solver has not run");
     if (Sv) *Sv = s->form.S.as<double>();
     if (U) *U = s->form.U.as<double>();
-    if (V) *V = s->cfg.jobv ? s->form.V.as<double>() : nullptr;
+    if (V)
+      *V = (s->cfg.jobv && s->cfg.former != MCNLA_FORM_ROW_BLOCK_SYMMETRIC) ? s->form.V.as<double>()
+                                                                              : nullptr;
     if (Qbar) *Qbar = s->Qbar.as<double>();
     return MCNLA_OK;
   });
@@ -1098,6 +1153,16 @@ mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* s
     const auto v = s->timer.parts();
     for (int i = 0; i < 4; ++i) times[i] = i < static_cast<int>(v.size()) ? v[i] : 0.0;
     if (sketch_kernel_s) *sketch_kernel_s = s->sketch.gemm_seconds();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_kernel_times(mcnla_solver s, double times[2]) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && times, MCNLA_ERR_INVALID, "null argument");
+    MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "solver has not run");
+    times[0] = s->sketch.gemm_seconds();
+    times[1] = s->gram.gemm_seconds();
     return MCNLA_OK;
   });
 }

@@ -279,19 +279,22 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
   __shared__ __attribute__((aligned(16))) double B2[BK * LDB];
 
   if (p.gate != nullptr && *p.gate < p.gate_t) return;
+  constexpr int64_t R = BM / BN;  // lower-triangle rows of tiles: row tm holds R (tm + 1) tiles
+  static_assert(BM % BN == 0, "lower-triangle map needs BM a multiple of BN");
   const int64_t tiles_m = ceil_div(p.M, BM), tiles_n = ceil_div(p.N, BN);
-  const int64_t ntiles = p.lower_only ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
+  const int64_t ntiles = p.lower_only ? R * tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
   const int64_t nblocks = ntiles * p.splits * p.batch;
   const int64_t logical = xcd_swizzle(blockIdx.x, nblocks);
   const int64_t bat = logical / (ntiles * p.splits);
   const int64_t split = (logical / ntiles) % p.splits;
   const int64_t t = logical % ntiles;
   int64_t tm, tn;
-  if (p.lower_only) {  // square tiles, lower triangle (BM == BN)
-    tm = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(t) + 1.0) - 1.0) * 0.5);
-    while (tm * (tm + 1) / 2 > t) --tm;
-    while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
-    tn = t - tm * (tm + 1) / 2;
+  if (p.lower_only) {  // tiles (tm, tn) with tn < R (tm + 1): every tile touching the diagonal
+    const int64_t u = t / R;  // R tm (tm+1)/2 <= t  <=>  tm (tm+1)/2 <= t / R
+    tm = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(u) + 1.0) - 1.0) * 0.5);
+    while (tm * (tm + 1) / 2 > u) --tm;
+    while ((tm + 1) * (tm + 2) / 2 <= u) ++tm;
+    tn = t - R * tm * (tm + 1) / 2;
   } else {
     tm = t / tiles_n;
     tn = t % tiles_n;
@@ -392,12 +395,14 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
       }
 }
 
-// Deterministic split-K reduction: C = alpha * sum_s partial[s] + beta * C.  With sym_tile > 0
-// only tiles (tm >= tn) of size sym_tile were computed; the upper triangle is mirrored.
+// Deterministic split-K reduction: C = alpha * sum_s partial[s] + beta * C.  With sym_bm > 0
+// only the lower-triangle tiles of a sym_bm x sym_bn tiling were computed (tile column
+// j / sym_bn < (i / sym_bm + 1) * sym_bm / sym_bn); the other entries are mirrored.
 __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t splits,
                                      int64_t batch, int64_t M, int64_t N, double alpha,
                                      double beta, double* __restrict__ C, int64_t ldc,
-                                     int64_t stride_c, int sym_tile, const int* gate, int gate_t) {
+                                     int64_t stride_c, int sym_bm, int sym_bn, const int* gate,
+                                     int gate_t) {
   if (gate != nullptr && *gate < gate_t) return;
   const int64_t per = M * N;
   const int64_t total = per * batch;
@@ -406,7 +411,7 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t
     const int64_t b = e / per, ij = e % per;
     const int64_t i = ij / N, j = ij % N;
     int64_t si = i, sj = j;
-    if (sym_tile > 0 && (i / sym_tile) < (j / sym_tile)) {
+    if (sym_bm > 0 && (j / sym_bn) >= (i / sym_bm + 1) * (sym_bm / sym_bn)) {
       si = j;
       sj = i;
     }
@@ -497,20 +502,16 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
                    p.shape == GemmShape::kSquare && p.M >= 128 && p.N >= 128 &&
                    even16(p.A, p.lda, p.stride_a) &&
                    even16(p.B, p.ldb, p.stride_b);
+  if (p.ev_begin) MCNLA_HIP(hipEventRecord(p.ev_begin, st));
   if (dma) {
-    // 256 x 128 tiles of 8 waves (64 x 64 each); lower-triangle schedules need square tiles:
-    // 128 x 128 of 8 waves (32 x 64 each).
-    const int64_t tiles = p.lower_only
-                              ? ceil_div(p.M, 128) * (ceil_div(p.M, 128) + 1) / 2
-                              : ceil_div(p.M, 256) * ceil_div(p.N, 128);
+    // 256 x 128 tiles of 8 waves (64 x 64 each); a lower-triangle schedule takes the tiles that
+    // touch the diagonal (2 (tm + 1) of them in tile row tm).
+    const int64_t tm = ceil_div(p.M, 256);
+    const int64_t tiles = p.lower_only ? tm * (tm + 1) : tm * ceil_div(p.N, 128);
     const int64_t nblocks = tiles * p.splits * p.batch;
     MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
-    if (p.lower_only)
-      hipLaunchKernelGGL((gemm_f64_dma_kernel<128, 128, 16, 32, 64>),
-                         dim3(static_cast<unsigned>(nblocks)), dim3(512), 0, st, p);
-    else
-      hipLaunchKernelGGL((gemm_f64_dma_kernel<256, 128, 16, 64, 64>),
-                         dim3(static_cast<unsigned>(nblocks)), dim3(512), 0, st, p);
+    hipLaunchKernelGGL((gemm_f64_dma_kernel<256, 128, 16, 64, 64>),
+                       dim3(static_cast<unsigned>(nblocks)), dim3(512), 0, st, p);
     MCNLA_LAUNCHED();
   } else if (p.a_f32) {
     if (p.b_f32) launch_types<float, float>(p, st);
@@ -519,15 +520,17 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
     if (p.b_f32) launch_types<double, float>(p, st);
     else launch_types<double, double>(p, st);
   }
+  if (p.ev_end) MCNLA_HIP(hipEventRecord(p.ev_end, st));
   if (p.partial != nullptr && p.C != nullptr) {
-    const int sym = p.lower_only ? (p.shape == GemmShape::kSquare ? 128 : 32) : 0;
+    const int sym_bm = !p.lower_only ? 0 : dma ? 256 : (p.shape == GemmShape::kSquare ? 128 : 32);
+    const int sym_bn = !p.lower_only ? 0 : dma ? 128 : sym_bm;
     const int64_t total = p.M * p.N * p.batch;
     const int threads = 256;
     int64_t blocks = ceil_div(total, threads);
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0,
                        st, p.partial, p.splits, p.batch, p.M, p.N, p.alpha, p.beta, p.C, p.ldc,
-                       p.stride_c, sym, p.gate, p.gate_t);
+                       p.stride_c, sym_bm, sym_bn, p.gate, p.gate_t);
     MCNLA_LAUNCHED();
   }
 }

@@ -22,7 +22,8 @@ namespace mcnla {
 struct IntegratorState {
   int converged_at;  // iteration index that met the stopping test, INT_MAX while running
   int nan_seen;      // a non-finite value appeared in the l x l step
-  int pad[2];
+  int ticket;        // workgroups done with the current fused reduction (reset by the last one)
+  int pad;
   double res;        // last residual (KN: ||s - 1||_2, WY: mu)
   double taug, zeta, phi, mu;  // Wen-Yin scalars
 };
@@ -108,6 +109,8 @@ __global__ void reduce_partials_kernel(int t, const IntegratorState* st,
 // semidefinite with lambda <= 1/4), and ||s - 1||_2 = ||C - I||_F.  Should an iteration not
 // converge (lambda at or above 1/4, non-finite input) the step falls back to a Jacobi
 // eigen-decomposition and the formulas above, which also reproduce the reference's NaN there.
+// Runs on NT threads (all of the workgroup).
+template <int NT>
 __device__ void kn_step_eigen(int t, IntegratorState* st, int l, double invN, double tol,
                               const double* __restrict__ P, int64_t nblk, double* __restrict__ F,
                               double* __restrict__ Cinv, double* dc, double* a, double* v,
@@ -115,16 +118,16 @@ __device__ void kn_step_eigen(int t, IntegratorState* st, int l, double invN, do
   __shared__ double s[kMaxL];
   __shared__ double cs[2 * kMaxL];
   __shared__ int pq[2 * kMaxL];
-  __shared__ double red[kThreads / kWave];
+  __shared__ double red[8];
   const int ld = lds_ld(l);
   const int ll = l * l;
-  for (int e = threadIdx.x; e < 2 * ll; e += kThreads) {  // Dc into dc, Dgc into a
+  for (int e = threadIdx.x; e < 2 * ll; e += NT) {  // Dc into dc, Dgc into a
     const double x = invN * sum_partials(P, nblk, 2 * ll, e);
     const int q = e % ll;
     (e < ll ? dc : a)[(q / l) * ld + q % l] = x;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ll; e += kThreads) {  // Z = sym(Dgc) - Dc Dc'
+  for (int e = threadIdx.x; e < ll; e += NT) {  // Z = sym(Dgc) - Dc Dc'
     const int i = e / l, j = e % l;
     double x = 0.5 * (a[i * ld + j] + a[j * ld + i]);
     for (int q = 0; q < l; ++q) x -= dc[i * ld + q] * dc[j * ld + q];
@@ -132,9 +135,9 @@ __device__ void kn_step_eigen(int t, IntegratorState* st, int l, double invN, do
   }
   __syncthreads();
   lds_jacobi(l, z, v, true, cs, pq, red);
-  for (int i = threadIdx.x; i < l; i += kThreads) s[i] = sqrt(0.5 + sqrt(0.25 - z[i * ld + i]));
+  for (int i = threadIdx.x; i < l; i += NT) s[i] = sqrt(0.5 + sqrt(0.25 - z[i * ld + i]));
   __syncthreads();
-  for (int e = threadIdx.x; e < ll; e += kThreads) {  // Cinv into `a`
+  for (int e = threadIdx.x; e < ll; e += NT) {  // Cinv into `a`
     const int i = e / l, j = e % l;
     double ci = 0.0;
     for (int q = 0; q < l; ++q) ci += v[i * ld + q] * v[j * ld + q] / s[q];
@@ -142,7 +145,7 @@ __device__ void kn_step_eigen(int t, IntegratorState* st, int l, double invN, do
     Cinv[e] = ci;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ll; e += kThreads) {  // F = C - Dc Cinv, C on the fly
+  for (int e = threadIdx.x; e < ll; e += NT) {  // F = C - Dc Cinv, C on the fly
     const int i = e / l, j = e % l;
     double c = 0.0;
     for (int q = 0; q < l; ++q) c += v[i * ld + q] * v[j * ld + q] * s[q];
@@ -164,22 +167,22 @@ __device__ void kn_step_eigen(int t, IntegratorState* st, int l, double invN, do
   }
 }
 
-__global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorState* st, int l,
-                                                          double invN, double tol,
-                                                          const double* __restrict__ P,
-                                                          int64_t nblk, double* __restrict__ F,
-                                                          double* __restrict__ Cinv) {
-  if (skip(st, t)) return;
-  __shared__ double sDc[kMaxL * (kMaxL + 1)];
-  __shared__ double sY[kMaxL * (kMaxL + 1)];
-  __shared__ double sW[kMaxL * (kMaxL + 1)];
-  __shared__ double sT[kMaxL * (kMaxL + 1)];
-  __shared__ double red[kThreads / kWave];
+// The NS form of the step; smem holds four (kMaxL x (kMaxL+1)) buffers, red NT/64 doubles.
+template <int NT>
+__device__ void kn_step_ns(int t, IntegratorState* st, int l, double invN, double tol,
+                           const double* __restrict__ P, int64_t nblk, double* __restrict__ F,
+                           double* __restrict__ Cinv, double* smem, double* red) {
+  constexpr int NW = NT / kWave;
+  constexpr int kBuf = kMaxL * (kMaxL + 1);
+  double* sDc = smem;
+  double* sY = smem + kBuf;
+  double* sW = smem + 2 * kBuf;
+  double* sT = smem + 3 * kBuf;
   const int np = pad16(l), ld = np + 1, ll = l * l;
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
   const int nt1 = np / 16, ntiles = nt1 * nt1;
   // Dc into sDc, Dgc into sT, both zero-padded to np x np
-  for (int e = tid; e < np * np; e += kThreads) {
+  for (int e = tid; e < np * np; e += NT) {
     const int i = e / np, j = e % np;
     const bool in = i < l && j < l;
     sDc[i * ld + j] = in ? invN * sum_partials(P, nblk, 2 * ll, i * l + j) : 0.0;
@@ -187,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorStat
   }
   __syncthreads();
   // Y = I - 4Z with Z = sym(Dgc) - Dc Dc'
-  for (int tile = wave; tile < ntiles; tile += kThreads / kWave) {
+  for (int tile = wave; tile < ntiles; tile += NW) {
     const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;
     const d4 acc = lds_mfma_tile<false, true>(np, ld, sDc, sDc, i0, j0);
 #pragma unroll
@@ -198,22 +201,22 @@ __global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorStat
     }
   }
   __syncthreads();
-  int ok = lds_ns_sqrt(np, ld, sY, sW, sT, red, 60);
+  int ok = lds_ns_sqrt<NW>(np, ld, sY, sW, sT, red, 60);
   if (ok > 0) {
-    for (int e = tid; e < np * np; e += kThreads) {
+    for (int e = tid; e < np * np; e += NT) {
       const int i = e / np, j = e % np;
       sY[i * ld + j] = 0.5 * (((i == j) ? 1.0 : 0.0) + sY[i * ld + j]);
     }
     __syncthreads();
-    ok = lds_ns_sqrt(np, ld, sY, sW, sT, red, 60);
+    ok = lds_ns_sqrt<NW>(np, ld, sY, sW, sT, red, 60);
   }
   if (ok <= 0) {  // rare: eigen-decomposition instead (uses its own layout of the buffers)
-    kn_step_eigen(t, st, l, invN, tol, P, nblk, F, Cinv, sDc, sT, sW, sY);
+    kn_step_eigen<NT>(t, st, l, invN, tol, P, nblk, F, Cinv, sDc, sT, sW, sY);
     return;
   }
   // C = sY, Cinv = sW; F = C - Dc Cinv; ||C - I||_F
   double rs = 0.0;
-  for (int tile = wave; tile < ntiles; tile += kThreads / kWave) {
+  for (int tile = wave; tile < ntiles; tile += NW) {
     const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;
     const d4 acc = lds_mfma_tile<false, false>(np, ld, sDc, sW, i0, j0);
 #pragma unroll
@@ -234,12 +237,77 @@ __global__ __launch_bounds__(kThreads) void kn_step_kernel(int t, IntegratorStat
   __syncthreads();
   if (tid == 0) {
     double tot = 0.0;
-    for (int w = 0; w < kThreads / kWave; ++w) tot += red[w];
+    for (int w = 0; w < NW; ++w) tot += red[w];
     const double r = sqrt(tot);
     st->res = r;
     if (!isfinite(r)) st->nan_seen = 1;
     if (!(r >= tol)) st->converged_at = t;
   }
+}
+
+// One KN iteration's reductions and l x l step in one launch (after the split-K GEMM that
+// leaves Pbg[s] = partial Bs * Bc over the s-th K range):
+//   1. workgroup b owns rows [128 b, 128 b + 128) of the Nl x l matrices: Bgc = invN sum_s Pbg[s]
+//      (fixed order), written out, and its block partials Bc'Bc, Bc'Bgc -> P[b];
+//   2. the last workgroup to finish (a ticket counter, taken after a device-scope fence; no
+//      waiting) sums P over blocks in block order and runs the NS step above.
+// The result is deterministic: no sum depends on which workgroup finishes last.
+constexpr int kStepThreads = 512;
+constexpr int kStepRows = 128;
+
+__global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
+    int t, IntegratorState* st, int64_t nl, int l, double invN, double tol,
+    const double* __restrict__ Pbg, int64_t splits, const double* __restrict__ B,
+    double* __restrict__ Bg, double* __restrict__ P, int64_t nblk, double* __restrict__ F,
+    double* __restrict__ Cinv) {
+  if (skip(st, t)) return;
+  constexpr int kBuf = kMaxL * (kMaxL + 1);
+  static_assert(2 * kStepRows * (kMaxL + 1) <= 4 * kBuf, "row blocks fit the step buffers");
+  __shared__ double smem[4 * kBuf];
+  __shared__ double red[kStepThreads / kWave];
+  __shared__ int last;
+  const int tid = threadIdx.x;
+  const int ldr = kMaxL + 1;
+  double* sB = smem;
+  double* sG = smem + kStepRows * ldr;
+  const int64_t r0 = blockIdx.x * static_cast<int64_t>(kStepRows);
+  const int64_t per = nl * l;
+  for (int e = tid; e < kStepRows * l; e += kStepThreads) {
+    const int rr = e / l, j = e % l;
+    const int64_t r = r0 + rr;
+    double b = 0.0, g = 0.0;
+    if (r < nl) {
+      const int64_t o = r * l + j;
+#pragma unroll 4
+      for (int64_t s = 0; s < splits; ++s) g += Pbg[s * per + o];
+      g *= invN;
+      b = B[o];
+      Bg[o] = g;
+    }
+    sB[rr * ldr + j] = b;
+    sG[rr * ldr + j] = g;
+  }
+  __syncthreads();
+  const int ll = l * l;
+  double* Pb = P + blockIdx.x * static_cast<int64_t>(2 * ll);
+  for (int q = tid; q < 2 * ll; q += kStepThreads) {
+    const int part = q / ll, i = (q / l) % l, j = q % l;
+    const double* y = part == 0 ? sB : sG;
+    double s0 = 0.0, s1 = 0.0;
+    for (int rr = 0; rr < kStepRows; rr += 2) {
+      s0 += sB[rr * ldr + i] * y[rr * ldr + j];
+      s1 += sB[(rr + 1) * ldr + i] * y[(rr + 1) * ldr + j];
+    }
+    Pb[q] = s0 + s1;
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) last = (atomicAdd(&st->ticket, 1) == static_cast<int>(nblk) - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (tid == 0) st->ticket = 0;  // ready for the next iteration's launch
+  kn_step_ns<kStepThreads>(t, st, l, invN, tol, P, nblk, F, Cinv, smem, red);
 }
 
 // B+ = Bc F + Bg Cinv; E+ = Ec F + invN Bc Cinv; FF+ = FFc F (block 0).
@@ -657,6 +725,7 @@ __global__ void wy_tau_kernel(int t, IntegratorState* st, int l, int odd,
 __global__ void init_state_kernel(IntegratorState* st) {
   st->converged_at = INT_MAX;
   st->nan_seen = 0;
+  st->ticket = 0;
   st->res = 0.0;
   st->taug = WyConst::tau0;
   st->zeta = 1.0;
@@ -718,7 +787,7 @@ __global__ __launch_bounds__(kThreads) void polish_kernel(int l, double N,
     sY[i * ld + j] = 0.5 * (sT[i * ld + j] + sT[j * ld + i]);
   }
   __syncthreads();
-  const int ok = lds_ns_sqrt(np, ld, sY, sW, sT, red, 60);
+  const int ok = lds_ns_sqrt<kThreads / kWave>(np, ld, sY, sW, sT, red, 60);
   for (int e = threadIdx.x; e < ll; e += kThreads) {
     const int i = e / l, j = e % l;
     Wout[e] = ok > 0 ? sW[i * ld + j] : ((i == j) ? 1.0 : 0.0);
@@ -840,8 +909,12 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
   const double invN = 1.0 / static_cast<double>(g.N);
   const int ll = l * l;
 
-  // Bout = Bs * Bin / N on the MFMA GEMM (split-K, deterministic reduction), gated on t.
+  // Bout = Bs * Bin / N on the MFMA GEMM (split-K, deterministic reduction), gated on t; with
+  // Bout null only the bs_splits partial products are left in L.Pbg ([split][Nl][l]).
   // Bs is exactly symmetric, so its rows serve as the M-contiguous operand.
+  const int64_t bs_kchunk = ceil_div(ceil_div(nl, L.splits), 16) * 16;
+  const int64_t bs_splits = ceil_div(nl, bs_kchunk);
+  const int64_t nblk_s = ceil_div(nl, kStepRows);
   auto bs_product = [&](const double* Bin, double* Bout, int t) {
     GemmF64Args p;
     p.M = nl;
@@ -857,7 +930,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
     p.alpha = invN;
     p.shape = l <= 32 ? GemmShape::kTallN32 : GemmShape::kTallN64;
     p.splits = L.splits;
-    p.kchunk = ceil_div(ceil_div(nl, L.splits), 16) * 16;
+    p.kchunk = bs_kchunk;
     p.partial = L.Pbg;
     p.gate = &L.st->converged_at;
     p.gate_t = t;
@@ -898,12 +971,10 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
       const int c = static_cast<int>(t & 1), p = c ^ 1;
       const int ti = static_cast<int>(t);
       if (!g.wen_yin) {
-        bs_product(L.B[c], L.Bg[0], ti);
-        hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, ti, L.st, nl, l,
-                           L.B[c], L.Bg[0], L.P, 2);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(kn_step_kernel, dim3(1), dim3(kThreads), 0, st, ti, L.st, l, invN, g.tol,
-                           L.P, L.nblk_p, L.F, L.C);
+        bs_product(L.B[c], nullptr, ti);  // split-K partials only
+        hipLaunchKernelGGL(kn_reduce_step_kernel, dim3(static_cast<unsigned>(nblk_s)),
+                           dim3(kStepThreads), 0, st, ti, L.st, nl, l, invN, g.tol, L.Pbg, bs_splits,
+                           L.B[c], L.Bg[0], L.P, nblk_s, L.F, L.C);
         MCNLA_LAUNCHED();
         hipLaunchKernelGGL(kn_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, ti,
                            L.st, nl, l, invN, L.B[c], L.Bg[0], L.B[p], L.E[c], L.E[p], L.FF[c],

@@ -33,6 +33,8 @@ struct GemmF64Args {
   // iteration loops that are enqueued ahead of their convergence test).
   const int* gate = nullptr;
   int gate_t = 0;
+  // Optional events recorded around the main GEMM launch (not the split-K reduction).
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 };
 
 void gemm_f64(GemmF64Args p, hipStream_t st);
@@ -117,12 +119,22 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
                        bool* numeric_ok);
 
 // ------------------------------------------------------------------------------------------------
+// Non-iterative integrators on Bs (combine.hip): the pairwise reduction tree and the extrinsic
+// mean.  Both leave the Nl x l coefficients with Qbar = Qs * coef in `coef`.
+size_t combine_workspace_bytes(bool extrinsic, int64_t N, int l);
+void integrate_reduction(const double* Bs, int64_t N, int l, double* coef, char* ws,
+                         size_t ws_bytes, hipStream_t st);
+void integrate_extrinsic(const double* Bs, int64_t N, int l, double* coef, char* ws,
+                         size_t ws_bytes, hipStream_t st);
+
+// ------------------------------------------------------------------------------------------------
 // Former helpers (former.hip).
 // Given Z = A' Qbar (n x l row-major) and Wg = Z'Z (l x l): eig, S (k), the l x k rotation
 // Wk (row-major), and, if V != null, V = Z Wk diag(1/S) (n x k row-major).
-// svd_sigma: compute S as ||Z w_j|| instead of sqrt(eig) (the SVD former's singular values).
+// symmetric: Wg = Qbar' A Qbar of a symmetric A; S and Wk take the k largest eigenpairs in
+// ascending order and S holds the eigenvalues themselves.
 void former_small(const double* Wg, int l, int k, double* eigw, double* eigv, double* Wk,
-                  double* S, hipStream_t st);
+                  double* S, hipStream_t st, bool symmetric = false);
 void scale_columns(double* X, int64_t rows, int k, const double* S, bool inverse, hipStream_t st);
 void column_norms(const double* X, int64_t rows, int k, double* out, hipStream_t st);
 

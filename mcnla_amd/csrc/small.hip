@@ -119,8 +119,10 @@ __global__ void apply_blockdiag_kernel(T* __restrict__ Y, int64_t m, int64_t N, 
 }
 
 // Former: eig of Wg (l x l) -> eigw (descending), eigv; Wk = eigv[:, :k]; S = sqrt(eigw[:k]).
+// symmetric (Wg = Qbar' A Qbar of a symmetric A): the k largest eigenpairs in ascending order,
+// S = the eigenvalues themselves (row_block_symmetric_former.hpp:65-66,127-130).
 __global__ __launch_bounds__(kSmallThreads) void former_small_kernel(const double* __restrict__ Wg,
-                                                                    int l, int k,
+                                                                    int l, int k, bool symmetric,
                                                                     double* __restrict__ eigw,
                                                                     double* __restrict__ eigv,
                                                                     double* __restrict__ Wk,
@@ -141,9 +143,10 @@ __global__ __launch_bounds__(kSmallThreads) void former_small_kernel(const doubl
   lds_eig_sorted(l, a, v, eigw, eigv, ord);
   for (int e = threadIdx.x; e < l * k; e += blockDim.x) {
     const int i = e / k, j = e % k;
-    Wk[e] = eigv[i * l + j];
+    Wk[e] = eigv[i * l + (symmetric ? k - 1 - j : j)];
   }
-  for (int j = threadIdx.x; j < k; j += blockDim.x) S[j] = sqrt(fmax(eigw[j], 0.0));
+  for (int j = threadIdx.x; j < k; j += blockDim.x)
+    S[j] = symmetric ? eigw[k - 1 - j] : sqrt(fmax(eigw[j], 0.0));
 }
 
 __global__ void scale_columns_kernel(double* __restrict__ X, int64_t rows, int k,
@@ -231,10 +234,10 @@ void apply_blockdiag(void* Y, bool y_f32, int64_t m, int64_t N, int l, const dou
 }
 
 void former_small(const double* Wg, int l, int k, double* eigw, double* eigv, double* Wk,
-                  double* S, hipStream_t st) {
+                  double* S, hipStream_t st, bool symmetric) {
   MCNLA_REQUIRE(l >= 1 && l <= kMaxL && k >= 1 && k <= l, MCNLA_ERR_INVALID, "former sizes");
-  hipLaunchKernelGGL(former_small_kernel, dim3(1), dim3(kSmallThreads), 0, st, Wg, l, k, eigw, eigv,
-                     Wk, S);
+  hipLaunchKernelGGL(former_small_kernel, dim3(1), dim3(kSmallThreads), 0, st, Wg, l, k, symmetric,
+                     eigw, eigv, Wk, S);
   MCNLA_LAUNCHED();
 }
 

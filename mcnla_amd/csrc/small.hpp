@@ -163,11 +163,14 @@ __device__ __forceinline__ double nan_max(double a, double b) {
 // Stops once max|T - I| < 1e-8 (the following update leaves an error of about 3 (1e-8)^2).
 // Returns the number of iterations, or -1 when that did not happen within maxit (M not positive
 // definite, too ill-conditioned, or not finite).  `red` needs one double per wave.  Every thread
-// of the workgroup (at most 4 waves) must call this.
+// of the workgroup (NW waves) must call this.
+template <int NW>
 __device__ inline int lds_ns_sqrt(int np, int ld, double* Y, double* W, double* T, double* red,
                                   int maxit) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int wave = tid / kWave, nw = nt / kWave, lane = tid & (kWave - 1);
+  constexpr int QMAX = (16 + NW - 1) / NW;  // output tiles per wave (at most 16 tiles, np <= 64)
+  constexpr int nt = NW * kWave, nw = NW;
+  const int tid = threadIdx.x;
+  const int wave = tid / kWave, lane = tid & (kWave - 1);
   const int nt1 = np / 16, ntiles = nt1 * nt1;
   for (int e = tid; e < np * np; e += nt) W[(e / np) * ld + e % np] = (e / np == e % np) ? 1.0 : 0.0;
   __syncthreads();
@@ -190,9 +193,9 @@ __device__ inline int lds_ns_sqrt(int np, int ld, double* Y, double* W, double* 
     __syncthreads();
     double d = 0.0;
     for (int w = 0; w < nw; ++w) d = nan_max(d, red[w]);
-    d4 ay[4], aw[4];
+    d4 ay[QMAX], aw[QMAX];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < QMAX; ++q) {
       const int tile = wave + q * nw;
       if (tile < ntiles) {
         const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;
@@ -202,7 +205,7 @@ __device__ inline int lds_ns_sqrt(int np, int ld, double* Y, double* W, double* 
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < QMAX; ++q) {
       const int tile = wave + q * nw;
       if (tile < ntiles) {
         const int i0 = (tile / nt1) * 16, j0 = (tile % nt1) * 16;

@@ -304,6 +304,27 @@ class RowBlockGramianWenYinIntegrator(_Integrator):
     _default_tol = 1e-3
 
 
+class _CombiningIntegrator(_Integrator):
+    """Integrators that combine the sketches in one pass (no iteration, no tolerance); the
+    reference constructs them from the parameters alone."""
+
+    def __init__(self, parameters, precision=_lib.PREC_F64):
+        super().__init__(parameters, 0, 0.0, precision)
+
+    def __call__(self, collection_q, *args, stream=None):
+        # the extrinsic mean takes (Qj, Q, Qbar) in the reference
+        # (row_block_extrinsic_mean_integrator.hh:109-111); with one rank Qj and Q coincide
+        return super().__call__(collection_q, args[-1], stream)
+
+
+class RowBlockReductionIntegrator(_CombiningIntegrator):
+    _kind = _lib.INT_ROW_BLOCK_REDUCTION
+
+
+class RowBlockExtrinsicMeanIntegrator(_CombiningIntegrator):
+    _kind = _lib.INT_ROW_BLOCK_EXTRINSIC_MEAN
+
+
 # ---------------------------------------------------------------------------------------------
 # Formers
 class _Former(_Stage):
@@ -366,6 +387,17 @@ class ColBlockGramianFormer(_Former):
 
 class RowBlockGramianFormer(_Former):
     _kind = _lib.FORM_ROW_BLOCK_GRAMIAN
+
+
+class RowBlockSymmetricFormer(_Former):
+    """Square symmetric A: vectorS() holds the k algebraically largest Ritz values of
+    Qbar' A Qbar in ascending order and matrixU() = Qbar W (row_block_symmetric_former.hpp)."""
+    _kind = _lib.FORM_ROW_BLOCK_SYMMETRIC
+
+    def matrixV(self):
+        raise RuntimeError("the symmetric former forms U only")
+
+    matrixVj = matrixV
 
 
 # ---------------------------------------------------------------------------------------------
@@ -475,6 +507,12 @@ class Solver:
         k = ctypes.c_double()
         _lib.check(_lib.lib().mcnla_solver_stage_times(self._h, t, ctypes.byref(k)))
         return list(t), k.value
+
+    def kernel_times(self):
+        """Seconds of single launches of the last solve: (sketch GEMM, Gram GEMM)."""
+        t = (ctypes.c_double * 2)()
+        _lib.check(_lib.lib().mcnla_solver_kernel_times(self._h, t))
+        return t[0], t[1]
 
     def workspace_bytes(self):
         b = ctypes.c_size_t()

@@ -15,6 +15,10 @@ Parity pins (each enforced by a test under ``tests/``):
   * KN (row-block, X-form, Gramian): ``data/qi.mtx -> data/qb_kn.mtx``, 81 iterations, 1e-8;
   * WY (row-block, Gramian): ``data/qi.mtx -> data/qb_wy.mtx``, 21 iterations, 1e-8;
   * formers: ``data/a.mtx + data/q.mtx -> data/u.mtx`` projectors at 1e-8, sigma vs numpy;
+  * reduction / extrinsic-mean integrators: ``data/qi.mtx -> data/qb_re.mtx / qb_em.mtx``
+    projectors at 1e-8, plus the two-sketch midpoint and identical-sketch closed forms;
+  * symmetric former: ``data/a_sym.mtx + data/q.mtx -> data/u_sym.mtx`` projectors at 1e-8,
+    sigma vs the eigenvalues of ``Qbar' A Qbar``;
   * sketch: closed form ``A @ Omega``;
   * ``omega_ref``: moments and determinism only -- the bit stream is *parity unpinned* because
     no reference fixture stores Omega or its seed.
@@ -58,6 +62,9 @@ def lib():
                                          _d, _d]
         L.orc_frerr.argtypes = [_i64, _i64, _i64, _d, _i64, ctypes.c_int, _d, _d, _d]
         L.orc_frerr.restype = ctypes.c_double
+        L.orc_reduction.argtypes = [_i64, _i64, _i64, _d, _d]
+        L.orc_extrinsic_mean.argtypes = [_i64, _i64, _i64, _d, _d]
+        L.orc_symmetric_former.argtypes = [_i64, _i64, _i64, _d, _i64, ctypes.c_int, _d, _d, _d]
         _LIB = L
     return _LIB
 
@@ -179,6 +186,34 @@ def gramian_former(A, Qbar, k, a_colmajor=True, jobv=True):
                              1 if a_colmajor else 0, _p(_c(Qbar)), _p(S), _p(U),
                              _p(V) if jobv else None)
     return S, U, (V if jobv else None)
+
+
+def reduction(Qs, l, N):
+    """Reduction integrator (pairwise tree); returns Qbar (m x l)."""
+    Qs = _c(Qs)
+    Qbar = np.empty((Qs.shape[0], l))
+    lib().orc_reduction(Qs.shape[0], l, N, _p(Qs), _p(Qbar))
+    return Qbar
+
+
+def extrinsic_mean(Qs, l, N):
+    """Extrinsic-mean integrator; returns Qbar (m x l, left singular vectors)."""
+    Qs = _c(Qs)
+    Qbar = np.empty((Qs.shape[0], l))
+    lib().orc_extrinsic_mean(Qs.shape[0], l, N, _p(Qs), _p(Qbar))
+    return Qbar
+
+
+def symmetric_former(A, Qbar, k, a_colmajor=True):
+    """Symmetric former for a square symmetric A: (S[k] ascending, U m x k)."""
+    m = A.shape[0]
+    l = Qbar.shape[1]
+    store = _c(A.T) if a_colmajor else _c(A)
+    S = np.empty(k)
+    U = np.empty((m, k))
+    lib().orc_symmetric_former(m, l, k, _p(store), m, 1 if a_colmajor else 0, _p(_c(Qbar)),
+                               _p(S), _p(U))
+    return S, U
 
 
 def frerr(A, S, U, V):

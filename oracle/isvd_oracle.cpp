@@ -14,6 +14,8 @@
 //   orc_kn_*                               -> data/qi.mtx -> data/qb_kn.mtx, 81 iterations, 1e-8
 //   orc_wy_*                               -> data/qi.mtx -> data/qb_wy.mtx, 21 iterations, 1e-8
 //   orc_svd_former / orc_gramian_former    -> data/a.mtx + data/q.mtx -> data/u.mtx projectors
+//   orc_reduction / orc_extrinsic_mean     -> data/qi.mtx -> data/qb_re.mtx / qb_em.mtx projectors
+//   orc_symmetric_former                   -> data/a_sym.mtx + data/q.mtx -> data/u_sym.mtx
 //   orc_omega_ref                          -> moments + determinism only ("parity unpinned" for
 //                                             the bit stream: no reference fixture stores Omega)
 //   orc_sketch                             -> closed form (numpy A @ Omega)
@@ -719,6 +721,98 @@ void orc_gramian_former(int64_t m, int64_t n, int64_t l, int64_t k, const double
     for (int64_t r = 0; r < n; ++r)
       for (int64_t j = 0; j < k; ++j) V_rowmajor[r * k + j] /= S[j];
   }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Reduction integrator (row_block_reduction_integrator.hpp:86-120): a pairwise tree over the
+// collection.  For n = N, then n = (n+1)/2 while n > 1, with h = n/2: for i < h,
+// B = Q_i' Q_{i+h}, svd(B) = W S T', Q_i = (Q_i W + Q_{i+h} T) diag(1/sqrt(2(1 + s))); entries at
+// positions >= h keep their current value (so for odd n the next level re-reads them, exactly as
+// the reference loop does).  Qbar = Q_0.
+void orc_reduction(int64_t m, int64_t l, int64_t N, const double* Qs, double* Qbar) {
+  const int64_t Nl = N * l;
+  std::vector<std::vector<double>> Q(N, std::vector<double>(m * l));
+  for (int64_t i = 0; i < N; ++i)
+    for (int64_t r = 0; r < m; ++r)
+      for (int64_t c = 0; c < l; ++c) Q[i][r * l + c] = Qs[r * Nl + i * l + c];
+  std::vector<double> B(l * l), s(l), Ub(l * l), V(l * l), tmp(m * l);
+  for (int64_t n = N; n > 1; n = (n + 1) / 2) {
+    const int64_t h = n / 2;
+    for (int64_t i = 0; i < h; ++i) {
+      mm(true, false, l, l, m, 1.0, Q[i].data(), l, Q[i + h].data(), l, 0.0, B.data(), l);
+      // svd(B) = W S T' via the one-sided Jacobi SVD of B (W = Ub, T = V)
+      onesided_svd(l, l, B.data(), s.data(), Ub.data(), V.data());
+      mm(false, false, m, l, l, 1.0, Q[i].data(), l, Ub.data(), l, 0.0, tmp.data(), l);
+      mm(false, false, m, l, l, 1.0, Q[i + h].data(), l, V.data(), l, 1.0, tmp.data(), l);
+      for (int64_t r = 0; r < m; ++r)
+        for (int64_t c = 0; c < l; ++c)
+          Q[i][r * l + c] = tmp[r * l + c] / std::sqrt(2.0 * (1.0 + s[c]));
+    }
+  }
+  for (int64_t e = 0; e < m * l; ++e) Qbar[e] = Q[0][e];
+}
+
+// Extrinsic-mean integrator (row_block_extrinsic_mean_integrator.hpp:105-162), one rank:
+// Bs = Qs'Qs; for each sketch G_i = B_i B_i' with B_i the i-th block row of Bs, eig(G_i) with the
+// eigenvectors as the rows of G_i (LAPACK ascending order); with G0 = G_0, row j of G_i is negated
+// when G_i(j,:) . (G0 B_i0')(j,:) < 0 (B_i0 = Q_i'Q_0); Qbar = sum_i Q_i G_i', then Qbar is
+// replaced by its left singular vectors (gesvd, descending).
+void orc_extrinsic_mean(int64_t m, int64_t l, int64_t N, const double* Qs, double* Qbar) {
+  const int64_t Nl = N * l;
+  std::vector<double> Bs(Nl * Nl);
+  mm(true, false, Nl, Nl, m, 1.0, Qs, Nl, Qs, Nl, 0.0, Bs.data(), Nl);
+  std::vector<std::vector<double>> G(N, std::vector<double>(l * l));
+  std::vector<double> Gi(l * l), w(l), V(l * l);
+  for (int64_t i = 0; i < N; ++i) {
+    const double* Bi = &Bs[i * l * Nl];
+    mm(false, true, l, l, Nl, 1.0, Bi, Nl, Bi, Nl, 0.0, Gi.data(), l);
+    jacobi_eig(l, Gi.data(), w.data(), V.data());
+    for (int64_t j = 0; j < l; ++j)  // row j of G_i = eigenvector j (ascending)
+      for (int64_t e = 0; e < l; ++e) G[i][j * l + e] = V[e * l + j];
+  }
+  std::vector<double> Bi0t(l * l), GB(l * l), X(m * l, 0.0);
+  for (int64_t i = 0; i < N; ++i) {
+    // Bi0' = (Q_i'Q_0)' = Q_0'Q_i = Bs block (0, i)
+    for (int64_t a = 0; a < l; ++a)
+      for (int64_t b = 0; b < l; ++b) Bi0t[a * l + b] = Bs[a * Nl + i * l + b];
+    mm(false, false, l, l, l, 1.0, G[0].data(), l, Bi0t.data(), l, 0.0, GB.data(), l);
+    for (int64_t j = 0; j < l; ++j)
+      if (dot(l, &G[i][j * l], &GB[j * l]) < 0)
+        for (int64_t e = 0; e < l; ++e) G[i][j * l + e] = -G[i][j * l + e];
+    // X += Q_i G_i'
+    for (int64_t r = 0; r < m; ++r)
+      for (int64_t c = 0; c < l; ++c) {
+        double acc = 0;
+        for (int64_t e = 0; e < l; ++e) acc += Qs[r * Nl + i * l + e] * G[i][c * l + e];
+        X[r * l + c] += acc;
+      }
+  }
+  std::vector<double> s(l), Vx(l * l);
+  onesided_svd(m, l, X.data(), s.data(), Qbar, Vx.data());
+}
+
+// Symmetric former (row_block_symmetric_former.hpp:65-66, 111-130), square A: Z = A'Qbar;
+// W = Z'Qbar = Qbar'A Qbar; eig(W) ascending; the last k eigenpairs (the k algebraically largest,
+// still ascending) give S and U = Qbar W[:, l-k:] (row-major m x k).
+void orc_symmetric_former(int64_t m, int64_t l, int64_t k, const double* A, int64_t lda,
+                          int a_colmajor, const double* Qbar, double* S, double* U_rowmajor) {
+  std::vector<double> Z(m * l, 0.0), W(l * l), w(l), V(l * l);
+  for (int64_t r = 0; r < m; ++r) {
+    const double* q = Qbar + r * l;
+    for (int64_t j = 0; j < m; ++j) {
+      const double a = a_colmajor ? A[r + j * lda] : A[r * lda + j];
+      double* z = &Z[j * l];
+      for (int64_t c = 0; c < l; ++c) z[c] += a * q[c];
+    }
+  }
+  mm(true, false, l, l, m, 1.0, Z.data(), l, Qbar, l, 0.0, W.data(), l);
+  jacobi_eig(l, W.data(), w.data(), V.data());
+  std::vector<double> Wk(l * k);
+  for (int64_t j = 0; j < k; ++j) {
+    S[j] = w[l - k + j];
+    for (int64_t e = 0; e < l; ++e) Wk[e * k + j] = V[e * l + (l - k + j)];
+  }
+  mm(false, false, m, k, l, 1.0, Qbar, l, Wk.data(), k, 0.0, U_rowmajor, k);
 }
 
 // The drivers' error check (src/isvd_dense_cb.cpp:308-332): ||A - U S V'||_F / ||A||_F with

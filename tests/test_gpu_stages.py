@@ -144,6 +144,56 @@ def test_integrator_fixture(gpu, kind, fixture, tol, iters):
     assert np.abs(Qbar.numpy() - qb).max() < 1e-8
 
 
+@pytest.mark.parametrize("kind,fixture,ofn", [
+    (mc.RowBlockReductionIntegrator, "qb_re.mtx", oracle.reduction),
+    (mc.RowBlockExtrinsicMeanIntegrator, "qb_em.mtx", oracle.extrinsic_mean),
+])
+def test_combining_integrator_fixture(gpu, kind, fixture, ofn):
+    # check/isvd/integrator/row_block_{reduction,extrinsic_mean}_integrator.cpp: Qbar Qbar' at 1e-8
+    note("integrator fixture %s" % kind.__name__)
+    qi, qb = mtx.load("qi.mtx"), mtx.load(fixture)
+    P = params(101, 11, 11, 0, 12)
+    it = kind(P)
+    Qbar = P.createMatrixQbar()
+    it(mc.DeviceArray.from_numpy(qi), Qbar)
+    Q = Qbar.numpy()
+    assert it.iteration() == 0
+    assert np.abs(proj(Q) - proj(qb)).max() < 1e-8
+    assert np.allclose(Q.T @ Q, np.eye(11), atol=1e-12)
+    # a larger case with an odd number of sketches (the tree re-reads entries of odd levels)
+    m, l, N = 3000, 24, 13
+    rng = np.random.default_rng(8)
+    base, _ = np.linalg.qr(rng.standard_normal((m, l)))
+    blocks = [np.linalg.qr(base + 0.2 * rng.standard_normal((m, l)))[0] for _ in range(N)]
+    Qs = np.hstack(blocks)
+    P2 = params(m, l, l, 0, N)
+    it2 = kind(P2)
+    Qbar2 = P2.createMatrixQbar()
+    it2(mc.DeviceArray.from_numpy(Qs), Qbar2)
+    Q2 = Qbar2.numpy()
+    assert np.abs(proj(Q2) - proj(ofn(Qs, l, N))).max() < 1e-10
+    assert np.allclose(Q2.T @ Q2, np.eye(l), atol=1e-12)
+
+
+def test_symmetric_former_fixture(gpu):
+    # check/isvd/former/row_block_symmetric_former.cpp: U U' at 1e-8 (k = 5, p = 6)
+    note("former fixture RowBlockSymmetricFormer")
+    a, q, u = mtx.load("a_sym.mtx"), mtx.load("q.mtx"), mtx.load("u_sym.mtx")
+    k = 5
+    P = params(101, 101, k, 6, 1)
+    f = mc.RowBlockSymmetricFormer(P, True)
+    f(mc.DeviceArray.from_numpy(np.asarray(a, order="C")), mc.DeviceArray.from_numpy(q))
+    U, S = f.matrixU(), f.vectorS()
+    assert np.abs(proj(U) - proj(u[:, :k])).max() < 1e-8
+    S_o, U_o = oracle.symmetric_former(a, q, k)
+    assert np.allclose(S, S_o, rtol=1e-12)
+    assert np.all(np.diff(S) >= 0)  # ascending, as the reference returns them
+    assert np.abs(proj(U) - proj(U_o)).max() < 1e-10
+    with pytest.raises(mc.McnlaError):  # a non-square A is rejected
+        mc.RowBlockSymmetricFormer(params(101, 102, k, 6, 1), True)(
+            mc.DeviceArray.from_numpy(np.zeros((101, 102))), mc.DeviceArray.from_numpy(q))
+
+
 def test_integrator_not_converged_returns_result(gpu):
     note("integrator maxiter exhaustion")
     qi = mtx.load("qi.mtx")

@@ -89,6 +89,48 @@ def test_orthogonalizer_fixture(qi):
         assert np.allclose(Q[:, b].T @ Q[:, b], np.eye(11), atol=1e-12)
 
 
+def test_reduction_and_extrinsic_fixtures(qi):
+    # check/isvd/integrator/row_block_reduction_integrator.cpp:66-80 and
+    # row_block_extrinsic_mean_integrator.cpp (Qbar Qbar' at 1e-8)
+    for fn, name in ((oracle.reduction, "qb_re.mtx"), (oracle.extrinsic_mean, "qb_em.mtx")):
+        Q = fn(qi, 11, 12)
+        assert np.abs(proj(Q) - proj(mtx.load(name))).max() < 1e-8
+        assert np.allclose(Q.T @ Q, np.eye(11), atol=1e-12)
+
+
+def test_reduction_and_extrinsic_closed_forms():
+    rng = np.random.default_rng(4)
+    m, l = 60, 4
+    Q0, _ = np.linalg.qr(rng.standard_normal((m, l)))
+    # identical sketches: both integrators return that subspace
+    same = np.hstack([Q0] * 4)
+    for fn in (oracle.reduction, oracle.extrinsic_mean):
+        assert np.abs(proj(fn(same, l, 4)) - proj(Q0)).max() < 1e-12
+    # two sketches: the reduction is the geodesic midpoint -- principal vectors (Q0 w_k + Q1 t_k)
+    # normalized, i.e. the principal angles to both inputs are half of the angles between them
+    Q1, _ = np.linalg.qr(Q0 + 0.3 * rng.standard_normal((m, l)))
+    Qm = oracle.reduction(np.hstack([Q0, Q1]), l, 2)
+    th = np.arccos(np.clip(np.linalg.svd(Q0.T @ Q1, compute_uv=False), -1, 1))
+    for Qx in (Q0, Q1):
+        th_m = np.arccos(np.clip(np.linalg.svd(Qx.T @ Qm, compute_uv=False), -1, 1))
+        assert np.allclose(np.sort(th_m), np.sort(th / 2), atol=1e-10)
+    # the extrinsic mean of two subspaces spans the top-l eigenvectors of Q0Q0' + Q1Q1'
+    Qe = oracle.extrinsic_mean(np.hstack([Q0, Q1]), l, 2)
+    w, V = np.linalg.eigh(proj(Q0) + proj(Q1))
+    assert np.abs(proj(Qe) - proj(V[:, -l:])).max() < 1e-10
+
+
+def test_symmetric_former_fixture():
+    # check/isvd/former/row_block_symmetric_former.cpp:45-66 (k = 5, p = 6; U U' at 1e-8)
+    a, q, u = mtx.load("a_sym.mtx"), mtx.load("q.mtx"), mtx.load("u_sym.mtx")
+    k = 5
+    S, U = oracle.symmetric_former(a, q, k)
+    assert np.abs(proj(U) - proj(u[:, :k])).max() < 1e-8
+    w = np.linalg.eigvalsh(q.T @ a @ q)  # LAPACK, ascending
+    assert np.allclose(S, w[-k:], rtol=1e-12)  # the k largest, ascending
+    assert np.allclose(a @ U - U * S, (np.eye(a.shape[0]) - proj(q)) @ a @ U, atol=1e-9 * S[-1])
+
+
 @pytest.mark.parametrize("colmajor", [True, False])
 def test_formers_fixture(colmajor):
     # check/isvd/former/svd_former.cpp:41,52-69; col_block_gramian_former.cpp:59-63;
