@@ -73,20 +73,40 @@ __global__ __launch_bounds__(kThreads) void gram_partials_kernel(int t, const In
   }
 }
 
-// out[q] = scale * sum_blk P[blk][q] for q < count, in a fixed order (deterministic); four
-// independent accumulators keep several loads in flight.
+// sum_blk P[blk][q] in block order (deterministic).  The loads of 16 blocks are issued before
+// any is added, so a thread waits for one memory round trip per 16 blocks, not per block.
 __device__ __forceinline__ double sum_partials(const double* __restrict__ P, int64_t nblk,
                                                int64_t count, int64_t q) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int64_t b = 0;
-  for (; b + 4 <= nblk; b += 4) {
-    s0 += P[b * count + q];
-    s1 += P[(b + 1) * count + q];
-    s2 += P[(b + 2) * count + q];
-    s3 += P[(b + 3) * count + q];
+  double total = 0.0;
+  for (int64_t b0 = 0; b0 < nblk; b0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = (b0 + j < nblk) ? P[(b0 + j) * count + q] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) total += v[j];
   }
-  for (; b < nblk; ++b) s0 += P[b * count + q];
-  return (s0 + s1) + (s2 + s3);
+  return total;
+}
+
+// Two such sums at once (32 loads in flight); q1 < 0 skips the second.
+__device__ __forceinline__ void sum_partials2(const double* __restrict__ P, int64_t nblk,
+                                              int64_t count, int64_t q0, int64_t q1, double& s0,
+                                              double& s1) {
+  s0 = 0.0;
+  s1 = 0.0;
+  for (int64_t b0 = 0; b0 < nblk; b0 += 16) {
+    double v[16], w[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      v[j] = (b0 + j < nblk) ? P[(b0 + j) * count + q0] : 0.0;
+      w[j] = (b0 + j < nblk && q1 >= 0) ? P[(b0 + j) * count + q1] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      s0 += v[j];
+      s1 += w[j];
+    }
+  }
 }
 
 __global__ void reduce_partials_kernel(int t, const IntegratorState* st,
@@ -184,9 +204,10 @@ __device__ void kn_step_ns(int t, IntegratorState* st, int l, double invN, doubl
   // Dc into sDc, Dgc into sT, both zero-padded to np x np
   for (int e = tid; e < np * np; e += NT) {
     const int i = e / np, j = e % np;
-    const bool in = i < l && j < l;
-    sDc[i * ld + j] = in ? invN * sum_partials(P, nblk, 2 * ll, i * l + j) : 0.0;
-    sT[i * ld + j] = in ? invN * sum_partials(P, nblk, 2 * ll, ll + i * l + j) : 0.0;
+    double dc = 0.0, dgc = 0.0;
+    if (i < l && j < l) sum_partials2(P, nblk, 2 * ll, i * l + j, ll + i * l + j, dc, dgc);
+    sDc[i * ld + j] = invN * dc;
+    sT[i * ld + j] = invN * dgc;
   }
   __syncthreads();
   // Y = I - 4Z with Z = sym(Dgc) - Dc Dc'
@@ -272,20 +293,28 @@ __global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
   double* sG = smem + kStepRows * ldr;
   const int64_t r0 = blockIdx.x * static_cast<int64_t>(kStepRows);
   const int64_t per = nl * l;
-  for (int e = tid; e < kStepRows * l; e += kStepThreads) {
-    const int rr = e / l, j = e % l;
-    const int64_t r = r0 + rr;
-    double b = 0.0, g = 0.0;
-    if (r < nl) {
-      const int64_t o = r * l + j;
-#pragma unroll 4
-      for (int64_t s = 0; s < splits; ++s) g += Pbg[s * per + o];
-      g *= invN;
-      b = B[o];
-      Bg[o] = g;
+  // elements e and e + kStepThreads together, so two sums' loads are in flight at once
+  for (int e = tid; e < kStepRows * l; e += 2 * kStepThreads) {
+    const int e2 = e + kStepThreads;
+    const int64_t ra = r0 + e / l, rb = r0 + e2 / l;
+    const bool va = ra < nl, vb = e2 < kStepRows * l && rb < nl;
+    const int64_t oa = ra * l + e % l, ob = rb * l + e2 % l;
+    double ga = 0.0, gb = 0.0;
+    if (va) sum_partials2(Pbg, splits, per, oa, vb ? ob : -1, ga, gb);
+    if (va) {
+      ga *= invN;
+      Bg[oa] = ga;
     }
-    sB[rr * ldr + j] = b;
-    sG[rr * ldr + j] = g;
+    sB[(e / l) * ldr + e % l] = va ? B[oa] : 0.0;
+    sG[(e / l) * ldr + e % l] = va ? ga : 0.0;
+    if (e2 < kStepRows * l) {
+      if (vb) {
+        gb *= invN;
+        Bg[ob] = gb;
+      }
+      sB[(e2 / l) * ldr + e2 % l] = vb ? B[ob] : 0.0;
+      sG[(e2 / l) * ldr + e2 % l] = vb ? gb : 0.0;
+    }
   }
   __syncthreads();
   const int ll = l * l;
