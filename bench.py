@@ -7,6 +7,9 @@ between them.  Prints one JSON line.
 A is synthetic: A = U50 diag(0.9^i) V50' + E with Haar U50 (m x 50), V50 (n x 50) and
 E_ij = 1e-3 g_ij / (sqrt(m) + sqrt(n)), g ~ N(0,1) (||E||_2 ~ 1e-3); it is resident in HBM
 before timing starts.
+
+The `serve` object is the serving loop on the mixed-size request stream (Poisson arrivals at 75% of
+the capacity measured by a closed-loop burst): solves/s and p50/p99 latency, submit to results.
 """
 import argparse
 import json
@@ -94,6 +97,122 @@ def cpu_baseline(m, n, k, p, N, iters, budget_rows):
             "seconds_per_solve": total}
 
 
+SERVE_M, SERVE_N, SERVE_R, SERVE_BASES = 50000, 5000, 128, 8
+
+
+def serve_stream_requests(count, seed=3):
+    """The serving stream: m log-uniform in [1000, 50000], n log-uniform in [500, 5000] resampled
+    until n <= m, k uniform in {5..50}, p = 10, N uniform in {8..32}, rho ~ U(0.8, 0.98) (mapped
+    to the nearest of the resident bases), Omega seed per request."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        m = int(round(np.exp(rng.uniform(np.log(1000), np.log(SERVE_M)))))
+        while True:
+            n = int(round(np.exp(rng.uniform(np.log(500), np.log(SERVE_N)))))
+            if n <= m:
+                break
+        k = int(rng.integers(5, 51))
+        N = int(rng.integers(8, 33))
+        rho = float(rng.uniform(0.8, 0.98))
+        base = min(SERVE_BASES - 1, int((rho - 0.8) / 0.18 * SERVE_BASES))
+        out.append((m, n, k, 10, N, base, 5000 + i))
+    return out
+
+
+def serve_bench(mc, args):
+    """Continuous-batching serving of the request stream: capacity from a closed-loop burst, then
+    Poisson arrivals at 75% of it; p50/p99 latency from submit to results ready."""
+    import threading
+    rhos = 0.8 + 0.18 * (np.arange(SERVE_BASES) + 0.5) / SERVE_BASES
+    bases = []
+    for b in range(SERVE_BASES):
+        U, V = make_factors(SERVE_M, SERVE_N, SERVE_R, 100 + b)
+        bases.append(mc.synth_lowrank(SERVE_M, SERVE_N, U, V, rhos[b] ** np.arange(SERVE_R), 0.0,
+                                      seed=200 + b, dtype=np.float32, order="F"))
+    log("serve: %d resident FP32 bases %d x %d (rank %d, sigma_i = rho^i)"
+        % (SERVE_BASES, SERVE_M, SERVE_N, SERVE_R))
+    reqs = serve_stream_requests(args.serve_requests + args.serve_burst)
+    burst, stream = reqs[:args.serve_burst], reqs[args.serve_burst:]
+    kw = dict(hbm_budget=int(args.serve_budget_gb * (1 << 30)), precision=mc.PREC_MIXED_BF16)
+
+    def submit(srv, r):
+        m, n, k, p, N, b, seed = r
+        return srv.submit(bases[b], m, n, k, p, N, seed=seed, lda=SERVE_M)
+
+    # the same requests alone (one slot, one at a time), to compare with their results in the mix
+    check_idx = list(range(0, min(64, len(stream)), 4))  # kept however the stream is cut
+    alone = {}
+    with mc.Server(slots=1, **kw) as srv:
+        for i in check_idx:
+            rid = submit(srv, stream[i])
+            srv.wait(rid)
+            r = srv.result(rid)
+            alone[i] = (r.S.numpy(), r.Qbar.numpy(), r.iterations)
+            srv.release(rid)
+
+    with mc.Server(slots=args.serve_slots, **kw) as srv:
+        t0 = time.perf_counter()
+        ids = [submit(srv, r) for r in burst]
+        for _ in ids:
+            srv.release(srv.next_done())
+        cap = len(burst) / (time.perf_counter() - t0)
+        log("serve: capacity %.1f solves/s (closed-loop burst of %d)" % (cap, len(burst)))
+        rate = 0.75 * cap
+        if len(stream) > rate * args.serve_seconds:  # keep the open-loop phase bounded
+            stream = stream[:max(64, int(rate * args.serve_seconds))]
+            log("serve: stream cut to %d requests (%.0f s at %.1f/s)"
+                % (len(stream), args.serve_seconds, rate))
+        srv.reset_stats()
+        arrivals = np.cumsum(np.random.default_rng(4).exponential(1.0 / rate, len(stream)))
+        matched, mismatched = [0], [0]
+        id_to_idx = {}
+        lock = threading.Lock()
+
+        def collect():
+            for _ in range(len(stream)):
+                rid = srv.next_done()
+                with lock:
+                    i = id_to_idx.get(rid)
+                if i in alone:
+                    r = srv.result(rid)
+                    got = (r.S.numpy(), r.Qbar.numpy(), r.iterations)
+                    ok = all(np.array_equal(x, y) for x, y in zip(got[:2], alone[i][:2]))
+                    ok = ok and got[2] == alone[i][2]
+                    (matched if ok else mismatched)[0] += 1
+                srv.release(rid)
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+        for i, r in enumerate(stream):
+            delay = t0 + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            with lock:
+                id_to_idx[submit(srv, r)] = i
+            if i % 512 == 511:
+                log("serve: %d/%d submitted" % (i + 1, len(stream)))
+        th.join()
+        st = srv.stats()
+    for b in bases:
+        b.free()
+    log("serve: p50 %.2f ms, p99 %.2f ms, %.1f solves/s, max running %d"
+        % (st["p50_s"] * 1e3, st["p99_s"] * 1e3, st["completed"] / st["span_s"], st["max_running"]))
+    return {"requests": len(stream), "arrivals": "Poisson", "rate_per_s": round(rate, 2),
+            "capacity_solves_per_s": round(cap, 2), "solves_per_s": round(st["completed"] / st["span_s"], 2),
+            "p50_ms": round(st["p50_s"] * 1e3, 3), "p99_ms": round(st["p99_s"] * 1e3, 3),
+            "mean_ms": round(st["mean_s"] * 1e3, 3), "max_ms": round(st["max_s"] * 1e3, 3),
+            "completed": st["completed"], "failed": st["failed"], "slots": args.serve_slots,
+            "max_running": st["max_running"], "hbm_budget_gb": args.serve_budget_gb,
+            "peak_gb": round(st["peak_bytes"] / 2 ** 30, 2), "precision": "bf16 sketch, f64 rest",
+            "alone_vs_mix_identical": "%d/%d" % (matched[0], matched[0] + mismatched[0]),
+            "workload": "m log-U[1000,50000], n log-U[500,5000] (n<=m), k U{5..50}, p=10, "
+                        "N U{8..32}, rho U(0.8,0.98) over %d resident FP32 %dx%d bases "
+                        "(sigma_i=rho^i, rank %d; a request is the leading m x n block), KN tol "
+                        "1e-4, SVD former, stream seed 3" % (SERVE_BASES, SERVE_M, SERVE_N, SERVE_R)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +227,12 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", default="bf16", choices=["bf16", "bf16x3"])
+    ap.add_argument("--serve-requests", type=int, default=4096)
+    ap.add_argument("--serve-burst", type=int, default=256)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=float, default=48.0)
+    ap.add_argument("--serve-seconds", type=float, default=90.0)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the iSVD path here is built for one MI355X "
@@ -208,9 +333,13 @@ def main():
         "stage_order": ["sketch", "orthogonalize", "integrate", "form"],
         "roofline": {k: v for k, v in kernels[dominant].items() if k != "ms"},
     }
+    iters64 = s64.iteration()
+    if not args.no_serve:
+        del s64, smx, A64, A32  # the serving stream's bases and arena take their place
+        out["serve"] = serve_bench(mc, args)
     if not args.no_cpu:
         log("bench: cpu baseline (oracle, %d-row sample)" % args.cpu_rows)
-        out["cpu_baseline"] = cpu_baseline(m, n, k, p, N, s64.iteration(), args.cpu_rows)
+        out["cpu_baseline"] = cpu_baseline(m, n, k, p, N, iters64, args.cpu_rows)
     print(json.dumps(out), flush=True)
 
 

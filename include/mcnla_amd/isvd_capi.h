@@ -212,6 +212,11 @@ mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_
 mcnla_status mcnla_solver_destroy(mcnla_solver s);
 /* Device bytes the solver holds (workspaces, not counting A). */
 mcnla_status mcnla_solver_workspace_bytes(mcnla_solver s, size_t* bytes);
+/* Device bytes a solver of this shape and configuration would hold, computed without allocating
+ * and without a device (what the serving loop reserves per request).  No reference counterpart:
+ * the reference sizes workspaces inside each stage's initialize() (stage_wrapper.hpp:52-61). */
+mcnla_status mcnla_solver_required_bytes(const mcnla_params* params,
+                                         const mcnla_solver_config* cfg, size_t* bytes);
 mcnla_status mcnla_solver_set_seed(mcnla_solver s, uint64_t seed);
 mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
                                 int a_order, void* stream);
@@ -226,6 +231,80 @@ mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* s
  * No reference counterpart (the reference times whole stages only, stage_wrapper.hpp:120-140);
  * this is for roofline accounting. */
 mcnla_status mcnla_solver_kernel_times(mcnla_solver s, double times[2]);
+
+/* ---------------------------------------------------------------------------------------------
+ * Serving loop: continuous batching of low-rank-approximation requests on one GPU.  No reference
+ * counterpart (the reference runs one solve per MPI job, src/isvd_dense_cb.cpp:46-302).
+ *
+ * The server owns one device arena of `hbm_budget` bytes, allocated at creation; nothing is
+ * allocated afterwards.  A request is admitted when one of `slots` workers is free and the arena
+ * holds a free region of the request's size (its solver's workspaces, plus a copy of A when A is
+ * on the host, plus its results).  Requests that arrive while others run are admitted as slots
+ * and memory free up; up to `max_bypass` later requests may be admitted ahead of a head request
+ * that does not fit yet.  Each worker owns one HIP stream, so admitted solves overlap on the
+ * device.  A finished request releases its workspaces at once and keeps only its results (S, U,
+ * V, Qbar) in the arena until mcnla_server_release.  A solve is deterministic: its results do
+ * not depend on what else runs. */
+typedef struct mcnla_server_s* mcnla_server;
+
+typedef struct {
+  mcnla_solver_config solver; /* per-request template (precision, integrator, former, tol, ...);
+                                 the seed comes with each request */
+  int slots;                  /* concurrent solves, each with a host thread and a stream; 1..16 */
+  size_t hbm_budget;          /* bytes of the device arena */
+  int max_bypass;             /* later requests admitted ahead of a blocked head, at most */
+} mcnla_server_config;
+
+typedef struct {
+  mcnla_params params;
+  const void* A;   /* device pointer, or host pointer when a_on_host (then copied into the arena) */
+  int64_t lda;
+  int a_dtype;     /* mcnla_dtype */
+  int a_order;     /* mcnla_order */
+  int a_on_host;
+  uint64_t seed;   /* Omega seed */
+} mcnla_request;
+
+typedef struct {
+  int status;           /* mcnla_status of the solve (MCNLA_ERR_NOT_CONVERGED still has results) */
+  int64_t iterations;   /* integrator iterations */
+  const double* S;      /* device pointers into the arena, valid until release: S (k) */
+  const double* U;      /* m x k, column-major (row-major for a Gramian former) */
+  const double* V;      /* n x k row-major, or NULL without jobv */
+  const double* Qbar;   /* m x (k+p) row-major */
+  double queue_s;       /* submit -> admission */
+  double run_s;         /* admission -> results ready */
+  double latency_s;     /* submit -> results ready */
+  const char* error;    /* message of a failed solve, "" otherwise; valid until release */
+} mcnla_served;
+
+typedef struct {
+  uint64_t submitted, completed, failed;
+  int max_running;       /* most solves admitted at once */
+  size_t peak_bytes;     /* most arena bytes reserved at once */
+  size_t budget_bytes;
+  double p50_s, p99_s, mean_s, max_s; /* latency (submit -> results ready) of completed requests */
+  double span_s;         /* first submit -> last completion */
+} mcnla_server_stats;
+
+void mcnla_server_config_default(mcnla_server_config* cfg);
+mcnla_status mcnla_server_create(const mcnla_server_config* cfg, mcnla_server* out);
+/* Waits for running solves; pending requests are dropped. */
+mcnla_status mcnla_server_destroy(mcnla_server s);
+/* Arena bytes the request would reserve. */
+mcnla_status mcnla_server_request_bytes(mcnla_server s, const mcnla_request* r, size_t* bytes);
+/* Queues a request (never blocks); MCNLA_ERR_NOMEM if it can never fit the budget. */
+mcnla_status mcnla_server_submit(mcnla_server s, const mcnla_request* r, uint64_t* id);
+/* Blocks until request `id` is finished or `timeout_s` passes (< 0: no limit); *done tells which. */
+mcnla_status mcnla_server_wait(mcnla_server s, uint64_t id, double timeout_s, int* done);
+/* Next finished request not yet reported by this call, in completion order; *id = 0 on timeout. */
+mcnla_status mcnla_server_next_done(mcnla_server s, double timeout_s, uint64_t* id);
+mcnla_status mcnla_server_result(mcnla_server s, uint64_t id, mcnla_served* out);
+/* Frees the request's results in the arena; the id is then unknown. */
+mcnla_status mcnla_server_release(mcnla_server s, uint64_t id);
+mcnla_status mcnla_server_get_stats(mcnla_server s, mcnla_server_stats* out);
+/* Clears the latency record and counters (not the requests). */
+mcnla_status mcnla_server_reset_stats(mcnla_server s);
 
 /* ---------------------------------------------------------------------------------------------
  * Synthetic test matrices (the reference demo's construction, demo/demo.cpp:250-293):

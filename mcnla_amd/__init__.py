@@ -18,6 +18,7 @@ from .isvd import (
     RowBlockKolmogorovNagumoIntegrator, RowBlockReductionIntegrator, RowBlockSymmetricFormer,
     RowBlockWenYinIntegrator, Solver, SvdFormer, synth_lowrank,
 )
+from .serve import Served, Server, required_bytes
 
 PREC_F64 = _lib.PREC_F64
 PREC_MIXED_BF16 = _lib.PREC_MIXED_BF16

@@ -52,6 +52,34 @@ class SolverConfig(ctypes.Structure):
                 ("omega_threads", ctypes.c_int)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("solver", SolverConfig), ("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("max_bypass", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("params", Params), ("A", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("a_dtype", ctypes.c_int), ("a_order", ctypes.c_int), ("a_on_host", ctypes.c_int),
+                ("seed", ctypes.c_uint64)]
+
+
+class Served(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int64),
+                ("S", ctypes.c_void_p), ("U", ctypes.c_void_p), ("V", ctypes.c_void_p),
+                ("Qbar", ctypes.c_void_p), ("queue_s", ctypes.c_double),
+                ("run_s", ctypes.c_double), ("latency_s", ctypes.c_double),
+                ("error", ctypes.c_char_p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("max_running", ctypes.c_int),
+                ("peak_bytes", ctypes.c_size_t), ("budget_bytes", ctypes.c_size_t),
+                ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double),
+                ("mean_s", ctypes.c_double), ("max_s", ctypes.c_double),
+                ("span_s", ctypes.c_double)]
+
+
 # Every exported symbol and its signature; tests check the library exports exactly these.
 _vp = ctypes.c_void_p
 _pp = ctypes.POINTER(ctypes.c_void_p)
@@ -106,6 +134,20 @@ SIGNATURES = {
     "mcnla_solver_iteration": (_st, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "mcnla_solver_stage_times": (_st, [_vp, _dp, _dp]),
     "mcnla_solver_kernel_times": (_st, [_vp, _dp]),
+    "mcnla_solver_required_bytes": (_st, [ctypes.POINTER(Params), ctypes.POINTER(SolverConfig),
+                                          ctypes.POINTER(ctypes.c_size_t)]),
+    "mcnla_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
+    "mcnla_server_create": (_st, [ctypes.POINTER(ServerConfig), _pp]),
+    "mcnla_server_destroy": (_st, [_vp]),
+    "mcnla_server_request_bytes": (_st, [_vp, ctypes.POINTER(Request),
+                                         ctypes.POINTER(ctypes.c_size_t)]),
+    "mcnla_server_submit": (_st, [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    "mcnla_server_wait": (_st, [_vp, _u64, _d, ctypes.POINTER(ctypes.c_int)]),
+    "mcnla_server_next_done": (_st, [_vp, _d, ctypes.POINTER(ctypes.c_uint64)]),
+    "mcnla_server_result": (_st, [_vp, _u64, ctypes.POINTER(Served)]),
+    "mcnla_server_release": (_st, [_vp, _u64]),
+    "mcnla_server_get_stats": (_st, [_vp, ctypes.POINTER(ServerStats)]),
+    "mcnla_server_reset_stats": (_st, [_vp]),
     "mcnla_synth_lowrank": (_st, [_i64, _i64, _i64, _vp, _vp, _vp, _d, _u64, _vp, _i64, _i, _i,
                                   _vp]),
 }

@@ -8,6 +8,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "solver.hpp"
 
 namespace mcnla {
 
@@ -18,6 +19,13 @@ thread_local std::string g_error;
 }
 void set_error(const std::string& msg) { g_error = msg; }
 const char* get_error() { return g_error.c_str(); }
+
+namespace {
+thread_local Carve* g_carve = nullptr;
+}
+Carve* current_carve() { return g_carve; }
+CarveScope::CarveScope(Carve* c) : prev_(g_carve) { g_carve = c; }
+CarveScope::~CarveScope() { g_carve = prev_; }
 
 namespace {
 
@@ -40,18 +48,35 @@ mcnla_status guard(F&& f) {
   }
 }
 
-// Device allocation owned by a stage (grows, never shrinks).
+// Device allocation owned by a stage (grows, never shrinks).  While a CarveScope is active on the
+// calling thread, ensure() takes its memory from that scope's region instead of hipMalloc (a
+// solver created inside a server's arena); such a buffer is borrowed, is never freed, and may not
+// grow once the scope has ended.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
   }
   void ensure(size_t n) {
     if (n <= bytes) return;
+    if (Carve* c = current_carve()) {
+      const size_t off = align_up(c->used, kCarveAlign);
+      MCNLA_REQUIRE(c->counting || off + n <= c->cap, MCNLA_ERR_NOMEM,
+                    "solver workspace exceeds its reserved region");
+      // Counting scopes only size a layout: their pointers are never dereferenced.
+      p = c->counting ? reinterpret_cast<void*>(kCarveAlign + off) : c->base + off;
+      bytes = n;
+      borrowed = true;
+      c->used = off + n;
+      return;
+    }
+    MCNLA_REQUIRE(!borrowed, MCNLA_ERR_STATE,
+                  "a solver carved from a reserved region cannot grow its workspace");
     if (p) MCNLA_HIP(hipFree(p));
     p = nullptr;
     bytes = 0;
@@ -147,6 +172,19 @@ struct SketchCore {
     if (gemm_beg) (void)hipEventDestroy(gemm_beg);
     if (gemm_end) (void)hipEventDestroy(gemm_end);
   }
+  static SketchBf16Args bf16_args(const mcnla_params& p, int precision) {
+    SketchBf16Args s;
+    s.m = p.nrow;
+    s.n = p.ncol;
+    s.nl = (p.rank + p.over_rank) * p.num_sketch;
+    s.terms = (precision == MCNLA_PREC_MIXED_BF16X3) ? 3 : 1;
+    return s;
+  }
+  // Allocates Omega and, for a mixed precision, the BF16 panels of A and Omega.
+  void prepare(const mcnla_params& p, int precision) {
+    omega.ensure(p.ncol * (p.rank + p.over_rank) * p.num_sketch * sizeof(double));
+    if (precision != MCNLA_PREC_F64) ws.ensure(sketch_bf16_workspace(bf16_args(p, precision)));
+  }
   void generate(const mcnla_params& p, uint64_t seed, int mode, int threads, bool col_block,
                 hipStream_t st) {
     const int64_t nl = (p.rank + p.over_rank) * p.num_sketch;
@@ -189,10 +227,7 @@ struct SketchCore {
       gemm_f64(g, st);
       MCNLA_HIP(hipEventRecord(gemm_end, st));
     } else {
-      SketchBf16Args s;
-      s.m = p.nrow;
-      s.n = p.ncol;
-      s.nl = nl;
+      SketchBf16Args s = bf16_args(p, precision);
       s.A = A;
       s.lda = lda;
       s.a_colmajor = (order == MCNLA_COL_MAJOR);
@@ -203,7 +238,6 @@ struct SketchCore {
       else
         s.Y = static_cast<float*>(Y);
       s.ldy = nl;
-      s.terms = (precision == MCNLA_PREC_MIXED_BF16X3) ? 3 : 1;
       ws.ensure(sketch_bf16_workspace(s));
       sketch_bf16(s, ws.p, ws.bytes, st, gemm_beg, gemm_end);
     }
@@ -232,30 +266,22 @@ struct GramCore {
     MCNLA_HIP(hipEventElapsedTime(&ms, beg, end));
     return ms * 1e-3;
   }
-  void run(const void* Y, bool y_f32, int64_t m, int64_t nl, double* G, hipStream_t st) {
-    if (!beg) {
-      MCNLA_HIP(hipEventCreate(&beg));
-      MCNLA_HIP(hipEventCreate(&end));
-    }
+  // Arguments of G = Y'Y except the pointers.
+  static GemmF64Args plan(bool y_f32, int64_t m, int64_t nl) {
     // Lower-triangle tiles: 256 x 128 on the FP64 fast path (tm (tm+1) tiles), else 128 x 128.
     // Split K so that the grid fills whole rounds of about 2304 workgroups (9 per CU).
     const int64_t tm = ceil_div(nl, 256), t = ceil_div(nl, 128);
     const int64_t tiles = (!y_f32 && nl >= 128) ? tm * (tm + 1) : t * (t + 1) / 2;
     GemmF64Args g;
-    g.ev_begin = beg;
-    g.ev_end = end;
     g.M = nl;
     g.N = nl;
     g.K = m;
-    g.A = Y;
     g.lda = nl;
     g.a_mcontig = true;
     g.a_f32 = y_f32;
-    g.B = Y;
     g.ldb = nl;
     g.b_ncontig = true;
     g.b_f32 = y_f32;
-    g.C = G;
     g.ldc = nl;
     g.lower_only = true;
     g.shape = GemmShape::kSquare;
@@ -263,7 +289,24 @@ struct GramCore {
     g.kchunk = ceil_div(m, g.splits);
     g.kchunk = ceil_div(g.kchunk, 16) * 16;
     g.splits = ceil_div(m, g.kchunk);
+    return g;
+  }
+  void prepare(bool y_f32, int64_t m, int64_t nl) {
+    const GemmF64Args g = plan(y_f32, m, nl);
     partial.ensure(gemm_f64_partial_elems(nl, nl, g.splits) * sizeof(double));
+  }
+  void run(const void* Y, bool y_f32, int64_t m, int64_t nl, double* G, hipStream_t st) {
+    if (!beg) {
+      MCNLA_HIP(hipEventCreate(&beg));
+      MCNLA_HIP(hipEventCreate(&end));
+    }
+    GemmF64Args g = plan(y_f32, m, nl);
+    g.ev_begin = beg;
+    g.ev_end = end;
+    g.A = Y;
+    g.B = Y;
+    g.C = G;
+    prepare(y_f32, m, nl);
     g.partial = partial.as<double>();
     gemm_f64(g, st);
   }
@@ -271,16 +314,18 @@ struct GramCore {
 
 // Z = A' Qbar (n x l), Wg = Z' Z, eig, and the outputs of either former.
 struct FormCore {
-  DevBuf Z, partial, Wg, eigw, eigv, Wk, S, U, V, Vtmp;
+  DevBuf Z, partial, Wg, eigw, eigv, Wk, S, U, V;
   // kind: MCNLA_FORM_*.  The symmetric former (square symmetric A) uses Wg = Z'Qbar = Qbar'A Qbar
   // in place of Z'Z and forms no V.
-  void run(const mcnla_params& p, int kind, bool jobv, const void* A, int64_t lda, int dtype,
-           int order, const double* Qbar, hipStream_t st) {
-    const bool svd = kind == MCNLA_FORM_SVD;
-    const bool sym = kind == MCNLA_FORM_ROW_BLOCK_SYMMETRIC;
-    if (sym) jobv = false;
-    const int64_t m = p.nrow, n = p.ncol, k = p.rank;
-    const int l = static_cast<int>(p.rank + p.over_rank);
+  // Split-K of Z = A'Qbar (K = m over ceil(n/128) row tiles) and of Wg = Z'Z (K = n).
+  static int64_t z_kchunk(int64_t m, int64_t n) {
+    return ceil_div(ceil_div(m, gemm_f64_splits_for(ceil_div(n, 128), m, 2048)), 16) * 16;
+  }
+  static int64_t w_kchunk(int64_t n) {
+    return ceil_div(ceil_div(n, gemm_f64_splits_for(1, n, 64)), 16) * 16;
+  }
+  void prepare(const mcnla_params& p) {
+    const int64_t m = p.nrow, n = p.ncol, k = p.rank, l = p.rank + p.over_rank;
     Z.ensure(n * l * sizeof(double));
     Wg.ensure(l * l * sizeof(double));
     eigw.ensure(l * sizeof(double));
@@ -289,6 +334,18 @@ struct FormCore {
     S.ensure(k * sizeof(double));
     U.ensure(m * k * sizeof(double));
     V.ensure(n * k * sizeof(double));
+    partial.ensure(std::max(gemm_f64_partial_elems(n, l, ceil_div(m, z_kchunk(m, n))),
+                            gemm_f64_partial_elems(l, l, ceil_div(n, w_kchunk(n)))) *
+                   sizeof(double));
+  }
+  void run(const mcnla_params& p, int kind, bool jobv, const void* A, int64_t lda, int dtype,
+           int order, const double* Qbar, hipStream_t st) {
+    const bool svd = kind == MCNLA_FORM_SVD;
+    const bool sym = kind == MCNLA_FORM_ROW_BLOCK_SYMMETRIC;
+    if (sym) jobv = false;
+    const int64_t m = p.nrow, n = p.ncol, k = p.rank;
+    const int l = static_cast<int>(p.rank + p.over_rank);
+    prepare(p);
     // Z = A' Qbar: element (j, r) of A' is A(r, j).
     GemmF64Args z;
     z.M = n;
@@ -303,10 +360,8 @@ struct FormCore {
     z.C = Z.as<double>();
     z.ldc = l;
     z.shape = tall_shape(l);
-    z.splits = gemm_f64_splits_for(ceil_div(n, 128), m, 2048);
-    z.kchunk = ceil_div(ceil_div(m, z.splits), 16) * 16;
+    z.kchunk = z_kchunk(m, n);
     z.splits = ceil_div(m, z.kchunk);
-    size_t need = gemm_f64_partial_elems(n, l, z.splits);
     // Wg = Z' Z (symmetric former: Z' Qbar, Qbar having n = m rows)
     GemmF64Args w;
     w.M = l;
@@ -320,11 +375,8 @@ struct FormCore {
     w.C = Wg.as<double>();
     w.ldc = l;
     w.shape = GemmShape::kSmall;
-    w.splits = gemm_f64_splits_for(1, n, 64);
-    w.kchunk = ceil_div(ceil_div(n, w.splits), 16) * 16;
+    w.kchunk = w_kchunk(n);
     w.splits = ceil_div(n, w.kchunk);
-    need = std::max(need, gemm_f64_partial_elems(l, l, w.splits));
-    partial.ensure(need * sizeof(double));
     z.partial = partial.as<double>();
     w.partial = partial.as<double>();
     gemm_f64(z, st);
@@ -604,6 +656,73 @@ struct mcnla_solver_s {
   bool computed = false;
   bool converged = false;
 };
+
+namespace mcnla {
+
+namespace {
+// Validates the configuration and allocates every device buffer the solve uses (the stages'
+// initialize()), so that a solve itself allocates nothing.
+std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
+                                           const mcnla_solver_config* cfg) {
+  check_params(params);
+  MCNLA_REQUIRE(cfg, MCNLA_ERR_INVALID, "null argument");
+  check_precision(cfg->precision);
+  MCNLA_REQUIRE(cfg->integrator >= MCNLA_INT_KN && cfg->integrator <= MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN,
+                MCNLA_ERR_INVALID, "unknown integrator kind");
+  MCNLA_REQUIRE(cfg->former >= MCNLA_FORM_SVD && cfg->former <= MCNLA_FORM_ROW_BLOCK_SYMMETRIC,
+                MCNLA_ERR_INVALID, "unknown former kind");
+  MCNLA_REQUIRE(cfg->former != MCNLA_FORM_ROW_BLOCK_SYMMETRIC || params->nrow == params->ncol,
+                MCNLA_ERR_INVALID, "the symmetric former needs a square A");
+  MCNLA_REQUIRE(cfg->omega_mode == MCNLA_OMEGA_PHILOX || cfg->omega_mode == MCNLA_OMEGA_REFERENCE,
+                MCNLA_ERR_INVALID, "unknown omega mode");
+  MCNLA_REQUIRE(cfg->maxiter >= 0 && cfg->tol >= 0.0, MCNLA_ERR_INVALID, "bad maxiter/tol");
+  auto s = std::make_unique<mcnla_solver_s>();
+  s->prm = *params;
+  s->cfg = *cfg;
+  const int64_t l = params->rank + params->over_rank, N = params->num_sketch, nl = l * N;
+  const int64_t m = params->nrow;
+  // Y is FP64 in every precision: the mixed sketch accumulates in FP32 and stores FP64, so that
+  // the FP64 Gram and basis products stream it at full rate.
+  s->Y.ensure(m * nl * sizeof(double));
+  s->G.ensure(nl * nl * sizeof(double));
+  s->Bs.ensure(nl * nl * sizeof(double));
+  s->tmp.ensure(nl * nl * sizeof(double));
+  s->W.ensure(N * l * l * sizeof(double));
+  s->w.ensure(N * l * sizeof(double));
+  s->V.ensure(N * l * l * sizeof(double));
+  s->T.ensure(N * l * l * sizeof(double));
+  s->ws.ensure(integration_workspace(cfg->integrator, N, static_cast<int>(l)));
+  s->coef.ensure(nl * l * sizeof(double));
+  s->M2.ensure(nl * l * sizeof(double));
+  s->Qbar.ensure(m * l * sizeof(double));
+  s->bad.ensure(sizeof(int));
+  s->sketch.prepare(*params, cfg->precision);
+  s->gram.prepare(false, m, nl);
+  s->form.prepare(*params);
+  return s;
+}
+}  // namespace
+
+size_t solver_required_bytes(const mcnla_params* params, const mcnla_solver_config* cfg) {
+  Carve c;
+  c.counting = true;
+  CarveScope scope(&c);
+  new_solver(params, cfg);
+  return align_up(c.used, kCarveAlign);
+}
+
+mcnla_solver solver_create_in(const mcnla_params* params, const mcnla_solver_config* cfg,
+                              void* base, size_t bytes) {
+  MCNLA_REQUIRE(base != nullptr && reinterpret_cast<uintptr_t>(base) % kCarveAlign == 0,
+                MCNLA_ERR_INVALID, "region base must be 256-byte aligned");
+  Carve c;
+  c.base = static_cast<char*>(base);
+  c.cap = bytes;
+  CarveScope scope(&c);
+  return new_solver(params, cfg).release();
+}
+
+}  // namespace mcnla
 
 using namespace mcnla;
 
@@ -971,41 +1090,17 @@ void mcnla_solver_config_default(mcnla_solver_config* cfg) {
 mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_config* cfg,
                                  mcnla_solver* out) {
   return guard([&] {
-    check_params(params);
-    MCNLA_REQUIRE(cfg && out, MCNLA_ERR_INVALID, "null argument");
-    check_precision(cfg->precision);
-    MCNLA_REQUIRE(cfg->integrator >= MCNLA_INT_KN && cfg->integrator <= MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN,
-                  MCNLA_ERR_INVALID, "unknown integrator kind");
-    MCNLA_REQUIRE(cfg->former >= MCNLA_FORM_SVD && cfg->former <= MCNLA_FORM_ROW_BLOCK_SYMMETRIC,
-                  MCNLA_ERR_INVALID, "unknown former kind");
-    MCNLA_REQUIRE(cfg->former != MCNLA_FORM_ROW_BLOCK_SYMMETRIC || params->nrow == params->ncol,
-                  MCNLA_ERR_INVALID, "the symmetric former needs a square A");
-    MCNLA_REQUIRE(cfg->omega_mode == MCNLA_OMEGA_PHILOX || cfg->omega_mode == MCNLA_OMEGA_REFERENCE,
-                  MCNLA_ERR_INVALID, "unknown omega mode");
-    MCNLA_REQUIRE(cfg->maxiter >= 0 && cfg->tol >= 0.0, MCNLA_ERR_INVALID, "bad maxiter/tol");
-    auto s = std::make_unique<mcnla_solver_s>();
-    s->prm = *params;
-    s->cfg = *cfg;
-    const int64_t l = params->rank + params->over_rank, N = params->num_sketch, nl = l * N;
-    const int64_t m = params->nrow;
-    // Allocate the resident workspaces up front (the stages' initialize()).  Y is FP64 in every
-    // precision: the mixed sketch accumulates in FP32 and stores FP64, so that the FP64 Gram and
-    // basis products stream it at full rate.
-    s->Y.ensure(m * nl * sizeof(double));
-    s->G.ensure(nl * nl * sizeof(double));
-    s->Bs.ensure(nl * nl * sizeof(double));
-    s->tmp.ensure(nl * nl * sizeof(double));
-    s->W.ensure(N * l * l * sizeof(double));
-    s->w.ensure(N * l * sizeof(double));
-    s->V.ensure(N * l * l * sizeof(double));
-    s->T.ensure(N * l * l * sizeof(double));
-    s->ws.ensure(integration_workspace(cfg->integrator, N, static_cast<int>(l)));
-    s->coef.ensure(nl * l * sizeof(double));
-    s->M2.ensure(nl * l * sizeof(double));
-    s->Qbar.ensure(m * l * sizeof(double));
-    s->bad.ensure(sizeof(int));
-    s->sketch.omega.ensure(params->ncol * nl * sizeof(double));
-    *out = s.release();
+    MCNLA_REQUIRE(out, MCNLA_ERR_INVALID, "null argument");
+    *out = new_solver(params, cfg).release();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_required_bytes(const mcnla_params* params, const mcnla_solver_config* cfg,
+                                         size_t* bytes) {
+  return guard([&] {
+    MCNLA_REQUIRE(bytes, MCNLA_ERR_INVALID, "null argument");
+    *bytes = solver_required_bytes(params, cfg);
     return MCNLA_OK;
   });
 }
@@ -1023,7 +1118,8 @@ mcnla_status mcnla_solver_workspace_bytes(mcnla_solver s, size_t* bytes) {
     size_t b = 0;
     for (const DevBuf* d : {&s->Y, &s->G, &s->W, &s->w, &s->V, &s->T, &s->Bs, &s->tmp, &s->ws,
                             &s->coef, &s->M2, &s->Qbar, &s->bad, &s->sketch.omega, &s->sketch.ws,
-                            &s->gram.partial, &s->form.Z, &s->form.partial, &s->form.U,
+                            &s->gram.partial, &s->form.Z, &s->form.partial, &s->form.Wg,
+                            &s->form.eigw, &s->form.eigv, &s->form.Wk, &s->form.S, &s->form.U,
                             &s->form.V})
       b += d->bytes;
     *bytes = b;

@@ -120,3 +120,20 @@ def test_python_mirror_names():
         assert hasattr(mc, name), name
     P = mc.Parameters().setSize(2000, 1000).setRank(10).setOverRank(10).setNumSketch(4)
     assert P.dimSketch() == 20 and P.dimSketchTotal() == 80 and P.nrowRank() == 2000
+
+
+def test_required_bytes_without_device():
+    # the serving loop sizes a request's arena region from this; it needs no GPU
+    m, n, k, p, N = 100000, 10000, 20, 12, 64
+    nl = (k + p) * N
+    f64 = mc.required_bytes(m, n, k, p, N)
+    mixed = mc.required_bytes(m, n, k, p, N, precision=mc.PREC_MIXED_BF16)
+    # Y (m x Nl FP64), G, Bs and a scratch Nl x Nl, Omega n x Nl, Qbar and U
+    floor = 8 * (m * nl + 3 * nl * nl + n * nl + m * (k + p) + m * k)
+    assert floor <= f64 < floor + 2 * 8 * 32 * nl * nl
+    # the mixed sketch adds BF16 panels of A (m x n) and Omega
+    assert mixed - f64 >= 2 * m * n
+    assert mc.required_bytes(1000, 500, 5, 10, 8) < mc.required_bytes(2000, 500, 5, 10, 8)
+    with pytest.raises(mc.McnlaError) as e:
+        mc.required_bytes(10, 5, 5, 10, 8)
+    assert e.value.status == _lib.ERR_INVALID
