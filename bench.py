@@ -224,7 +224,7 @@ def main():
     ap.add_argument("--p", type=int, default=12)
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--cpu-rows", type=int, default=200)
+    ap.add_argument("--cpu-rows", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", default="bf16", choices=["bf16", "bf16x3"])
     ap.add_argument("--serve-requests", type=int, default=4096)
@@ -233,6 +233,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=48.0)
     ap.add_argument("--serve-seconds", type=float, default=90.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the iSVD path here is built for one MI355X "
@@ -243,6 +244,17 @@ def main():
     import mcnla_amd as mc
     if mc.device_count() < 1:
         raise SystemExit("no GPU visible")
+    if args.calibrate_copy_mb:
+        # a plain device-to-device copy of known size: the reference reading for the HBM
+        # traffic counters (FETCH_SIZE / WRITE_SIZE) of the roofline's `traffic`
+        nbytes = int(args.calibrate_copy_mb) << 20
+        src = mc.DeviceArray((nbytes // 8,), np.float64)
+        dst = mc.DeviceArray((nbytes // 8,), np.float64)
+        for _ in range(3):
+            mc.lib().mcnla_memcpy_dtod(dst.ptr, src.ptr, nbytes, None)
+        mc.synchronize()
+        log("bench: copied %d bytes device-to-device 3 times" % nbytes)
+        return
     m, n, k, p, N = args.m, args.n, args.k, args.p, args.N
     l = k + p
     log("bench: A %d x %d, k=%d p=%d N=%d" % (m, n, k, p, N))

@@ -18,6 +18,17 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// acc += a * b on the matrix cores with the accumulator tied to the destination.  Through the
+// builtin, the register allocator may give each MFMA a fresh destination (the untied form's
+// destination is early-clobber) and then copy accumulators back: with 256 live accumulator
+// registers that is more AGPR moves than MFMAs, and every move waits for the MFMA it reads.
+// Inline asm is outside the compiler's hazard tracking, so a caller keeps consecutive MFMAs on
+// one accumulator far apart and pads with s_nop before the first MFMA and before VALU reads of
+// the results.
+__device__ __forceinline__ void mfma_bf16_tied(f4& acc, const bf16x8& a, const bf16x8& b) {
+  asm volatile("v_mfma_f32_16x16x32_bf16 %0, %1, %2, %0" : "+a"(acc) : "v"(a), "v"(b));
+}
+
 constexpr int kBM = 128, kBN = 128;
 constexpr int kThreads = 256;
 constexpr int kTile = 64;  // pack transpose tile
@@ -332,18 +343,19 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 8; ++j) mfma_bf16_tied(acc[i][j], af[i], bfr[j]);
   };
   issue(0, S0);
   issue(1, S1);
   issue(2, S2);
+  asm volatile("s_nop 7");  // zeroed accumulators (VALU writes) before their first MFMA read
   for (int64_t kt = 0; kt < nkt; kt += kStages) {
     step(kt, S0, S3);
     step(kt + 1, S1, S0);
     step(kt + 2, S2, S1);
     step(kt + 3, S3, S2);
   }
+  asm volatile("s_nop 7\n s_nop 7\n s_nop 7\n s_nop 7");  // last MFMAs drain before VALU reads
   wait_vmcnt<0>();  // no DMA may still target LDS when the workgroup ends
   const int64_t m0 = tm * kBR, n0 = tn * kBR;
 #pragma unroll
