@@ -395,6 +395,140 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
       }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Tall-skinny products C (M x N, N <= 64) = A * B where A's rows run along K (element (i, k) at
+// A[i*lda + k]) and B's rows along N (element (k, j) at B[k*ldb + j]): the former's Z = A'Qbar
+// of a column-major A (FP32 or FP64), and Qbar = Y (T M) of the row-major collection Y.  A is
+// streamed once, so the kernel is bound by HBM or by FP64 MFMA, whichever is tighter.
+//
+// 128 x LP output tiles (LP = 32 or 64), four waves of 32 x LP.  Each step moves a 128 x 32 slab
+// of A (two threads per row, 16 consecutive k each: 128 or 256 contiguous bytes per row) and a
+// 32 x LP slab of B through LDS in FP64, k-major, with row strides of 128 bytes modulo 256 so
+// that the fragment reads of 32 lanes cover the 64 banks once.  The next step's slabs load into
+// registers while the MFMAs of the current one run.
+template <typename TA, int LP, bool VEC>
+__global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
+  constexpr int BM = 128, BK = 32;
+  constexpr int SA = BM + 16, SB = LP + 16;  // LDS row strides (doubles)
+  constexpr int NB = BK * LP / 256;          // B elements per thread
+  constexpr int VA = VEC ? 16 / static_cast<int>(sizeof(TA)) : 1;
+  typedef typename VecOf<TA, VA>::type V;
+  __shared__ double sA[BK * SA];
+  __shared__ double sB[BK * SB];
+  if (p.gate != nullptr && *p.gate < p.gate_t) return;
+  const int64_t tiles_m = ceil_div(p.M, BM);
+  const int64_t tile = blockIdx.x % tiles_m, split = blockIdx.x / tiles_m;
+  const int64_t i0 = tile * BM;
+  const int64_t kbeg = split * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const TA* A = static_cast<const TA*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+  // A slab: thread -> row ra = tid / 2, k offset ka = 16 * (tid % 2)
+  const int ra = tid / 2, ka = 16 * (tid % 2);
+  const bool row_ok = i0 + ra < p.M;
+  const TA* arow = A + (row_ok ? (i0 + ra) : 0) * p.lda;
+  double va[16], vb[NB];
+  auto load = [&](int64_t k0) {
+    const int64_t kk = k0 + ka;
+    bool done = false;
+    if constexpr (VEC) {
+      if (row_ok && kk + 16 <= kend) {
+#pragma unroll
+        for (int v = 0; v < 16 / VA; ++v) {
+          const V x = *reinterpret_cast<const V*>(arow + kk + v * VA);
+#pragma unroll
+          for (int e = 0; e < VA; ++e) va[v * VA + e] = static_cast<double>(x[e]);
+        }
+        done = true;
+      }
+    }
+    if (!done) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        va[e] = (row_ok && kk + e < kend) ? static_cast<double>(arow[kk + e]) : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int idx = e * 256 + tid, kb = idx / LP, jb = idx % LP;
+      vb[e] = (jb < p.N && k0 + kb < kend) ? B[(k0 + kb) * p.ldb + jb] : 0.0;
+    }
+  };
+  auto store = [&] {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) sA[(ka + e) * SA + ra] = va[e];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int idx = e * 256 + tid;
+      sB[(idx / LP) * SB + idx % LP] = vb[e];
+    }
+  };
+  constexpr int TJ = LP / 16;
+  d4 acc[2][TJ];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < TJ; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const int fr = lane / 16, fc = lane % 16;
+  if (kbeg < kend) load(kbeg);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();  // the previous step's fragment reads are done
+    store();
+    __syncthreads();
+    if (k0 + BK < kend) load(k0 + BK);
+#pragma unroll
+    for (int s4 = 0; s4 < BK / 4; ++s4) {
+      const int kr = 4 * s4 + fr;
+      double af[2], bf[TJ];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) af[a] = sA[kr * SA + wave * 32 + a * 16 + fc];
+#pragma unroll
+      for (int b = 0; b < TJ; ++b) bf[b] = sB[kr * SB + b * 16 + fc];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < TJ; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+  }
+  // D element r of a 16 x 16 tile sits at row lane/16 + 4r, column lane%16
+  const bool to_partial = p.partial != nullptr;
+  double* P = to_partial ? p.partial + split * p.M * p.N : nullptr;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < TJ; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = i0 + wave * 32 + a * 16 + fr + 4 * r;
+        const int64_t gj = b * 16 + fc;
+        if (gi >= p.M || gj >= p.N) continue;
+        const double v = acc[a][b][r];
+        if (to_partial) {
+          P[gi * p.N + gj] = v;
+        } else {
+          double* c = p.C + gi * p.ldc + gj;
+          *c = (p.beta == 0.0) ? p.alpha * v : p.alpha * v + p.beta * *c;
+        }
+      }
+}
+
+template <typename TA, int LP>
+void launch_tall(const GemmF64Args& p, hipStream_t st) {
+  const int64_t nblocks = ceil_div(p.M, 128) * p.splits;
+  MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
+  constexpr int va = 16 / static_cast<int>(sizeof(TA));
+  const bool vec = reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && p.lda % va == 0 &&
+                   p.kchunk % va == 0;
+  if (vec)
+    hipLaunchKernelGGL((tall_kernel<TA, LP, true>), dim3(static_cast<unsigned>(nblocks)),
+                       dim3(256), 0, st, p);
+  else
+    hipLaunchKernelGGL((tall_kernel<TA, LP, false>), dim3(static_cast<unsigned>(nblocks)),
+                       dim3(256), 0, st, p);
+  MCNLA_LAUNCHED();
+}
+
 // Deterministic split-K reduction: C = alpha * sum_s partial[s] + beta * C.  With sym_bm > 0
 // only the lower-triangle tiles of a sym_bm x sym_bn tiling were computed (tile column
 // j / sym_bn < (i / sym_bm + 1) * sym_bm / sym_bn); the other entries are mirrored.
@@ -502,8 +636,19 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
                    p.shape == GemmShape::kSquare && p.M >= 128 && p.N >= 128 &&
                    even16(p.A, p.lda, p.stride_a) &&
                    even16(p.B, p.ldb, p.stride_b);
+  const bool tall = !p.a_mcontig && p.b_ncontig && !p.b_f32 && p.N <= 64 && p.batch == 1 &&
+                    !p.lower_only && p.M >= 128 &&
+                    (p.shape == GemmShape::kTallN32 || p.shape == GemmShape::kTallN64);
   if (p.ev_begin) MCNLA_HIP(hipEventRecord(p.ev_begin, st));
-  if (dma) {
+  if (tall) {
+    if (p.a_f32) {
+      if (p.N <= 32) launch_tall<float, 32>(p, st);
+      else launch_tall<float, 64>(p, st);
+    } else {
+      if (p.N <= 32) launch_tall<double, 32>(p, st);
+      else launch_tall<double, 64>(p, st);
+    }
+  } else if (dma) {
     // 256 x 128 tiles of 8 waves (64 x 64 each); a lower-triangle schedule takes the tiles that
     // touch the diagonal (2 (tm + 1) of them in tile row tm).
     const int64_t tm = ceil_div(p.M, 256);
