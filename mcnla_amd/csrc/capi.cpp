@@ -651,10 +651,19 @@ struct mcnla_solver_s {
   mcnla::GramCore gram;
   mcnla::FormCore form;
   mcnla::DevBuf Y, G, W, w, V, T, Bs, tmp, ws, coef, M2, Qbar, bad;
+  mcnla::DevBuf L0, T0, W0;  // first block's Cholesky factor, eigen transform, and rotation
   mcnla::Timer timer;
+  // second stream for the first block's eigen-decomposition, which runs beside the integrator
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int64_t iteration = 0;
   bool computed = false;
   bool converged = false;
+  ~mcnla_solver_s() {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
+  }
 };
 
 namespace mcnla {
@@ -686,7 +695,7 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
   s->Y.ensure(m * nl * sizeof(double));
   s->G.ensure(nl * nl * sizeof(double));
   s->Bs.ensure(nl * nl * sizeof(double));
-  s->tmp.ensure(nl * nl * sizeof(double));
+  s->tmp.ensure(nl * l * sizeof(double));
   s->W.ensure(N * l * l * sizeof(double));
   s->w.ensure(N * l * sizeof(double));
   s->V.ensure(N * l * l * sizeof(double));
@@ -696,6 +705,9 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
   s->M2.ensure(nl * l * sizeof(double));
   s->Qbar.ensure(m * l * sizeof(double));
   s->bad.ensure(sizeof(int));
+  s->L0.ensure(l * l * sizeof(double));
+  s->T0.ensure(l * l * sizeof(double));
+  s->W0.ensure(l * l * sizeof(double));
   s->sketch.prepare(*params, cfg->precision);
   s->gram.prepare(false, m, nl);
   s->form.prepare(*params);
@@ -1151,21 +1163,75 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
                        c.sketcher != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN, st);
     s->sketch.project(p, c.precision, A, lda, a_dtype, a_order, s->Y.p, st, true);
     s->timer.mark(st);
-    // 2. orthogonalize through the Gram matrix: G = Y'Y, T_i from eig(G_ii), Bs = T' G T
+    // 2. orthogonalize through the Gram matrix: G = Y'Y, T_i from G_ii, Bs = T' G T.  The
+    // reference's T_i = V_i S_i^{-1/2} (eig of G_ii) needs a Jacobi eigen-solve per block; for
+    // the KN and WY integrators, which are equivariant in the basis of the first sketch (Qbar
+    // turns into Qbar R when Q_0 turns into Q_0 R) and depend on the other sketches only through
+    // their spans, T_i = L_i^{-T} (Cholesky, G_ii = L_i L_i') gives the same subspaces, and the
+    // reference's basis is restored at the end by W_0 = L_0' V_0 S_0^{-1/2} = Q_0' Q_0^ref, with
+    // the one eigen-solve it needs running on a second stream beside the integrator.  The
+    // combining integrators depend on every basis and keep the eigen transforms.
     s->gram.run(s->Y.p, false, m, nl, s->G.as<double>(), st);
     gram_diag_blocks(s->G.as<double>(), nl, s->W.as<double>(), N, l, st);
     MCNLA_HIP(hipMemsetAsync(s->bad.p, 0, sizeof(int), st));
-    batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), N, l, st);
-    orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T.as<double>(), N, l, s->bad.as<int>(),
-                    st);
-    congruence_blockdiag(s->G.as<double>(), s->T.as<double>(), s->Bs.as<double>(),
-                         s->tmp.as<double>(), N, l, st);
+    const bool rotate = !is_combining(c.integrator);
+    if (rotate) {
+      if (!s->side) {
+        MCNLA_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+        MCNLA_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+        MCNLA_HIP(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+      }
+      MCNLA_HIP(hipEventRecord(s->ev_fork, st));
+      MCNLA_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+      batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), 1, l, s->side);
+      orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T0.as<double>(), 1, l,
+                      s->bad.as<int>(), s->side);
+      MCNLA_HIP(hipEventRecord(s->ev_join, s->side));
+      cholesky_transforms(s->W.as<double>(), N, l, s->T.as<double>(), s->L0.as<double>(),
+                          s->bad.as<int>(), st);
+    } else {
+      batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), N, l, st);
+      orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T.as<double>(), N, l,
+                      s->bad.as<int>(), st);
+    }
+    congruence_blockdiag(s->G.as<double>(), s->T.as<double>(), s->Bs.as<double>(), N, l, st);
     s->timer.mark(st);
     // 3. integrate on Bs, then Qbar = Y * (T * coef) in one pass over Y
     bool conv = false, ok = true;
     s->iteration = run_integration(c.integrator, s->Bs.as<double>(), N, l, c.maxiter, c.tol,
                                    s->coef.as<double>(), s->ws, st, &conv, &ok);
     s->converged = conv;
+    const double* cf = s->coef.as<double>();
+    if (rotate) {
+      MCNLA_HIP(hipStreamWaitEvent(st, s->ev_join, 0));
+      GemmF64Args wr;  // W_0 = L_0' T_0^ref
+      wr.M = l;
+      wr.N = l;
+      wr.K = l;
+      wr.A = s->L0.p;
+      wr.lda = l;
+      wr.a_mcontig = true;
+      wr.B = s->T0.p;
+      wr.ldb = l;
+      wr.C = s->W0.as<double>();
+      wr.ldc = l;
+      wr.shape = GemmShape::kSmall;
+      gemm_f64(wr, st);
+      GemmF64Args cr;  // coef W_0 (Nl x l) into the scratch
+      cr.M = nl;
+      cr.N = l;
+      cr.K = l;
+      cr.A = s->coef.p;
+      cr.lda = l;
+      cr.a_mcontig = false;
+      cr.B = s->W0.p;
+      cr.ldb = l;
+      cr.C = s->tmp.as<double>();
+      cr.ldc = l;
+      cr.shape = tall_shape(l);
+      gemm_f64(cr, st);
+      cf = s->tmp.as<double>();
+    }
     GemmF64Args tm;  // M2_i = T_i * coef_i  (batched l x l)
     tm.M = l;
     tm.N = l;
@@ -1173,7 +1239,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     tm.A = s->T.p;
     tm.lda = l;
     tm.a_mcontig = false;
-    tm.B = s->coef.p;
+    tm.B = cf;
     tm.ldb = l;
     tm.C = s->M2.as<double>();
     tm.ldc = l;

@@ -531,7 +531,8 @@ void launch_tall(const GemmF64Args& p, hipStream_t st) {
 
 // Deterministic split-K reduction: C = alpha * sum_s partial[s] + beta * C.  With sym_bm > 0
 // only the lower-triangle tiles of a sym_bm x sym_bn tiling were computed (tile column
-// j / sym_bn < (i / sym_bm + 1) * sym_bm / sym_bn); the other entries are mirrored.
+// j / sym_bn < (i / sym_bm + 1) * sym_bm / sym_bn): each computed entry is summed where it lies
+// (coalesced reads) and also written to its mirror when the mirror was not computed.
 __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t splits,
                                      int64_t batch, int64_t M, int64_t N, double alpha,
                                      double beta, double* __restrict__ C, int64_t ldc,
@@ -540,20 +541,23 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t
   if (gate != nullptr && *gate < gate_t) return;
   const int64_t per = M * N;
   const int64_t total = per * batch;
+  auto computed = [&](int64_t i, int64_t j) {
+    return sym_bm <= 0 || (j / sym_bn) < (i / sym_bm + 1) * (sym_bm / sym_bn);
+  };
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = e / per, ij = e % per;
     const int64_t i = ij / N, j = ij % N;
-    int64_t si = i, sj = j;
-    if (sym_bm > 0 && (j / sym_bn) >= (i / sym_bm + 1) * (sym_bm / sym_bn)) {
-      si = j;
-      sj = i;
-    }
-    const double* pb = partial + b * splits * per;
+    if (!computed(i, j)) continue;
+    const double* pb = partial + b * splits * per + ij;
     double s = 0.0;
-    for (int64_t q = 0; q < splits; ++q) s += pb[q * per + si * N + sj];
+    for (int64_t q = 0; q < splits; ++q) s += pb[q * per];
     double* c = C + b * stride_c + i * ldc + j;
     *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
+    if (!computed(j, i)) {
+      double* cm = C + b * stride_c + j * ldc + i;
+      *cm = (beta == 0.0) ? alpha * s : alpha * s + beta * *cm;
+    }
   }
 }
 

@@ -88,9 +88,14 @@ void orth_transforms(const double* w, const double* V, double* Tst, int64_t N, i
 void gram_diag_blocks(const double* G, int64_t ldg, double* W, int64_t N, int l, hipStream_t st);
 
 // Bs = T' G T with T = blockdiag(T_0..T_{N-1}) (Tst as above); Bs and G are Nl x Nl row-major.
-// tmp: Nl x Nl scratch.
-void congruence_blockdiag(const double* G, const double* Tst, double* Bs, double* tmp, int64_t N,
-                          int l, hipStream_t st);
+void congruence_blockdiag(const double* G, const double* Tst, double* Bs, int64_t N, int l,
+                          hipStream_t st);
+
+// T_i = L_i^{-T} with W_i = L_i L_i' (Cholesky) for the N blocks W (N x l x l): Y_i T_i has
+// orthonormal columns when W_i = Y_i' Y_i.  L_0 goes to L0 (l x l, lower; may be null).
+// Non-positive pivots are flagged in *bad (atomic add).
+void cholesky_transforms(const double* W, int64_t N, int l, double* Tst, double* L0, int* bad,
+                         hipStream_t st);
 
 // In-place per-sketch apply of the collection: Q_i = Y_i T_i  (Y row-major m x Nl, FP64 or FP32).
 void apply_blockdiag(void* Y, bool y_f32, int64_t m, int64_t N, int l, const double* Tst,

@@ -59,42 +59,105 @@ __global__ void gram_diag_blocks_kernel(const double* __restrict__ G, int64_t ld
   }
 }
 
-// H = G * blockdiag(T): H[r][c] = sum_e G[r][blk(c)*l + e] * T_blk(c)[e][c % l]
-__global__ void right_blockdiag_kernel(const double* __restrict__ G, const double* __restrict__ T,
-                                       double* __restrict__ H, int64_t nl, int l) {
-  const int64_t total = nl * nl;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e / nl, c = e % nl;
-    const int64_t blk = c / l;
-    const int cc = c % l;
-    const double* g = G + r * nl + blk * l;
-    const double* t = T + blk * l * l + cc;
+// Bs = blockdiag(T)' G blockdiag(T), block (I, J) = T_I' G_IJ T_J for I >= J: one workgroup per
+// block pair, G_IJ, T_I, T_J and H = G_IJ T_J in LDS (dynamic, 4 l (l + 1) doubles).  Each sum
+// runs over q in ascending order.  Bs is symmetric in exact arithmetic; the entries with global
+// row >= column are computed and written to both halves, so that it is exactly symmetric (its
+// rows can then stand in for its columns).
+__global__ __launch_bounds__(256) void congruence_kernel(const double* __restrict__ G,
+                                                        const double* __restrict__ T,
+                                                        double* __restrict__ Bs, int64_t N,
+                                                        int l) {
+  extern __shared__ double smem[];
+  const int ld = l + 1;
+  double* sG = smem;
+  double* sTi = sG + l * ld;
+  double* sTj = sTi + l * ld;
+  double* sH = sTj + l * ld;
+  const int64_t nl = N * l;
+  // block pair p -> (I, J), I >= J, p = I (I + 1) / 2 + J
+  const int64_t p = blockIdx.x;
+  int64_t I = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(p) + 1.0) - 1.0) / 2.0);
+  while (I * (I + 1) / 2 > p) --I;
+  while ((I + 1) * (I + 2) / 2 <= p) ++I;
+  const int64_t J = p - I * (I + 1) / 2;
+  const int ll = l * l;
+  for (int e = threadIdx.x; e < ll; e += blockDim.x) {
+    const int a = e / l, b = e % l;
+    sG[a * ld + b] = G[(I * l + a) * nl + J * l + b];
+    sTi[a * ld + b] = T[I * ll + e];
+    sTj[a * ld + b] = T[J * ll + e];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ll; e += blockDim.x) {
+    const int a = e / l, c = e % l;
     double acc = 0.0;
-    for (int q = 0; q < l; ++q) acc += g[q] * t[q * l];
-    H[e] = acc;
+    for (int q = 0; q < l; ++q) acc += sG[a * ld + q] * sTj[q * ld + c];
+    sH[a * ld + c] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ll; e += blockDim.x) {
+    const int r = e / l, c = e % l;
+    if (I == J && r < c) continue;
+    double acc = 0.0;
+    for (int q = 0; q < l; ++q) acc += sTi[q * ld + r] * sH[q * ld + c];
+    Bs[(I * l + r) * nl + J * l + c] = acc;
+    Bs[(J * l + c) * nl + I * l + r] = acc;
   }
 }
 
-// Bs = blockdiag(T)' * H: Bs[r][c] = sum_e T_blk(r)[e][r % l] * H[blk(r)*l + e][c].  Bs is
-// symmetric in exact arithmetic; the entry with r >= c is computed and written to both halves so
-// that it is exactly symmetric (its rows can then stand in for its columns).
-__global__ void left_blockdiag_t_kernel(const double* __restrict__ H, const double* __restrict__ T,
-                                        double* __restrict__ Bs, int64_t nl, int l) {
-  const int64_t total = nl * nl;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e / nl, c = e % nl;
-    if (r < c) continue;
-    const int64_t blk = r / l;
-    const int rr = r % l;
-    const double* h = H + blk * l * nl + c;
-    const double* t = T + blk * l * l + rr;
-    double acc = 0.0;
-    for (int q = 0; q < l; ++q) acc += t[q * l] * h[q * nl];
-    Bs[e] = acc;
-    Bs[c * nl + r] = acc;
+// T_i = L_i^{-T} from the Cholesky factorization W_i = L_i L_i' (one workgroup per block), so
+// that Y_i T_i has orthonormal columns when W_i = Y_i' Y_i.  L_0 is also written to L0 (lower
+// triangle, zeros above).  A pivot that is not positive flags *bad.
+__global__ __launch_bounds__(256) void cholesky_transforms_kernel(const double* __restrict__ W,
+                                                                 double* __restrict__ T,
+                                                                 double* __restrict__ L0, int l,
+                                                                 int* bad) {
+  __shared__ double a[kLds];
+  __shared__ double x[kLds];
+  const int ld = lds_ld(l);
+  const int64_t b = blockIdx.x;
+  const double* Wb = W + b * l * l;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < l * l; e += blockDim.x) {
+    const int i = e / l, j = e % l;
+    a[i * ld + j] = (i >= j) ? Wb[i * l + j] : 0.0;
+    x[i * ld + j] = 0.0;
   }
+  __syncthreads();
+  bool fail = false;
+  for (int j = 0; j < l; ++j) {
+    const double d = a[j * ld + j];
+    const double piv = (d > 0.0) ? sqrt(d) : 1.0;
+    fail = fail || !(d > 0.0);
+    __syncthreads();  // every thread has read the pivot before it is overwritten
+    if (tid == 0) a[j * ld + j] = piv;
+    for (int r = j + 1 + tid; r < l; r += blockDim.x) a[r * ld + j] /= piv;
+    __syncthreads();
+    const int w = l - 1 - j;
+    for (int e = tid; e < w * w; e += blockDim.x) {
+      const int r = j + 1 + e / w, c = j + 1 + e % w;
+      if (c <= r) a[r * ld + c] -= a[r * ld + j] * a[c * ld + j];
+    }
+    __syncthreads();
+  }
+  // X = L^{-1}: thread c solves L x = e_c by forward substitution (column c of X in LDS)
+  for (int c = tid; c < l; c += blockDim.x) {
+    x[c * ld + c] = 1.0 / a[c * ld + c];
+    for (int r = c + 1; r < l; ++r) {
+      double s = 0.0;
+      for (int q = c; q < r; ++q) s += a[r * ld + q] * x[q * ld + c];
+      x[r * ld + c] = -s / a[r * ld + r];
+    }
+  }
+  __syncthreads();
+  double* Tb = T + b * l * l;
+  for (int e = tid; e < l * l; e += blockDim.x) {
+    const int i = e / l, j = e % l;
+    Tb[e] = x[j * ld + i];  // T = X'
+    if (b == 0 && L0 != nullptr) L0[e] = (i >= j) ? a[i * ld + j] : 0.0;
+  }
+  if (fail && tid == 0) atomicAdd(bad, 1);
 }
 
 // Q_i = Y_i T_i in place; one thread per (row, sketch).
@@ -209,14 +272,23 @@ void gram_diag_blocks(const double* G, int64_t ldg, double* W, int64_t N, int l,
   MCNLA_LAUNCHED();
 }
 
-void congruence_blockdiag(const double* G, const double* Tst, double* Bs, double* tmp, int64_t N,
-                          int l, hipStream_t st) {
-  const int64_t nl = N * l;
-  hipLaunchKernelGGL(right_blockdiag_kernel, dim3(grid_for(nl * nl)), dim3(256), 0, st, G, Tst, tmp,
-                     nl, l);
+void congruence_blockdiag(const double* G, const double* Tst, double* Bs, int64_t N, int l,
+                          hipStream_t st) {
+  MCNLA_REQUIRE(l >= 1 && l <= kMaxL, MCNLA_ERR_INVALID, "l > 64");
+  const int64_t pairs = N * (N + 1) / 2;
+  if (pairs == 0) return;
+  const size_t lds = static_cast<size_t>(4) * l * (l + 1) * sizeof(double);
+  hipLaunchKernelGGL(congruence_kernel, dim3(static_cast<unsigned>(pairs)), dim3(256), lds, st, G,
+                     Tst, Bs, N, l);
   MCNLA_LAUNCHED();
-  hipLaunchKernelGGL(left_blockdiag_t_kernel, dim3(grid_for(nl * nl)), dim3(256), 0, st, tmp, Tst,
-                     Bs, nl, l);
+}
+
+void cholesky_transforms(const double* W, int64_t N, int l, double* Tst, double* L0, int* bad,
+                         hipStream_t st) {
+  MCNLA_REQUIRE(l >= 1 && l <= kMaxL, MCNLA_ERR_INVALID, "l > 64");
+  if (N <= 0) return;
+  hipLaunchKernelGGL(cholesky_transforms_kernel, dim3(static_cast<unsigned>(N)), dim3(256), 0, st,
+                     W, Tst, L0, l, bad);
   MCNLA_LAUNCHED();
 }
 

@@ -129,7 +129,7 @@ def test_required_bytes_without_device():
     f64 = mc.required_bytes(m, n, k, p, N)
     mixed = mc.required_bytes(m, n, k, p, N, precision=mc.PREC_MIXED_BF16)
     # Y (m x Nl FP64), G, Bs and a scratch Nl x Nl, Omega n x Nl, Qbar and U
-    floor = 8 * (m * nl + 3 * nl * nl + n * nl + m * (k + p) + m * k)
+    floor = 8 * (m * nl + 2 * nl * nl + n * nl + m * (k + p) + m * k)
     assert floor <= f64 < floor + 2 * 8 * 32 * nl * nl
     # the mixed sketch adds BF16 panels of A (m x n) and Omega
     assert mixed - f64 >= 2 * m * n
