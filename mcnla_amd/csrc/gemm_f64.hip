@@ -264,7 +264,9 @@ __device__ __forceinline__ void wait_vmcnt() {
 template <int BM, int BN, int BK, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kernel(GemmF64Args p) {
   constexpr int NW = (BM / WM) * (BN / WN);
-  constexpr int LDA = BM + 4, LDB = BN + 4;
+  // LDS row strides of 128 bytes modulo 256: a fragment read (lanes 0-15 on k-row r, 16-31 on
+  // r + 1, 8 bytes each) then covers the 64 banks once per 32 lanes.
+  constexpr int LDA = BM + 16, LDB = BN + 16;
   constexpr int RA = BM / 128, RB = BN / 128;          // wave instructions per tile row
   constexpr int LA = BK * RA / NW, LB = BK * RB / NW;  // per wave and stage
   constexpr int L = LA + LB;
@@ -337,15 +339,18 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
                                        0, 0);
     }
   };
+  // Branch-free pipeline: the stage two ahead is issued every step (clamped to the last stage,
+  // whose rows are then read again and never used), and the step count is padded to a multiple
+  // of three with steps whose A fragments are all zero.  Without branches in the loop the
+  // compiler keeps its per-buffer wait tracking, so the only DMA waits are the explicit ones.
   auto step = [&](int64_t kt, const double* as, const double* bs, double* An, double* Bn) {
     asm volatile("" ::: "memory");
-    if (kt + 1 < ntk) wait_vmcnt<L>();  // this stage landed; the next may still be in flight
-    else wait_vmcnt<0>();
-    __builtin_amdgcn_s_barrier();       // ... for every wave; and stage kt-1 is fully read
+    wait_vmcnt<L>();               // this stage landed; the next may still be in flight
+    __builtin_amdgcn_s_barrier();  // ... for every wave; and stage kt-1 is fully read
     asm volatile("" ::: "memory");
-    if (kt + 2 < ntk) issue(kt + 2, An, Bn);
+    issue(kt + 2 < ntk ? kt + 2 : ntk - 1, An, Bn);
     const int64_t left = kend - (kbeg + kt * BK);
-    const int kvalid = left < BK ? static_cast<int>(left) : BK;
+    const int kvalid = left < BK ? static_cast<int>(left) : BK;  // <= 0 past the last stage
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int krow = kk + lane / 16;
@@ -364,12 +369,16 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
   };
-  if (ntk > 0) issue(0, A0, B0);
-  if (ntk > 1) issue(1, A1, B1);
-  for (int64_t kt = 0; kt < ntk; kt += 3) {
-    step(kt, A0, B0, A2, B2);
-    if (kt + 1 < ntk) step(kt + 1, A1, B1, A0, B0);
-    if (kt + 2 < ntk) step(kt + 2, A2, B2, A1, B1);
+  if (ntk > 0) {
+    issue(0, A0, B0);
+    issue(ntk > 1 ? 1 : 0, A1, B1);
+    const int64_t nsteps = ceil_div(ntk, 3) * 3;
+    for (int64_t kt = 0; kt < nsteps; kt += 3) {
+      step(kt, A0, B0, A2, B2);
+      step(kt + 1, A1, B1, A0, B0);
+      step(kt + 2, A2, B2, A1, B1);
+    }
+    wait_vmcnt<0>();  // no DMA may still target LDS when the workgroup ends
   }
 
   double* __restrict__ C = p.partial ? p.partial + (bat * p.splits + split) * p.M * p.N
