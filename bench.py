@@ -213,6 +213,48 @@ def serve_bench(mc, args):
                         "1e-4, SVD former, stream seed 3" % (SERVE_BASES, SERVE_M, SERVE_N, SERVE_R)}
 
 
+def c4_bench(mc, args):
+    """The tall row-block workload with the Wen-Yin integrator: A 500000 x 2000 FP32 row-major
+    with sigma_i = 1/(1+i)^2, k=50, p=14, N=128 (Nl = 8192 > n).  The solver's automatic route
+    forms G = Omega'(A'A)Omega instead of the 500000 x 8192 collection; one solve on the sketch
+    route (FP64 sketch and Gram) is the reference for the sigma error."""
+    m, n, k, p, N = 500000, 2000, 50, 14, 128
+    V, _ = np.linalg.qr(np.random.default_rng(41).standard_normal((n, n)))
+    dU = mc.gaussian((m, n), seed=40, scale=1.0 / np.sqrt(m))  # columns orthonormal to ~sqrt(n/m)
+    sig = 1.0 / (1.0 + np.arange(n)) ** 2
+    A = mc.synth_lowrank(m, n, dU, V, sig, 0.0, seed=1, dtype=np.float32, order="C")
+    dU.free()
+    log("c4: A %d x %d FP32 row-major resident (sigma_i = 1/(1+i)^2)" % (m, n))
+    P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+    kw = dict(sketcher=mc.RowBlockGaussianProjectionSketcher,
+              integrator=mc.RowBlockWenYinIntegrator, former=mc.RowBlockGramianFormer, seed=2)
+    s = mc.Solver(P, precision=mc.PREC_MIXED_BF16, **kw)
+    dt, kern, stages = timed_solves(mc, s, A, args.c4_steps, 1)
+    S = s.vectorS()
+    it = s.iteration()
+    log("c4: %.1f ms/solve (auto route), %d WY iterations" % (dt / args.c4_steps * 1e3, it))
+    ref = mc.Solver(P, route=mc.ROUTE_SKETCH, **kw)
+    dref, _, sref = timed_solves(mc, ref, A, 1, 0)
+    Sr = ref.vectorS()
+    log("c4: sketch route (FP64 sketch + Gram of Y) %.1f ms/solve, %d iterations"
+        % (dref * 1e3, ref.iteration()))
+    out = {"workload": "iSVD 500000x2000 FP32 row-major, sigma_i=1/(1+i)^2, k=50 p=14 N=128 "
+                       "(Nl=8192), row-block Wen-Yin tol 1e-3, row-block Gramian former",
+           "route": "gram_of_a (auto)", "solves_per_s": round(args.c4_steps / dt, 3),
+           "ms_per_solve": round(dt / args.c4_steps * 1e3, 3), "steps": args.c4_steps,
+           "iterations": it, "stage_ms": [round(x * 1e3, 3) for x in stages.mean(axis=0)],
+           "sketch_route_ms_per_solve": round(dref * 1e3, 3),
+           "sketch_route_stage_ms": [round(x * 1e3, 3) for x in sref.mean(axis=0)],
+           "sketch_route_iterations": ref.iteration(),
+           "sigma_rel_err_vs_sketch_route": float(np.max(np.abs(S - Sr) / Sr)),
+           "sigma_vs_generator_max_rel": float(np.max(np.abs(S - sig[:k]) / sig[:k])),
+           "data": "A = U diag(1/(1+i)^2) V', V Haar 2000x2000, U = N(0,1)/sqrt(m) (columns "
+                   "orthonormal to ~6%), Omega Philox seed 2"}
+    del s, ref
+    A.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +276,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=90.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0)
+    ap.add_argument("--c4-steps", type=int, default=3)
+    ap.add_argument("--no-c4", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the iSVD path here is built for one MI355X "
@@ -346,8 +390,10 @@ def main():
         "roofline": {k: v for k, v in kernels[dominant].items() if k != "ms"},
     }
     iters64 = s64.iteration()
+    del s64, smx, A64, A32  # the other workloads' matrices and workspaces take their place
+    if not args.no_c4:
+        out["c4"] = c4_bench(mc, args)
     if not args.no_serve:
-        del s64, smx, A64, A32  # the serving stream's bases and arena take their place
         out["serve"] = serve_bench(mc, args)
     if not args.no_cpu:
         log("bench: cpu baseline (oracle, %d-row sample)" % args.cpu_rows)

@@ -201,7 +201,15 @@ typedef struct {
   uint64_t seed;
   int omega_mode;      /* mcnla_omega_mode */
   int omega_threads;   /* REFERENCE mode: OpenMP thread count of the stream to reproduce */
+  int route;           /* mcnla_route: how G = Y'Y and Qbar are formed (results agree to rounding) */
 } mcnla_solver_config;
+
+/* Two exact formulations of the fused solve.  SKETCH: Y = A Omega, G = Y'Y, Qbar = Y (T M), the
+ * former reads A again (m Nl^2 FP64 MFMA work for G).  GRAM_OF_A: H = A'A, G = Omega' H Omega,
+ * Qbar = A (Omega T M) and Z = A'Qbar = H (Omega T M), so Y is never formed (m n^2 FP64 MFMA work;
+ * the better choice when n is small against Nl, e.g. 500000 x 2000 with Nl = 8192).  AUTO picks
+ * the one with less work for the shape and precision. */
+typedef enum { MCNLA_ROUTE_AUTO = 0, MCNLA_ROUTE_SKETCH = 1, MCNLA_ROUTE_GRAM_OF_A = 2 } mcnla_route;
 
 /* Fills `cfg` with the defaults of the reference's isvd_dense_cb driver as used on this path:
  * col-block Gaussian sketcher, Gramian orthogonalizer, row-block KN (maxiter 256, tol 1e-4),
@@ -313,6 +321,11 @@ mcnla_status mcnla_server_reset_stats(mcnla_server s);
 mcnla_status mcnla_synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
                                  const double* sigma, double noise, uint64_t seed, void* A,
                                  int64_t lda, int a_dtype, int a_order, void* stream);
+/* out[e] = scale * g_e with g the Philox N(0,1) stream of `seed` (device buffer of `count`
+ * doubles): random factors for the synthetic matrices.  No reference counterpart (the reference
+ * demo draws its factors on the host, demo/demo.cpp:250-293). */
+mcnla_status mcnla_fill_gaussian(double* out, int64_t count, uint64_t seed, double scale,
+                                 void* stream);
 
 #ifdef __cplusplus
 }

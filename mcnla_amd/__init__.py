@@ -16,7 +16,7 @@ from .isvd import (
     RowBlockExtrinsicMeanIntegrator, RowBlockGramianFormer, RowBlockGramianKolmogorovNagumoIntegrator,
     RowBlockGramianOrthogonalizer, RowBlockGramianWenYinIntegrator,
     RowBlockKolmogorovNagumoIntegrator, RowBlockReductionIntegrator, RowBlockSymmetricFormer,
-    RowBlockWenYinIntegrator, Solver, SvdFormer, synth_lowrank,
+    RowBlockWenYinIntegrator, Solver, SvdFormer, gaussian, synth_lowrank,
 )
 from .serve import Served, Server, required_bytes
 
@@ -25,5 +25,8 @@ PREC_MIXED_BF16 = _lib.PREC_MIXED_BF16
 PREC_MIXED_BF16X3 = _lib.PREC_MIXED_BF16X3
 OMEGA_PHILOX = _lib.OMEGA_PHILOX
 OMEGA_REFERENCE = _lib.OMEGA_REFERENCE
+ROUTE_AUTO = _lib.ROUTE_AUTO
+ROUTE_SKETCH = _lib.ROUTE_SKETCH
+ROUTE_GRAM_OF_A = _lib.ROUTE_GRAM_OF_A
 
 __version__ = "0.1.0"

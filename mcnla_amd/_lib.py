@@ -19,6 +19,7 @@ PREC_F64, PREC_MIXED_BF16, PREC_MIXED_BF16X3 = 0, 1, 2
 # kinds
 SKETCHER_COL_BLOCK_GAUSSIAN, SKETCHER_ROW_BLOCK_GAUSSIAN, SKETCHER_GAUSSIAN = 0, 1, 2
 OMEGA_PHILOX, OMEGA_REFERENCE = 0, 1
+ROUTE_AUTO, ROUTE_SKETCH, ROUTE_GRAM_OF_A = 0, 1, 2
 ORTH_GRAMIAN, ORTH_ROW_BLOCK_GRAMIAN = 0, 1
 (INT_KN, INT_ROW_BLOCK_KN, INT_ROW_BLOCK_GRAMIAN_KN, INT_ROW_BLOCK_WY, INT_ROW_BLOCK_GRAMIAN_WY,
  INT_ROW_BLOCK_REDUCTION, INT_ROW_BLOCK_EXTRINSIC_MEAN) = range(7)
@@ -49,7 +50,7 @@ class SolverConfig(ctypes.Structure):
                 ("precision", ctypes.c_int), ("jobv", ctypes.c_int),
                 ("maxiter", ctypes.c_int64), ("tol", ctypes.c_double),
                 ("seed", ctypes.c_uint64), ("omega_mode", ctypes.c_int),
-                ("omega_threads", ctypes.c_int)]
+                ("omega_threads", ctypes.c_int), ("route", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -148,6 +149,7 @@ SIGNATURES = {
     "mcnla_server_release": (_st, [_vp, _u64]),
     "mcnla_server_get_stats": (_st, [_vp, ctypes.POINTER(ServerStats)]),
     "mcnla_server_reset_stats": (_st, [_vp]),
+    "mcnla_fill_gaussian": (_st, [_vp, _i64, _u64, _d, _vp]),
     "mcnla_synth_lowrank": (_st, [_i64, _i64, _i64, _vp, _vp, _vp, _d, _u64, _vp, _i64, _i, _i,
                                   _vp]),
 }

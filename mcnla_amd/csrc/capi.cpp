@@ -338,8 +338,9 @@ struct FormCore {
                             gemm_f64_partial_elems(l, l, ceil_div(n, w_kchunk(n)))) *
                    sizeof(double));
   }
+  // z_ready: Z = A'Qbar is already in Z (the solver's Gram-of-A route forms it without A).
   void run(const mcnla_params& p, int kind, bool jobv, const void* A, int64_t lda, int dtype,
-           int order, const double* Qbar, hipStream_t st) {
+           int order, const double* Qbar, hipStream_t st, bool z_ready = false) {
     const bool svd = kind == MCNLA_FORM_SVD;
     const bool sym = kind == MCNLA_FORM_ROW_BLOCK_SYMMETRIC;
     if (sym) jobv = false;
@@ -379,7 +380,7 @@ struct FormCore {
     w.splits = ceil_div(n, w.kchunk);
     z.partial = partial.as<double>();
     w.partial = partial.as<double>();
-    gemm_f64(z, st);
+    if (!z_ready) gemm_f64(z, st);
     gemm_f64(w, st);
     former_small(Wg.as<double>(), l, static_cast<int>(k), eigw.as<double>(), eigv.as<double>(),
                  Wk.as<double>(), S.as<double>(), st, sym);
@@ -652,6 +653,9 @@ struct mcnla_solver_s {
   mcnla::FormCore form;
   mcnla::DevBuf Y, G, W, w, V, T, Bs, tmp, ws, coef, M2, Qbar, bad;
   mcnla::DevBuf L0, T0, W0;  // first block's Cholesky factor, eigen transform, and rotation
+  // Gram-of-A route: A as row-major FP64, H = A'A, P = H Omega, X = Omega (T M), split-K scratch
+  mcnla::DevBuf A64, H, P, X, gp;
+  int route = MCNLA_ROUTE_SKETCH;
   mcnla::Timer timer;
   // second stream for the first block's eigen-decomposition, which runs beside the integrator
   hipStream_t side = nullptr;
@@ -669,6 +673,58 @@ struct mcnla_solver_s {
 namespace mcnla {
 
 namespace {
+// The route a solve of this shape takes: the one with less FP64-MFMA-equivalent work.  The sketch
+// route costs the Gram m Nl^2 plus the sketch 2 m n Nl (at about 1/16 of that on BF16 MFMA for
+// the mixed precisions); the Gram-of-A route costs A'A (m n^2), H Omega (2 n^2 Nl) and
+// Omega'(H Omega) (n Nl^2).
+int resolve_route(const mcnla_params& p, const mcnla_solver_config& c) {
+  if (c.route == MCNLA_ROUTE_SKETCH || c.route == MCNLA_ROUTE_GRAM_OF_A) return c.route;
+  const double m = static_cast<double>(p.nrow), n = static_cast<double>(p.ncol);
+  const double nl = static_cast<double>((p.rank + p.over_rank) * p.num_sketch);
+  const double sketch = 2.0 * m * n * nl / (c.precision == MCNLA_PREC_F64 ? 1.0 : 16.0);
+  const double via_y = m * nl * nl + sketch;
+  const double via_a = m * n * n + 2.0 * n * n * nl + n * nl * nl;
+  return via_a < via_y ? MCNLA_ROUTE_GRAM_OF_A : MCNLA_ROUTE_SKETCH;
+}
+
+// P = H Omega (n x Nl): H symmetric, so its rows serve as the M-contiguous operand.
+GemmF64Args plan_h_omega(int64_t n, int64_t nl) {
+  GemmF64Args g;
+  g.M = n;
+  g.N = nl;
+  g.K = n;
+  g.lda = n;
+  g.a_mcontig = true;
+  g.ldb = nl;
+  g.b_ncontig = true;
+  g.ldc = nl;
+  g.shape = GemmShape::kSquare;
+  g.splits = gemm_f64_splits_for(ceil_div(n, 256) * ceil_div(nl, 128), n, 2304);
+  g.kchunk = ceil_div(ceil_div(n, g.splits), 16) * 16;
+  g.splits = ceil_div(n, g.kchunk);
+  return g;
+}
+
+// G = Omega' P (Nl x Nl, symmetric): lower-triangle tiles.
+GemmF64Args plan_omega_p(int64_t n, int64_t nl) {
+  GemmF64Args g;
+  g.M = nl;
+  g.N = nl;
+  g.K = n;
+  g.lda = nl;
+  g.a_mcontig = true;
+  g.ldb = nl;
+  g.b_ncontig = true;
+  g.ldc = nl;
+  g.lower_only = true;
+  g.shape = GemmShape::kSquare;
+  const int64_t tm = ceil_div(nl, 256), t = ceil_div(nl, 128);
+  g.splits = gemm_f64_splits_for(nl >= 128 ? tm * (tm + 1) : t * (t + 1) / 2, n, 2304);
+  g.kchunk = ceil_div(ceil_div(n, g.splits), 16) * 16;
+  g.splits = ceil_div(n, g.kchunk);
+  return g;
+}
+
 // Validates the configuration and allocates every device buffer the solve uses (the stages'
 // initialize()), so that a solve itself allocates nothing.
 std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
@@ -690,9 +746,22 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
   s->cfg = *cfg;
   const int64_t l = params->rank + params->over_rank, N = params->num_sketch, nl = l * N;
   const int64_t m = params->nrow;
-  // Y is FP64 in every precision: the mixed sketch accumulates in FP32 and stores FP64, so that
-  // the FP64 Gram and basis products stream it at full rate.
-  s->Y.ensure(m * nl * sizeof(double));
+  s->route = resolve_route(*params, *cfg);
+  const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
+  const int64_t n = params->ncol;
+  if (!via_a) {
+    // Y is FP64 in every precision: the mixed sketch accumulates in FP32 and stores FP64, so that
+    // the FP64 Gram and basis products stream it at full rate.
+    s->Y.ensure(m * nl * sizeof(double));
+  } else {
+    s->A64.ensure(m * n * sizeof(double));
+    s->H.ensure(n * n * sizeof(double));
+    s->P.ensure(n * nl * sizeof(double));
+    s->X.ensure(n * l * sizeof(double));
+    s->gp.ensure(std::max(gemm_f64_partial_elems(n, nl, plan_h_omega(n, nl).splits),
+                          gemm_f64_partial_elems(nl, nl, plan_omega_p(n, nl).splits)) *
+                 sizeof(double));
+  }
   s->G.ensure(nl * nl * sizeof(double));
   s->Bs.ensure(nl * nl * sizeof(double));
   s->tmp.ensure(nl * l * sizeof(double));
@@ -708,8 +777,8 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
   s->L0.ensure(l * l * sizeof(double));
   s->T0.ensure(l * l * sizeof(double));
   s->W0.ensure(l * l * sizeof(double));
-  s->sketch.prepare(*params, cfg->precision);
-  s->gram.prepare(false, m, nl);
+  s->sketch.prepare(*params, via_a ? static_cast<int>(MCNLA_PREC_F64) : cfg->precision);
+  s->gram.prepare(false, m, via_a ? n : nl);
   s->form.prepare(*params);
   return s;
 }
@@ -1097,6 +1166,7 @@ void mcnla_solver_config_default(mcnla_solver_config* cfg) {
   cfg->seed = 0;
   cfg->omega_mode = MCNLA_OMEGA_PHILOX;
   cfg->omega_threads = 1;
+  cfg->route = MCNLA_ROUTE_AUTO;
 }
 
 mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_config* cfg,
@@ -1128,7 +1198,8 @@ mcnla_status mcnla_solver_workspace_bytes(mcnla_solver s, size_t* bytes) {
   return guard([&] {
     MCNLA_REQUIRE(s && bytes, MCNLA_ERR_INVALID, "null argument");
     size_t b = 0;
-    for (const DevBuf* d : {&s->Y, &s->G, &s->W, &s->w, &s->V, &s->T, &s->Bs, &s->tmp, &s->ws,
+    for (const DevBuf* d : {&s->Y, &s->A64, &s->H, &s->P, &s->X, &s->gp, &s->L0, &s->T0, &s->W0,
+                            &s->G, &s->W, &s->w, &s->V, &s->T, &s->Bs, &s->tmp, &s->ws,
                             &s->coef, &s->M2, &s->Qbar, &s->bad, &s->sketch.omega, &s->sketch.ws,
                             &s->gram.partial, &s->form.Z, &s->form.partial, &s->form.Wg,
                             &s->form.eigw, &s->form.eigv, &s->form.Wk, &s->form.S, &s->form.U,
@@ -1158,10 +1229,25 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     const int64_t N = p.num_sketch, nl = l * N, m = p.nrow;
     hipStream_t st = S(stream);
     s->timer.start(st);
-    // 1. sketch: Y = A * Omega
+    const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
+    const int64_t n = p.ncol;
+    // 1. sketch: Y = A * Omega; on the Gram-of-A route, H = A'A (A as row-major FP64) instead
     s->sketch.generate(p, c.seed, c.omega_mode, c.omega_threads,
                        c.sketcher != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN, st);
-    s->sketch.project(p, c.precision, A, lda, a_dtype, a_order, s->Y.p, st, true);
+    const double* Ar = nullptr;
+    int64_t ldar = n;
+    if (!via_a) {
+      s->sketch.project(p, c.precision, A, lda, a_dtype, a_order, s->Y.p, st, true);
+    } else {
+      if (a_dtype == MCNLA_F64 && a_order == MCNLA_ROW_MAJOR && lda == n) {
+        Ar = static_cast<const double*>(A);
+      } else {
+        to_rowmajor_f64(A, lda, a_dtype == MCNLA_F32, a_order == MCNLA_COL_MAJOR, m, n,
+                        s->A64.as<double>(), st);
+        Ar = s->A64.as<double>();
+      }
+      s->gram.run(Ar, false, m, n, s->H.as<double>(), st);
+    }
     s->timer.mark(st);
     // 2. orthogonalize through the Gram matrix: G = Y'Y, T_i from G_ii, Bs = T' G T.  The
     // reference's T_i = V_i S_i^{-1/2} (eig of G_ii) needs a Jacobi eigen-solve per block; for
@@ -1171,7 +1257,22 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     // reference's basis is restored at the end by W_0 = L_0' V_0 S_0^{-1/2} = Q_0' Q_0^ref, with
     // the one eigen-solve it needs running on a second stream beside the integrator.  The
     // combining integrators depend on every basis and keep the eigen transforms.
-    s->gram.run(s->Y.p, false, m, nl, s->G.as<double>(), st);
+    if (!via_a) {
+      s->gram.run(s->Y.p, false, m, nl, s->G.as<double>(), st);
+    } else {  // G = Omega' (H Omega)
+      GemmF64Args hp = plan_h_omega(n, nl);
+      hp.A = s->H.p;
+      hp.B = s->sketch.omega.p;
+      hp.C = s->P.as<double>();
+      hp.partial = s->gp.as<double>();
+      gemm_f64(hp, st);
+      GemmF64Args pg = plan_omega_p(n, nl);
+      pg.A = s->sketch.omega.p;
+      pg.B = s->P.p;
+      pg.C = s->G.as<double>();
+      pg.partial = s->gp.as<double>();
+      gemm_f64(pg, st);
+    }
     gram_diag_blocks(s->G.as<double>(), nl, s->W.as<double>(), N, l, st);
     MCNLA_HIP(hipMemsetAsync(s->bad.p, 0, sizeof(int), st));
     const bool rotate = !is_combining(c.integrator);
@@ -1262,11 +1363,37 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     q.C = s->Qbar.as<double>();
     q.ldc = l;
     q.shape = tall_shape(l);
+    if (via_a) {  // X = Omega (T M) (n x l), then Qbar = A X
+      GemmF64Args x = q;
+      x.M = n;
+      x.A = s->sketch.omega.p;
+      x.C = s->X.as<double>();
+      gemm_f64(x, st);
+      q.K = n;
+      q.A = Ar;
+      q.lda = ldar;
+      q.B = s->X.p;
+    }
     gemm_f64(q, st);
     s->timer.mark(st);
-    // 4. form
-    s->form.run(p, c.former, c.jobv != 0, A, lda, a_dtype, a_order,
-                s->Qbar.as<double>(), st);
+    // 4. form; on the Gram-of-A route Z = A'Qbar = H X without another pass over A
+    if (via_a) {
+      GemmF64Args z;
+      z.M = n;
+      z.N = l;
+      z.K = n;
+      z.A = s->H.p;
+      z.lda = n;
+      z.a_mcontig = false;
+      z.B = s->X.p;
+      z.ldb = l;
+      z.C = s->form.Z.as<double>();
+      z.ldc = l;
+      z.shape = tall_shape(l);
+      gemm_f64(z, st);
+    }
+    s->form.run(p, c.former, c.jobv != 0, A, lda, a_dtype, a_order, s->Qbar.as<double>(), st,
+                via_a);
     s->timer.mark(st);
     s->computed = true;
     int bad = 0;
@@ -1336,8 +1463,45 @@ mcnla_status mcnla_synth_lowrank(int64_t m, int64_t n, int64_t r, const double* 
     MCNLA_REQUIRE(m >= 1 && n >= 1 && r >= 0 && A, MCNLA_ERR_INVALID, "bad synth arguments");
     MCNLA_REQUIRE(r == 0 || (U && V && sigma), MCNLA_ERR_INVALID, "null factors");
     MCNLA_REQUIRE(lda >= (a_order == MCNLA_COL_MAJOR ? m : n), MCNLA_ERR_INVALID, "lda too small");
-    synth_lowrank(m, n, r, U, V, sigma, noise, seed, A, lda, a_dtype == MCNLA_F32,
-                  a_order == MCNLA_COL_MAJOR, S(stream));
+    hipStream_t st = S(stream);
+    if (r <= 64) {
+      synth_lowrank(m, n, r, U, V, sigma, noise, seed, A, lda, a_dtype == MCNLA_F32,
+                    a_order == MCNLA_COL_MAJOR, st);
+      return MCNLA_OK;
+    }
+    // High rank: T = U (V diag(sigma))' on the FP64 MFMA GEMM, then noise and storage order.
+    DevBuf Vs, Tm;
+    Vs.ensure(static_cast<size_t>(n) * r * sizeof(double));
+    Tm.ensure(static_cast<size_t>(m) * n * sizeof(double));
+    MCNLA_HIP(hipMemcpyAsync(Vs.p, V, static_cast<size_t>(n) * r * sizeof(double),
+                             hipMemcpyDeviceToDevice, st));
+    scale_columns(Vs.as<double>(), n, static_cast<int>(r), sigma, false, st);
+    GemmF64Args g;
+    g.M = m;
+    g.N = n;
+    g.K = r;
+    g.A = U;
+    g.lda = r;
+    g.a_mcontig = false;
+    g.B = Vs.p;
+    g.ldb = r;
+    g.b_ncontig = false;
+    g.C = Tm.as<double>();
+    g.ldc = n;
+    g.shape = GemmShape::kSquare;
+    gemm_f64(g, st);
+    synth_finish(m, n, Tm.as<double>(), noise, seed, A, lda, a_dtype == MCNLA_F32,
+                 a_order == MCNLA_COL_MAJOR, st);
+    MCNLA_HIP(hipStreamSynchronize(st));  // the temporaries are freed on return
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_fill_gaussian(double* out, int64_t count, uint64_t seed, double scale,
+                                 void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(count >= 0 && (count == 0 || out), MCNLA_ERR_INVALID, "bad fill arguments");
+    gaussian_fill(out, count, seed, scale, S(stream));
     return MCNLA_OK;
   });
 }

@@ -65,6 +65,11 @@ size_t sketch_bf16_workspace(const SketchBf16Args& a);
 // ------------------------------------------------------------------------------------------------
 // Random numbers and synthetic data (rng.hip).
 void omega_philox(double* out, int64_t count, uint64_t seed, hipStream_t st);
+// A = Tm + noise * g (Tm row-major m x n FP64), in A's dtype and storage order.
+void synth_finish(int64_t m, int64_t n, const double* Tm, double noise, uint64_t seed, void* A,
+                  int64_t lda, bool a_f32, bool a_colmajor, hipStream_t st);
+// out[e] = scale * g(e), g the Philox N(0,1) stream of `seed`.
+void gaussian_fill(double* out, int64_t count, uint64_t seed, double scale, hipStream_t st);
 void synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
                    const double* sigma, double noise, uint64_t seed, void* A, int64_t lda,
                    bool a_f32, bool a_colmajor, hipStream_t st);
@@ -96,6 +101,10 @@ void congruence_blockdiag(const double* G, const double* Tst, double* Bs, int64_
 // Non-positive pivots are flagged in *bad (atomic add).
 void cholesky_transforms(const double* W, int64_t N, int l, double* Tst, double* L0, int* bad,
                          hipStream_t st);
+
+// out (m x n, row-major FP64) = A (FP32 or FP64, row- or column-major with leading dimension lda).
+void to_rowmajor_f64(const void* A, int64_t lda, bool a_f32, bool colmajor, int64_t m, int64_t n,
+                     double* out, hipStream_t st);
 
 // In-place per-sketch apply of the collection: Q_i = Y_i T_i  (Y row-major m x Nl, FP64 or FP32).
 void apply_blockdiag(void* Y, bool y_f32, int64_t m, int64_t N, int l, const double* Tst,

@@ -82,7 +82,60 @@ __global__ void synth_kernel(int64_t m, int64_t n, int64_t r, const double* __re
   }
 }
 
+// A = T + noise * g with T row-major m x n FP64 (the low-rank part from a GEMM), written in A's
+// storage order; the noise is the same stream as in synth_kernel.
+template <typename T>
+__global__ void synth_finish_kernel(int64_t m, int64_t n, const double* __restrict__ Tm,
+                                    double noise, uint64_t seed, T* __restrict__ A, int64_t lda,
+                                    bool colmajor) {
+  const int64_t total = m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = colmajor ? e % m : e / n;
+    const int64_t j = colmajor ? e / m : e % n;
+    double acc = Tm[i * n + j];
+    if (noise != 0.0) {
+      const int64_t canon = i + j * m;
+      double z0, z1;
+      gaussian_pair(seed, static_cast<uint64_t>(canon >> 1), z0, z1);
+      acc += noise * ((canon & 1) ? z1 : z0);
+    }
+    A[colmajor ? (i + j * lda) : (i * lda + j)] = static_cast<T>(acc);
+  }
+}
+
+__global__ void scale_stream_kernel(double* __restrict__ out, int64_t count, double scale) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] *= scale;
+}
+
 }  // namespace
+
+void synth_finish(int64_t m, int64_t n, const double* Tm, double noise, uint64_t seed, void* A,
+                  int64_t lda, bool a_f32, bool a_colmajor, hipStream_t st) {
+  if (m <= 0 || n <= 0) return;
+  int64_t blocks = ceil_div(m * n, 256);
+  if (blocks > 65536) blocks = 65536;
+  if (a_f32)
+    hipLaunchKernelGGL(synth_finish_kernel<float>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                       st, m, n, Tm, noise, seed, static_cast<float*>(A), lda, a_colmajor);
+  else
+    hipLaunchKernelGGL(synth_finish_kernel<double>, dim3(static_cast<unsigned>(blocks)), dim3(256),
+                       0, st, m, n, Tm, noise, seed, static_cast<double*>(A), lda, a_colmajor);
+  MCNLA_LAUNCHED();
+}
+
+void gaussian_fill(double* out, int64_t count, uint64_t seed, double scale, hipStream_t st) {
+  omega_philox(out, count, seed, st);
+  if (scale != 1.0 && count > 0) {
+    int64_t blocks = ceil_div(count, 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(scale_stream_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
+                       out, count, scale);
+    MCNLA_LAUNCHED();
+  }
+}
 
 void omega_philox(double* out, int64_t count, uint64_t seed, hipStream_t st) {
   if (count <= 0) return;

@@ -160,6 +160,33 @@ __global__ __launch_bounds__(256) void cholesky_transforms_kernel(const double* 
   if (fail && tid == 0) atomicAdd(bad, 1);
 }
 
+// out (m x n, row-major FP64, ld n) = A (FP32 or FP64, row- or column-major, lda), through
+// 32 x 32 LDS tiles so that reads and writes are both coalesced.
+template <typename TS, bool COLMAJOR>
+__global__ __launch_bounds__(256) void to_rowmajor_f64_kernel(const TS* __restrict__ A, int64_t lda,
+                                                             int64_t m, int64_t n,
+                                                             double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int64_t r0 = static_cast<int64_t>(blockIdx.y) * 32, c0 = static_cast<int64_t>(blockIdx.x) * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (COLMAJOR) {  // column c0 + ty + 8q, rows r0 + tx contiguous
+      const int64_t r = r0 + tx, c = c0 + ty + 8 * q;
+      tile[tx][ty + 8 * q] = (r < m && c < n) ? static_cast<double>(A[r + c * lda]) : 0.0;
+    } else {
+      const int64_t r = r0 + ty + 8 * q, c = c0 + tx;
+      tile[ty + 8 * q][tx] = (r < m && c < n) ? static_cast<double>(A[r * lda + c]) : 0.0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t r = r0 + ty + 8 * q, c = c0 + tx;
+    if (r < m && c < n) out[r * n + c] = tile[ty + 8 * q][tx];
+  }
+}
+
 // Q_i = Y_i T_i in place; one thread per (row, sketch).
 template <typename T>
 __global__ void apply_blockdiag_kernel(T* __restrict__ Y, int64_t m, int64_t N, int l,
@@ -280,6 +307,29 @@ void congruence_blockdiag(const double* G, const double* Tst, double* Bs, int64_
   const size_t lds = static_cast<size_t>(4) * l * (l + 1) * sizeof(double);
   hipLaunchKernelGGL(congruence_kernel, dim3(static_cast<unsigned>(pairs)), dim3(256), lds, st, G,
                      Tst, Bs, N, l);
+  MCNLA_LAUNCHED();
+}
+
+void to_rowmajor_f64(const void* A, int64_t lda, bool a_f32, bool colmajor, int64_t m, int64_t n,
+                     double* out, hipStream_t st) {
+  if (m == 0 || n == 0) return;
+  MCNLA_REQUIRE(ceil_div(m, 32) < 65536, MCNLA_ERR_INVALID, "too many rows for the converter");
+  const dim3 grid(static_cast<unsigned>(ceil_div(n, 32)), static_cast<unsigned>(ceil_div(m, 32)));
+  if (a_f32) {
+    if (colmajor)
+      hipLaunchKernelGGL((to_rowmajor_f64_kernel<float, true>), grid, dim3(256), 0, st,
+                         static_cast<const float*>(A), lda, m, n, out);
+    else
+      hipLaunchKernelGGL((to_rowmajor_f64_kernel<float, false>), grid, dim3(256), 0, st,
+                         static_cast<const float*>(A), lda, m, n, out);
+  } else {
+    if (colmajor)
+      hipLaunchKernelGGL((to_rowmajor_f64_kernel<double, true>), grid, dim3(256), 0, st,
+                         static_cast<const double*>(A), lda, m, n, out);
+    else
+      hipLaunchKernelGGL((to_rowmajor_f64_kernel<double, false>), grid, dim3(256), 0, st,
+                         static_cast<const double*>(A), lda, m, n, out);
+  }
   MCNLA_LAUNCHED();
 }
 

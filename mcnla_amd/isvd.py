@@ -441,7 +441,7 @@ class Solver:
     def __init__(self, parameters, sketcher=ColBlockGaussianProjectionSketcher,
                  integrator=RowBlockKolmogorovNagumoIntegrator, former=SvdFormer,
                  precision=_lib.PREC_F64, max_iteration=256, tolerance=None, seed=0,
-                 omega_mode=_lib.OMEGA_PHILOX, omega_threads=1, jobv=True):
+                 omega_mode=_lib.OMEGA_PHILOX, omega_threads=1, jobv=True, route=_lib.ROUTE_AUTO):
         self.parameters = parameters
         cfg = _lib.SolverConfig()
         _lib.lib().mcnla_solver_config_default(ctypes.byref(cfg))
@@ -455,6 +455,7 @@ class Solver:
         cfg.seed = int(seed)
         cfg.omega_mode = omega_mode
         cfg.omega_threads = int(omega_threads)
+        cfg.route = int(route)
         self.config = cfg
         self.former_kind = former._kind
         p = parameters.c_params()
@@ -530,10 +531,15 @@ class Solver:
 def synth_lowrank(m, n, U, V, sigma, noise=0.0, seed=1, dtype=np.float64, order="F",
                   stream=None):
     """A = U diag(sigma) V' + noise * G on the device (G ~ N(0,1), Philox(seed)).  U (m x r) and
-    V (n x r) are host arrays; returns a DeviceArray."""
+    V (n x r) are host arrays or row-major FP64 DeviceArrays; returns a DeviceArray."""
     r = len(sigma)
-    dU = DeviceArray.from_numpy(np.ascontiguousarray(U, dtype=np.float64))
-    dV = DeviceArray.from_numpy(np.ascontiguousarray(V, dtype=np.float64))
+
+    def dev(X):
+        if isinstance(X, DeviceArray):
+            assert X.dtype == np.float64 and X.order == "C"
+            return X
+        return DeviceArray.from_numpy(np.ascontiguousarray(X, dtype=np.float64))
+    dU, dV = dev(U), dev(V)
     ds = DeviceArray.from_numpy(np.ascontiguousarray(sigma, dtype=np.float64))
     A = DeviceArray((m, n), dtype, order)
     _lib.check(_lib.lib().mcnla_synth_lowrank(m, n, r, dU.ptr, dV.ptr, ds.ptr, float(noise),
@@ -541,3 +547,12 @@ def synth_lowrank(m, n, U, V, sigma, noise=0.0, seed=1, dtype=np.float64, order=
                                               A.order_code, stream))
     _lib.check(_lib.lib().mcnla_stream_synchronize(stream))
     return A
+
+
+def gaussian(shape, seed=0, scale=1.0, stream=None):
+    """A row-major FP64 DeviceArray of scale * N(0,1) (the Philox stream of `seed`)."""
+    out = DeviceArray(shape, np.float64, "C")
+    count = int(np.prod(out.shape))
+    _lib.check(_lib.lib().mcnla_fill_gaussian(out.ptr, count, int(seed), float(scale), stream))
+    _lib.check(_lib.lib().mcnla_stream_synchronize(stream))
+    return out

@@ -159,10 +159,12 @@ class Server:
 
 
 def required_bytes(m, n, k, p, N, precision=_lib.PREC_F64,
-                   integrator=RowBlockKolmogorovNagumoIntegrator, former=SvdFormer, jobv=True):
+                   integrator=RowBlockKolmogorovNagumoIntegrator, former=SvdFormer, jobv=True,
+                   route=_lib.ROUTE_AUTO):
     """Device bytes of a solver of this shape (no device needed)."""
     cfg = _lib.SolverConfig()
     _lib.lib().mcnla_solver_config_default(ctypes.byref(cfg))
+    cfg.route = int(route)
     cfg.integrator = integrator._kind
     cfg.former = former._kind
     cfg.precision = precision

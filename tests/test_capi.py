@@ -137,3 +137,15 @@ def test_required_bytes_without_device():
     with pytest.raises(mc.McnlaError) as e:
         mc.required_bytes(10, 5, 5, 10, 8)
     assert e.value.status == _lib.ERR_INVALID
+
+
+def test_route_choice_by_shape():
+    # n small against Nl (500000 x 2000, Nl = 8192): the Gram-of-A route holds A'A and A as FP64
+    # instead of the m x Nl collection; at 100000 x 10000, Nl = 2048 the sketch route is kept
+    m, n, k, p, N = 500000, 2000, 50, 14, 128
+    auto = mc.required_bytes(m, n, k, p, N, precision=mc.PREC_MIXED_BF16)
+    via_a = mc.required_bytes(m, n, k, p, N, precision=mc.PREC_MIXED_BF16, route=mc.ROUTE_GRAM_OF_A)
+    via_y = mc.required_bytes(m, n, k, p, N, precision=mc.PREC_MIXED_BF16, route=mc.ROUTE_SKETCH)
+    assert auto == via_a and via_y > via_a + 8 * m * (k + p) * N - 8 * m * n - 16 * n * (k + p) * N
+    m, n, k, p, N = 100000, 10000, 20, 12, 64
+    assert mc.required_bytes(m, n, k, p, N) == mc.required_bytes(m, n, k, p, N, route=mc.ROUTE_SKETCH)

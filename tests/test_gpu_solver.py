@@ -229,3 +229,41 @@ def test_invalid_solve_arguments(gpu):
     with pytest.raises(mc.McnlaError) as e:
         s.solve(bad)
     assert e.value.status == mc._lib.ERR_INVALID
+
+
+@pytest.mark.parametrize("dtype,order", [(np.float64, "C"), (np.float64, "F"), (np.float32, "F"),
+                                         (np.float32, "C")])
+def test_solver_gram_of_a_route(gpu, dtype, order):
+    """n < Nl: the Gram-of-A route (H = A'A, G = Omega' H Omega, Qbar = A (Omega T M),
+    Z = H (Omega T M); Y never formed) against the oracle pipeline on the same reference Omega,
+    and against the sketch route.  Both are exact reformulations, so they agree to rounding."""
+    m, n, k, p, N = 3000, 150, 10, 10, 12  # Nl = 240 > n
+    note("solver Gram-of-A route %s %s" % (np.dtype(dtype).name, order))
+    rng = np.random.default_rng(21)
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A64 = ((U * (0.9 ** np.arange(n) + 1e-6)) @ V.T).astype(dtype).astype(np.float64)
+    l = k + p
+    om = oracle.omega_ref(SEED, n, N * l, col_block=True)
+    Q, _ = oracle.gramian_orthogonalize(oracle.sketch(A64, om), l, N)
+    Qb_o, it_o = oracle.kn_rowblock(Q, l, N, 256, 1e-4)
+    S_o, U_o, V_o = oracle.svd_former(A64, Qb_o, k)
+    P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+    dA = mc.DeviceArray.from_numpy(np.asarray(A64.astype(dtype), order=order), order=order)
+    outs = {}
+    for route in (mc.ROUTE_GRAM_OF_A, mc.ROUTE_SKETCH):
+        s = mc.Solver(P, seed=SEED, omega_mode=mc.OMEGA_REFERENCE, route=route)
+        s.solve(dA)
+        assert s.converged
+        outs[route] = (s.vectorS(), s.matrixU(), s.matrixV(), s.matrixQbar(), s.iteration())
+    S, Uu, Vv, Qb, it = outs[mc.ROUTE_GRAM_OF_A]
+    assert abs(it - it_o) <= 1
+    assert np.max(np.abs(S - S_o) / S_o) < 1e-9
+    assert np.abs(proj(Uu) - proj(U_o)).max() < 1e-8
+    assert np.abs(proj(Vv) - proj(V_o)).max() < 1e-8
+    assert np.linalg.norm(proj(Qb) - proj(Qb_o), 2) < 1e-7
+    assert np.allclose(Qb.T @ Qb, np.eye(l), atol=1e-12)
+    S2, U2, V2, Qb2, it2 = outs[mc.ROUTE_SKETCH]
+    assert it2 == it
+    assert np.max(np.abs(S - S2) / S2) < 1e-10
+    assert np.abs(proj(Uu) - proj(U2)).max() < 1e-9
