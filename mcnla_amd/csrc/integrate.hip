@@ -391,17 +391,25 @@ __global__ __launch_bounds__(kThreads) void kn_update_kernel(
 __device__ void lds_inverse(int n, double* a, int* piv, double* rowk, double* colk) {
   const int ld = n + 1;
   for (int k = 0; k < n; ++k) {
-    if (threadIdx.x == 0) {
-      int p = k;
-      double best = fabs(a[k * ld + k]);
-      for (int i = k + 1; i < n; ++i) {
+    if (threadIdx.x < kWave) {  // partial pivoting: first row of largest |a_ik|, i >= k
+      int p = n;
+      double best = -1.0;
+      for (int i = k + static_cast<int>(threadIdx.x); i < n; i += kWave) {
         const double x = fabs(a[i * ld + k]);
         if (x > best) {
           best = x;
           p = i;
         }
       }
-      piv[k] = p;
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, kWave);
+        const int op = __shfl_xor(p, off, kWave);
+        if (ob > best || (ob == best && op < p)) {
+          best = ob;
+          p = op;
+        }
+      }
+      if (threadIdx.x == 0) piv[k] = (p < n) ? p : k;
     }
     __syncthreads();
     const int p = piv[k];
@@ -454,12 +462,20 @@ __global__ __launch_bounds__(kThreads) void wy_search_kernel(int t, IntegratorSt
                                                             double* __restrict__ Fc,
                                                             double* __restrict__ Fgc) {
   if (skip(st, t)) return;
+  // The reference inverts C = [[Dc/2 - I/tau, I/2], [-Dgc/2, -Dc/2 - I/tau]] (2l x 2l) for every
+  // trial tau and takes Fgc = X12 Dc - X11, Fc = X22 Dc - X21 + I from X = C^{-1}, i.e.
+  // [Fgc; Fc - I] = C^{-1} [-I; Dc].  Eliminating the second block row of that solve gives
+  //   Fgc = (4/tau) (K + (4/tau^2) I)^{-1},  Fc = -I - (Dc - (2/tau) I) Fgc,  K = Dgc - Dc^2,
+  // one l x l inverse per trial instead of a 2l x 2l one (K is the same for every trial).
   extern __shared__ double dyn[];
-  const int n2 = 2 * l, ld = n2 + 1;
-  double* a = dyn;                     // n2 x ld
-  double* rowk = a + n2 * ld;          // n2
-  double* colk = rowk + n2;            // n2
-  double* red = colk + n2;             // kThreads / 64
+  const int ld = l + 1;
+  double* a = dyn;                     // l x ld: K + (4/tau^2) I, inverted in place
+  double* sK = a + l * ld;             // l x ld: K
+  double* fg = sK + l * ld;            // l x ld: Fgc
+  double* fc = fg + l * ld;            // l x ld: Fc
+  double* rowk = fc + l * ld;          // l
+  double* colk = rowk + l;             // l
+  double* red = colk + l;              // kThreads / 64
   int* piv = reinterpret_cast<int*>(red + kThreads / kWave);
   __shared__ double phit_s;
   __shared__ int accept_s;
@@ -467,27 +483,33 @@ __global__ __launch_bounds__(kThreads) void wy_search_kernel(int t, IntegratorSt
   const double* Dgc = D3 + l * l;
   const double* Hgg = D3 + 2 * l * l;
   const double phi = st->phi, mu = st->mu;
+  for (int e = threadIdx.x; e < l * l; e += kThreads) {
+    const int i = e / l, j = e % l;
+    double d2 = 0.0;
+    for (int q = 0; q < l; ++q) d2 += Dc[i * l + q] * Dc[q * l + j];
+    sK[i * ld + j] = Dgc[i * l + j] - d2;
+  }
+  __syncthreads();
   double tau = st->taug;
   double phit = phi;
   for (int ti = 0; ti < WyConst::taumaxiter; ++ti, tau *= WyConst::beta) {
-    for (int e = threadIdx.x; e < n2 * n2; e += kThreads) {
-      const int i = e / n2, j = e % n2;
-      double x = 0.0;
-      if (i < l && j < l) x = 0.5 * Dc[i * l + j] - (i == j ? 1.0 / tau : 0.0);
-      else if (i < l) x = (i == j - l) ? 0.5 : 0.0;
-      else if (j < l) x = -0.5 * Dgc[(i - l) * l + j];
-      else x = -0.5 * Dc[(i - l) * l + (j - l)] - (i == j ? 1.0 / tau : 0.0);
-      a[i * ld + j] = x;
+    const double c = 4.0 / (tau * tau);
+    for (int e = threadIdx.x; e < l * l; e += kThreads) {
+      const int i = e / l, j = e % l;
+      a[i * ld + j] = sK[i * ld + j] + (i == j ? c : 0.0);
     }
     __syncthreads();
-    lds_inverse(n2, a, piv, rowk, colk);
-    // C#1 := C#2 * Dc - C#1 in place (each entry reads only its own old value), then C21 += I:
-    // Fgc = C11 = a[0:l][0:l], Fc = C21 = a[l:2l][0:l].
-    for (int e = threadIdx.x; e < n2 * l; e += kThreads) {
+    lds_inverse(l, a, piv, rowk, colk);
+    for (int e = threadIdx.x; e < l * l; e += kThreads) {
+      const int i = e / l, j = e % l;
+      fg[i * ld + j] = (4.0 / tau) * a[i * ld + j];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < l * l; e += kThreads) {
       const int i = e / l, j = e % l;
       double acc = 0.0;
-      for (int q = 0; q < l; ++q) acc += a[i * ld + l + q] * Dc[q * l + j];
-      a[i * ld + j] = acc - a[i * ld + j] + ((i - l) == j ? 1.0 : 0.0);
+      for (int q = 0; q < l; ++q) acc += Dc[i * l + q] * fg[q * ld + j];
+      fc[i * ld + j] = (2.0 / tau) * fg[i * ld + j] - acc - (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
     // phit = (1/2N) ||Bc Fc + Bg Fgc||^2 = 1/2 tr(Fc'Dc Fc + 2 Fc'Dgc Fgc + Fgc'Hgg Fgc)
@@ -496,21 +518,21 @@ __global__ __launch_bounds__(kThreads) void wy_search_kernel(int t, IntegratorSt
       const int i = e / l, j = e % l;  // entry (i, j) of the three l x l products
       double x1 = 0.0, x2 = 0.0, x3 = 0.0;
       for (int q = 0; q < l; ++q) {
-        const double fc = a[(l + q) * ld + j], fg = a[q * ld + j];
-        x1 += Dc[i * l + q] * fc;
-        x2 += Dgc[i * l + q] * fg;
-        x3 += Hgg[i * l + q] * fg;
+        const double fcq = fc[q * ld + j], fgq = fg[q * ld + j];
+        x1 += Dc[i * l + q] * fcq;
+        x2 += Dgc[i * l + q] * fgq;
+        x3 += Hgg[i * l + q] * fgq;
       }
-      const double fci = a[(l + i) * ld + j], fgi = a[i * ld + j];
+      const double fci = fc[i * ld + j], fgi = fg[i * ld + j];
       part += fci * x1 + 2.0 * fci * x2 + fgi * x3;
     }
     part = wave_sum(part);
     if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = part;
     __syncthreads();
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < kThreads / kWave; ++w) s += red[w];
-      phit_s = 0.5 * s;
+      double sum = 0.0;
+      for (int w = 0; w < kThreads / kWave; ++w) sum += red[w];
+      phit_s = 0.5 * sum;
       accept_s = (phit_s >= phi + tau * WyConst::sigma * mu) ? 1 : 0;
     }
     __syncthreads();
@@ -519,8 +541,8 @@ __global__ __launch_bounds__(kThreads) void wy_search_kernel(int t, IntegratorSt
   }
   for (int e = threadIdx.x; e < l * l; e += kThreads) {
     const int i = e / l, j = e % l;
-    Fgc[e] = a[i * ld + j];
-    Fc[e] = a[(l + i) * ld + j];
+    Fgc[e] = fg[i * ld + j];
+    Fc[e] = fc[i * ld + j];
   }
   if (threadIdx.x == 0) {
     const double zeta = st->zeta;
@@ -973,8 +995,8 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
                      L.B[0], L.E[0], L.FF[0]);
   MCNLA_LAUNCHED();
 
-  const size_t wy_lds = (static_cast<size_t>(2 * l) * (2 * l + 1) + 4 * l + kThreads / kWave) *
-                            sizeof(double) + 2 * l * sizeof(int) + 64;
+  const size_t wy_lds = (static_cast<size_t>(4) * l * (l + 1) + 2 * l + kThreads / kWave) *
+                            sizeof(double) + l * sizeof(int) + 64;
   if (g.wen_yin) {
     MCNLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(wy_search_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
