@@ -228,7 +228,7 @@ def c4_bench(mc, args):
     P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
     kw = dict(sketcher=mc.RowBlockGaussianProjectionSketcher,
               integrator=mc.RowBlockWenYinIntegrator, former=mc.RowBlockGramianFormer, seed=2)
-    s = mc.Solver(P, precision=mc.PREC_MIXED_BF16, **kw)
+    s = mc.Solver(P, precision=mc.PREC_MIXED_BF16, graph=args.graph, **kw)
     dt, kern, stages = timed_solves(mc, s, A, args.c4_steps, 1)
     S = s.vectorS()
     it = s.iteration()
@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0)
     ap.add_argument("--c4-steps", type=int, default=3)
+    ap.add_argument("--graph", type=int, default=0, help="replay the integrator iterations as "
+                    "HIP graphs recorded in the warm-up solve")
     ap.add_argument("--no-c4", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -310,8 +312,8 @@ def main():
     log("bench: A resident (FP64 %.1f GB, FP32 %.1f GB)" % (A64.nbytes / 1e9, A32.nbytes / 1e9))
     P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
     prec = mc.PREC_MIXED_BF16 if args.mixed == "bf16" else mc.PREC_MIXED_BF16X3
-    s64 = mc.Solver(P, seed=2)
-    smx = mc.Solver(P, seed=2, precision=prec)
+    s64 = mc.Solver(P, seed=2, graph=args.graph)
+    smx = mc.Solver(P, seed=2, precision=prec, graph=args.graph)
 
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else max(3, args.steps // 2)
     dt64, k64, st64 = timed_solves(mc, s64, A64, fp64_steps, 1)

@@ -202,6 +202,8 @@ typedef struct {
   int omega_mode;      /* mcnla_omega_mode */
   int omega_threads;   /* REFERENCE mode: OpenMP thread count of the stream to reproduce */
   int route;           /* mcnla_route: how G = Y'Y and Qbar are formed (results agree to rounding) */
+  int graph;           /* 1: record the integrator's chunks of iterations as HIP graphs on the
+                          first solve and replay them in later solves of this plan (same results) */
 } mcnla_solver_config;
 
 /* Two exact formulations of the fused solve.  SKETCH: Y = A Omega, G = Y'Y, Qbar = Y (T M), the

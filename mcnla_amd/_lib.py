@@ -50,7 +50,8 @@ class SolverConfig(ctypes.Structure):
                 ("precision", ctypes.c_int), ("jobv", ctypes.c_int),
                 ("maxiter", ctypes.c_int64), ("tol", ctypes.c_double),
                 ("seed", ctypes.c_uint64), ("omega_mode", ctypes.c_int),
-                ("omega_threads", ctypes.c_int), ("route", ctypes.c_int)]
+                ("omega_threads", ctypes.c_int), ("route", ctypes.c_int),
+                ("graph", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):

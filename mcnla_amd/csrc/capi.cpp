@@ -528,7 +528,8 @@ size_t integration_workspace(int kind, int64_t N, int l) {
 // Runs integrator `kind` on the Gram matrix Bs (Nl x Nl) of an orthonormal collection; writes
 // the Nl x l coefficients of Qbar.  Returns the iteration count (0 for the combining kinds).
 int64_t run_integration(int kind, const double* Bs, int64_t N, int l, int64_t maxiter, double tol,
-                        double* coef, DevBuf& ws, hipStream_t st, bool* conv, bool* ok) {
+                        double* coef, DevBuf& ws, hipStream_t st, bool* conv, bool* ok,
+                        IterationGraphs* graphs = nullptr) {
   ws.ensure(integration_workspace(kind, N, l));
   if (is_combining(kind)) {
     if (kind == MCNLA_INT_ROW_BLOCK_REDUCTION)
@@ -548,6 +549,7 @@ int64_t run_integration(int kind, const double* Bs, int64_t N, int l, int64_t ma
   gi.Bs = const_cast<double*>(Bs);
   gi.ws = ws.as<char>();
   gi.ws_bytes = ws.bytes;
+  gi.graphs = graphs;
   return integrate_gram(gi, coef, st, conv, ok);
 }
 
@@ -660,6 +662,7 @@ struct mcnla_solver_s {
   // second stream for the first block's eigen-decomposition, which runs beside the integrator
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  std::unique_ptr<mcnla::IterationGraphs> graphs;  // cfg.graph: recorded iteration chunks
   int64_t iteration = 0;
   bool computed = false;
   bool converged = false;
@@ -1167,6 +1170,7 @@ void mcnla_solver_config_default(mcnla_solver_config* cfg) {
   cfg->omega_mode = MCNLA_OMEGA_PHILOX;
   cfg->omega_threads = 1;
   cfg->route = MCNLA_ROUTE_AUTO;
+  cfg->graph = 0;
 }
 
 mcnla_status mcnla_solver_create(const mcnla_params* params, const mcnla_solver_config* cfg,
@@ -1299,8 +1303,9 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     s->timer.mark(st);
     // 3. integrate on Bs, then Qbar = Y * (T * coef) in one pass over Y
     bool conv = false, ok = true;
+    if (c.graph && !s->graphs) s->graphs = std::make_unique<IterationGraphs>();
     s->iteration = run_integration(c.integrator, s->Bs.as<double>(), N, l, c.maxiter, c.tol,
-                                   s->coef.as<double>(), s->ws, st, &conv, &ok);
+                                   s->coef.as<double>(), s->ws, st, &conv, &ok, s->graphs.get());
     s->converged = conv;
     const double* cf = s->coef.as<double>();
     if (rotate) {

@@ -950,6 +950,11 @@ Layout layout(char* base, int64_t N, int l) {
 
 size_t integrator_workspace_bytes(int64_t N, int l) { return layout(nullptr, N, l).bytes; }
 
+IterationGraphs::~IterationGraphs() {
+  for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
+  if (cap) (void)hipStreamDestroy(cap);
+}
+
 int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bool* converged,
                        bool* numeric_ok) {
   MCNLA_REQUIRE(g.l >= 1 && g.l <= kMaxL, MCNLA_ERR_INVALID, "integrator needs 1 <= l <= 64");
@@ -966,6 +971,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
   const int64_t bs_kchunk = ceil_div(ceil_div(nl, L.splits), 16) * 16;
   const int64_t bs_splits = ceil_div(nl, bs_kchunk);
   const int64_t nblk_s = ceil_div(nl, kStepRows);
+  hipStream_t ls = st;  // where the iteration launches go: a capture stream while recording
   auto bs_product = [&](const double* Bin, double* Bout, int t) {
     GemmF64Args p;
     p.M = nl;
@@ -985,7 +991,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
     p.partial = L.Pbg;
     p.gate = &L.st->converged_at;
     p.gate_t = t;
-    gemm_f64(p, st);
+    gemm_f64(p, ls);
   };
   const unsigned gp = static_cast<unsigned>(L.nblk_p);
 
@@ -1018,50 +1024,82 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
   int host_conv = INT_MAX;
   while (t < g.maxiter) {
     const int64_t tend = (t + chunk < g.maxiter) ? t + chunk : g.maxiter;
-    for (; t < tend; ++t) {
-      const int c = static_cast<int>(t & 1), p = c ^ 1;
-      const int ti = static_cast<int>(t);
-      if (!g.wen_yin) {
-        bs_product(L.B[c], nullptr, ti);  // split-K partials only
-        hipLaunchKernelGGL(kn_reduce_step_kernel, dim3(static_cast<unsigned>(nblk_s)),
-                           dim3(kStepThreads), 0, st, ti, L.st, nl, l, invN, g.tol, L.Pbg, bs_splits,
-                           L.B[c], L.Bg[0], L.P, nblk_s, L.F, L.C);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(kn_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, ti,
-                           L.st, nl, l, invN, L.B[c], L.Bg[0], L.B[p], L.E[c], L.E[p], L.FF[c],
-                           L.FF[p], L.F, L.C);
-        MCNLA_LAUNCHED();
-      } else {
-        const int odd = p;  // the reference toggles is_odd before using it: odd = !(t & 1)
-        hipLaunchKernelGGL(wy_search_kernel, dim3(1), dim3(kThreads), wy_lds, st, ti, L.st, l,
-                           L.D3[c], L.F, L.C);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(wy_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, ti,
-                           L.st, nl, l, invN, L.B[c], L.Bg[c], L.B[p], L.U1, L.BsU1, L.E[c],
-                           L.E[p], L.FF[c], L.FF[p], L.F, L.C);
-        MCNLA_LAUNCHED();
-        bs_product(L.B[p], L.Bg[p], ti);
-        hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, ti, L.st, nl, l,
-                           L.B[p], L.Bg[p], L.P, 3);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll, kThreads)), dim3(kThreads),
-                           0, st, ti, L.st, L.P, L.nblk_p, 3 * ll, invN, L.D3[p]);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(wy_check_kernel, dim3(1), dim3(kThreads), 0, st, ti, L.st, l, g.tol,
-                           L.D3[p]);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(wy_terms_kernel, dim3(static_cast<unsigned>(L.nblk_t)), dim3(kThreads), 0,
-                           st, ti, L.st, nl, l, invN, odd, L.B[c], L.B[p], L.Bg[c], L.Bg[p], L.U1,
-                           L.BsU1, L.D3[p], L.E[c], L.FF[c], L.P);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll + 2, kThreads)),
-                           dim3(kThreads), 0, st, ti, L.st, L.P, L.nblk_t, 3 * ll + 2, 1.0, L.T);
-        MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(wy_tau_kernel, dim3(1), dim3(kThreads), 0, st, ti, L.st, l, odd, L.D3[c],
-                           L.D3[p], L.F, L.T);
-        MCNLA_LAUNCHED();
-      }
+    hipGraphExec_t exec = g.graphs ? g.graphs->find(t, tend) : nullptr;
+    if (exec == nullptr && g.graphs) {  // record this chunk once
+      if (!g.graphs->cap) MCNLA_HIP(hipStreamCreateWithFlags(&g.graphs->cap, hipStreamNonBlocking));
+      ls = g.graphs->cap;
+      MCNLA_HIP(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
     }
+    const int64_t t0 = t;
+    const bool recording = exec == nullptr && g.graphs != nullptr;
+    if (exec != nullptr) {
+      t = tend;
+    } else try {
+      for (; t < tend; ++t) {
+        const int c = static_cast<int>(t & 1), p = c ^ 1;
+        const int ti = static_cast<int>(t);
+        if (!g.wen_yin) {
+          bs_product(L.B[c], nullptr, ti);  // split-K partials only
+          hipLaunchKernelGGL(kn_reduce_step_kernel, dim3(static_cast<unsigned>(nblk_s)),
+                             dim3(kStepThreads), 0, ls, ti, L.st, nl, l, invN, g.tol, L.Pbg, bs_splits,
+                             L.B[c], L.Bg[0], L.P, nblk_s, L.F, L.C);
+          MCNLA_LAUNCHED();
+          hipLaunchKernelGGL(kn_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
+                             L.st, nl, l, invN, L.B[c], L.Bg[0], L.B[p], L.E[c], L.E[p], L.FF[c],
+                             L.FF[p], L.F, L.C);
+          MCNLA_LAUNCHED();
+        } else {
+          const int odd = p;  // the reference toggles is_odd before using it: odd = !(t & 1)
+          hipLaunchKernelGGL(wy_search_kernel, dim3(1), dim3(kThreads), wy_lds, ls, ti, L.st, l,
+                             L.D3[c], L.F, L.C);
+          MCNLA_LAUNCHED();
+          hipLaunchKernelGGL(wy_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
+                             L.st, nl, l, invN, L.B[c], L.Bg[c], L.B[p], L.U1, L.BsU1, L.E[c],
+                             L.E[p], L.FF[c], L.FF[p], L.F, L.C);
+          MCNLA_LAUNCHED();
+          bs_product(L.B[p], L.Bg[p], ti);
+          hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, ls, ti, L.st, nl, l,
+                             L.B[p], L.Bg[p], L.P, 3);
+          MCNLA_LAUNCHED();
+          hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll, kThreads)), dim3(kThreads),
+                             0, ls, ti, L.st, L.P, L.nblk_p, 3 * ll, invN, L.D3[p]);
+          MCNLA_LAUNCHED();
+          hipLaunchKernelGGL(wy_check_kernel, dim3(1), dim3(kThreads), 0, ls, ti, L.st, l, g.tol,
+                             L.D3[p]);
+          MCNLA_LAUNCHED();
+          hipLaunchKernelGGL(wy_terms_kernel, dim3(static_cast<unsigned>(L.nblk_t)), dim3(kThreads), 0,
+                             ls, ti, L.st, nl, l, invN, odd, L.B[c], L.B[p], L.Bg[c], L.Bg[p], L.U1,
+                             L.BsU1, L.D3[p], L.E[c], L.FF[c], L.P);
+          MCNLA_LAUNCHED();
+          hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll + 2, kThreads)),
+                             dim3(kThreads), 0, ls, ti, L.st, L.P, L.nblk_t, 3 * ll + 2, 1.0, L.T);
+          MCNLA_LAUNCHED();
+          hipLaunchKernelGGL(wy_tau_kernel, dim3(1), dim3(kThreads), 0, ls, ti, L.st, l, odd, L.D3[c],
+                             L.D3[p], L.F, L.T);
+          MCNLA_LAUNCHED();
+        }
+      }
+    } catch (...) {
+      if (recording) {  // leave the capture stream usable
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(ls, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        ls = st;
+      }
+      throw;
+    }
+    if (recording) {
+      hipGraph_t graph = nullptr;
+      MCNLA_HIP(hipStreamEndCapture(ls, &graph));
+      ls = st;
+      hipGraphExec_t made = nullptr;
+      const hipError_t e = hipGraphInstantiate(&made, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      MCNLA_HIP(e);
+      g.graphs->execs.push_back({{t0, tend}, made});
+      exec = made;
+    }
+    if (exec != nullptr) MCNLA_HIP(hipGraphLaunch(exec, st));
     MCNLA_HIP(hipMemcpyAsync(&host_conv, &L.st->converged_at, sizeof(int), hipMemcpyDeviceToHost, st));
     MCNLA_HIP(hipStreamSynchronize(st));
     if (host_conv != INT_MAX) break;

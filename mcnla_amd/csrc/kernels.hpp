@@ -1,6 +1,9 @@
 // Host-side launchers of the HIP kernels of libmcnla_amd.
 #pragma once
 
+#include <utility>
+#include <vector>
+
 #include "common.hpp"
 
 namespace mcnla {
@@ -114,6 +117,24 @@ void apply_blockdiag(void* Y, bool y_f32, int64_t m, int64_t N, int l, const dou
 // Integrator state machine on the Gram matrix Bs (integrate.hip).
 struct IntegratorState;  // device-side scalars (flags, counters, tau, phi, ...)
 
+// HIP graphs of the integrator's chunks of iterations, recorded once and replayed by later solves
+// of the same plan.  A chunk [t0, t1) is the same launches with the same arguments every time (the
+// iteration index is a kernel argument, the buffers belong to the plan), so its graph can be
+// replayed as is.  Owned by the caller; the capture stream is created on first use.
+struct IterationGraphs {
+  std::vector<std::pair<std::pair<int64_t, int64_t>, hipGraphExec_t>> execs;
+  hipStream_t cap = nullptr;
+  IterationGraphs() = default;
+  IterationGraphs(const IterationGraphs&) = delete;
+  IterationGraphs& operator=(const IterationGraphs&) = delete;
+  ~IterationGraphs();
+  hipGraphExec_t find(int64_t t0, int64_t t1) const {
+    for (const auto& e : execs)
+      if (e.first.first == t0 && e.first.second == t1) return e.second;
+    return nullptr;
+  }
+};
+
 struct GramIntegrator {
   int64_t N = 0;
   int l = 0;
@@ -124,6 +145,7 @@ struct GramIntegrator {
   double* Bs = nullptr;   // Nl x Nl
   char* ws = nullptr;
   size_t ws_bytes = 0;
+  IterationGraphs* graphs = nullptr;  // record/replay the iteration chunks as HIP graphs
 };
 size_t integrator_workspace_bytes(int64_t N, int l);
 // Runs the whole iteration on the device.  On return (stream-ordered) the Nl x l coefficient

@@ -441,7 +441,8 @@ class Solver:
     def __init__(self, parameters, sketcher=ColBlockGaussianProjectionSketcher,
                  integrator=RowBlockKolmogorovNagumoIntegrator, former=SvdFormer,
                  precision=_lib.PREC_F64, max_iteration=256, tolerance=None, seed=0,
-                 omega_mode=_lib.OMEGA_PHILOX, omega_threads=1, jobv=True, route=_lib.ROUTE_AUTO):
+                 omega_mode=_lib.OMEGA_PHILOX, omega_threads=1, jobv=True, route=_lib.ROUTE_AUTO,
+                 graph=False):
         self.parameters = parameters
         cfg = _lib.SolverConfig()
         _lib.lib().mcnla_solver_config_default(ctypes.byref(cfg))
@@ -456,6 +457,7 @@ class Solver:
         cfg.omega_mode = omega_mode
         cfg.omega_threads = int(omega_threads)
         cfg.route = int(route)
+        cfg.graph = 1 if graph else 0
         self.config = cfg
         self.former_kind = former._kind
         p = parameters.c_params()

@@ -267,3 +267,24 @@ def test_solver_gram_of_a_route(gpu, dtype, order):
     assert it2 == it
     assert np.max(np.abs(S - S2) / S2) < 1e-10
     assert np.abs(proj(Uu) - proj(U2)).max() < 1e-9
+
+
+@pytest.mark.parametrize("integ", [mc.RowBlockKolmogorovNagumoIntegrator, mc.RowBlockWenYinIntegrator])
+def test_solver_graph_replay(gpu, problem, integ):
+    """With graph=True the first solve records the integrator's chunks of iterations as HIP graphs
+    and later solves replay them; every replay equals the eager solver bit for bit, with other
+    eager work and synchronizations between the replays."""
+    note("solver graph replay %s" % integ.__name__)
+    A = mc.DeviceArray.from_numpy(np.asfortranarray(problem["A"]))
+    eager = mc.Solver(params(), integrator=integ, seed=SEED).solve(A)
+    ref = (eager.vectorS(), eager.matrixU(), eager.matrixQbar(), eager.iteration())
+    other = mc.Solver(params(), integrator=integ, seed=SEED + 1)
+    g = mc.Solver(params(), integrator=integ, seed=SEED, graph=True)
+    for rep in range(3):
+        g.solve(A)
+        got = (g.vectorS(), g.matrixU(), g.matrixQbar(), g.iteration())
+        for x, y in zip(got[:3], ref[:3]):
+            assert np.array_equal(x, y), "replay %d differs" % rep
+        assert got[3] == ref[3]
+        other.solve(A)  # eager work between the replays
+        mc.synchronize()
