@@ -271,7 +271,7 @@ def main():
     ap.add_argument("--mixed", default="bf16", choices=["bf16", "bf16x3"])
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-burst", type=int, default=256)
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-budget-gb", type=float, default=48.0)
     ap.add_argument("--serve-seconds", type=float, default=90.0)
     ap.add_argument("--no-serve", action="store_true")

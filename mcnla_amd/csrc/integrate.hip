@@ -966,8 +966,8 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
   const int ll = l * l;
 
   // Bout = Bs * Bin / N on the MFMA GEMM (split-K, deterministic reduction), gated on t; with
-  // Bout null only the bs_splits partial products are left in L.Pbg ([split][Nl][l]).
-  // Bs is exactly symmetric, so its rows serve as the M-contiguous operand.
+  // Bout null only the bs_splits partial products are left in L.Pbg ([split][Nl][l]).  Its rows
+  // run along K, which takes the tall-skinny kernel (row slabs through LDS).
   const int64_t bs_kchunk = ceil_div(ceil_div(nl, L.splits), 16) * 16;
   const int64_t bs_splits = ceil_div(nl, bs_kchunk);
   const int64_t nblk_s = ceil_div(nl, kStepRows);
@@ -979,7 +979,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
     p.K = nl;
     p.A = g.Bs;
     p.lda = nl;
-    p.a_mcontig = true;
+    p.a_mcontig = false;
     p.B = Bin;
     p.ldb = l;
     p.C = Bout;

@@ -312,7 +312,7 @@ extern "C" {
 void mcnla_server_config_default(mcnla_server_config* cfg) {
   if (!cfg) return;
   mcnla_solver_config_default(&cfg->solver);
-  cfg->slots = 4;
+  cfg->slots = 8;
   cfg->hbm_budget = static_cast<size_t>(16) << 30;
   cfg->max_bypass = 8;
 }

@@ -39,7 +39,7 @@ class Served:
 
 
 class Server:
-    def __init__(self, slots=4, hbm_budget=16 << 30, max_bypass=8,
+    def __init__(self, slots=8, hbm_budget=16 << 30, max_bypass=8,
                  sketcher=ColBlockGaussianProjectionSketcher,
                  integrator=RowBlockKolmogorovNagumoIntegrator, former=SvdFormer,
                  precision=_lib.PREC_F64, max_iteration=256, tolerance=None, jobv=True,
