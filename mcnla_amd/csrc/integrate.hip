@@ -910,9 +910,9 @@ Layout layout(char* base, int64_t N, int l) {
   L.l = l;
   L.nblk_p = ceil_div(L.nl, kPRows);
   L.nblk_t = ceil_div(L.nl, kRows);
-  {  // about 256 workgroups, at least 128 k per split
+  {  // about 256 workgroups (512 for large Nl, where each split stays long), >= 128 k per split
     const int64_t tiles = ceil_div(L.nl, 128);
-    int64_t sp = ceil_div(256, tiles);
+    int64_t sp = ceil_div(L.nl > 2048 ? 512 : 256, tiles);
     const int64_t cap = ceil_div(L.nl, 128);
     L.splits = sp < cap ? sp : cap;
     if (L.splits < 1) L.splits = 1;
