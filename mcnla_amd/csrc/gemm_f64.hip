@@ -304,10 +304,14 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
   const int64_t m0 = tm * BM, n0 = tn * BN;
   const int64_t kbeg = split * p.kchunk;
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
-  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
   const int wm0 = (wave / (BN / WN)) * WM, wn0 = (wave % (BN / WN)) * WN;
   const double* __restrict__ A = static_cast<const double*>(p.A) + bat * p.stride_a;
   const double* __restrict__ B = static_cast<const double*>(p.B) + bat * p.stride_b;
+  // A wave whose WM x WN block lies strictly above the diagonal of a symmetric output only moves
+  // tiles for the others; the reduction never reads its entries.
+  const bool live = !p.lower_only || m0 + wm0 + WM - 1 >= n0 + wn0;
 
   d4 acc[FM][FN];
 #pragma unroll
@@ -351,6 +355,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
     issue(kt + 2 < ntk ? kt + 2 : ntk - 1, An, Bn);
     const int64_t left = kend - (kbeg + kt * BK);
     const int kvalid = left < BK ? static_cast<int>(left) : BK;  // <= 0 past the last stage
+    if (!live) return;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int krow = kk + lane / 16;
@@ -550,9 +555,9 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t
   if (gate != nullptr && *gate < gate_t) return;
   const int64_t per = M * N;
   const int64_t total = per * batch;
-  auto computed = [&](int64_t i, int64_t j) {
-    return sym_bm <= 0 || (j / sym_bn) < (i / sym_bm + 1) * (sym_bm / sym_bn);
-  };
+  // Symmetric outputs take every entry from the lower triangle (the kernels may leave the strictly
+  // upper part of their diagonal tiles uncomputed) and mirror it, so C is exactly symmetric.
+  auto computed = [&](int64_t i, int64_t j) { return sym_bm <= 0 || j <= i; };
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = e / per, ij = e % per;
