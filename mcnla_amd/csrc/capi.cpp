@@ -286,8 +286,7 @@ struct GramCore {
     g.lower_only = true;
     g.shape = GemmShape::kSquare;
     g.splits = gemm_f64_splits_for(tiles, m, 2304);
-    g.kchunk = ceil_div(m, g.splits);
-    g.kchunk = ceil_div(g.kchunk, 16) * 16;
+    g.kchunk = ceil_div(m, g.splits);  // equal splits (the GEMM rounds them up where it must)
     g.splits = ceil_div(m, g.kchunk);
     return g;
   }

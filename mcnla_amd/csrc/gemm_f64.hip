@@ -281,23 +281,41 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
   __shared__ __attribute__((aligned(16))) double B2[BK * LDB];
 
   if (p.gate != nullptr && *p.gate < p.gate_t) return;
-  constexpr int64_t R = BM / BN;  // lower-triangle rows of tiles: row tm holds R (tm + 1) tiles
-  static_assert(BM % BN == 0, "lower-triangle map needs BM a multiple of BN");
+  constexpr int64_t R = BM / BN;
+  static_assert(R == 2, "the lower-triangle schedule assumes BM = 2 BN");
   const int64_t tiles_m = ceil_div(p.M, BM), tiles_n = ceil_div(p.N, BN);
-  const int64_t ntiles = p.lower_only ? R * tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
-  const int64_t nblocks = ntiles * p.splits * p.batch;
-  const int64_t logical = xcd_swizzle(blockIdx.x, nblocks);
-  const int64_t bat = logical / (ntiles * p.splits);
-  const int64_t split = (logical / ntiles) % p.splits;
-  const int64_t t = logical % ntiles;
-  int64_t tm, tn;
-  if (p.lower_only) {  // tiles (tm, tn) with tn < R (tm + 1): every tile touching the diagonal
-    const int64_t u = t / R;  // R tm (tm+1)/2 <= t  <=>  tm (tm+1)/2 <= t / R
-    tm = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(u) + 1.0) - 1.0) * 0.5);
-    while (tm * (tm + 1) / 2 > u) --tm;
-    while ((tm + 1) * (tm + 2) / 2 <= u) ++tm;
-    tn = t - R * tm * (tm + 1) / 2;
+  int64_t bat, split, tm, tn;
+  if (p.lower_only) {
+    // Tile row tm of a symmetric output holds the 2 tm + 1 tiles left of the diagonal 256-block's
+    // right half (full work, tm^2 before row tm) and that right half, whose upper waves skip their
+    // MFMAs (about half the work).  The full tiles of every split come first and the half tiles
+    // last, each group spread over the XCDs, so the short workgroups fill the end of the grid.
+    const int64_t nf = tiles_m * tiles_m, nh = tiles_m;
+    const int64_t fb = nf * p.splits * p.batch, hb = nh * p.splits * p.batch;
+    const int64_t b = blockIdx.x;
+    if (b < fb) {
+      const int64_t lg = xcd_swizzle(b, fb);
+      bat = lg / (nf * p.splits);
+      split = (lg / nf) % p.splits;
+      const int64_t t = lg % nf;
+      tm = static_cast<int64_t>(sqrt(static_cast<double>(t)));
+      while (tm * tm > t) --tm;
+      while ((tm + 1) * (tm + 1) <= t) ++tm;
+      tn = t - tm * tm;
+    } else {
+      const int64_t lg = xcd_swizzle(b - fb, hb);
+      bat = lg / (nh * p.splits);
+      split = (lg / nh) % p.splits;
+      tm = lg % nh;
+      tn = R * tm + 1;
+      if (tn * BN >= p.N) return;  // the diagonal block's right half lies beyond the last column
+    }
   } else {
+    const int64_t ntiles = tiles_m * tiles_n;
+    const int64_t lg = xcd_swizzle(blockIdx.x, ntiles * p.splits * p.batch);
+    bat = lg / (ntiles * p.splits);
+    split = (lg / ntiles) % p.splits;
+    const int64_t t = lg % ntiles;
     tm = t / tiles_n;
     tn = t % tiles_n;
   }
@@ -544,9 +562,9 @@ void launch_tall(const GemmF64Args& p, hipStream_t st) {
 }
 
 // Deterministic split-K reduction: C = alpha * sum_s partial[s] + beta * C.  With sym_bm > 0
-// only the lower-triangle tiles of a sym_bm x sym_bn tiling were computed (tile column
-// j / sym_bn < (i / sym_bm + 1) * sym_bm / sym_bn): each computed entry is summed where it lies
-// (coalesced reads) and also written to its mirror when the mirror was not computed.
+// (a symmetric output of which the kernels computed at least the lower triangle, in tiles of
+// sym_bm x sym_bn) each lower entry is summed where it lies (coalesced reads) and written to
+// both (i, j) and (j, i).
 __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t splits,
                                      int64_t batch, int64_t M, int64_t N, double alpha,
                                      double beta, double* __restrict__ C, int64_t ldc,
@@ -638,15 +656,6 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
   if (p.M == 0 || p.N == 0) return;
   if (p.splits < 1) p.splits = 1;
   if (p.kchunk <= 0) p.kchunk = ceil_div(p.K, p.splits);
-  // kchunk must be a multiple of the K tile (16) so split ranges tile exactly.
-  p.kchunk = ceil_div(p.kchunk, 16) * 16;
-  if (p.K > 0) p.splits = ceil_div(p.K, p.kchunk);
-  if (p.splits == 1 && !p.lower_only && p.C != nullptr) p.partial = nullptr;
-  MCNLA_REQUIRE(!(p.splits > 1 && p.partial == nullptr), MCNLA_ERR_INVALID,
-                "split-K GEMM needs a partial buffer");
-  MCNLA_REQUIRE(!p.lower_only || p.M == p.N, MCNLA_ERR_INVALID, "lower_only needs M == N");
-  MCNLA_REQUIRE(!p.lower_only || p.shape == GemmShape::kSquare || p.shape == GemmShape::kSmall,
-                MCNLA_ERR_INVALID, "lower_only needs square tiles");
   auto even16 = [](const void* ptr, int64_t ld, int64_t stride) {
     return reinterpret_cast<uintptr_t>(ptr) % 16 == 0 && ld % 2 == 0 && stride % 2 == 0;
   };
@@ -654,6 +663,16 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
                    p.shape == GemmShape::kSquare && p.M >= 128 && p.N >= 128 &&
                    even16(p.A, p.lda, p.stride_a) &&
                    even16(p.B, p.ldb, p.stride_b);
+  // The register-staged kernels need split ranges that are whole K tiles (16); the DMA kernel
+  // masks a partial last tile, so its splits can be equal.
+  if (!dma) p.kchunk = ceil_div(p.kchunk, 16) * 16;
+  if (p.K > 0) p.splits = ceil_div(p.K, p.kchunk);
+  if (p.splits == 1 && !p.lower_only && p.C != nullptr) p.partial = nullptr;
+  MCNLA_REQUIRE(!(p.splits > 1 && p.partial == nullptr), MCNLA_ERR_INVALID,
+                "split-K GEMM needs a partial buffer");
+  MCNLA_REQUIRE(!p.lower_only || p.M == p.N, MCNLA_ERR_INVALID, "lower_only needs M == N");
+  MCNLA_REQUIRE(!p.lower_only || p.shape == GemmShape::kSquare || p.shape == GemmShape::kSmall,
+                MCNLA_ERR_INVALID, "lower_only needs square tiles");
   const bool tall = !p.a_mcontig && p.b_ncontig && !p.b_f32 && p.N <= 64 && p.batch == 1 &&
                     !p.lower_only && p.M >= 128 &&
                     (p.shape == GemmShape::kTallN32 || p.shape == GemmShape::kTallN64);
@@ -670,7 +689,7 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
     // 256 x 128 tiles of 8 waves (64 x 64 each); a lower-triangle schedule takes the tiles that
     // touch the diagonal (2 (tm + 1) of them in tile row tm).
     const int64_t tm = ceil_div(p.M, 256);
-    const int64_t tiles = p.lower_only ? tm * (tm + 1) : tm * ceil_div(p.N, 128);
+    const int64_t tiles = p.lower_only ? tm * tm + tm : tm * ceil_div(p.N, 128);
     const int64_t nblocks = tiles * p.splits * p.batch;
     MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
     hipLaunchKernelGGL((gemm_f64_dma_kernel<256, 128, 16, 64, 64>),

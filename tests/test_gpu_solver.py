@@ -182,11 +182,13 @@ def test_solver_mixed_vs_f64_headline_spectrum(gpu, prec, bound):
     assert np.max(np.abs(S64 - sig[:20]) / sig[:20]) < 1e-2
 
 
-def test_solver_ragged_tiles(gpu):
-    """Sizes that leave partial tiles on every edge of the sketch and Gram products (m=998, n=333,
-    l=17, N=10, so Nl=170 spans one full and one partial 128-tile, K=333 and K=998 are not
-    multiples of the K tile) against the oracle pipeline on the reference Omega."""
-    m, n, k, p, N, seed = 998, 333, 13, 4, 10, 9
+@pytest.mark.parametrize("m,n,k,p,N,seed", [(998, 333, 13, 4, 10, 9), (1201, 703, 20, 10, 10, 3)])
+def test_solver_ragged_tiles(gpu, m, n, k, p, N, seed):
+    """Sizes that leave partial tiles on every edge of the sketch and Gram products against the
+    oracle pipeline on the reference Omega: m=998, n=333, l=17, N=10 (Nl=170 spans one full and
+    one partial 128-tile; K=333 and K=998 are not multiples of the K tile), and m=1201, n=703,
+    l=30, N=10 (Nl=300: the last 256-row block of the Gram ends before its right 128-column half,
+    which is then never read)."""
     note("solver ragged tiles %dx%d l=%d N=%d" % (m, n, k + p, N))
     rng = np.random.default_rng(5)
     U, _ = np.linalg.qr(rng.standard_normal((m, n)))
