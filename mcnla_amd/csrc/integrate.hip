@@ -385,153 +385,221 @@ __global__ __launch_bounds__(kThreads) void kn_update_kernel(
 // ||Bc Fc + Bg Fgc||_F^2 per trial step; with Dc = Bc'Bc/N, Dgc = Bc'Bg/N and Hgg = Bg'Bg/N it is
 // N * (tr(Fc'Dc Fc) + 2 tr(Fc'Dgc Fgc) + tr(Fgc'Hgg Fgc)), so every trial is l x l work.
 
-// In-place Gauss-Jordan inverse with partial pivoting of the n x n matrix in LDS (ld = n + 1):
-// the inverse LAPACK getrf+getri computes (getrfi.hpp:117-124).  `piv` needs n ints; `red`
-// kThreads/64 ints + doubles.
-__device__ void lds_inverse(int n, double* a, int* piv, double* rowk, double* colk) {
-  const int ld = n + 1;
-  for (int k = 0; k < n; ++k) {
-    if (threadIdx.x < kWave) {  // partial pivoting: first row of largest |a_ik|, i >= k
-      int p = n;
-      double best = -1.0;
-      for (int i = k + static_cast<int>(threadIdx.x); i < n; i += kWave) {
-        const double x = fabs(a[i * ld + k]);
-        if (x > best) {
-          best = x;
-          p = i;
-        }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off, kWave);
-        const int op = __shfl_xor(p, off, kWave);
-        if (ob > best || (ob == best && op < p)) {
-          best = ob;
-          p = op;
-        }
-      }
-      if (threadIdx.x == 0) piv[k] = (p < n) ? p : k;
-    }
-    __syncthreads();
-    const int p = piv[k];
-    if (p != k)
-      for (int j = threadIdx.x; j < n; j += kThreads) {
-        const double x = a[k * ld + j];
-        a[k * ld + j] = a[p * ld + j];
-        a[p * ld + j] = x;
-      }
-    __syncthreads();
-    const double inv = 1.0 / a[k * ld + k];
-    for (int j = threadIdx.x; j < n; j += kThreads) {
-      rowk[j] = (j == k) ? inv : a[k * ld + j] * inv;
-      colk[j] = a[j * ld + k];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < n * n; e += kThreads) {
-      const int i = e / n, j = e % n;
-      if (i == k) {
-        a[i * ld + j] = rowk[j];
-      } else {
-        const double f = colk[i];
-        a[i * ld + j] = (j == k) ? -f * rowk[j] : a[i * ld + j] - f * rowk[j];
-      }
-    }
-    __syncthreads();
-  }
-  // undo the row interchanges as column interchanges, last pivot first
-  for (int k = n - 1; k >= 0; --k) {
-    const int p = piv[k];
-    if (p != k)
-      for (int i = threadIdx.x; i < n; i += kThreads) {
-        const double x = a[i * ld + k];
-        a[i * ld + k] = a[i * ld + p];
-        a[i * ld + p] = x;
-      }
-    __syncthreads();
-  }
-}
-
 struct WyConst {
   static constexpr double tau0 = 1.0, taumax = 1e14, taumin = 1e-14, beta = 0.5, sigma = 1e-4,
                           eta = 0.85;
   static constexpr int taumaxiter = 20;
 };
 
-// Line search + phi/zeta update.  D3 = [Dc; Dgc; Hgg] of parity c.  Writes Fc, Fgc.
-__global__ __launch_bounds__(kThreads) void wy_search_kernel(int t, IntegratorState* st, int l,
-                                                            const double* __restrict__ D3,
-                                                            double* __restrict__ Fc,
-                                                            double* __restrict__ Fgc) {
+// The l x l Wen-Yin kernels run on one workgroup of 16 waves.  An l x l matrix (l <= 64, padded
+// with zeros to np = pad16(l)) is cut into (np/16)^2 tiles of 16 x 16: wave w owns tile
+// (w / nt1, w % nt1), nt1 = np/16, and lane ln its entries (i0 + ln/16 + 4r, j0 + ln%16),
+// r = 0..3 -- the layout of a v_mfma_f64_16x16x4f64 result, so a product lands in the registers
+// of the entries it defines.  Four np x (np + 1) LDS buffers.
+constexpr int kWyWaves = 16;
+constexpr int kWyThreads = kWyWaves * kWave;
+
+size_t wy_lds_bytes(int l) {
+  const size_t np = pad16(l);
+  return 4 * np * (np + 1) * sizeof(double) + 2 * kMaxL * sizeof(int) + kWyWaves * sizeof(double);
+}
+
+struct WyTile {
+  int i0, j0, oj, ib;  // tile origin, owned column, first owned row
+  bool act;            // the wave owns a tile
+  __device__ WyTile(int np) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave, nt1 = np / 16;
+    act = wave < nt1 * nt1;
+    i0 = act ? (wave / nt1) * 16 : 0;
+    j0 = act ? (wave % nt1) * 16 : 0;
+    oj = j0 + (lane & 15);
+    ib = i0 + (lane >> 4);
+  }
+  __device__ int row(int r) const { return ib + 4 * r; }
+};
+
+// Gauss-Jordan inverse with partial pivoting of the l x l matrix in X (LDS, leading dimension
+// ld): the inverse LAPACK getrf+getri computes (getrfi.hpp:117-124).  Rows are never swapped:
+// step k eliminates column k with the unused row p_k of largest |x_ik| (first on ties) and keeps
+// the new column of the inverse in place of column k, so on return M[p_k][q] with p_q = j holds
+// inv[k][j] (piv[k] = p_k) -- the same operations as with explicit row and column interchanges.
+// Each step reads one buffer and writes the other, so it needs one barrier; wave 0 also forms
+// the next pivot column and picks the next pivot.  Every thread must call this; returns the
+// buffer that holds M.
+__device__ double* wy_gj_inverse(int l, int ld, double* X, double* Y, int* piv, const WyTile& w) {
+  const int lane = threadIdx.x % kWave;
+  const bool wave0 = threadIdx.x < kWave;
+  bool used = false;  // wave 0: row `lane` was a pivot
+  // pick the pivot of column k among the unused rows; y = |column k| of this lane's row
+  auto choose = [&](double y, int k) {
+    const bool cand = lane < l && !used;
+    double key = cand ? fabs(y) : -1.0;
+    if (cand && key != key) key = -0.5;  // NaN only when no finite candidate is left
+    int idx = cand ? lane : kWave;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ok = __shfl_xor(key, off, kWave);
+      const int oi = __shfl_xor(idx, off, kWave);
+      if (ok > key || (ok == key && oi < idx)) {
+        key = ok;
+        idx = oi;
+      }
+    }
+    if (lane == idx) used = true;
+    if (lane == 0) piv[k] = idx;
+  };
+  if (wave0) choose(lane < l ? X[lane * ld] : 0.0, 0);
+  __syncthreads();
+  for (int k = 0; k < l; ++k) {
+    const int p = piv[k];
+    const double pv = 1.0 / X[p * ld + k];
+    if (w.act && w.oj < l) {
+      const int j = w.oj;
+      const double rk = (j == k) ? pv : X[p * ld + j] * pv;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = w.row(r);
+        if (i >= l) continue;
+        if (i == p) {
+          Y[i * ld + j] = rk;
+        } else {
+          const double f = X[i * ld + k];
+          Y[i * ld + j] = (j == k) ? -f * rk : X[i * ld + j] - f * rk;
+        }
+      }
+    }
+    if (wave0 && k + 1 < l) {
+      double y = 0.0;
+      if (lane < l) {
+        const double rk1 = X[p * ld + k + 1] * pv;
+        y = (lane == p) ? rk1 : X[lane * ld + k + 1] - X[lane * ld + k] * rk1;
+      }
+      choose(y, k + 1);
+    }
+    __syncthreads();
+    double* z = X;
+    X = Y;
+    Y = z;
+  }
+  return X;
+}
+
+// Line search + phi/zeta update (row_block_gramian_wen_yin_integrator.hpp:181-222).  D3 = [Dc;
+// Dgc; Hgg] of parity c.  Writes Fc, Fgc.
+//
+// The reference inverts C = [[Dc/2 - I/tau, I/2], [-Dgc/2, -Dc/2 - I/tau]] (2l x 2l) for every
+// trial tau and takes Fgc = X12 Dc - X11, Fc = X22 Dc - X21 + I from X = C^{-1}, i.e.
+// [Fgc; Fc - I] = C^{-1} [-I; Dc].  Eliminating the second block row of that solve gives
+//   Fgc = (4/tau) (K + (4/tau^2) I)^{-1},  Fc = -I - (Dc - s I) Fgc,  K = Dgc - Dc^2, s = 2/tau,
+// one l x l inverse per trial.  Its test value (1/2N) ||Bc Fc + Bg Fgc||_F^2 with Dc = Bc'Bc/N,
+// Dgc = Bc'Bg/N, Hgg = Bg'Bg/N is, with F = Fgc,
+//   phit = 1/2 [tr(Dc) - 2 tr((K + s Dc) F) + tr(F' X(s) F)],  X(s) = X0 + s X1 + s^2 Dc,
+//   X0 = Dc^3 - Dc Dgc - (Dc Dgc)' + Hgg,  X1 = Dgc + Dgc' - 2 Dc^2,
+// so a trial costs one inverse and one product.  K, X0 and X1 stay in registers (owned entries).
+__global__ __launch_bounds__(kWyThreads) void wy_search_kernel(int t, IntegratorState* st, int l,
+                                                              const double* __restrict__ D3,
+                                                              double* __restrict__ Fc,
+                                                              double* __restrict__ Fgc) {
   if (skip(st, t)) return;
-  // The reference inverts C = [[Dc/2 - I/tau, I/2], [-Dgc/2, -Dc/2 - I/tau]] (2l x 2l) for every
-  // trial tau and takes Fgc = X12 Dc - X11, Fc = X22 Dc - X21 + I from X = C^{-1}, i.e.
-  // [Fgc; Fc - I] = C^{-1} [-I; Dc].  Eliminating the second block row of that solve gives
-  //   Fgc = (4/tau) (K + (4/tau^2) I)^{-1},  Fc = -I - (Dc - (2/tau) I) Fgc,  K = Dgc - Dc^2,
-  // one l x l inverse per trial instead of a 2l x 2l one (K is the same for every trial).
   extern __shared__ double dyn[];
-  const int ld = l + 1;
-  double* a = dyn;                     // l x ld: K + (4/tau^2) I, inverted in place
-  double* sK = a + l * ld;             // l x ld: K
-  double* fg = sK + l * ld;            // l x ld: Fgc
-  double* fc = fg + l * ld;            // l x ld: Fc
-  double* rowk = fc + l * ld;          // l
-  double* colk = rowk + l;             // l
-  double* red = colk + l;              // kThreads / 64
-  int* piv = reinterpret_cast<int*>(red + kThreads / kWave);
-  __shared__ double phit_s;
+  __shared__ double phit_s, trdc_s;
   __shared__ int accept_s;
+  const int np = pad16(l), ld = np + 1, nb = np * ld;
+  double* B0 = dyn;  // Dc
+  double* B1 = B0 + nb;
+  double* B2 = B1 + nb;
+  double* B3 = B2 + nb;
+  int* piv = reinterpret_cast<int*>(B3 + nb);
+  int* pinv = piv + kMaxL;
+  double* red = reinterpret_cast<double*>(pinv + kMaxL);
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const WyTile w(np);
+  const int oj = w.oj;
   const double* Dc = D3;
   const double* Dgc = D3 + l * l;
   const double* Hgg = D3 + 2 * l * l;
-  const double phi = st->phi, mu = st->mu;
-  for (int e = threadIdx.x; e < l * l; e += kThreads) {
-    const int i = e / l, j = e % l;
-    double d2 = 0.0;
-    for (int q = 0; q < l; ++q) d2 += Dc[i * l + q] * Dc[q * l + j];
-    sK[i * ld + j] = Dgc[i * l + j] - d2;
+  for (int e = tid; e < np * np; e += kWyThreads) {
+    const int i = e / np, j = e % np;
+    const bool in = i < l && j < l;
+    B0[i * ld + j] = in ? Dc[i * l + j] : 0.0;
+    B1[i * ld + j] = in ? Dgc[i * l + j] : 0.0;
   }
   __syncthreads();
-  double tau = st->taug;
-  double phit = phi;
+  if (w.act) {  // Dc^2 into B2
+    const d4 a = lds_mfma_tile<false, false>(np, ld, B0, B0, w.i0, w.j0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) B2[w.row(r) * ld + oj] = a[r];
+  }
+  __syncthreads();
+  double K[4] = {0, 0, 0, 0}, X0[4] = {0, 0, 0, 0}, X1[4] = {0, 0, 0, 0};
+  if (w.act) {
+    const d4 c3 = lds_mfma_tile<false, false>(np, ld, B0, B2, w.i0, w.j0);  // Dc^3
+    const d4 cg = lds_mfma_tile<false, false>(np, ld, B0, B1, w.i0, w.j0);  // Dc Dgc
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = w.row(r);
+      const double g = B1[i * ld + oj], d2 = B2[i * ld + oj];
+      K[r] = g - d2;
+      X1[r] = g + B1[oj * ld + i] - 2.0 * d2;
+      X0[r] = c3[r];
+      B3[i * ld + oj] = cg[r];
+    }
+  }
+  if (wave == 0) {
+    const double d = wave_sum(lane < l ? B0[lane * ld + lane] : 0.0);
+    if (lane == 0) trdc_s = d;
+  }
+  __syncthreads();
+  if (w.act) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = w.row(r);
+      const double h = (i < l && oj < l) ? Hgg[i * l + oj] : 0.0;
+      X0[r] = X0[r] - B3[i * ld + oj] - B3[oj * ld + i] + h;
+    }
+  }
+  __syncthreads();  // B1..B3 are free from here
+  const double phi = st->phi, mu = st->mu, trdc = trdc_s;
+  double tau = st->taug, phit = phi, s = 2.0 / tau;
+  double* Fb = B2;
   for (int ti = 0; ti < WyConst::taumaxiter; ++ti, tau *= WyConst::beta) {
     const double c = 4.0 / (tau * tau);
-    for (int e = threadIdx.x; e < l * l; e += kThreads) {
-      const int i = e / l, j = e % l;
-      a[i * ld + j] = sK[i * ld + j] + (i == j ? c : 0.0);
-    }
-    __syncthreads();
-    lds_inverse(l, a, piv, rowk, colk);
-    for (int e = threadIdx.x; e < l * l; e += kThreads) {
-      const int i = e / l, j = e % l;
-      fg[i * ld + j] = (4.0 / tau) * a[i * ld + j];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < l * l; e += kThreads) {
-      const int i = e / l, j = e % l;
-      double acc = 0.0;
-      for (int q = 0; q < l; ++q) acc += Dc[i * l + q] * fg[q * ld + j];
-      fc[i * ld + j] = (2.0 / tau) * fg[i * ld + j] - acc - (i == j ? 1.0 : 0.0);
-    }
-    __syncthreads();
-    // phit = (1/2N) ||Bc Fc + Bg Fgc||^2 = 1/2 tr(Fc'Dc Fc + 2 Fc'Dgc Fgc + Fgc'Hgg Fgc)
-    double part = 0.0;
-    for (int e = threadIdx.x; e < l * l; e += kThreads) {
-      const int i = e / l, j = e % l;  // entry (i, j) of the three l x l products
-      double x1 = 0.0, x2 = 0.0, x3 = 0.0;
-      for (int q = 0; q < l; ++q) {
-        const double fcq = fc[q * ld + j], fgq = fg[q * ld + j];
-        x1 += Dc[i * l + q] * fcq;
-        x2 += Dgc[i * l + q] * fgq;
-        x3 += Hgg[i * l + q] * fgq;
+    s = 2.0 / tau;
+    if (w.act) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = w.row(r);
+        B1[i * ld + oj] = K[r] + (i == oj ? c : 0.0);
       }
-      const double fci = fc[i * ld + j], fgi = fg[i * ld + j];
-      part += fci * x1 + 2.0 * fci * x2 + fgi * x3;
+    }
+    __syncthreads();
+    const double* R = wy_gj_inverse(l, ld, B1, B2, piv, w);
+    Fb = (R == B1) ? B2 : B1;
+    if (tid < l) pinv[piv[tid]] = tid;
+    __syncthreads();
+    if (w.act) {  // F = (4/tau) a^{-1} (gathered from the pivoted form), Xs = X0 + s X1 + s^2 Dc
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = w.row(r);
+        Fb[i * ld + oj] = (i < l && oj < l) ? (4.0 / tau) * R[piv[i] * ld + pinv[oj]] : 0.0;
+        B3[i * ld + oj] = X0[r] + s * X1[r] + s * s * B0[i * ld + oj];
+      }
+    }
+    __syncthreads();
+    double part = 0.0;
+    if (w.act) {
+      const d4 xf = lds_mfma_tile<false, false>(np, ld, B3, Fb, w.i0, w.j0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = w.row(r);
+        part += Fb[i * ld + oj] * xf[r] - 2.0 * (K[r] + s * B0[i * ld + oj]) * Fb[oj * ld + i];
+      }
     }
     part = wave_sum(part);
-    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = part;
+    if (lane == 0) red[wave] = part;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double sum = 0.0;
-      for (int w = 0; w < kThreads / kWave; ++w) sum += red[w];
+    if (tid == 0) {
+      double sum = trdc;
+      for (int q = 0; q < kWyWaves; ++q) sum += red[q];
       phit_s = 0.5 * sum;
       accept_s = (phit_s >= phi + tau * WyConst::sigma * mu) ? 1 : 0;
     }
@@ -539,12 +607,19 @@ __global__ __launch_bounds__(kThreads) void wy_search_kernel(int t, IntegratorSt
     phit = phit_s;
     if (accept_s) break;
   }
-  for (int e = threadIdx.x; e < l * l; e += kThreads) {
-    const int i = e / l, j = e % l;
-    Fgc[e] = fg[i * ld + j];
-    Fc[e] = fc[i * ld + j];
+  if (w.act) {  // Fgc = F, Fc = s F - Dc F - I
+    const d4 df = lds_mfma_tile<false, false>(np, ld, B0, Fb, w.i0, w.j0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = w.row(r);
+      if (i < l && oj < l) {
+        const double f = Fb[i * ld + oj];
+        Fgc[i * l + oj] = f;
+        Fc[i * l + oj] = s * f - df[r] - (i == oj ? 1.0 : 0.0);
+      }
+    }
   }
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     const double zeta = st->zeta;
     st->phi = (WyConst::eta * zeta * phi + phit) / (WyConst::eta * zeta + 1.0);
     st->zeta = WyConst::eta * zeta + 1.0;
@@ -709,54 +784,75 @@ __global__ __launch_bounds__(kThreads) void wy_terms_kernel(
 // ||Qc Phi1 + Qs U1||^2 then equals the row-block form's explicit sum of squares, and the step
 // size no longer amplifies the drift of Qc'Qc from I (with the identity, a 1e-15 perturbation of
 // the input moves the result by ~1e-5 at the CPU-runnable workload).
-__global__ void wy_tau_kernel(int t, IntegratorState* st, int l, int odd,
-                              const double* __restrict__ Dc, const double* __restrict__ Dp,
-                              const double* __restrict__ Fc, const double* __restrict__ T) {
+__global__ __launch_bounds__(kWyThreads) void wy_tau_kernel(int t, IntegratorState* st, int l,
+                                                           int odd, const double* __restrict__ Dc,
+                                                           const double* __restrict__ Dp,
+                                                           const double* __restrict__ Fc,
+                                                           const double* __restrict__ T) {
   if (skip(st, t) || st->converged_at == t) return;
-  __shared__ double sP1[kMaxL * kMaxL];
-  __shared__ double sP2[kMaxL * kMaxL];
-  __shared__ double red7[7][kThreads / kWave];
+  extern __shared__ double dyn[];
+  __shared__ double red7[7][kWyWaves];
+  const int np = pad16(l), ld = np + 1, nb = np * ld;
+  double* sF = dyn;        // Fc
+  double* sD = sF + nb;    // D+
+  double* sQ = sD + nb;    // Qc'Qc
+  double* sP2 = sQ + nb;   // Phi2 = Dc - Fc D+
   const double* BtU1 = T;
   const double* BtU2 = T + l * l;
   const double* QQ = T + 2 * l * l;
   const double u1bsu = T[3 * l * l], u2bsu = T[3 * l * l + 1];
-  // Phi1 = Fc - I, Phi2 = Dc - Fc D+
-  for (int e = threadIdx.x; e < l * l; e += blockDim.x) {
-    const int i = e / l, j = e % l;
-    double fd = 0.0;
-    for (int q = 0; q < l; ++q) fd += Fc[i * l + q] * Dp[q * l + j];
-    sP1[e] = Fc[e] - (i == j ? 1.0 : 0.0);
-    sP2[e] = Dc[e] - fd;
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const WyTile w(np);
+  const int oj = w.oj;
+  for (int e = tid; e < np * np; e += kWyThreads) {
+    const int i = e / np, j = e % np;
+    const bool in = i < l && j < l;
+    sF[i * ld + j] = in ? Fc[i * l + j] : 0.0;
+    sD[i * ld + j] = in ? Dp[i * l + j] : 0.0;
+    sQ[i * ld + j] = in ? QQ[i * l + j] : 0.0;
   }
   __syncthreads();
-  // <X, QQ Y> = sum_ij X_ij (QQ Y)_ij
-  double p11 = 0, p1bu1 = 0, p12 = 0, p1bu2 = 0, p2bu1 = 0, p22 = 0, p2bu2 = 0;
-  for (int e = threadIdx.x; e < l * l; e += blockDim.x) {
-    const int i = e / l, j = e % l;
-    double q1 = 0.0, q2 = 0.0;
-    for (int q = 0; q < l; ++q) {
-      q1 += QQ[i * l + q] * sP1[q * l + j];
-      q2 += QQ[i * l + q] * sP2[q * l + j];
+  if (w.act) {
+    const d4 fd = lds_mfma_tile<false, false>(np, ld, sF, sD, w.i0, w.j0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = w.row(r);
+      sP2[i * ld + oj] = ((i < l && oj < l) ? Dc[i * l + oj] : 0.0) - fd[r];
     }
-    const double ph1 = sP1[e], ph2 = sP2[e];
-    p11 += ph1 * q1;
-    p12 += ph1 * q2;
-    p22 += ph2 * q2;
-    p1bu1 += ph1 * BtU1[e];
-    p1bu2 += ph1 * BtU2[e];
-    p2bu1 += ph2 * BtU1[e];
-    p2bu2 += ph2 * BtU2[e];
-  }
-  double vals[7] = {p11, p1bu1, p12, p1bu2, p2bu1, p22, p2bu2};
-  for (int k = 0; k < 7; ++k) {
-    const double v = wave_sum(vals[k]);
-    if (threadIdx.x % kWave == 0) red7[k][threadIdx.x / kWave] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  // <X, QQ Y> = sum_ij X_ij (QQ Y)_ij with Phi1 = Fc - I, so QQ Phi1 = QQ Fc - QQ
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};  // p11, p1bu1, p12, p1bu2, p2bu1, p22, p2bu2
+  if (w.act) {
+    const d4 q1 = lds_mfma_tile<false, false>(np, ld, sQ, sF, w.i0, w.j0);
+    const d4 q2 = lds_mfma_tile<false, false>(np, ld, sQ, sP2, w.i0, w.j0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = w.row(r);
+      if (i < l && oj < l) {
+        const double ph1 = sF[i * ld + oj] - (i == oj ? 1.0 : 0.0), ph2 = sP2[i * ld + oj];
+        const double qp1 = q1[r] - sQ[i * ld + oj];
+        const double bu1 = BtU1[i * l + oj], bu2 = BtU2[i * l + oj];
+        v[0] += ph1 * qp1;
+        v[1] += ph1 * bu1;
+        v[2] += ph1 * q2[r];
+        v[3] += ph1 * bu2;
+        v[4] += ph2 * bu1;
+        v[5] += ph2 * q2[r];
+        v[6] += ph2 * bu2;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const double x = wave_sum(v[k]);
+    if (lane == 0) red7[k][wave] = x;
+  }
+  __syncthreads();
+  if (tid == 0) {
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int k = 0; k < 7; ++k)
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s[k] += red7[k][w];
+      for (int q = 0; q < kWyWaves; ++q) s[k] += red7[k][q];
     double t1, t2;
     if (odd) {
       t1 = s[0] + 2.0 * s[1] + u1bsu;
@@ -800,18 +896,24 @@ __global__ void init_coef_kernel(const double* __restrict__ Bs, int64_t nl, int 
 }
 
 // WY initial scalars: phi = ||Bc||^2 / 2N = tr(Dc)/2, mu = sum|diag(Dgc)| - ||Dc||^2.
+// One wave.
 __global__ void wy_init_scalars_kernel(IntegratorState* st, int l, const double* __restrict__ D3) {
-  if (threadIdx.x != 0) return;
   const double* Dc = D3;
   const double* Dgc = D3 + l * l;
-  double tr = 0.0, mu = 0.0;
-  for (int i = 0; i < l; ++i) {
+  const int lane = threadIdx.x;
+  double tr = 0.0, dg = 0.0, sq = 0.0;
+  for (int i = lane; i < l; i += kWave) {
     tr += Dc[i * l + i];
-    mu += fabs(Dgc[i * l + i]);
+    dg += fabs(Dgc[i * l + i]);
   }
-  for (int e = 0; e < l * l; ++e) mu -= Dc[e] * Dc[e];
-  st->phi = 0.5 * tr;
-  st->mu = mu;
+  for (int e = lane; e < l * l; e += kWave) sq += Dc[e] * Dc[e];
+  tr = wave_sum(tr);
+  dg = wave_sum(dg);
+  sq = wave_sum(sq);
+  if (lane == 0) {
+    st->phi = 0.5 * tr;
+    st->mu = dg - sq;
+  }
 }
 
 // Final re-orthonormalization of the basis Qbar = Qs M: Gm = M' Bs M = N * sum_blk P[blk][1]
@@ -1001,12 +1103,12 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
                      L.B[0], L.E[0], L.FF[0]);
   MCNLA_LAUNCHED();
 
-  const size_t wy_lds = (static_cast<size_t>(4) * l * (l + 1) + 2 * l + kThreads / kWave) *
-                            sizeof(double) + l * sizeof(int) + 64;
+  const size_t wy_lds = wy_lds_bytes(l);
   if (g.wen_yin) {
-    MCNLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(wy_search_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(wy_lds)));
+    for (const void* f : {reinterpret_cast<const void*>(wy_search_kernel),
+                          reinterpret_cast<const void*>(wy_tau_kernel)})
+      MCNLA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(wy_lds)));
     // Bg0 = Bs B0 / N and [Dc; Dgc; Hgg] of parity 0.
     bs_product(L.B[0], L.Bg[0], -1);
     hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, -1, L.st, nl, l,
@@ -1050,7 +1152,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
           MCNLA_LAUNCHED();
         } else {
           const int odd = p;  // the reference toggles is_odd before using it: odd = !(t & 1)
-          hipLaunchKernelGGL(wy_search_kernel, dim3(1), dim3(kThreads), wy_lds, ls, ti, L.st, l,
+          hipLaunchKernelGGL(wy_search_kernel, dim3(1), dim3(kWyThreads), wy_lds, ls, ti, L.st, l,
                              L.D3[c], L.F, L.C);
           MCNLA_LAUNCHED();
           hipLaunchKernelGGL(wy_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
@@ -1074,8 +1176,8 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
           hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll + 2, kThreads)),
                              dim3(kThreads), 0, ls, ti, L.st, L.P, L.nblk_t, 3 * ll + 2, 1.0, L.T);
           MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(wy_tau_kernel, dim3(1), dim3(kThreads), 0, ls, ti, L.st, l, odd, L.D3[c],
-                             L.D3[p], L.F, L.T);
+          hipLaunchKernelGGL(wy_tau_kernel, dim3(1), dim3(kWyThreads), wy_lds, ls, ti, L.st, l, odd,
+                             L.D3[c], L.D3[p], L.F, L.T);
           MCNLA_LAUNCHED();
         }
       }
