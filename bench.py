@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay the integrator iterations as "
                     "HIP graphs recorded in the warm-up solve")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--only-c4", action="store_true", help="run the row-block Wen-Yin workload "
+                    "alone and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the iSVD path here is built for one MI355X "
@@ -300,6 +302,9 @@ def main():
             mc.lib().mcnla_memcpy_dtod(dst.ptr, src.ptr, nbytes, None)
         mc.synchronize()
         log("bench: copied %d bytes device-to-device 3 times" % nbytes)
+        return
+    if args.only_c4:
+        print(json.dumps({"c4": c4_bench(mc, args)}), flush=True)
         return
     m, n, k, p, N = args.m, args.n, args.k, args.p, args.N
     l = k + p
