@@ -707,6 +707,12 @@ GemmF64Args plan_h_omega(int64_t n, int64_t nl) {
   return g;
 }
 
+// Split-K factor of the n x l products X = Omega (T M) (K = Nl) and Z = H X (K = n): n is small
+// on this route, so without splits they would run on n/128 workgroups.
+int64_t n_by_l_splits(int64_t n, int64_t K) {
+  return gemm_f64_splits_for(ceil_div(n, 128), K, 512);
+}
+
 // G = Omega' P (Nl x Nl, symmetric): lower-triangle tiles.
 GemmF64Args plan_omega_p(int64_t n, int64_t nl) {
   GemmF64Args g;
@@ -760,8 +766,10 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
     s->H.ensure(n * n * sizeof(double));
     s->P.ensure(n * nl * sizeof(double));
     s->X.ensure(n * l * sizeof(double));
-    s->gp.ensure(std::max(gemm_f64_partial_elems(n, nl, plan_h_omega(n, nl).splits),
-                          gemm_f64_partial_elems(nl, nl, plan_omega_p(n, nl).splits)) *
+    s->gp.ensure(std::max({gemm_f64_partial_elems(n, nl, plan_h_omega(n, nl).splits),
+                           gemm_f64_partial_elems(nl, nl, plan_omega_p(n, nl).splits),
+                           gemm_f64_partial_elems(n, l, n_by_l_splits(n, nl)),
+                           gemm_f64_partial_elems(n, l, n_by_l_splits(n, n))}) *
                  sizeof(double));
   }
   s->G.ensure(nl * nl * sizeof(double));
@@ -1372,11 +1380,19 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
       x.M = n;
       x.A = s->sketch.omega.p;
       x.C = s->X.as<double>();
+      x.splits = n_by_l_splits(n, nl);
+      x.partial = s->gp.as<double>();
       gemm_f64(x, st);
       q.K = n;
-      q.A = Ar;
-      q.lda = ldar;
       q.B = s->X.p;
+      if (a_order == MCNLA_ROW_MAJOR) {  // stream the caller's A (FP32 reads half the bytes)
+        q.A = A;
+        q.a_f32 = a_dtype == MCNLA_F32;
+        q.lda = lda;
+      } else {
+        q.A = Ar;
+        q.lda = ldar;
+      }
     }
     gemm_f64(q, st);
     s->timer.mark(st);
@@ -1394,6 +1410,8 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
       z.C = s->form.Z.as<double>();
       z.ldc = l;
       z.shape = tall_shape(l);
+      z.splits = n_by_l_splits(n, n);
+      z.partial = s->gp.as<double>();
       gemm_f64(z, st);
     }
     s->form.run(p, c.former, c.jobv != 0, A, lda, a_dtype, a_order, s->Qbar.as<double>(), st,

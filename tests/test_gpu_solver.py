@@ -233,14 +233,17 @@ def test_invalid_solve_arguments(gpu):
     assert e.value.status == mc._lib.ERR_INVALID
 
 
-@pytest.mark.parametrize("dtype,order", [(np.float64, "C"), (np.float64, "F"), (np.float32, "F"),
-                                         (np.float32, "C")])
-def test_solver_gram_of_a_route(gpu, dtype, order):
+@pytest.mark.parametrize("dtype,order,shape", [
+    (np.float64, "C", (3000, 150, 10, 10, 12)), (np.float64, "F", (3000, 150, 10, 10, 12)),
+    (np.float32, "F", (3000, 150, 10, 10, 12)), (np.float32, "C", (3000, 150, 10, 10, 12)),
+    (np.float32, "C", (4000, 600, 10, 10, 40)), (np.float64, "F", (4000, 600, 10, 10, 40))])
+def test_solver_gram_of_a_route(gpu, dtype, order, shape):
     """n < Nl: the Gram-of-A route (H = A'A, G = Omega' H Omega, Qbar = A (Omega T M),
     Z = H (Omega T M); Y never formed) against the oracle pipeline on the same reference Omega,
-    and against the sketch route.  Both are exact reformulations, so they agree to rounding."""
-    m, n, k, p, N = 3000, 150, 10, 10, 12  # Nl = 240 > n
-    note("solver Gram-of-A route %s %s" % (np.dtype(dtype).name, order))
+    and against the sketch route.  Both are exact reformulations, so they agree to rounding.
+    At n = 600, Nl = 800 the n x l products X = Omega (T M) and Z = H X run split over K."""
+    m, n, k, p, N = shape  # Nl > n
+    note("solver Gram-of-A route %s %s %s" % (np.dtype(dtype).name, order, shape))
     rng = np.random.default_rng(21)
     U, _ = np.linalg.qr(rng.standard_normal((m, n)))
     V, _ = np.linalg.qr(rng.standard_normal((n, n)))
