@@ -36,8 +36,9 @@ class DeviceArray:
         return out
 
     @classmethod
-    def view(cls, ptr, shape, dtype=np.float64, order="C"):
-        """A non-owning view of memory the library owns (stage results)."""
+    def view(cls, ptr, shape, dtype=np.float64, order="C", ld=None):
+        """A non-owning view of device memory (stage results, or a block of a larger matrix with
+        leading dimension ``ld``)."""
         out = cls.__new__(cls)
         out.shape = tuple(int(s) for s in shape)
         out.dtype = np.dtype(dtype)
@@ -45,6 +46,8 @@ class DeviceArray:
         out.nbytes = int(np.prod(out.shape)) * out.dtype.itemsize
         out.ptr = ptr
         out._owner = False
+        if ld is not None:
+            out._ld = int(ld)
         return out
 
     def numpy(self):
@@ -55,6 +58,8 @@ class DeviceArray:
 
     @property
     def ld(self):
+        if getattr(self, "_ld", None) is not None:
+            return self._ld
         if len(self.shape) == 1:
             return self.shape[0]
         return self.shape[1] if self.order == "C" else self.shape[0]

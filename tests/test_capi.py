@@ -149,3 +149,11 @@ def test_route_choice_by_shape():
     assert auto == via_a and via_y > via_a + 8 * m * (k + p) * N - 8 * m * n - 16 * n * (k + p) * N
     m, n, k, p, N = 100000, 10000, 20, 12, 64
     assert mc.required_bytes(m, n, k, p, N) == mc.required_bytes(m, n, k, p, N, route=mc.ROUTE_SKETCH)
+
+
+def test_device_view_leading_dimension():
+    """A view of a block of a larger matrix carries the larger matrix's leading dimension (no
+    device memory is touched)."""
+    v = mc.DeviceArray.view(0x1000, (300, 40), np.float32, "F", ld=301)
+    assert v.ld == 301 and v.order_code == _lib.COL_MAJOR and v.dtype_code == _lib.F32
+    assert mc.DeviceArray.view(0x1000, (300, 40), np.float64, "C").ld == 40
