@@ -298,15 +298,21 @@ template <int KIND, typename _Val>
 class GaussianSketcherImpl : public StageBase {
  public:
   explicit GaussianSketcherImpl(const Parameters<_Val>& p, index_t seed = 0, index_t exponent = 0)
-      : params_(p), seed_(seed) {
-    if (exponent != 0) throw Error(MCNLA_ERR_INVALID, "power iterations (exponent > 0) are not supported");
-  }
+      : params_(p), seed_(seed), exponent_(exponent) {}
   void initialize() {
     mcnla_stage h = nullptr;
     check(mcnla_sketcher_create(&params_.c(), KIND, precision_, &h), "sketcher");
     reset(h);
     check(mcnla_sketcher_set_seed(h_, static_cast<uint64_t>(seed_), omega_mode_, threads_), "setSeed");
+    check(mcnla_sketcher_set_exponent(h_, exponent_), "setExponent");
   }
+  // Power iterations (gaussian_projection_sketcher.hh:87; GaussianProjectionSketcher only).
+  GaussianSketcherImpl& setExponent(index_t exponent) {
+    exponent_ = exponent;
+    if (h_) check(mcnla_sketcher_set_exponent(h_, exponent_), "setExponent");
+    return *this;
+  }
+  index_t exponent() const { return exponent_; }
   GaussianSketcherImpl& setSeed(index_t seed) {
     seed_ = seed;
     if (h_) check(mcnla_sketcher_set_seed(h_, static_cast<uint64_t>(seed_), omega_mode_, threads_), "setSeed");
@@ -336,6 +342,7 @@ class GaussianSketcherImpl : public StageBase {
  private:
   const Parameters<_Val>& params_;
   index_t seed_;
+  index_t exponent_;
   int omega_mode_ = MCNLA_OMEGA_PHILOX, threads_ = 1, precision_ = MCNLA_PREC_F64;
 };
 
@@ -345,6 +352,50 @@ template <typename _Val> using RowBlockGaussianProjectionSketcher =
     GaussianSketcherImpl<MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN, _Val>;
 template <typename _Val> using GaussianProjectionSketcher =
     GaussianSketcherImpl<MCNLA_SKETCHER_GAUSSIAN, _Val>;
+
+// Column sampling (column_sampling_sketcher.hh, row_block_column_sampling_sketcher.hh): Y's
+// columns are columns of A drawn by the reference's seeded index stream; `threads` is the OpenMP
+// thread count of the run whose stream should be reproduced.
+template <int KIND, typename _Val>
+class ColumnSamplingSketcherImpl : public StageBase {
+ public:
+  explicit ColumnSamplingSketcherImpl(const Parameters<_Val>& p, index_t seed = 0)
+      : params_(p), seed_(seed) {}
+  void initialize() {
+    mcnla_stage h = nullptr;
+    check(mcnla_sketcher_create(&params_.c(), KIND, MCNLA_PREC_F64, &h), "sketcher");
+    reset(h);
+    check(mcnla_sketcher_set_seed(h_, static_cast<uint64_t>(seed_), MCNLA_OMEGA_REFERENCE, threads_),
+          "setSeed");
+  }
+  ColumnSamplingSketcherImpl& setSeed(index_t seed, int threads = 1) {
+    seed_ = seed;
+    threads_ = threads;
+    if (h_)
+      check(mcnla_sketcher_set_seed(h_, static_cast<uint64_t>(seed_), MCNLA_OMEGA_REFERENCE, threads_),
+            "setSeed");
+    return *this;
+  }
+  index_t seed() const { return seed_; }
+  template <class _Matrix>
+  void operator()(const _Matrix& a, DenseMatrixCollectionColBlockRowMajor<_Val>& q) {
+    if (!h_) initialize();
+    check(mcnla_sketcher_run(h_, a.storage().device(), a.pitch(), _Matrix::dtype(), _Matrix::order(),
+                             q.unfold().storage().deviceMutable(), nullptr),
+          "sketcher");
+    computed_ = true;
+  }
+
+ private:
+  const Parameters<_Val>& params_;
+  index_t seed_;
+  int threads_ = 1;
+};
+
+template <typename _Val> using ColumnSamplingSketcher =
+    ColumnSamplingSketcherImpl<MCNLA_SKETCHER_COLUMN_SAMPLING, _Val>;
+template <typename _Val> using RowBlockColumnSamplingSketcher =
+    ColumnSamplingSketcherImpl<MCNLA_SKETCHER_ROW_BLOCK_COLUMN_SAMPLING, _Val>;
 
 // ---------------------------------------------------------------------------------------------
 // Orthogonalizers (include/mcnla/isvd/orthogonalizer/*gramian*.hh).

@@ -86,12 +86,18 @@ typedef struct mcnla_stage_s* mcnla_stage;
 
 /* Sketchers.  Counterparts of ColBlockGaussianProjectionSketcher
  * (include/mcnla/isvd/sketcher/col_block_gaussian_projection_sketcher.hpp:49-102),
- * RowBlockGaussianProjectionSketcher (row_block_gaussian_projection_sketcher.hpp:51-114) and
- * GaussianProjectionSketcher with exponent 0 (gaussian_projection_sketcher.hpp:66-103). */
+ * RowBlockGaussianProjectionSketcher (row_block_gaussian_projection_sketcher.hpp:51-114),
+ * GaussianProjectionSketcher with its power iterations (gaussian_projection_sketcher.hpp:66-103),
+ * ColumnSamplingSketcher (column_sampling_sketcher.hpp:49-99) and
+ * RowBlockColumnSamplingSketcher (row_block_column_sampling_sketcher.hpp:49-99).  The column
+ * sampling kinds copy columns of A (MCNLA_PREC_F64 only) and are stage sketchers: the fused
+ * solver rejects them. */
 typedef enum {
   MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN = 0,
   MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN = 1,
-  MCNLA_SKETCHER_GAUSSIAN = 2
+  MCNLA_SKETCHER_GAUSSIAN = 2,
+  MCNLA_SKETCHER_COLUMN_SAMPLING = 3,
+  MCNLA_SKETCHER_ROW_BLOCK_COLUMN_SAMPLING = 4
 } mcnla_sketcher_kind;
 
 /* Omega generation.  PHILOX: counter-based N(0,1) generated on the device (Philox4x32-10 +
@@ -111,8 +117,23 @@ mcnla_status mcnla_sketcher_set_seed(mcnla_stage s, uint64_t seed, int omega_mod
  * MCNLA_PREC_F64, FP32 for the mixed precisions. */
 mcnla_status mcnla_sketcher_run(mcnla_stage s, const void* A, int64_t lda, int a_dtype,
                                 int a_order, void* Y, void* stream);
+/* setExponent (gaussian_projection_sketcher.hh:87): q power iterations after Y = A Omega,
+ * Omega := A'Y, Y := A Omega (gaussian_projection_sketcher.hpp:96-100), so Y = (A A')^q A Omega
+ * and Omega ends as A' times the previous Y.  GAUSSIAN kind with MCNLA_PREC_F64 only; the other
+ * kinds accept 0 (row_block_gaussian_projection_sketcher.hpp:158-168 allows no other value). */
+mcnla_status mcnla_sketcher_set_exponent(mcnla_stage s, int64_t exponent);
 /* Device pointer to the n x (N*l) row-major FP64 Omega of the last run (for checking). */
 mcnla_status mcnla_sketcher_omega(mcnla_stage s, const double** omega);
+/* Device pointer to the N*l column indices (int64) the last column-sampling run copied: column
+ * j of Y is column idx[j] of A (column_sampling_sketcher.hpp:92-96). */
+mcnla_status mcnla_sketcher_indices(mcnla_stage s, const int64_t** idx);
+/* The column-sampling index stream on the host: `len` raw std::mt19937 draws of the seed chain
+ * of `sketcher_kind` (col-block or row-block, as for Omega) in `threads` chunks, each stored in
+ * the reference's 32-bit index_t and mapped to abs(draw) % ncol (streams.hpp:118-150,
+ * uniform_bits.hpp:39-53, uniform_bits_detail_nomkl.hh:33-40, column_sampling_sketcher.hpp:95).
+ * Every sketch draws its indices this way (both Omega modes). */
+mcnla_status mcnla_column_indices_reference(uint64_t seed, int sketcher_kind, int threads,
+                                            int64_t len, int64_t ncol, int64_t* out);
 /* The REFERENCE-mode stream on the host (no device involved): `len` N(0,1) values for `seed`,
  * with the col-block (kind COL_BLOCK_GAUSSIAN / GAUSSIAN) or row-block seed chain and `threads`
  * chunks (streams.hpp:118-150, gaussian.hpp:39-56).  Used by the drivers and for checking. */

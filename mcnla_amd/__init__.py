@@ -9,6 +9,7 @@ from ._lib import McnlaError, lib
 from .device import DeviceArray, device_count, mem_info, synchronize
 from .isvd import (
     ColBlockGaussianProjectionSketcher, ColBlockGramianFormer, CollectionFromPartialSumToRowBlockConverter,
+    ColumnSamplingSketcher, RowBlockColumnSamplingSketcher, column_indices_reference,
     CollectionFromRowBlockConverter, CollectionToRowBlockConverter, DummyConverter,
     GaussianProjectionSketcher, GramianFormer, GramianOrthogonalizer, KolmogorovNagumoIntegrator,
     MatrixFromColBlockToAllConverter, MatrixFromRowBlockConverter, MatrixFromRowBlockToAllConverter,

@@ -17,7 +17,8 @@ COL_MAJOR, ROW_MAJOR = 0, 1
 # mcnla_precision
 PREC_F64, PREC_MIXED_BF16, PREC_MIXED_BF16X3 = 0, 1, 2
 # kinds
-SKETCHER_COL_BLOCK_GAUSSIAN, SKETCHER_ROW_BLOCK_GAUSSIAN, SKETCHER_GAUSSIAN = 0, 1, 2
+(SKETCHER_COL_BLOCK_GAUSSIAN, SKETCHER_ROW_BLOCK_GAUSSIAN, SKETCHER_GAUSSIAN,
+ SKETCHER_COLUMN_SAMPLING, SKETCHER_ROW_BLOCK_COLUMN_SAMPLING) = range(5)
 OMEGA_PHILOX, OMEGA_REFERENCE = 0, 1
 ROUTE_AUTO, ROUTE_SKETCH, ROUTE_GRAM_OF_A = 0, 1, 2
 ORTH_GRAMIAN, ORTH_ROW_BLOCK_GRAMIAN = 0, 1
@@ -113,7 +114,10 @@ SIGNATURES = {
     "mcnla_sketcher_set_seed": (_st, [_vp, _u64, _i, _i]),
     "mcnla_sketcher_run": (_st, [_vp, _vp, _i64, _i, _i, _vp, _vp]),
     "mcnla_sketcher_omega": (_st, [_vp, _pp]),
+    "mcnla_sketcher_set_exponent": (_st, [_vp, _i64]),
+    "mcnla_sketcher_indices": (_st, [_vp, _pp]),
     "mcnla_omega_reference": (_st, [_u64, _i, _i, _i64, _dp]),
+    "mcnla_column_indices_reference": (_st, [_u64, _i, _i, _i64, _i64, ctypes.POINTER(_i64)]),
     "mcnla_orthogonalizer_create": (_st, [ctypes.POINTER(Params), _i, _i, _pp]),
     "mcnla_orthogonalizer_run": (_st, [_vp, _vp, _vp]),
     "mcnla_integrator_create": (_st, [ctypes.POINTER(Params), _i, _i, _i64, _d, _pp]),

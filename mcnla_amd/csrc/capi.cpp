@@ -13,6 +13,8 @@
 namespace mcnla {
 
 void omega_reference(uint32_t seed, bool col_block, int threads, int64_t len, double* out);
+void column_indices_reference(uint32_t seed, bool col_block, int threads, int64_t len,
+                              int64_t ncol, int64_t* out);
 
 namespace {
 thread_local std::string g_error;
@@ -457,9 +459,21 @@ struct Sketcher : mcnla_stage_s {
   uint64_t seed = 0;
   int mode = MCNLA_OMEGA_PHILOX;
   int threads = 1;
+  int64_t exponent = 0;
   SketchCore core;
+  DevBuf idx, partial;             // column-sampling indices; split-K partials of A'Y
+  std::vector<int64_t> host_idx;
   Sketcher() : mcnla_stage_s(kSketcher) {}
+  bool sampling() const {
+    return kind == MCNLA_SKETCHER_COLUMN_SAMPLING || kind == MCNLA_SKETCHER_ROW_BLOCK_COLUMN_SAMPLING;
+  }
 };
+
+// The row-block sketchers draw from Streams(seed); the others from Streams(seed, root, comm),
+// whose per-rank seed_seq adds one step (streams.hpp:118-150).
+bool col_block_chain(int kind) {
+  return kind != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN && kind != MCNLA_SKETCHER_ROW_BLOCK_COLUMN_SAMPLING;
+}
 
 struct Orthogonalizer : mcnla_stage_s {
   DevBuf W, w, V, T, partial, bad;
@@ -485,6 +499,8 @@ const char* sketcher_name(int kind) {
   switch (kind) {
     case MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN: return "Gaussian Projection Sketcher (Column-Block Version)";
     case MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN: return "Gaussian Projection Sketcher (Row-Block Version)";
+    case MCNLA_SKETCHER_COLUMN_SAMPLING: return "Column Sampling Sketcher";
+    case MCNLA_SKETCHER_ROW_BLOCK_COLUMN_SAMPLING: return "Column Sampling Sketcher (Row-Block Version)";
     default: return "Gaussian Projection Sketcher";
   }
 }
@@ -740,6 +756,10 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
   check_params(params);
   MCNLA_REQUIRE(cfg, MCNLA_ERR_INVALID, "null argument");
   check_precision(cfg->precision);
+  MCNLA_REQUIRE(cfg->sketcher >= MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN &&
+                cfg->sketcher <= MCNLA_SKETCHER_GAUSSIAN, MCNLA_ERR_INVALID,
+                "the solver sketches with a Gaussian projection (column sampling is a stage "
+                "sketcher)");
   MCNLA_REQUIRE(cfg->integrator >= MCNLA_INT_KN && cfg->integrator <= MCNLA_INT_ROW_BLOCK_EXTRINSIC_MEAN,
                 MCNLA_ERR_INVALID, "unknown integrator kind");
   MCNLA_REQUIRE(cfg->former >= MCNLA_FORM_SVD && cfg->former <= MCNLA_FORM_ROW_BLOCK_SYMMETRIC,
@@ -936,7 +956,8 @@ mcnla_status mcnla_sketcher_create(const mcnla_params* params, int kind, int pre
     check_params(params);
     check_precision(precision);
     MCNLA_REQUIRE(out, MCNLA_ERR_INVALID, "out is null");
-    MCNLA_REQUIRE(kind >= MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN && kind <= MCNLA_SKETCHER_GAUSSIAN,
+    MCNLA_REQUIRE(kind >= MCNLA_SKETCHER_COL_BLOCK_GAUSSIAN &&
+                  kind <= MCNLA_SKETCHER_ROW_BLOCK_COLUMN_SAMPLING,
                   MCNLA_ERR_INVALID, "unknown sketcher kind");
     auto s = std::make_unique<Sketcher>();
     s->kind = kind;
@@ -944,7 +965,13 @@ mcnla_status mcnla_sketcher_create(const mcnla_params* params, int kind, int pre
     s->prm = *params;
     s->name = sketcher_name(kind);
     s->part_names = "random generating / projection";
-    s->core.omega.ensure(params->ncol * s->nl() * sizeof(double));
+    if (s->sampling()) {
+      MCNLA_REQUIRE(precision == MCNLA_PREC_F64, MCNLA_ERR_INVALID,
+                    "column sampling copies A: use MCNLA_PREC_F64");
+      s->idx.ensure(s->nl() * sizeof(int64_t));
+    } else {
+      s->core.omega.ensure(params->ncol * s->nl() * sizeof(double));
+    }
     *out = s.release();
     return MCNLA_OK;
   });
@@ -971,13 +998,66 @@ mcnla_status mcnla_sketcher_run(mcnla_stage s, const void* A, int64_t lda, int a
     check_a(k->prm, A, lda, a_dtype, a_order);
     MCNLA_REQUIRE(Y, MCNLA_ERR_INVALID, "Y is null");
     hipStream_t st = S(stream);
+    const auto& p = k->prm;
+    const int64_t nl = k->nl();
     k->timer.start(st);
-    k->core.generate(k->prm, k->seed, k->mode, k->threads,
-                     k->kind != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN, st);
+    if (k->sampling()) {  // column_sampling_sketcher.hpp:84-96
+      k->host_idx.resize(nl);
+      column_indices_reference(static_cast<uint32_t>(k->seed), col_block_chain(k->kind), k->threads,
+                               nl, p.ncol, k->host_idx.data());
+      MCNLA_HIP(hipMemcpyAsync(k->idx.p, k->host_idx.data(), nl * sizeof(int64_t),
+                               hipMemcpyHostToDevice, st));
+      MCNLA_HIP(hipStreamSynchronize(st));  // host_idx is reused by the next run
+      k->timer.mark(st);
+      gather_columns(A, lda, a_dtype == MCNLA_F32, a_order == MCNLA_COL_MAJOR, p.nrow,
+                     k->idx.as<int64_t>(), nl, static_cast<double*>(Y), st);
+      k->timer.mark(st);
+      k->computed = true;
+      return MCNLA_OK;
+    }
+    k->core.generate(p, k->seed, k->mode, k->threads, col_block_chain(k->kind), st);
     k->timer.mark(st);
-    k->core.project(k->prm, k->precision, A, lda, a_dtype, a_order, Y, st);
+    k->core.project(p, k->precision, A, lda, a_dtype, a_order, Y, st);
+    for (int64_t q = 0; q < k->exponent; ++q) {  // gaussian_projection_sketcher.hpp:96-100
+      GemmF64Args t;  // Omega := A' Y  (n x Nl)
+      t.M = p.ncol;
+      t.N = nl;
+      t.K = p.nrow;
+      t.A = A;
+      t.lda = lda;
+      t.a_mcontig = (a_order == MCNLA_ROW_MAJOR);  // row j of A' is column j of A
+      t.a_f32 = (a_dtype == MCNLA_F32);
+      t.B = Y;
+      t.ldb = nl;
+      t.b_ncontig = true;
+      t.C = k->core.omega.as<double>();
+      t.ldc = nl;
+      t.shape = GemmShape::kSquare;
+      t.splits = gemm_f64_splits_for(ceil_div(p.ncol, 128) * ceil_div(nl, 128), p.nrow, 1024);
+      if (t.splits > 1) {
+        k->partial.ensure(gemm_f64_partial_elems(p.ncol, nl, t.splits) * sizeof(double));
+        t.partial = k->partial.as<double>();
+      }
+      gemm_f64(t, st);
+      k->core.project(p, k->precision, A, lda, a_dtype, a_order, Y, st);  // Y := A Omega
+    }
     k->timer.mark(st);
     k->computed = true;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_sketcher_set_exponent(mcnla_stage s, int64_t exponent) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kSketcher, MCNLA_ERR_INVALID, "not a sketcher");
+    auto* k = static_cast<Sketcher*>(s);
+    MCNLA_REQUIRE(exponent >= 0, MCNLA_ERR_INVALID, "negative exponent");
+    MCNLA_REQUIRE(exponent == 0 || k->kind == MCNLA_SKETCHER_GAUSSIAN, MCNLA_ERR_INVALID,
+                  "only the Gaussian projection sketcher has power iterations");
+    MCNLA_REQUIRE(exponent == 0 || k->precision == MCNLA_PREC_F64, MCNLA_ERR_INVALID,
+                  "power iterations need MCNLA_PREC_F64");
+    k->exponent = exponent;
+    k->computed = false;
     return MCNLA_OK;
   });
 }
@@ -986,7 +1066,33 @@ mcnla_status mcnla_sketcher_omega(mcnla_stage s, const double** omega) {
   return guard([&] {
     MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kSketcher, MCNLA_ERR_INVALID, "not a sketcher");
     MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "sketcher has not run");
+    MCNLA_REQUIRE(!static_cast<Sketcher*>(s)->sampling(), MCNLA_ERR_INVALID,
+                  "a column-sampling sketcher has no Omega");
     *omega = static_cast<Sketcher*>(s)->core.omega.as<double>();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_sketcher_indices(mcnla_stage s, const int64_t** idx) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kSketcher, MCNLA_ERR_INVALID, "not a sketcher");
+    MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "sketcher has not run");
+    MCNLA_REQUIRE(static_cast<Sketcher*>(s)->sampling(), MCNLA_ERR_INVALID,
+                  "not a column-sampling sketcher");
+    MCNLA_REQUIRE(idx, MCNLA_ERR_INVALID, "idx is null");
+    *idx = static_cast<Sketcher*>(s)->idx.as<int64_t>();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_column_indices_reference(uint64_t seed, int sketcher_kind, int threads,
+                                            int64_t len, int64_t ncol, int64_t* out) {
+  return guard([&] {
+    MCNLA_REQUIRE(out || len == 0, MCNLA_ERR_INVALID, "out is null");
+    MCNLA_REQUIRE(len >= 0 && ncol >= 1, MCNLA_ERR_INVALID, "bad length or column count");
+    MCNLA_REQUIRE(threads >= 1 && threads <= 1024, MCNLA_ERR_INVALID, "threads out of range");
+    column_indices_reference(static_cast<uint32_t>(seed), col_block_chain(sketcher_kind), threads,
+                             len, ncol, out);
     return MCNLA_OK;
   });
 }
@@ -997,8 +1103,7 @@ mcnla_status mcnla_omega_reference(uint64_t seed, int sketcher_kind, int threads
     MCNLA_REQUIRE(out || len == 0, MCNLA_ERR_INVALID, "out is null");
     MCNLA_REQUIRE(len >= 0, MCNLA_ERR_INVALID, "negative length");
     MCNLA_REQUIRE(threads >= 1 && threads <= 1024, MCNLA_ERR_INVALID, "threads out of range");
-    omega_reference(static_cast<uint32_t>(seed), sketcher_kind != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN,
-                    threads, len, out);
+    omega_reference(static_cast<uint32_t>(seed), col_block_chain(sketcher_kind), threads, len, out);
     return MCNLA_OK;
   });
 }
@@ -1244,7 +1349,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     const int64_t n = p.ncol;
     // 1. sketch: Y = A * Omega; on the Gram-of-A route, H = A'A (A as row-major FP64) instead
     s->sketch.generate(p, c.seed, c.omega_mode, c.omega_threads,
-                       c.sketcher != MCNLA_SKETCHER_ROW_BLOCK_GAUSSIAN, st);
+                       col_block_chain(c.sketcher), st);
     const double* Ar = nullptr;
     int64_t ldar = n;
     if (!via_a) {

@@ -73,6 +73,9 @@ void synth_finish(int64_t m, int64_t n, const double* Tm, double noise, uint64_t
                   int64_t lda, bool a_f32, bool a_colmajor, hipStream_t st);
 // out[e] = scale * g(e), g the Philox N(0,1) stream of `seed`.
 void gaussian_fill(double* out, int64_t count, uint64_t seed, double scale, hipStream_t st);
+// Y (m x nl row-major FP64) = the columns idx[0..nl) of A (FP32 or FP64, either order).
+void gather_columns(const void* A, int64_t lda, bool a_f32, bool a_colmajor, int64_t m,
+                    const int64_t* idx, int64_t nl, double* Y, hipStream_t st);
 void synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
                    const double* sigma, double noise, uint64_t seed, void* A, int64_t lda,
                    bool a_f32, bool a_colmajor, hipStream_t st);

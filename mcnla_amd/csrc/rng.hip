@@ -104,6 +104,22 @@ __global__ void synth_finish_kernel(int64_t m, int64_t n, const double* __restri
   }
 }
 
+// Y (m x nl, row-major FP64) = columns idx[0..nl) of A (element (i, k) at A[i + k*lda] when
+// col-major, A[i*lda + k] when row-major).  One element per thread, grid-stride; Y is written in
+// order.
+template <typename T>
+__global__ void gather_columns_kernel(const T* __restrict__ A, int64_t lda, bool colmajor,
+                                      int64_t m, const int64_t* __restrict__ idx, int64_t nl,
+                                      double* __restrict__ Y) {
+  const int64_t total = m * nl;
+  for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = e / nl, c = e % nl;
+    const int64_t k = idx[c];
+    Y[e] = static_cast<double>(colmajor ? A[i + k * lda] : A[i * lda + k]);
+  }
+}
+
 __global__ void scale_stream_kernel(double* __restrict__ out, int64_t count, double scale) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
        e += (int64_t)gridDim.x * blockDim.x)
@@ -123,6 +139,20 @@ void synth_finish(int64_t m, int64_t n, const double* Tm, double noise, uint64_t
   else
     hipLaunchKernelGGL(synth_finish_kernel<double>, dim3(static_cast<unsigned>(blocks)), dim3(256),
                        0, st, m, n, Tm, noise, seed, static_cast<double*>(A), lda, a_colmajor);
+  MCNLA_LAUNCHED();
+}
+
+void gather_columns(const void* A, int64_t lda, bool a_f32, bool a_colmajor, int64_t m,
+                    const int64_t* idx, int64_t nl, double* Y, hipStream_t st) {
+  if (m <= 0 || nl <= 0) return;
+  int64_t blocks = ceil_div(m * nl, 256);
+  if (blocks > 65536) blocks = 65536;
+  if (a_f32)
+    hipLaunchKernelGGL(gather_columns_kernel<float>, dim3(static_cast<unsigned>(blocks)), dim3(256),
+                       0, st, static_cast<const float*>(A), lda, a_colmajor, m, idx, nl, Y);
+  else
+    hipLaunchKernelGGL(gather_columns_kernel<double>, dim3(static_cast<unsigned>(blocks)), dim3(256),
+                       0, st, static_cast<const double*>(A), lda, a_colmajor, m, idx, nl, Y);
   MCNLA_LAUNCHED();
 }
 

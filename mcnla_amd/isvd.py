@@ -157,18 +157,35 @@ class _GaussianSketcher(_Stage):
     def __init__(self, parameters, seed=0, exponent=0, precision=_lib.PREC_F64,
                  omega_mode=_lib.OMEGA_PHILOX, omega_threads=1):
         super().__init__(parameters, precision)
-        if exponent != 0:
-            raise ValueError("power iterations (exponent > 0) are not supported")
         self._seed = seed
         self._mode = omega_mode
         self._threads = omega_threads
+        self._exponent = int(exponent)
+        if self._exponent != 0:
+            self._handle()  # the library checks the kind and precision now
 
     def _make(self):
         p = self.parameters.c_params()
         h = _out_handle(_lib.lib().mcnla_sketcher_create, ctypes.byref(p), self._kind,
                         self.precision)
-        _lib.check(_lib.lib().mcnla_sketcher_set_seed(h, self._seed, self._mode, self._threads))
+        try:
+            _lib.check(_lib.lib().mcnla_sketcher_set_seed(h, self._seed, self._mode,
+                                                         self._threads))
+            _lib.check(_lib.lib().mcnla_sketcher_set_exponent(h, self._exponent))
+        except Exception:
+            _lib.lib().mcnla_stage_destroy(h)
+            raise
         return h
+
+    def exponent(self):
+        return self._exponent
+
+    def setExponent(self, exponent):
+        """Power iterations (gaussian_projection_sketcher.hh:87): Y = (A A')^q A Omega."""
+        self._exponent = int(exponent)
+        if self._h is not None:
+            _lib.check(_lib.lib().mcnla_sketcher_set_exponent(self._h, self._exponent))
+        return self
 
     def setSeed(self, seed, omega_mode=None, omega_threads=None):
         self._seed = int(seed)
@@ -207,6 +224,41 @@ class RowBlockGaussianProjectionSketcher(_GaussianSketcher):
 
 class GaussianProjectionSketcher(_GaussianSketcher):
     _kind = _lib.SKETCHER_GAUSSIAN
+
+
+class _ColumnSamplingSketcher(_GaussianSketcher):
+    """Y's columns are columns of A picked by the reference's seeded index stream
+    (column_sampling_sketcher.hpp:84-96); FP64 only, no Omega."""
+
+    def __init__(self, parameters, seed=0, omega_threads=1):
+        super().__init__(parameters, seed=seed, precision=_lib.PREC_F64,
+                         omega_mode=_lib.OMEGA_REFERENCE, omega_threads=omega_threads)
+
+    def omega(self):
+        raise AttributeError("a column-sampling sketcher has no Omega")
+
+    def indices(self):
+        """The column of A behind each column of Y (last run)."""
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().mcnla_sketcher_indices(self._handle(), ctypes.byref(p)))
+        return DeviceArray.view(p.value, (self.parameters.dimSketchTotal(),), np.int64).numpy()
+
+
+class ColumnSamplingSketcher(_ColumnSamplingSketcher):
+    _kind = _lib.SKETCHER_COLUMN_SAMPLING
+
+
+class RowBlockColumnSamplingSketcher(_ColumnSamplingSketcher):
+    _kind = _lib.SKETCHER_ROW_BLOCK_COLUMN_SAMPLING
+
+
+def column_indices_reference(seed, ncol, length, kind=ColumnSamplingSketcher, threads=1):
+    """The column-sampling index stream on the host (no device)."""
+    out = np.empty(int(length), np.int64)
+    _lib.check(_lib.lib().mcnla_column_indices_reference(
+        int(seed), kind._kind, int(threads), int(length), int(ncol),
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+    return out
 
 
 # ---------------------------------------------------------------------------------------------

@@ -49,6 +49,9 @@ def lib():
         L.orc_omega_ref.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _i64, _d]
         L.orc_sketch.argtypes = [_i64, _i64, _i64, _d, _i64, ctypes.c_int, _d, _d]
+        L.orc_column_indices.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _i64, _i64,
+                                         ctypes.POINTER(ctypes.c_int64)]
         L.orc_gramian_orthogonalize.argtypes = [_i64, _i64, _i64, _d]
         L.orc_gramian_orthogonalize.restype = ctypes.c_int
         for name in ("orc_kn_rowblock", "orc_kn_xform", "orc_kn_gramian", "orc_wy_rowblock",
@@ -113,6 +116,30 @@ def omega_ref(seed, n, Nl, col_block=True, nprocs=1, rank=0, threads=1):
     lib().orc_omega_ref(seed & 0xFFFFFFFF, 1 if col_block else 0, nprocs, rank, threads, n * Nl,
                         _p(out))
     return out
+
+
+def column_indices_ref(seed, ncol, length, col_block=True, nprocs=1, rank=0, threads=1):
+    """The column-sampling sketchers' index stream for one rank (orc_column_indices)."""
+    out = np.empty(int(length), np.int64)
+    lib().orc_column_indices(seed & 0xFFFFFFFF, 1 if col_block else 0, nprocs, rank, threads,
+                             int(length), int(ncol),
+                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return out
+
+
+def column_sample(A, idx):
+    """Y = the columns idx of A (column_sampling_sketcher.hpp:94-96)."""
+    return np.ascontiguousarray(np.asarray(A)[:, np.asarray(idx)])
+
+
+def power_sketch(A, omega, q, a_colmajor=True):
+    """Y = A Omega, then q times Omega := A'Y, Y := A Omega (gaussian_projection_sketcher.hpp
+    :96-100).  Returns (Y, final Omega)."""
+    Y = sketch(A, omega, a_colmajor)
+    for _ in range(q):
+        omega = sketch(np.ascontiguousarray(np.asarray(A).T), Y, a_colmajor)
+        Y = sketch(A, omega, a_colmajor)
+    return Y, omega
 
 
 def sketch(A, omega, a_colmajor=True):

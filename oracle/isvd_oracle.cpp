@@ -18,10 +18,13 @@
 //   orc_symmetric_former                   -> data/a_sym.mtx + data/q.mtx -> data/u_sym.mtx
 //   orc_omega_ref                          -> moments + determinism only ("parity unpinned" for
 //                                             the bit stream: no reference fixture stores Omega)
+//   orc_column_indices                     -> range, uniformity and the 32-bit wrap only
+//                                             ("parity unpinned" for the bit stream, as Omega)
 //   orc_sketch                             -> closed form (numpy A @ Omega)
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -287,6 +290,36 @@ void orc_omega_ref(uint32_t seed, int col_block, int nprocs, int rank, int threa
     std::mt19937 gen(per_thread[t]);
     std::normal_distribution<> dis(0.0, 1.0);
     for (int64_t i = 0; i < chunk; ++i) out[start + i] = dis(gen);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Column-sampling indices (column_sampling_sketcher.hpp:82-96, row_block_column_sampling_sketcher.hpp
+// :80-96): the Omega seed chain above, each thread's chunk filled with raw mt19937 draws
+// (uniform_bits.hpp:39-53, uniform_bits_detail_nomkl.hh:33-40) into the 32-bit index_t of the
+// default build (def.hpp:34-38: draws of 2^31 and above wrap to negative), then abs(v) % ncol.
+void orc_column_indices(uint32_t seed, int col_block, int nprocs, int rank, int threads,
+                        int64_t len, int64_t ncol, int64_t* out) {
+  uint32_t s0 = seed;
+  if (col_block) {
+    std::seed_seq seq{seed};
+    std::vector<uint32_t> per_rank(nprocs);
+    seq.generate(per_rank.begin(), per_rank.end());
+    s0 = per_rank[rank];
+  }
+  std::seed_seq seq2{s0};
+  std::vector<uint32_t> per_thread(threads);
+  seq2.generate(per_thread.begin(), per_thread.end());
+  for (int t = 0; t < threads; ++t) {
+    int64_t chunk = len / threads;
+    const int64_t start = chunk * t;
+    if (t == threads - 1) chunk = len - start;
+    std::mt19937 gen(per_thread[t]);
+    for (int64_t i = 0; i < chunk; ++i) {
+      const uint32_t raw = static_cast<uint32_t>(gen());
+      const int64_t v = raw >= 0x80000000u ? static_cast<int64_t>(raw) - 0x100000000LL : raw;
+      out[start + i] = std::llabs(v) % ncol;
+    }
   }
 }
 

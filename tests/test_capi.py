@@ -157,3 +157,39 @@ def test_device_view_leading_dimension():
     v = mc.DeviceArray.view(0x1000, (300, 40), np.float32, "F", ld=301)
     assert v.ld == 301 and v.order_code == _lib.COL_MAJOR and v.dtype_code == _lib.F32
     assert mc.DeviceArray.view(0x1000, (300, 40), np.float64, "C").ld == 40
+
+
+def test_column_indices_match_oracle():
+    """The column-sampling index stream of the library's host generator equals the oracle's for
+    both seed chains and several thread chunkings; every index lies in [0, ncol) and the stream is
+    uniform (chi-square over 50 bins).  The bit stream itself is parity unpinned, as Omega's: no
+    reference fixture stores it."""
+    import oracle
+    for kind, cb in ((mc.ColumnSamplingSketcher, True), (mc.RowBlockColumnSamplingSketcher, False)):
+        for threads in (1, 3, 8):
+            got = mc.column_indices_reference(2024, 997, 1500, kind=kind, threads=threads)
+            assert np.array_equal(got, oracle.column_indices_ref(2024, 997, 1500, col_block=cb,
+                                                                 threads=threads))
+            assert got.min() >= 0 and got.max() < 997
+    big = mc.column_indices_reference(5, 50, 200000)
+    counts = np.bincount(big, minlength=50)
+    chi2 = ((counts - 4000.0) ** 2 / 4000.0).sum()
+    assert chi2 < 100.0  # 49 degrees of freedom: p ~ 3e-5 at 100
+
+
+def test_sketcher_variants_rejected_where_the_reference_has_none():
+    L = mc.lib()
+    h = ctypes.c_void_p()
+    prm = _params()
+    # column sampling copies A: no mixed precision
+    assert L.mcnla_sketcher_create(ctypes.byref(prm), _lib.SKETCHER_COLUMN_SAMPLING,
+                                   _lib.PREC_MIXED_BF16, ctypes.byref(h)) == _lib.ERR_INVALID
+    assert L.mcnla_sketcher_create(ctypes.byref(prm), 5, 0, ctypes.byref(h)) == _lib.ERR_INVALID
+    # the fused solver sketches with a Gaussian projection only
+    with pytest.raises(mc.McnlaError) as e:
+        cfg = _lib.SolverConfig()
+        L.mcnla_solver_config_default(ctypes.byref(cfg))
+        cfg.sketcher = _lib.SKETCHER_COLUMN_SAMPLING
+        b = ctypes.c_size_t()
+        _lib.check(L.mcnla_solver_required_bytes(ctypes.byref(prm), ctypes.byref(cfg), ctypes.byref(b)))
+    assert e.value.status == _lib.ERR_INVALID

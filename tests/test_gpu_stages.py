@@ -101,6 +101,50 @@ def test_sketch_mixed(gpu, order):
     assert np.linalg.norm(Y3 - exact) / np.linalg.norm(exact) < 5e-5
 
 
+@pytest.mark.parametrize("kind,cb", [(mc.ColumnSamplingSketcher, True),
+                                     (mc.RowBlockColumnSamplingSketcher, False)])
+@pytest.mark.parametrize("dtype,order", [(np.float64, "F"), (np.float32, "C")])
+def test_column_sampling_sketcher(gpu, kind, cb, dtype, order):
+    """Column sampling (column_sampling_sketcher.hpp:84-96): Y's columns are exactly the columns
+    of A at the oracle's index stream (3-thread chunking)."""
+    m, n, k, p, N = 301, 257, 7, 5, 6
+    note("column sampling %s %s %s" % (kind.__name__, np.dtype(dtype).name, order))
+    A = np.random.default_rng(6).standard_normal((m, n)).astype(dtype)
+    P = params(m, n, k, p, N)
+    sk = kind(P, seed=31, omega_threads=3)
+    Y = P.createCollectionQ()
+    sk(mc.DeviceArray.from_numpy(np.asarray(A, order=order), order=order), Y)
+    idx = sk.indices()
+    assert np.array_equal(idx, oracle.column_indices_ref(31, n, N * (k + p), col_block=cb,
+                                                         threads=3))
+    assert np.array_equal(Y.numpy(), oracle.column_sample(A.astype(np.float64), idx))
+    assert sk.time() > 0
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_power_iteration_sketcher(gpu, order):
+    """GaussianProjectionSketcher with exponent 2 (gaussian_projection_sketcher.hpp:96-100) on the
+    reference Omega against the oracle: Y = (AA')^2 A Omega and the final Omega = A'(AA')A Omega."""
+    m, n, k, p, N, q = 700, 300, 6, 4, 3, 2
+    note("power iteration sketcher order=%s" % order)
+    A = np.random.default_rng(12).standard_normal((m, n)) / np.sqrt(m)
+    P = params(m, n, k, p, N)
+    sk = mc.GaussianProjectionSketcher(P, seed=44, exponent=q, omega_mode=mc.OMEGA_REFERENCE)
+    assert sk.exponent() == q
+    Y = P.createCollectionQ()
+    sk(mc.DeviceArray.from_numpy(np.asarray(A, order=order), order=order), Y)
+    om0 = oracle.omega_ref(44, n, N * (k + p), col_block=True)
+    Y_o, om_o = oracle.power_sketch(A, om0, q, a_colmajor=(order == "F"))
+    assert np.abs(Y.numpy() - Y_o).max() < 1e-12 * np.abs(Y_o).max()
+    assert np.abs(sk.omega() - om_o).max() < 1e-12 * np.abs(om_o).max()
+    with pytest.raises(mc.McnlaError) as e:  # the col-block sketcher has no power iterations
+        mc.ColBlockGaussianProjectionSketcher(P, seed=1, exponent=1)
+    assert e.value.status == mc._lib.ERR_INVALID
+    with pytest.raises(mc.McnlaError) as e:  # FP64 only
+        mc.GaussianProjectionSketcher(P, seed=1, exponent=1, precision=mc.PREC_MIXED_BF16)
+    assert e.value.status == mc._lib.ERR_INVALID
+
+
 # ------------------------------------------------------------------------------------------------
 @pytest.mark.parametrize("kind", [mc.GramianOrthogonalizer, mc.RowBlockGramianOrthogonalizer])
 def test_orthogonalizer_fixture(gpu, kind):

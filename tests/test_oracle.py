@@ -216,3 +216,26 @@ def test_planted_mistake_is_caught(qi):
     assert np.abs(Q - qb).max() > 1e-3
     Q, it = oracle.kn_rowblock(qi, 11, 12, 256, 1e-3)  # wrong tolerance -> wrong count
     assert it != 81
+
+
+def test_power_sketch_closed_form():
+    """q power iterations (gaussian_projection_sketcher.hpp:96-100) give Y = (A A')^q A Omega and
+    leave Omega = A' (A A')^(q-1) A Omega_0; the restatement is checked against those closed
+    forms in both storage orders."""
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((60, 40)) / 8.0
+    om = rng.standard_normal((40, 12))
+    AAt = A @ A.T
+    for q in (0, 1, 3):
+        for cm in (True, False):
+            Y, om_q = oracle.power_sketch(A, om, q, a_colmajor=cm)
+            want = np.linalg.matrix_power(AAt, q) @ A @ om
+            assert np.abs(Y - want).max() < 1e-13 * np.abs(want).max()
+            if q:
+                want_om = A.T @ np.linalg.matrix_power(AAt, q - 1) @ A @ om
+                assert np.abs(om_q - want_om).max() < 1e-13 * np.abs(want_om).max()
+
+
+def test_column_sample_is_a_gather():
+    A = np.arange(30.0).reshape(5, 6)
+    assert np.array_equal(oracle.column_sample(A, [5, 0, 0, 3]), A[:, [5, 0, 0, 3]])
