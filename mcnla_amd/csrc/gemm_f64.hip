@@ -374,23 +374,37 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
     const int64_t left = kend - (kbeg + kt * BK);
     const int kvalid = left < BK ? static_cast<int>(left) : BK;  // <= 0 past the last stage
     if (!live) return;
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
+    double af[2][FM], bf[2][FN];
+    auto frags = [&](int kk, int b) {
       const int krow = kk + lane / 16;
-      double af[FM], bf[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
         const double v = as[krow * LDA + wm0 + i * 16 + lane % 16];
-        af[i] = krow < kvalid ? v : 0.0;
+        af[b][i] = krow < kvalid ? v : 0.0;
       }
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bf[j] = bs[krow * LDB + wn0 + j * 16 + lane % 16];
+      for (int j = 0; j < FN; ++j) bf[b][j] = bs[krow * LDB + wn0 + j * 16 + lane % 16];
+    };
+    frags(0, 0);
+    __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int b = (kk / 4) & 1;
+      if (kk + 4 < BK) frags(kk + 4, b ^ 1);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[b][i], bf[b][j], acc[i][j], 0, 0, 0);
+      if (kk + 4 < BK) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // one fragment read pair between every four MFMAs
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        }
+      }
     }
+    __builtin_amdgcn_s_setprio(0);
   };
   if (ntk > 0) {
     issue(0, A0, B0);
