@@ -268,13 +268,13 @@ __device__ void kn_step_ns(int t, IntegratorState* st, int l, double invN, doubl
 
 // One KN iteration's reductions and l x l step in one launch (after the split-K GEMM that
 // leaves Pbg[s] = partial Bs * Bc over the s-th K range):
-//   1. workgroup b owns rows [128 b, 128 b + 128) of the Nl x l matrices: Bgc = invN sum_s Pbg[s]
-//      (fixed order), written out, and its block partials Bc'Bc, Bc'Bgc -> P[b];
+//   1. workgroup b owns rows [64 b, 64 b + 64) of the Nl x l matrices: Bgc = invN sum_s Pbg[s]
+//      (fixed order), written out, and its block partials Bc'Bc, Bc'Bgc -> P[b] (f64 MFMA);
 //   2. the last workgroup to finish (a ticket counter, taken after a device-scope fence; no
 //      waiting) sums P over blocks in block order and runs the NS step above.
 // The result is deterministic: no sum depends on which workgroup finishes last.
 constexpr int kStepThreads = 512;
-constexpr int kStepRows = 128;
+constexpr int kStepRows = 64;
 
 __global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
     int t, IntegratorState* st, int64_t nl, int l, double invN, double tol,
@@ -319,15 +319,26 @@ __global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
   __syncthreads();
   const int ll = l * l;
   double* Pb = P + blockIdx.x * static_cast<int64_t>(2 * ll);
-  for (int q = tid; q < 2 * ll; q += kStepThreads) {
-    const int part = q / ll, i = (q / l) % l, j = q % l;
-    const double* y = part == 0 ? sB : sG;
-    double s0 = 0.0, s1 = 0.0;
-    for (int rr = 0; rr < kStepRows; rr += 2) {
-      s0 += sB[rr * ldr + i] * y[rr * ldr + j];
-      s1 += sB[(rr + 1) * ldr + i] * y[(rr + 1) * ldr + j];
+  {  // [B'B, B'G] over the block's rows on f64 MFMA tiles: 2 (np/16)^2 tiles over the waves
+    constexpr int NW = kStepThreads / kWave;
+    const int wave = tid / kWave, lane = tid % kWave;
+    const int nt1 = pad16(l) / 16, tiles = nt1 * nt1;
+    for (int tile = wave; tile < 2 * tiles; tile += NW) {
+      const int part = tile / tiles, i0 = ((tile % tiles) / nt1) * 16, j0 = (tile % nt1) * 16;
+      const double* y = part == 0 ? sB : sG;
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+      for (int k0 = 0; k0 < kStepRows; k0 += 4) {
+        const int k = k0 + lane / 16;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sB[k * ldr + i0 + lane % 16],
+                                                   y[k * ldr + j0 + lane % 16], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + lane / 16 + 4 * r, j = j0 + lane % 16;
+        if (i < l && j < l) Pb[part * ll + i * l + j] = acc[r];
+      }
     }
-    Pb[q] = s0 + s1;
   }
   __threadfence();
   __syncthreads();
