@@ -408,16 +408,24 @@ __global__ __launch_bounds__(kThreads) void kn_update_kernel(
   int64_t r0;
   int c0;
   if (update_tile(nl, l, r0, c0)) {
+    // every panel fragment is loaded before the first MFMA (one memory round trip per wave)
+    double bc[kMaxL / 4], bg[kMaxL / 4], ec[kMaxL / 4];
+#pragma unroll
+    for (int g = 0; g < kMaxL / 4; ++g) {
+      bc[g] = panel_frag(Bc, nl, l, r0, 4 * g);
+      bg[g] = panel_frag(Bg, nl, l, r0, 4 * g);
+      ec[g] = panel_frag(Ec, nl, l, r0, 4 * g);
+    }
     d4 ab = {0.0, 0.0, 0.0, 0.0}, ae = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < np; k0 += 4) {
-      const int k = k0 + lane / 16;
-      const double bc = panel_frag(Bc, nl, l, r0, k0), bg = panel_frag(Bg, nl, l, r0, k0);
-      const double ec = panel_frag(Ec, nl, l, r0, k0);
+#pragma unroll
+    for (int g = 0; g < kMaxL / 4; ++g) {
+      if (4 * g >= np) break;
+      const int k = 4 * g + lane / 16;
       const double f = sF[k * ld + c0 + lane % 16], ci = sC[k * ld + c0 + lane % 16];
-      ab = __builtin_amdgcn_mfma_f64_16x16x4f64(bc, f, ab, 0, 0, 0);
-      ab = __builtin_amdgcn_mfma_f64_16x16x4f64(bg, ci, ab, 0, 0, 0);
-      ae = __builtin_amdgcn_mfma_f64_16x16x4f64(ec, f, ae, 0, 0, 0);
-      ae = __builtin_amdgcn_mfma_f64_16x16x4f64(invN * bc, ci, ae, 0, 0, 0);
+      ab = __builtin_amdgcn_mfma_f64_16x16x4f64(bc[g], f, ab, 0, 0, 0);
+      ab = __builtin_amdgcn_mfma_f64_16x16x4f64(bg[g], ci, ab, 0, 0, 0);
+      ae = __builtin_amdgcn_mfma_f64_16x16x4f64(ec[g], f, ae, 0, 0, 0);
+      ae = __builtin_amdgcn_mfma_f64_16x16x4f64(invN * bc[g], ci, ae, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -709,16 +717,23 @@ __global__ __launch_bounds__(kThreads) void wy_update_kernel(
   int64_t r0;
   int c0;
   if (update_tile(nl, l, r0, c0)) {
+    double bc[kMaxL / 4], bg[kMaxL / 4], ec[kMaxL / 4];
+#pragma unroll
+    for (int q = 0; q < kMaxL / 4; ++q) {
+      bc[q] = panel_frag(Bc, nl, l, r0, 4 * q);
+      bg[q] = panel_frag(Bg, nl, l, r0, 4 * q);
+      ec[q] = panel_frag(Ec, nl, l, r0, 4 * q);
+    }
     d4 bcf = {0.0, 0.0, 0.0, 0.0}, bgg = bcf, bcg = bcf, ecf = bcf;
-    for (int k0 = 0; k0 < np; k0 += 4) {
-      const int k = k0 + lane / 16;
-      const double bc = panel_frag(Bc, nl, l, r0, k0), bg = panel_frag(Bg, nl, l, r0, k0);
-      const double ec = panel_frag(Ec, nl, l, r0, k0);
+#pragma unroll
+    for (int q = 0; q < kMaxL / 4; ++q) {
+      if (4 * q >= np) break;
+      const int k = 4 * q + lane / 16;
       const double f = sF[k * ld + c0 + lane % 16], g = sG[k * ld + c0 + lane % 16];
-      bcf = __builtin_amdgcn_mfma_f64_16x16x4f64(bc, f, bcf, 0, 0, 0);
-      bgg = __builtin_amdgcn_mfma_f64_16x16x4f64(bg, g, bgg, 0, 0, 0);
-      bcg = __builtin_amdgcn_mfma_f64_16x16x4f64(bc, g, bcg, 0, 0, 0);
-      ecf = __builtin_amdgcn_mfma_f64_16x16x4f64(ec, f, ecf, 0, 0, 0);
+      bcf = __builtin_amdgcn_mfma_f64_16x16x4f64(bc[q], f, bcf, 0, 0, 0);
+      bgg = __builtin_amdgcn_mfma_f64_16x16x4f64(bg[q], g, bgg, 0, 0, 0);
+      bcg = __builtin_amdgcn_mfma_f64_16x16x4f64(bc[q], g, bcg, 0, 0, 0);
+      ecf = __builtin_amdgcn_mfma_f64_16x16x4f64(ec[q], f, ecf, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {

@@ -51,46 +51,53 @@ __device__ inline int lds_jacobi(int n, double* a, double* v, bool init_v, doubl
             s = t * c;
             *rotated = 1;
           }
-        } else {
-          p = q = -1;
-        }
+        }  // q == n: the bye of an odd n; p takes part in no rotation this round
         cs[2 * k] = c;
         cs[2 * k + 1] = s;
         pq[2 * k] = p;
         pq[2 * k + 1] = q;
       }
       __syncthreads();
-      // rows p, q of a
-      for (int e = tid; e < npairs * n; e += nt) {
-        const int k = e / n, j = e % n;
-        const int p = pq[2 * k], q = pq[2 * k + 1];
-        if (p < 0 || cs[2 * k + 1] == 0.0) continue;
-        const double c = cs[2 * k], s = cs[2 * k + 1];
-        const double ap = a[p * ld + j], aq = a[q * ld + j];
-        a[p * ld + j] = c * ap - s * aq;
-        a[q * ld + j] = s * ap + c * aq;
-      }
-      __syncthreads();
-      // columns p, q of a and of v
-      for (int e = tid; e < npairs * n; e += nt) {
-        const int k = e / n, i = e % n;
-        const int p = pq[2 * k], q = pq[2 * k + 1];
-        if (p < 0 || cs[2 * k + 1] == 0.0) continue;
-        const double c = cs[2 * k], s = cs[2 * k + 1];
-        const double ap = a[i * ld + p], aq = a[i * ld + q];
-        a[i * ld + p] = c * ap - s * aq;
-        a[i * ld + q] = s * ap + c * aq;
-        const double vp = v[i * ld + p], vq = v[i * ld + q];
-        v[i * ld + p] = c * vp - s * vq;
-        v[i * ld + q] = s * vp + c * vq;
-      }
-      __syncthreads();
-      // the rotation annihilates a_pq analytically; drop its rounding residue
-      for (int k = tid; k < npairs; k += nt) {
-        const int p = pq[2 * k], q = pq[2 * k + 1];
-        if (p >= 0 && cs[2 * k + 1] != 0.0) {
-          a[p * ld + q] = 0.0;
-          a[q * ld + p] = 0.0;
+      // A := J'AJ one 2 x 2 block (pair ka's rows, pair kb's columns) per work item: the block's
+      // rows are rotated first, then its columns -- the same operations as rotating all rows and
+      // then all columns, with no barrier in between since the blocks are disjoint.  The entry
+      // a_pq that a rotation annihilates analytically is set to zero.  V := VJ in the same pass.
+      for (int e = tid; e < npairs * npairs + npairs * n; e += nt) {
+        if (e < npairs * npairs) {
+          const int ka = e / npairs, kb = e % npairs;
+          const int r0 = pq[2 * ka], r1 = pq[2 * ka + 1], c0 = pq[2 * kb], c1 = pq[2 * kb + 1];
+          const double ca = cs[2 * ka], sa = cs[2 * ka + 1], cb = cs[2 * kb], sb = cs[2 * kb + 1];
+          const bool ra = sa != 0.0, rb = sb != 0.0;  // a pair with the bye (index n) has s = 0
+          if (!ra && !rb) continue;
+          const bool v1 = r1 < n, w1 = c1 < n;  // r0, c0 < n always
+          double x00 = a[r0 * ld + c0], x01 = w1 ? a[r0 * ld + c1] : 0.0;
+          double x10 = v1 ? a[r1 * ld + c0] : 0.0, x11 = (v1 && w1) ? a[r1 * ld + c1] : 0.0;
+          if (ra) {
+            const double y00 = ca * x00 - sa * x10, y01 = ca * x01 - sa * x11;
+            const double y10 = sa * x00 + ca * x10, y11 = sa * x01 + ca * x11;
+            x00 = y00; x01 = y01; x10 = y10; x11 = y11;
+          }
+          if (rb) {
+            const double y00 = cb * x00 - sb * x01, y01 = sb * x00 + cb * x01;
+            const double y10 = cb * x10 - sb * x11, y11 = sb * x10 + cb * x11;
+            x00 = y00; x01 = y01; x10 = y10; x11 = y11;
+          }
+          if (ka == kb && ra) {
+            x01 = 0.0;
+            x10 = 0.0;
+          }
+          a[r0 * ld + c0] = x00;
+          if (w1) a[r0 * ld + c1] = x01;
+          if (v1) a[r1 * ld + c0] = x10;
+          if (v1 && w1) a[r1 * ld + c1] = x11;
+        } else {
+          const int f = e - npairs * npairs, k = f / n, i = f % n;
+          const int p = pq[2 * k], q = pq[2 * k + 1];
+          if (cs[2 * k + 1] == 0.0) continue;
+          const double c = cs[2 * k], s = cs[2 * k + 1];
+          const double vp = v[i * ld + p], vq = v[i * ld + q];
+          v[i * ld + p] = c * vp - s * vq;
+          v[i * ld + q] = s * vp + c * vq;
         }
       }
       __syncthreads();
