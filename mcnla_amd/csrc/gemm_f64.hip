@@ -255,6 +255,12 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
 // with s_waitcnt vmcnt and a bare s_barrier.
 typedef __attribute__((address_space(3))) void lds_void;
 
+// One 16-byte-per-lane HBM -> LDS transfer, as a plain (non-template) function: the builtin
+// called with template-dependent operand types makes the host pass drop the kernel's launch stub.
+__device__ __forceinline__ void dma16(const void* g, lds_void* l) {
+  __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+}
+
 template <int VM>
 __device__ __forceinline__ void wait_vmcnt() {
   // gfx9 s_waitcnt fields: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt high bits [15:14]
@@ -347,8 +353,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
       const int64_t gk = (k0 + kk < kend) ? k0 + kk : kbeg;
       int64_t gm = m0 + ch * 128 + lane * 2;
       if (gm >= p.M) gm = m0;
-      __builtin_amdgcn_global_load_lds(A + gm + gk * p.lda, (lds_void*)&Ad[kk * LDA + ch * 128], 16,
-                                       0, 0);
+      dma16(A + gm + gk * p.lda, (lds_void*)&Ad[kk * LDA + ch * 128]);
     }
 #pragma unroll
     for (int q = 0; q < LB; ++q) {
@@ -357,8 +362,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
       const int64_t gk = (k0 + kk < kend) ? k0 + kk : kbeg;
       int64_t gn = n0 + ch * 128 + lane * 2;
       if (gn >= p.N) gn = n0;
-      __builtin_amdgcn_global_load_lds(B + gn + gk * p.ldb, (lds_void*)&Bd[kk * LDB + ch * 128], 16,
-                                       0, 0);
+      dma16(B + gn + gk * p.ldb, (lds_void*)&Bd[kk * LDB + ch * 128]);
     }
   };
   // Branch-free pipeline: the stage two ahead is issued every step (clamped to the last stage,
