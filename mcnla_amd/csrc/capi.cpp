@@ -319,8 +319,8 @@ struct FormCore {
   // kind: MCNLA_FORM_*.  The symmetric former (square symmetric A) uses Wg = Z'Qbar = Qbar'A Qbar
   // in place of Z'Z and forms no V.
   // Split-K of Z = A'Qbar (K = m over ceil(n/128) row tiles) and of Wg = Z'Z (K = n).
-  static int64_t z_kchunk(int64_t m, int64_t n) {
-    return ceil_div(ceil_div(m, gemm_f64_splits_for(ceil_div(n, 128), m, 2048)), 16) * 16;
+  static int64_t z_kchunk(int64_t m, int64_t n) {  // about 4096 workgroups (measured best)
+    return ceil_div(ceil_div(m, gemm_f64_splits_for(ceil_div(n, 128), m, 4096)), 16) * 16;
   }
   static int64_t w_kchunk(int64_t n) {
     return ceil_div(ceil_div(n, gemm_f64_splits_for(1, n, 64)), 16) * 16;

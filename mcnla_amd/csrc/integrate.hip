@@ -357,99 +357,45 @@ __global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
   kn_step_ns<kStepThreads>(t, st, l, invN, tol, P, nblk, F, Cinv, smem, red);
 }
 
-// Products of Nl x l row panels with l x l matrices on f64 MFMA tiles: every wave owns one
-// 16 x 16 output tile (update_tile).  The panels' fragments are read straight from global memory
-// (rows of l doubles), the l x l matrices from LDS (zero-padded to np = pad16(l)).
-constexpr int kUpdWaves = kThreads / kWave;
-
-// Stages the l x l matrices M[0..n) into LDS s[n][np][np+1], zero-padded.
-template <int NM>
-__device__ void stage_small(const double* const (&M)[NM], int l, double* s) {
-  const int np = pad16(l), ld = np + 1;
-  for (int e = threadIdx.x; e < NM * np * np; e += blockDim.x) {
-    const int m = e / (np * np), i = (e / np) % np, j = e % np;
-    s[m * np * ld + i * ld + j] = (i < l && j < l) ? M[m][i * l + j] : 0.0;
-  }
-}
-
-// The output tile of this wave: rows r0 .. r0+15 of the Nl x l result, columns c0 .. c0+15.
-// Returns false when the wave has no tile.
-__device__ __forceinline__ bool update_tile(int64_t nl, int l, int64_t& r0, int& c0) {
-  const int nt1 = pad16(l) / 16;
-  const int64_t t = blockIdx.x * static_cast<int64_t>(kUpdWaves) + threadIdx.x / kWave;
-  r0 = (t / nt1) * 16;
-  c0 = static_cast<int>(t % nt1) * 16;
-  return r0 < nl;
-}
-
-// A fragment of rows r0.. of the row-major Nl x l panel X at k-group k0 (zero past nl and l).
-__device__ __forceinline__ double panel_frag(const double* __restrict__ X, int64_t nl, int l,
-                                             int64_t r0, int k0) {
-  const int lane = threadIdx.x % kWave;
-  const int64_t r = r0 + lane % 16;
-  const int k = k0 + lane / 16;
-  return (r < nl && k < l) ? X[r * l + k] : 0.0;
-}
-
-// B+ = Bc F + Bg Cinv; E+ = Ec F + invN Bc Cinv; FF+ = FFc F (workgroup 0).
+// B+ = Bc F + Bg Cinv; E+ = Ec F + invN Bc Cinv; FF+ = FFc F (block 0).
 __global__ __launch_bounds__(kThreads) void kn_update_kernel(
     int t, const IntegratorState* st, int64_t nl, int l, double invN,
     const double* __restrict__ Bc, const double* __restrict__ Bg, double* __restrict__ Bp,
     const double* __restrict__ Ec, double* __restrict__ Ep, const double* __restrict__ FFc,
     double* __restrict__ FFp, const double* __restrict__ F, const double* __restrict__ Cinv) {
   if (skip(st, t)) return;
-  __shared__ double sm[2 * kMaxL * (kMaxL + 1)];
-  const int np = pad16(l), ld = np + 1;
-  const double* sF = sm;
-  const double* sC = sm + np * ld;
-  stage_small<2>({F, Cinv}, l, sm);
+  __shared__ double sF[kMaxL * kMaxL];
+  __shared__ double sC[kMaxL * kMaxL];
+  for (int e = threadIdx.x; e < l * l; e += kThreads) {
+    sF[e] = F[e];
+    sC[e] = Cinv[e];
+  }
   __syncthreads();
-  const int lane = threadIdx.x % kWave;
-  int64_t r0;
-  int c0;
-  if (update_tile(nl, l, r0, c0)) {
-    // every panel fragment is loaded before the first MFMA (one memory round trip per wave)
-    double bc[kMaxL / 4], bg[kMaxL / 4], ec[kMaxL / 4];
-#pragma unroll
-    for (int g = 0; g < kMaxL / 4; ++g) {
-      bc[g] = panel_frag(Bc, nl, l, r0, 4 * g);
-      bg[g] = panel_frag(Bg, nl, l, r0, 4 * g);
-      ec[g] = panel_frag(Ec, nl, l, r0, 4 * g);
+  const int64_t total = nl * l;
+  for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kThreads) {
+    const int64_t r = e / l;
+    const int j = e % l;
+    const double* bc = Bc + r * l;
+    const double* bg = Bg + r * l;
+    const double* ec = Ec + r * l;
+    double b = 0.0, ee = 0.0;
+    for (int q = 0; q < l; ++q) {
+      const double f = sF[q * l + j], ci = sC[q * l + j];
+      b += bc[q] * f + bg[q] * ci;
+      ee += ec[q] * f + invN * bc[q] * ci;
     }
-    d4 ab = {0.0, 0.0, 0.0, 0.0}, ae = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int g = 0; g < kMaxL / 4; ++g) {
-      if (4 * g >= np) break;
-      const int k = 4 * g + lane / 16;
-      const double f = sF[k * ld + c0 + lane % 16], ci = sC[k * ld + c0 + lane % 16];
-      ab = __builtin_amdgcn_mfma_f64_16x16x4f64(bc[g], f, ab, 0, 0, 0);
-      ab = __builtin_amdgcn_mfma_f64_16x16x4f64(bg[g], ci, ab, 0, 0, 0);
-      ae = __builtin_amdgcn_mfma_f64_16x16x4f64(ec[g], f, ae, 0, 0, 0);
-      ae = __builtin_amdgcn_mfma_f64_16x16x4f64(invN * bc[g], ci, ae, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t i = r0 + lane / 16 + 4 * r;
-      const int j = c0 + lane % 16;
-      if (i < nl && j < l) {
-        Bp[i * l + j] = ab[r];
-        Ep[i * l + j] = ae[r];
-      }
-    }
+    Bp[e] = b;
+    Ep[e] = ee;
   }
   if (blockIdx.x == 0) {
     for (int e = threadIdx.x; e < l * l; e += kThreads) {
       const int i = e / l, j = e % l;
       double acc = 0.0;
-      for (int q = 0; q < l; ++q) acc += FFc[i * l + q] * sF[q * ld + j];
+      for (int q = 0; q < l; ++q) acc += FFc[i * l + q] * sF[q * l + j];
       FFp[e] = acc;
     }
   }
-}
-
-// Workgroups for an Nl x l update: one 16 x 16 tile per wave.
-unsigned update_grid(int64_t nl, int l) {
-  return static_cast<unsigned>(ceil_div(ceil_div(nl, 16) * (pad16(l) / 16), kUpdWaves));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -698,8 +644,7 @@ __global__ __launch_bounds__(kWyThreads) void wy_search_kernel(int t, Integrator
   }
 }
 
-// B+ = Bc Fc + Bg Fgc; U1 = Bc Fgc / N; BsU1 = Bg Fgc (= Bs U1); E+ = U1 + Ec Fc; FF+ = FFc Fc
-// (workgroup 0), on the MFMA tiles of kn_update_kernel.
+// B+ = Bc Fc + Bg Fgc; U1 = Bc Fgc / N; BsU1 = Bg Fgc (= Bs U1); E+ = U1 + Ec Fc; FF+ = FFc Fc.
 __global__ __launch_bounds__(kThreads) void wy_update_kernel(
     int t, const IntegratorState* st, int64_t nl, int l, double invN,
     const double* __restrict__ Bc, const double* __restrict__ Bg, double* __restrict__ Bp,
@@ -707,52 +652,39 @@ __global__ __launch_bounds__(kThreads) void wy_update_kernel(
     double* __restrict__ Ep, const double* __restrict__ FFc, double* __restrict__ FFp,
     const double* __restrict__ Fc, const double* __restrict__ Fgc) {
   if (skip(st, t)) return;
-  __shared__ double sm[2 * kMaxL * (kMaxL + 1)];
-  const int np = pad16(l), ld = np + 1;
-  const double* sF = sm;
-  const double* sG = sm + np * ld;
-  stage_small<2>({Fc, Fgc}, l, sm);
+  __shared__ double sF[kMaxL * kMaxL];
+  __shared__ double sG[kMaxL * kMaxL];
+  for (int e = threadIdx.x; e < l * l; e += kThreads) {
+    sF[e] = Fc[e];
+    sG[e] = Fgc[e];
+  }
   __syncthreads();
-  const int lane = threadIdx.x % kWave;
-  int64_t r0;
-  int c0;
-  if (update_tile(nl, l, r0, c0)) {
-    double bc[kMaxL / 4], bg[kMaxL / 4], ec[kMaxL / 4];
-#pragma unroll
-    for (int q = 0; q < kMaxL / 4; ++q) {
-      bc[q] = panel_frag(Bc, nl, l, r0, 4 * q);
-      bg[q] = panel_frag(Bg, nl, l, r0, 4 * q);
-      ec[q] = panel_frag(Ec, nl, l, r0, 4 * q);
+  const int64_t total = nl * l;
+  for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kThreads) {
+    const int64_t r = e / l;
+    const int j = e % l;
+    const double* bc = Bc + r * l;
+    const double* bg = Bg + r * l;
+    const double* ec = Ec + r * l;
+    double bcf = 0.0, bgg = 0.0, bcg = 0.0, ecf = 0.0;
+    for (int q = 0; q < l; ++q) {
+      const double f = sF[q * l + j], g = sG[q * l + j];
+      bcf += bc[q] * f;
+      bgg += bg[q] * g;
+      bcg += bc[q] * g;
+      ecf += ec[q] * f;
     }
-    d4 bcf = {0.0, 0.0, 0.0, 0.0}, bgg = bcf, bcg = bcf, ecf = bcf;
-#pragma unroll
-    for (int q = 0; q < kMaxL / 4; ++q) {
-      if (4 * q >= np) break;
-      const int k = 4 * q + lane / 16;
-      const double f = sF[k * ld + c0 + lane % 16], g = sG[k * ld + c0 + lane % 16];
-      bcf = __builtin_amdgcn_mfma_f64_16x16x4f64(bc[q], f, bcf, 0, 0, 0);
-      bgg = __builtin_amdgcn_mfma_f64_16x16x4f64(bg[q], g, bgg, 0, 0, 0);
-      bcg = __builtin_amdgcn_mfma_f64_16x16x4f64(bc[q], g, bcg, 0, 0, 0);
-      ecf = __builtin_amdgcn_mfma_f64_16x16x4f64(ec[q], f, ecf, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t i = r0 + lane / 16 + 4 * r;
-      const int j = c0 + lane % 16;
-      if (i < nl && j < l) {
-        const int64_t e = i * l + j;
-        Bp[e] = bcf[r] + bgg[r];
-        U1[e] = invN * bcg[r];
-        BsU1[e] = bgg[r];
-        Ep[e] = invN * bcg[r] + ecf[r];
-      }
-    }
+    Bp[e] = bcf + bgg;
+    U1[e] = invN * bcg;
+    BsU1[e] = bgg;
+    Ep[e] = invN * bcg + ecf;
   }
   if (blockIdx.x == 0) {
     for (int e = threadIdx.x; e < l * l; e += kThreads) {
       const int i = e / l, j = e % l;
       double acc = 0.0;
-      for (int q = 0; q < l; ++q) acc += FFc[i * l + q] * sF[q * ld + j];
+      for (int q = 0; q < l; ++q) acc += FFc[i * l + q] * sF[q * l + j];
       FFp[e] = acc;
     }
   }
@@ -1232,7 +1164,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
                              dim3(kStepThreads), 0, ls, ti, L.st, nl, l, invN, g.tol, L.Pbg, bs_splits,
                              L.B[c], L.Bg[0], L.P, nblk_s, L.F, L.C);
           MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(kn_update_kernel, dim3(update_grid(nl, l)), dim3(kThreads), 0, ls, ti,
+          hipLaunchKernelGGL(kn_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
                              L.st, nl, l, invN, L.B[c], L.Bg[0], L.B[p], L.E[c], L.E[p], L.FF[c],
                              L.FF[p], L.F, L.C);
           MCNLA_LAUNCHED();
@@ -1241,7 +1173,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
           hipLaunchKernelGGL(wy_search_kernel, dim3(1), dim3(kWyThreads), wy_lds, ls, ti, L.st, l,
                              L.D3[c], L.F, L.C);
           MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(wy_update_kernel, dim3(update_grid(nl, l)), dim3(kThreads), 0, ls, ti,
+          hipLaunchKernelGGL(wy_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
                              L.st, nl, l, invN, L.B[c], L.Bg[c], L.B[p], L.U1, L.BsU1, L.E[c],
                              L.E[p], L.FF[c], L.FF[p], L.F, L.C);
           MCNLA_LAUNCHED();
