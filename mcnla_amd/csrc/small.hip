@@ -211,7 +211,10 @@ __global__ void apply_blockdiag_kernel(T* __restrict__ Y, int64_t m, int64_t N, 
 // Former: eig of Wg (l x l) -> eigw (descending), eigv; Wk = eigv[:, :k]; S = sqrt(eigw[:k]).
 // symmetric (Wg = Qbar' A Qbar of a symmetric A): the k largest eigenpairs in ascending order,
 // S = the eigenvalues themselves (row_block_symmetric_former.hpp:65-66,127-130).
-__global__ __launch_bounds__(kSmallThreads) void former_small_kernel(const double* __restrict__ Wg,
+// One l x l problem on one workgroup: 16 waves share each Jacobi round's work.
+constexpr int kFormerThreads = 1024;
+
+__global__ __launch_bounds__(kFormerThreads) void former_small_kernel(const double* __restrict__ Wg,
                                                                     int l, int k, bool symmetric,
                                                                     double* __restrict__ eigw,
                                                                     double* __restrict__ eigv,
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(kSmallThreads) void former_small_kernel(const doubl
   __shared__ double v[kLds];
   __shared__ double cs[2 * kMaxL];
   __shared__ int pq[2 * kMaxL];
-  __shared__ double red[2 * (kSmallThreads / kWave)];
+  __shared__ double red[2 * (kFormerThreads / kWave)];
   __shared__ int ord[kMaxL];
   const int ld = lds_ld(l);
   for (int e = threadIdx.x; e < l * l; e += blockDim.x) {
@@ -358,7 +361,7 @@ void apply_blockdiag(void* Y, bool y_f32, int64_t m, int64_t N, int l, const dou
 void former_small(const double* Wg, int l, int k, double* eigw, double* eigv, double* Wk,
                   double* S, hipStream_t st, bool symmetric) {
   MCNLA_REQUIRE(l >= 1 && l <= kMaxL && k >= 1 && k <= l, MCNLA_ERR_INVALID, "former sizes");
-  hipLaunchKernelGGL(former_small_kernel, dim3(1), dim3(kSmallThreads), 0, st, Wg, l, k, symmetric,
+  hipLaunchKernelGGL(former_small_kernel, dim3(1), dim3(kFormerThreads), 0, st, Wg, l, k, symmetric,
                      eigw, eigv, Wk, S);
   MCNLA_LAUNCHED();
 }
