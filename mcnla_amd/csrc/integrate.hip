@@ -939,27 +939,28 @@ __global__ void wy_init_scalars_kernel(IntegratorState* st, int l, const double*
 // root; Qbar W is the orthonormal basis nearest to Qbar (Loewdin).  The iterations keep Qbar
 // orthonormal only to the rounding they accumulate (as the reference's own updates do); this
 // step removes it.  W = I when Gm is not positive definite.
-__global__ __launch_bounds__(kThreads) void polish_kernel(int l, double N,
-                                                         const double* __restrict__ P,
-                                                         int64_t nblk, double* __restrict__ Wout) {
+// (16 waves: one 16 x 16 tile of each product per wave at l = 64.)
+__global__ __launch_bounds__(kWyThreads) void polish_kernel(int l, double N,
+                                                           const double* __restrict__ P,
+                                                           int64_t nblk, double* __restrict__ Wout) {
   __shared__ double sY[kMaxL * (kMaxL + 1)];
   __shared__ double sW[kMaxL * (kMaxL + 1)];
   __shared__ double sT[kMaxL * (kMaxL + 1)];
-  __shared__ double red[kThreads / kWave];
+  __shared__ double red[kWyWaves];
   const int np = pad16(l), ld = np + 1, ll = l * l;
-  for (int e = threadIdx.x; e < np * np; e += kThreads) {
+  for (int e = threadIdx.x; e < np * np; e += kWyThreads) {
     const int i = e / np, j = e % np;
     sT[i * ld + j] = (i < l && j < l) ? N * sum_partials(P, nblk, 2 * ll, ll + i * l + j)
                                       : ((i == j) ? 1.0 : 0.0);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < np * np; e += kThreads) {
+  for (int e = threadIdx.x; e < np * np; e += kWyThreads) {
     const int i = e / np, j = e % np;
     sY[i * ld + j] = 0.5 * (sT[i * ld + j] + sT[j * ld + i]);
   }
   __syncthreads();
-  const int ok = lds_ns_sqrt<kThreads / kWave>(np, ld, sY, sW, sT, red, 60);
-  for (int e = threadIdx.x; e < ll; e += kThreads) {
+  const int ok = lds_ns_sqrt<kWyWaves>(np, ld, sY, sW, sT, red, 60);
+  for (int e = threadIdx.x; e < ll; e += kWyThreads) {
     const int i = e / l, j = e % l;
     Wout[e] = ok > 0 ? sW[i * ld + j] : ((i == j) ? 1.0 : 0.0);
   }
@@ -1234,7 +1235,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
   hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, -1, L.st, nl, l, L.U1,
                      L.BsU1, L.P, 2);
   MCNLA_LAUNCHED();
-  hipLaunchKernelGGL(polish_kernel, dim3(1), dim3(kThreads), 0, st, l, static_cast<double>(g.N),
+  hipLaunchKernelGGL(polish_kernel, dim3(1), dim3(kWyThreads), 0, st, l, static_cast<double>(g.N),
                      L.P, L.nblk_p, L.F);
   MCNLA_LAUNCHED();
   hipLaunchKernelGGL(right_mul_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, nl, l, L.U1,
