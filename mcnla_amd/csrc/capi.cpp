@@ -743,7 +743,9 @@ GemmF64Args plan_omega_p(int64_t n, int64_t nl) {
   g.lower_only = true;
   g.shape = GemmShape::kSquare;
   const int64_t tm = ceil_div(nl, 256), t = ceil_div(nl, 128);
-  g.splits = gemm_f64_splits_for(nl >= 128 ? tm * (tm + 1) : t * (t + 1) / 2, n, 2304);
+  // about 4 rounds of the grid without splitting K where the tiles allow it: a split writes and
+  // re-reads Nl^2 doubles of partials (1.6 GB per 3 splits at Nl = 8192)
+  g.splits = gemm_f64_splits_for(nl >= 128 ? tm * (tm + 1) : t * (t + 1) / 2, n, 1024);
   g.kchunk = ceil_div(ceil_div(n, g.splits), 16) * 16;
   g.splits = ceil_div(n, g.kchunk);
   return g;
