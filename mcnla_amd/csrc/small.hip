@@ -60,20 +60,20 @@ __global__ void gram_diag_blocks_kernel(const double* __restrict__ G, int64_t ld
 }
 
 // Bs = blockdiag(T)' G blockdiag(T), block (I, J) = T_I' G_IJ T_J for I >= J: one workgroup per
-// block pair, G_IJ, T_I, T_J and H = G_IJ T_J in LDS (dynamic, 4 l (l + 1) doubles).  Each sum
-// runs over q in ascending order.  Bs is symmetric in exact arithmetic; the entries with global
-// row >= column are computed and written to both halves, so that it is exactly symmetric (its
-// rows can then stand in for its columns).
+// block pair, G_IJ, T_I, T_J and H = G_IJ T_J in LDS (dynamic, 4 np (np + 1) doubles, np =
+// pad16(l), zero-padded) and both products on f64 MFMA tiles.  Bs is symmetric in exact
+// arithmetic; the entries with global row >= column are computed and written to both halves, so
+// that it is exactly symmetric (its rows can then stand in for its columns).
 __global__ __launch_bounds__(256) void congruence_kernel(const double* __restrict__ G,
                                                         const double* __restrict__ T,
                                                         double* __restrict__ Bs, int64_t N,
                                                         int l) {
   extern __shared__ double smem[];
-  const int ld = l + 1;
+  const int np = pad16(l), ld = np + 1, nb = np * ld;
   double* sG = smem;
-  double* sTi = sG + l * ld;
-  double* sTj = sTi + l * ld;
-  double* sH = sTj + l * ld;
+  double* sTi = sG + nb;
+  double* sTj = sTi + nb;
+  double* sH = sTj + nb;
   const int64_t nl = N * l;
   // block pair p -> (I, J), I >= J, p = I (I + 1) / 2 + J
   const int64_t p = blockIdx.x;
@@ -82,27 +82,33 @@ __global__ __launch_bounds__(256) void congruence_kernel(const double* __restric
   while ((I + 1) * (I + 2) / 2 <= p) ++I;
   const int64_t J = p - I * (I + 1) / 2;
   const int ll = l * l;
-  for (int e = threadIdx.x; e < ll; e += blockDim.x) {
-    const int a = e / l, b = e % l;
-    sG[a * ld + b] = G[(I * l + a) * nl + J * l + b];
-    sTi[a * ld + b] = T[I * ll + e];
-    sTj[a * ld + b] = T[J * ll + e];
+  for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
+    const int a = e / np, b = e % np;
+    const bool in = a < l && b < l;
+    sG[a * ld + b] = in ? G[(I * l + a) * nl + J * l + b] : 0.0;
+    sTi[a * ld + b] = in ? T[I * ll + a * l + b] : 0.0;
+    sTj[a * ld + b] = in ? T[J * ll + a * l + b] : 0.0;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ll; e += blockDim.x) {
-    const int a = e / l, c = e % l;
-    double acc = 0.0;
-    for (int q = 0; q < l; ++q) acc += sG[a * ld + q] * sTj[q * ld + c];
-    sH[a * ld + c] = acc;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int nt1 = np / 16, tiles = nt1 * nt1, nw = blockDim.x / kWave;
+  for (int t = wave; t < tiles; t += nw) {  // H = G_IJ T_J
+    const int i0 = (t / nt1) * 16, j0 = (t % nt1) * 16;
+    const d4 acc = lds_mfma_tile<false, false>(np, ld, sG, sTj, i0, j0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sH[(i0 + lane / 16 + 4 * r) * ld + j0 + lane % 16] = acc[r];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ll; e += blockDim.x) {
-    const int r = e / l, c = e % l;
-    if (I == J && r < c) continue;
-    double acc = 0.0;
-    for (int q = 0; q < l; ++q) acc += sTi[q * ld + r] * sH[q * ld + c];
-    Bs[(I * l + r) * nl + J * l + c] = acc;
-    Bs[(J * l + c) * nl + I * l + r] = acc;
+  for (int t = wave; t < tiles; t += nw) {  // T_I' H
+    const int i0 = (t / nt1) * 16, j0 = (t % nt1) * 16;
+    const d4 acc = lds_mfma_tile<true, false>(np, ld, sTi, sH, i0, j0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = i0 + lane / 16 + 4 * q, c = j0 + lane % 16;
+      if (r >= l || c >= l || (I == J && r < c)) continue;
+      Bs[(I * l + r) * nl + J * l + c] = acc[q];
+      Bs[(J * l + c) * nl + I * l + r] = acc[q];
+    }
   }
 }
 
@@ -307,7 +313,7 @@ void congruence_blockdiag(const double* G, const double* Tst, double* Bs, int64_
   MCNLA_REQUIRE(l >= 1 && l <= kMaxL, MCNLA_ERR_INVALID, "l > 64");
   const int64_t pairs = N * (N + 1) / 2;
   if (pairs == 0) return;
-  const size_t lds = static_cast<size_t>(4) * l * (l + 1) * sizeof(double);
+  const size_t lds = static_cast<size_t>(4) * pad16(l) * (pad16(l) + 1) * sizeof(double);
   hipLaunchKernelGGL(congruence_kernel, dim3(static_cast<unsigned>(pairs)), dim3(256), lds, st, G,
                      Tst, Bs, N, l);
   MCNLA_LAUNCHED();
