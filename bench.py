@@ -229,6 +229,7 @@ def c4_bench(mc, args):
     kw = dict(sketcher=mc.RowBlockGaussianProjectionSketcher,
               integrator=mc.RowBlockWenYinIntegrator, former=mc.RowBlockGramianFormer, seed=2)
     s = mc.Solver(P, precision=mc.PREC_MIXED_BF16, graph=args.graph, **kw)
+    tuning = tune(s, A, args, "c4")
     dt, kern, stages = timed_solves(mc, s, A, args.c4_steps, 1)
     S = s.vectorS()
     it = s.iteration()
@@ -243,6 +244,7 @@ def c4_bench(mc, args):
            "route": "gram_of_a (auto)", "solves_per_s": round(args.c4_steps / dt, 3),
            "ms_per_solve": round(dt / args.c4_steps * 1e3, 3), "steps": args.c4_steps,
            "iterations": it, "stage_ms": [round(x * 1e3, 3) for x in stages.mean(axis=0)],
+           "tuning": tuning,
            "sketch_route_ms_per_solve": round(dref * 1e3, 3),
            "sketch_route_stage_ms": [round(x * 1e3, 3) for x in sref.mean(axis=0)],
            "sketch_route_iterations": ref.iteration(),
@@ -253,6 +255,15 @@ def c4_bench(mc, args):
     del s, ref
     A.free()
     return out
+
+
+def tune(solver, A, args, name):
+    """Runs the autotuner on this solver (untimed set-up) unless --autotune 0; returns its plan."""
+    if args.autotune:
+        t0 = time.perf_counter()
+        solver.autotune(A, reps=2)
+        log("%s: autotuned in %.1f s: %s" % (name, time.perf_counter() - t0, solver.tuning()))
+    return solver.tuning()
 
 
 def main():
@@ -280,6 +291,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay the integrator iterations as "
                     "HIP graphs recorded in the warm-up solve")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--autotune", type=int, default=1, help="tune each solver's split-K plans "
+                    "before its warm-up (untimed); 0 keeps the default plans")
     ap.add_argument("--only-c4", action="store_true", help="run the row-block Wen-Yin workload "
                     "alone and print its record (for profiling)")
     args = ap.parse_args()
@@ -320,6 +333,8 @@ def main():
     s64 = mc.Solver(P, seed=2, graph=args.graph)
     smx = mc.Solver(P, seed=2, precision=prec, graph=args.graph)
 
+    tune64 = tune(s64, A64, args, "fp64")
+    tunemx = tune(smx, A32, args, "mixed")
     fp64_steps = args.fp64_steps if args.fp64_steps is not None else max(3, args.steps // 2)
     dt64, k64, st64 = timed_solves(mc, s64, A64, fp64_steps, 1)
     g64 = k64[:, 0]
@@ -382,13 +397,13 @@ def main():
                    "m": m, "n": n, "k": k, "p": p, "N": N, "precision": args.mixed},
         "fp64": {"value": round(fp64_value, 3), "unit": "solves/s",
                  "ms_per_step": round(dt64 / fp64_steps * 1e3, 3), "steps": fp64_steps,
-                 "iterations": s64.iteration(),
+                 "iterations": s64.iteration(), "tuning": tune64,
                  "stage_ms": [round(x * 1e3, 3) for x in st64.mean(axis=0)],
                  "sketch_gemm_ms": round(g64.mean() * 1e3, 3),
                  "gram_gemm_ms": round(k64[:, 1].mean() * 1e3, 3),
                  "sketch_gemm_tflops": round(ach_64 / 1e12, 2),
                  "sketch_gemm_frac_of_f64_peak": round(ach_64 / MI355X_PEAK["f64"], 3)},
-        "mixed": {"iterations": smx.iteration(),
+        "mixed": {"iterations": smx.iteration(), "tuning": tunemx,
                   "stage_ms": [round(x * 1e3, 3) for x in stmx.mean(axis=0)],
                   "sigma_rel_err_vs_fp64": rel, "subspace_dist_vs_fp64": dist,
                   "speedup_vs_fp64": round(value / fp64_value, 2),

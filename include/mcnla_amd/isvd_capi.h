@@ -264,6 +264,47 @@ mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* s
 mcnla_status mcnla_solver_kernel_times(mcnla_solver s, double times[2]);
 
 /* ---------------------------------------------------------------------------------------------
+ * Autotuner.  No reference counterpart (the reference has no device plans to tune).  The FP64
+ * GEMMs of a solve split K so that their grid holds about `target` workgroups; the defaults are
+ * the values measured best at the 100000 x 10000 (Nl = 2048) and 500000 x 2000 (Nl = 8192)
+ * shapes.  A split count changes only the order of the split-K sums, so a tuned plan gives the
+ * same results to rounding, and the same plan always gives the same bits.
+ *
+ * Tuned plans are kept in a process-wide table keyed by (route, precision, m, n, l, N); a solver
+ * created after an entry exists for its key uses it (servers too, per request shape). */
+typedef struct {
+  int64_t gram;     /* G = Y'Y on the sketch route; H = A'A on the Gram-of-A route */
+  int64_t former;   /* Z = A'Qbar of the former (sketch route) */
+  int64_t h_omega;  /* P = H Omega (Gram-of-A route) */
+  int64_t omega_p;  /* G = Omega'P (Gram-of-A route) */
+  int64_t n_by_l;   /* X = Omega (T M) and Z = H X (Gram-of-A route) */
+} mcnla_tuning;
+
+/* The defaults (no table entry). */
+void mcnla_tuning_default(mcnla_tuning* t);
+/* The plan this solver runs with. */
+mcnla_status mcnla_solver_get_tuning(mcnla_solver s, mcnla_tuning* t);
+/* Times candidate targets for each GEMM of this solver's route on A (as mcnla_solver_solve takes
+ * it; a few solves per candidate, `reps` >= 1 of them timed), one GEMM at a time with the others
+ * fixed, keeps a candidate only if its stage is faster than the incumbent's by more than 1 %,
+ * applies the result to the solver and records it in the table.  The solver's results afterwards
+ * are those of a solve with the tuned plan.  Synchronous.  MCNLA_ERR_STATE for a solver carved
+ * from a reserved region (a server's), whose workspace cannot grow. */
+mcnla_status mcnla_solver_autotune(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
+                                   int a_order, int reps, void* stream, mcnla_tuning* out);
+/* Table access: record / look up the plan for a shape and configuration (route AUTO is resolved
+ * as the solver would); *found = 0 and the defaults when there is no entry. */
+mcnla_status mcnla_tune_set(const mcnla_params* params, const mcnla_solver_config* cfg,
+                            const mcnla_tuning* t);
+mcnla_status mcnla_tune_get(const mcnla_params* params, const mcnla_solver_config* cfg,
+                            mcnla_tuning* t, int* found);
+void mcnla_tune_clear(void);
+/* Text persistence, one entry a line: "route precision m n l N gram former h_omega omega_p
+ * n_by_l".  Load merges into the table; a malformed line is MCNLA_ERR_INVALID and loads nothing. */
+mcnla_status mcnla_tune_save(const char* path);
+mcnla_status mcnla_tune_load(const char* path);
+
+/* ---------------------------------------------------------------------------------------------
  * Serving loop: continuous batching of low-rank-approximation requests on one GPU.  No reference
  * counterpart (the reference runs one solve per MPI job, src/isvd_dense_cb.cpp:46-302).
  *

@@ -55,6 +55,15 @@ class SolverConfig(ctypes.Structure):
                 ("graph", ctypes.c_int)]
 
 
+class Tuning(ctypes.Structure):
+    """Split-K workgroup targets of a solve's FP64 GEMMs (mcnla_tuning)."""
+    _fields_ = [("gram", ctypes.c_int64), ("former", ctypes.c_int64), ("h_omega", ctypes.c_int64),
+                ("omega_p", ctypes.c_int64), ("n_by_l", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("solver", SolverConfig), ("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
                 ("max_bypass", ctypes.c_int)]
@@ -140,6 +149,16 @@ SIGNATURES = {
     "mcnla_solver_iteration": (_st, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "mcnla_solver_stage_times": (_st, [_vp, _dp, _dp]),
     "mcnla_solver_kernel_times": (_st, [_vp, _dp]),
+    "mcnla_tuning_default": (None, [ctypes.POINTER(Tuning)]),
+    "mcnla_solver_get_tuning": (_st, [_vp, ctypes.POINTER(Tuning)]),
+    "mcnla_solver_autotune": (_st, [_vp, _vp, _i64, _i, _i, _i, _vp, ctypes.POINTER(Tuning)]),
+    "mcnla_tune_set": (_st, [ctypes.POINTER(Params), ctypes.POINTER(SolverConfig),
+                             ctypes.POINTER(Tuning)]),
+    "mcnla_tune_get": (_st, [ctypes.POINTER(Params), ctypes.POINTER(SolverConfig),
+                             ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_int)]),
+    "mcnla_tune_clear": (None, []),
+    "mcnla_tune_save": (_st, [ctypes.c_char_p]),
+    "mcnla_tune_load": (_st, [ctypes.c_char_p]),
     "mcnla_solver_required_bytes": (_st, [ctypes.POINTER(Params), ctypes.POINTER(SolverConfig),
                                           ctypes.POINTER(ctypes.c_size_t)]),
     "mcnla_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),

@@ -1,7 +1,10 @@
 // C ABI of libmcnla_amd: stage handles (the reference's iSVD stage plugins) and the fused solver.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -48,6 +51,40 @@ mcnla_status guard(F&& f) {
     set_error("unknown error");
     return MCNLA_ERR_INVALID;
   }
+}
+
+// ---- tuned GEMM plans: workgroup targets of the split-K FP64 GEMMs per solve shape ----
+constexpr mcnla_tuning kDefaultTuning = {2304, 4096, 2304, 1024, 512};
+constexpr int64_t kMaxTarget = 1 << 20;
+
+struct TuneKey {
+  int64_t v[6];  // route, precision, m, n, l, N
+  bool operator<(const TuneKey& o) const {
+    return std::lexicographical_compare(v, v + 6, o.v, o.v + 6);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, mcnla_tuning>& tune_table() {
+  static std::map<TuneKey, mcnla_tuning> t;
+  return t;
+}
+TuneKey tune_key(const mcnla_params& p, int route, int precision) {
+  return TuneKey{{route, precision, p.nrow, p.ncol, p.rank + p.over_rank, p.num_sketch}};
+}
+bool valid_tuning(const mcnla_tuning& t) {
+  for (int64_t v : {t.gram, t.former, t.h_omega, t.omega_p, t.n_by_l})
+    if (v < 1 || v > kMaxTarget) return false;
+  return true;
+}
+mcnla_tuning lookup_tuning(const TuneKey& k, bool* found) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = tune_table().find(k);
+  if (found) *found = it != tune_table().end();
+  return it == tune_table().end() ? kDefaultTuning : it->second;
+}
+void store_tuning(const TuneKey& k, const mcnla_tuning& t) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_table()[k] = t;
 }
 
 // Device allocation owned by a stage (grows, never shrinks).  While a CarveScope is active on the
@@ -268,10 +305,12 @@ struct GramCore {
     MCNLA_HIP(hipEventElapsedTime(&ms, beg, end));
     return ms * 1e-3;
   }
+  // workgroups the split-K grid aims at (about 9 per CU by default)
+  int64_t target = kDefaultTuning.gram;
   // Arguments of G = Y'Y except the pointers.
-  static GemmF64Args plan(bool y_f32, int64_t m, int64_t nl) {
+  GemmF64Args plan(bool y_f32, int64_t m, int64_t nl) const {
     // Lower-triangle tiles: 256 x 128 on the FP64 fast path (tm (tm+1) tiles), else 128 x 128.
-    // Split K so that the grid fills whole rounds of about 2304 workgroups (9 per CU).
+    // Split K so that the grid fills whole rounds of about `target` workgroups.
     const int64_t tm = ceil_div(nl, 256), t = ceil_div(nl, 128);
     const int64_t tiles = (!y_f32 && nl >= 128) ? tm * (tm + 1) : t * (t + 1) / 2;
     GemmF64Args g;
@@ -287,7 +326,7 @@ struct GramCore {
     g.ldc = nl;
     g.lower_only = true;
     g.shape = GemmShape::kSquare;
-    g.splits = gemm_f64_splits_for(tiles, m, 2304);
+    g.splits = gemm_f64_splits_for(tiles, m, target);
     g.kchunk = ceil_div(m, g.splits);  // equal splits (the GEMM rounds them up where it must)
     g.splits = ceil_div(m, g.kchunk);
     return g;
@@ -319,8 +358,9 @@ struct FormCore {
   // kind: MCNLA_FORM_*.  The symmetric former (square symmetric A) uses Wg = Z'Qbar = Qbar'A Qbar
   // in place of Z'Z and forms no V.
   // Split-K of Z = A'Qbar (K = m over ceil(n/128) row tiles) and of Wg = Z'Z (K = n).
-  static int64_t z_kchunk(int64_t m, int64_t n) {  // about 4096 workgroups (measured best)
-    return ceil_div(ceil_div(m, gemm_f64_splits_for(ceil_div(n, 128), m, 4096)), 16) * 16;
+  int64_t z_target = kDefaultTuning.former;  // workgroups of Z = A'Qbar's split-K grid
+  int64_t z_kchunk(int64_t m, int64_t n) const {
+    return ceil_div(ceil_div(m, gemm_f64_splits_for(ceil_div(n, 128), m, z_target)), 16) * 16;
   }
   static int64_t w_kchunk(int64_t n) {
     return ceil_div(ceil_div(n, gemm_f64_splits_for(1, n, 64)), 16) * 16;
@@ -673,6 +713,7 @@ struct mcnla_solver_s {
   // Gram-of-A route: A as row-major FP64, H = A'A, P = H Omega, X = Omega (T M), split-K scratch
   mcnla::DevBuf A64, H, P, X, gp;
   int route = MCNLA_ROUTE_SKETCH;
+  mcnla_tuning tune{};  // split-K workgroup targets of the FP64 GEMMs
   mcnla::Timer timer;
   // second stream for the first block's eigen-decomposition, which runs beside the integrator
   hipStream_t side = nullptr;
@@ -706,7 +747,7 @@ int resolve_route(const mcnla_params& p, const mcnla_solver_config& c) {
 }
 
 // P = H Omega (n x Nl): H symmetric, so its rows serve as the M-contiguous operand.
-GemmF64Args plan_h_omega(int64_t n, int64_t nl) {
+GemmF64Args plan_h_omega(int64_t n, int64_t nl, int64_t target) {
   GemmF64Args g;
   g.M = n;
   g.N = nl;
@@ -717,7 +758,7 @@ GemmF64Args plan_h_omega(int64_t n, int64_t nl) {
   g.b_ncontig = true;
   g.ldc = nl;
   g.shape = GemmShape::kSquare;
-  g.splits = gemm_f64_splits_for(ceil_div(n, 256) * ceil_div(nl, 128), n, 2304);
+  g.splits = gemm_f64_splits_for(ceil_div(n, 256) * ceil_div(nl, 128), n, target);
   g.kchunk = ceil_div(ceil_div(n, g.splits), 16) * 16;
   g.splits = ceil_div(n, g.kchunk);
   return g;
@@ -725,12 +766,12 @@ GemmF64Args plan_h_omega(int64_t n, int64_t nl) {
 
 // Split-K factor of the n x l products X = Omega (T M) (K = Nl) and Z = H X (K = n): n is small
 // on this route, so without splits they would run on n/128 workgroups.
-int64_t n_by_l_splits(int64_t n, int64_t K) {
-  return gemm_f64_splits_for(ceil_div(n, 128), K, 512);
+int64_t n_by_l_splits(int64_t n, int64_t K, int64_t target) {
+  return gemm_f64_splits_for(ceil_div(n, 128), K, target);
 }
 
 // G = Omega' P (Nl x Nl, symmetric): lower-triangle tiles.
-GemmF64Args plan_omega_p(int64_t n, int64_t nl) {
+GemmF64Args plan_omega_p(int64_t n, int64_t nl, int64_t target) {
   GemmF64Args g;
   g.M = nl;
   g.N = nl;
@@ -743,12 +784,30 @@ GemmF64Args plan_omega_p(int64_t n, int64_t nl) {
   g.lower_only = true;
   g.shape = GemmShape::kSquare;
   const int64_t tm = ceil_div(nl, 256), t = ceil_div(nl, 128);
-  // about 4 rounds of the grid without splitting K where the tiles allow it: a split writes and
-  // re-reads Nl^2 doubles of partials (1.6 GB per 3 splits at Nl = 8192)
-  g.splits = gemm_f64_splits_for(nl >= 128 ? tm * (tm + 1) : t * (t + 1) / 2, n, 1024);
+  // by default about 4 rounds of the grid without splitting K where the tiles allow it: a split
+  // writes and re-reads Nl^2 doubles of partials (1.6 GB per 3 splits at Nl = 8192)
+  g.splits = gemm_f64_splits_for(nl >= 128 ? tm * (tm + 1) : t * (t + 1) / 2, n, target);
   g.kchunk = ceil_div(ceil_div(n, g.splits), 16) * 16;
   g.splits = ceil_div(n, g.kchunk);
   return g;
+}
+
+// Sets the solver's GEMM plan and sizes the split-K scratch it needs (grows only).
+void apply_tuning(mcnla_solver_s* s, const mcnla_tuning& t) {
+  s->tune = t;
+  s->gram.target = t.gram;
+  s->form.z_target = t.former;
+  const int64_t l = s->prm.rank + s->prm.over_rank, nl = l * s->prm.num_sketch;
+  const int64_t m = s->prm.nrow, n = s->prm.ncol;
+  const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
+  if (via_a)
+    s->gp.ensure(std::max({gemm_f64_partial_elems(n, nl, plan_h_omega(n, nl, t.h_omega).splits),
+                           gemm_f64_partial_elems(nl, nl, plan_omega_p(n, nl, t.omega_p).splits),
+                           gemm_f64_partial_elems(n, l, n_by_l_splits(n, nl, t.n_by_l)),
+                           gemm_f64_partial_elems(n, l, n_by_l_splits(n, n, t.n_by_l))}) *
+                 sizeof(double));
+  s->gram.prepare(false, m, via_a ? n : nl);
+  s->form.prepare(s->prm);
 }
 
 // Validates the configuration and allocates every device buffer the solve uses (the stages'
@@ -788,12 +847,8 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
     s->H.ensure(n * n * sizeof(double));
     s->P.ensure(n * nl * sizeof(double));
     s->X.ensure(n * l * sizeof(double));
-    s->gp.ensure(std::max({gemm_f64_partial_elems(n, nl, plan_h_omega(n, nl).splits),
-                           gemm_f64_partial_elems(nl, nl, plan_omega_p(n, nl).splits),
-                           gemm_f64_partial_elems(n, l, n_by_l_splits(n, nl)),
-                           gemm_f64_partial_elems(n, l, n_by_l_splits(n, n))}) *
-                 sizeof(double));
   }
+  apply_tuning(s.get(), lookup_tuning(tune_key(*params, s->route, cfg->precision), nullptr));
   s->G.ensure(nl * nl * sizeof(double));
   s->Bs.ensure(nl * nl * sizeof(double));
   s->tmp.ensure(nl * l * sizeof(double));
@@ -810,8 +865,6 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
   s->T0.ensure(l * l * sizeof(double));
   s->W0.ensure(l * l * sizeof(double));
   s->sketch.prepare(*params, via_a ? static_cast<int>(MCNLA_PREC_F64) : cfg->precision);
-  s->gram.prepare(false, m, via_a ? n : nl);
-  s->form.prepare(*params);
   return s;
 }
 }  // namespace
@@ -1378,13 +1431,13 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     if (!via_a) {
       s->gram.run(s->Y.p, false, m, nl, s->G.as<double>(), st);
     } else {  // G = Omega' (H Omega)
-      GemmF64Args hp = plan_h_omega(n, nl);
+      GemmF64Args hp = plan_h_omega(n, nl, s->tune.h_omega);
       hp.A = s->H.p;
       hp.B = s->sketch.omega.p;
       hp.C = s->P.as<double>();
       hp.partial = s->gp.as<double>();
       gemm_f64(hp, st);
-      GemmF64Args pg = plan_omega_p(n, nl);
+      GemmF64Args pg = plan_omega_p(n, nl, s->tune.omega_p);
       pg.A = s->sketch.omega.p;
       pg.B = s->P.p;
       pg.C = s->G.as<double>();
@@ -1487,7 +1540,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
       x.M = n;
       x.A = s->sketch.omega.p;
       x.C = s->X.as<double>();
-      x.splits = n_by_l_splits(n, nl);
+      x.splits = n_by_l_splits(n, nl, s->tune.n_by_l);
       x.partial = s->gp.as<double>();
       gemm_f64(x, st);
       q.K = n;
@@ -1517,7 +1570,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
       z.C = s->form.Z.as<double>();
       z.ldc = l;
       z.shape = tall_shape(l);
-      z.splits = n_by_l_splits(n, n);
+      z.splits = n_by_l_splits(n, n, s->tune.n_by_l);
       z.partial = s->gp.as<double>();
       gemm_f64(z, st);
     }
@@ -1581,6 +1634,178 @@ mcnla_status mcnla_solver_kernel_times(mcnla_solver s, double times[2]) {
     MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "solver has not run");
     times[0] = s->sketch.gemm_seconds();
     times[1] = s->gram.gemm_seconds();
+    return MCNLA_OK;
+  });
+}
+
+void mcnla_tuning_default(mcnla_tuning* t) {
+  if (t) *t = kDefaultTuning;
+}
+
+mcnla_status mcnla_solver_get_tuning(mcnla_solver s, mcnla_tuning* t) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && t, MCNLA_ERR_INVALID, "null argument");
+    *t = s->tune;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_autotune(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
+                                   int a_order, int reps, void* stream, mcnla_tuning* out) {
+  return guard([&] {
+    MCNLA_REQUIRE(s, MCNLA_ERR_INVALID, "solver is null");
+    MCNLA_REQUIRE(reps >= 1 && reps <= 64, MCNLA_ERR_INVALID, "reps must be in [1, 64]");
+    MCNLA_REQUIRE(!s->G.borrowed, MCNLA_ERR_STATE,
+                  "a solver carved from a reserved region cannot be tuned (its split-K scratch "
+                  "cannot grow)");
+    check_a(s->prm, A, lda, a_dtype, a_order);
+    // One knob per GEMM of the route, with the solve stages ([sketch, orthogonalize, integrate,
+    // form]) that hold it.
+    struct Knob {
+      int64_t mcnla_tuning::*field;
+      std::vector<int64_t> candidates;
+      unsigned stages;
+    };
+    std::vector<Knob> knobs;
+    if (s->route == MCNLA_ROUTE_GRAM_OF_A) {
+      knobs = {{&mcnla_tuning::gram, {1152, 2304, 4608}, 1u},
+               {&mcnla_tuning::h_omega, {1152, 2304, 4608}, 2u},
+               {&mcnla_tuning::omega_p, {256, 512, 1024, 2048}, 2u},
+               {&mcnla_tuning::n_by_l, {128, 256, 512, 1024, 2048}, 12u}};
+    } else {
+      knobs = {{&mcnla_tuning::gram, {576, 1152, 2304, 4608, 9216}, 2u},
+               {&mcnla_tuning::former, {1024, 2048, 4096, 8192}, 8u}};
+    }
+    auto solve = [&] {
+      const mcnla_status r = mcnla_solver_solve(s, A, lda, a_dtype, a_order, stream);
+      MCNLA_REQUIRE(r == MCNLA_OK || r == MCNLA_ERR_NOT_CONVERGED, r, get_error());
+    };
+    // best time of `reps` solves over the knob's stages, after one untimed solve
+    auto measure = [&](unsigned stages) {
+      solve();
+      double best = 0.0;
+      for (int i = 0; i < reps; ++i) {
+        solve();
+        const std::vector<double> v = s->timer.parts();
+        double t = 0.0;
+        for (int j = 0; j < 4 && j < static_cast<int>(v.size()); ++j)
+          if (stages & (1u << j)) t += v[j];
+        if (i == 0 || t < best) best = t;
+      }
+      return best;
+    };
+    mcnla_tuning t = s->tune;
+    for (const Knob& k : knobs) {
+      int64_t best_v = t.*k.field;
+      double best_t = measure(k.stages);
+      for (int64_t v : k.candidates) {
+        if (v == best_v) continue;
+        mcnla_tuning c = t;
+        c.*k.field = v;
+        apply_tuning(s, c);
+        const double ct = measure(k.stages);
+        if (ct < 0.99 * best_t) {
+          best_t = ct;
+          best_v = v;
+        }
+      }
+      t.*k.field = best_v;
+      apply_tuning(s, t);
+    }
+    solve();  // leave the results of the tuned plan
+    store_tuning(tune_key(s->prm, s->route, s->cfg.precision), t);
+    if (out) *out = t;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_tune_set(const mcnla_params* params, const mcnla_solver_config* cfg,
+                            const mcnla_tuning* t) {
+  return guard([&] {
+    MCNLA_REQUIRE(params && cfg && t, MCNLA_ERR_INVALID, "null argument");
+    check_params(params);
+    MCNLA_REQUIRE(valid_tuning(*t), MCNLA_ERR_INVALID, "tuning targets must be in [1, 2^20]");
+    store_tuning(tune_key(*params, resolve_route(*params, *cfg), cfg->precision), *t);
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_tune_get(const mcnla_params* params, const mcnla_solver_config* cfg,
+                            mcnla_tuning* t, int* found) {
+  return guard([&] {
+    MCNLA_REQUIRE(params && cfg && t, MCNLA_ERR_INVALID, "null argument");
+    check_params(params);
+    bool f = false;
+    *t = lookup_tuning(tune_key(*params, resolve_route(*params, *cfg), cfg->precision), &f);
+    if (found) *found = f ? 1 : 0;
+    return MCNLA_OK;
+  });
+}
+
+void mcnla_tune_clear(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_table().clear();
+}
+
+mcnla_status mcnla_tune_save(const char* path) {
+  return guard([&] {
+    MCNLA_REQUIRE(path, MCNLA_ERR_INVALID, "null argument");
+    std::string text;
+    {
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      for (const auto& e : tune_table()) {
+        const TuneKey& k = e.first;
+        const mcnla_tuning& t = e.second;
+        char line[256];
+        std::snprintf(line, sizeof line,
+                      "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld\n",
+                      (long long)k.v[0], (long long)k.v[1], (long long)k.v[2], (long long)k.v[3],
+                      (long long)k.v[4], (long long)k.v[5], (long long)t.gram,
+                      (long long)t.former, (long long)t.h_omega, (long long)t.omega_p,
+                      (long long)t.n_by_l);
+        text += line;
+      }
+    }
+    FILE* f = std::fopen(path, "w");
+    MCNLA_REQUIRE(f, MCNLA_ERR_INVALID, std::string("cannot open ") + path);
+    const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+    MCNLA_REQUIRE(std::fclose(f) == 0 && ok, MCNLA_ERR_INVALID,
+                  std::string("cannot write ") + path);
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_tune_load(const char* path) {
+  return guard([&] {
+    MCNLA_REQUIRE(path, MCNLA_ERR_INVALID, "null argument");
+    FILE* f = std::fopen(path, "r");
+    MCNLA_REQUIRE(f, MCNLA_ERR_INVALID, std::string("cannot open ") + path);
+    std::vector<std::pair<TuneKey, mcnla_tuning>> entries;
+    char line[512];
+    int lineno = 0;
+    std::string bad;
+    while (bad.empty() && std::fgets(line, sizeof line, f)) {
+      ++lineno;
+      long long v[11];
+      char extra;
+      const int got = std::sscanf(line, "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %c",
+                                  &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8],
+                                  &v[9], &v[10], &extra);
+      bool blank = true;
+      for (const char* c = line; *c; ++c)
+        if (*c != ' ' && *c != '\t' && *c != '\n' && *c != '\r') blank = false;
+      if (blank) continue;
+      TuneKey k{{v[0], v[1], v[2], v[3], v[4], v[5]}};
+      const mcnla_tuning t{v[6], v[7], v[8], v[9], v[10]};
+      const bool key_ok = got == 11 &&
+                          (k.v[0] == MCNLA_ROUTE_SKETCH || k.v[0] == MCNLA_ROUTE_GRAM_OF_A) &&
+                          k.v[2] >= 1 && k.v[3] >= 1 && k.v[4] >= 1 && k.v[5] >= 1;
+      if (!key_ok || !valid_tuning(t)) bad = path + std::string(":") + std::to_string(lineno);
+      else entries.emplace_back(k, t);
+    }
+    std::fclose(f);
+    MCNLA_REQUIRE(bad.empty(), MCNLA_ERR_INVALID, "malformed tuning entry at " + bad);
+    for (const auto& e : entries) store_tuning(e.first, e.second);
     return MCNLA_OK;
   });
 }

@@ -574,12 +574,69 @@ class Solver:
         _lib.check(_lib.lib().mcnla_solver_workspace_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def tuning(self):
+        """The split-K workgroup targets this solver's FP64 GEMMs run with."""
+        t = _lib.Tuning()
+        _lib.check(_lib.lib().mcnla_solver_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, matrix_a, reps=2, stream=None):
+        """Times candidate split-K targets of each GEMM of this solver's route on ``matrix_a``,
+        keeps the fastest (by more than 1 %), records it for solvers of the same shape and
+        returns it; the solver then holds the results of a solve with the tuned plan."""
+        t = _lib.Tuning()
+        _lib.check(_lib.lib().mcnla_solver_autotune(self._h, matrix_a.ptr, matrix_a.ld,
+                                                    matrix_a.dtype_code, matrix_a.order_code,
+                                                    int(reps), stream, ctypes.byref(t)))
+        return t.as_dict()
+
     def __del__(self):
         try:
             if self._h:
                 _lib.lib().mcnla_solver_destroy(self._h)
         except Exception:
             pass
+
+
+def _tune_args(parameters, precision, route):
+    cfg = _lib.SolverConfig()
+    _lib.lib().mcnla_solver_config_default(ctypes.byref(cfg))
+    cfg.precision = precision
+    cfg.route = int(route)
+    return parameters.c_params(), cfg
+
+
+def tuning_default():
+    t = _lib.Tuning()
+    _lib.lib().mcnla_tuning_default(ctypes.byref(t))
+    return t.as_dict()
+
+
+def tune_get(parameters, precision=_lib.PREC_F64, route=_lib.ROUTE_AUTO):
+    """(targets, found) recorded for solvers of this shape, precision and route."""
+    p, cfg = _tune_args(parameters, precision, route)
+    t, found = _lib.Tuning(), ctypes.c_int()
+    _lib.check(_lib.lib().mcnla_tune_get(ctypes.byref(p), ctypes.byref(cfg), ctypes.byref(t),
+                                         ctypes.byref(found)))
+    return t.as_dict(), bool(found.value)
+
+
+def tune_set(parameters, targets, precision=_lib.PREC_F64, route=_lib.ROUTE_AUTO):
+    p, cfg = _tune_args(parameters, precision, route)
+    t = _lib.Tuning(**{k: int(v) for k, v in targets.items()})
+    _lib.check(_lib.lib().mcnla_tune_set(ctypes.byref(p), ctypes.byref(cfg), ctypes.byref(t)))
+
+
+def tune_clear():
+    _lib.lib().mcnla_tune_clear()
+
+
+def tune_save(path):
+    _lib.check(_lib.lib().mcnla_tune_save(str(path).encode()))
+
+
+def tune_load(path):
+    _lib.check(_lib.lib().mcnla_tune_load(str(path).encode()))
 
 
 def synth_lowrank(m, n, U, V, sigma, noise=0.0, seed=1, dtype=np.float64, order="F",

@@ -193,3 +193,51 @@ def test_sketcher_variants_rejected_where_the_reference_has_none():
         b = ctypes.c_size_t()
         _lib.check(L.mcnla_solver_required_bytes(ctypes.byref(prm), ctypes.byref(cfg), ctypes.byref(b)))
     assert e.value.status == _lib.ERR_INVALID
+
+
+def _shape(m, n, k, p, N):
+    return mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+
+
+def test_tune_table_round_trip(tmp_path):
+    """The autotuner's plan table on the host: defaults, set/get per shape class, route AUTO
+    resolved as the solver resolves it, save/load, rejection of bad targets and bad files."""
+    mc.tune_clear()
+    P = _shape(100000, 10000, 20, 12, 64)
+    d = mc.tuning_default()
+    assert d == {"gram": 2304, "former": 4096, "h_omega": 2304, "omega_p": 1024, "n_by_l": 512}
+    got, found = mc.tune_get(P, precision=_lib.PREC_MIXED_BF16)
+    assert not found and got == d
+    t = dict(d, gram=4608, former=2048)
+    mc.tune_set(P, t, precision=_lib.PREC_MIXED_BF16)
+    assert mc.tune_get(P, precision=_lib.PREC_MIXED_BF16) == (t, True)
+    # AUTO resolves to the sketch route at this shape: the same entry
+    assert mc.tune_get(P, precision=_lib.PREC_MIXED_BF16, route=_lib.ROUTE_SKETCH) == (t, True)
+    assert not mc.tune_get(P, precision=_lib.PREC_MIXED_BF16, route=_lib.ROUTE_GRAM_OF_A)[1]
+    assert not mc.tune_get(P, precision=_lib.PREC_F64)[1]  # precision is part of the key
+    Q = _shape(500000, 2000, 50, 14, 128)  # resolves to the Gram-of-A route
+    u = dict(d, omega_p=512, n_by_l=256)
+    mc.tune_set(Q, u)
+    assert mc.tune_get(Q, route=_lib.ROUTE_GRAM_OF_A) == (u, True)
+    path = tmp_path / "plans.txt"
+    mc.tune_save(path)
+    lines = path.read_text().split("\n")
+    assert len([x for x in lines if x]) == 2
+    mc.tune_clear()
+    assert not mc.tune_get(P, precision=_lib.PREC_MIXED_BF16)[1]
+    mc.tune_load(path)
+    assert mc.tune_get(P, precision=_lib.PREC_MIXED_BF16) == (t, True)
+    assert mc.tune_get(Q) == (u, True)
+    with pytest.raises(mc.McnlaError):
+        mc.tune_set(P, dict(d, gram=0))
+    with pytest.raises(mc.McnlaError):
+        mc.tune_set(P, dict(d, n_by_l=(1 << 20) + 1))
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1 0 100 100 8 4 2304 4096 2304 1024 512\n1 0 100 100 8 4 2304 4096\n")
+    mc.tune_clear()
+    with pytest.raises(mc.McnlaError):
+        mc.tune_load(bad)
+    assert not mc.tune_get(_shape(100, 100, 4, 4, 4))[1]  # nothing of a bad file loads
+    with pytest.raises(mc.McnlaError):
+        mc.tune_load(tmp_path / "missing.txt")
+    mc.tune_clear()

@@ -293,3 +293,42 @@ def test_solver_graph_replay(gpu, problem, integ):
         assert got[3] == ref[3]
         other.solve(A)  # eager work between the replays
         mc.synchronize()
+
+
+@pytest.mark.parametrize("route", [mc._lib.ROUTE_SKETCH, mc._lib.ROUTE_GRAM_OF_A])
+def test_solver_autotune(gpu, route):
+    """The autotuner picks split-K targets among its candidates, records them for the shape, and
+    a tuned plan changes results only by the order of split-K sums; a later solver of the same
+    shape picks the plan up and reproduces the tuned solver's bits."""
+    note("autotune route %d" % route)
+    mc.tune_clear()
+    m, n, k, p, N = 12000, 1500, 20, 12, 32
+    rng = np.random.default_rng(5)
+    U, _ = np.linalg.qr(rng.standard_normal((m, 200)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, 200)))
+    dA = mc.DeviceArray.from_numpy(np.asfortranarray((U * 0.9 ** np.arange(200)) @ V.T))
+    P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+    d = mc.tuning_default()
+    s0 = mc.Solver(P, route=route, seed=3)
+    assert s0.tuning() == d
+    s0.solve(dA)
+    S0, it0 = s0.vectorS(), s0.iteration()
+    s = mc.Solver(P, route=route, seed=3)
+    t = s.autotune(dA, reps=1)
+    note("tuned %s" % t)
+    tuned = {"gram": {576, 1152, 2304, 4608, 9216}, "former": {1024, 2048, 4096, 8192}}
+    if route == mc._lib.ROUTE_GRAM_OF_A:
+        tuned = {"gram": {1152, 2304, 4608}, "h_omega": {1152, 2304, 4608},
+                 "omega_p": {256, 512, 1024, 2048}, "n_by_l": {128, 256, 512, 1024, 2048}}
+    for key, v in t.items():
+        assert v in tuned[key] if key in tuned else v == d[key]
+    assert s.tuning() == t
+    assert mc.tune_get(P, route=route) == (t, True)
+    S1 = s.vectorS()
+    assert abs(s.iteration() - it0) <= 1
+    assert np.max(np.abs(S1 - S0) / S0) < 1e-10
+    s2 = mc.Solver(P, route=route, seed=3)
+    assert s2.tuning() == t
+    s2.solve(dA)
+    assert np.array_equal(s2.vectorS(), S1)  # same plan, same bits
+    mc.tune_clear()
