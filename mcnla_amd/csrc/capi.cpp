@@ -380,8 +380,11 @@ struct FormCore {
                    sizeof(double));
   }
   // z_ready: Z = A'Qbar is already in Z (the solver's Gram-of-A route forms it without A).
+  // mixed: a mixed precision forms Z of an FP32 A on FP32 MFMA (FP32 sums within each split,
+  // FP64 across splits); everything after Z stays FP64.
   void run(const mcnla_params& p, int kind, bool jobv, const void* A, int64_t lda, int dtype,
-           int order, const double* Qbar, hipStream_t st, bool z_ready = false) {
+           int order, const double* Qbar, hipStream_t st, bool z_ready = false,
+           bool mixed = false) {
     const bool svd = kind == MCNLA_FORM_SVD;
     const bool sym = kind == MCNLA_FORM_ROW_BLOCK_SYMMETRIC;
     if (sym) jobv = false;
@@ -402,6 +405,7 @@ struct FormCore {
     z.C = Z.as<double>();
     z.ldc = l;
     z.shape = tall_shape(l);
+    z.f32_mfma = mixed;
     z.kchunk = z_kchunk(m, n);
     z.splits = ceil_div(m, z.kchunk);
     // Wg = Z' Z (symmetric former: Z' Qbar, Qbar having n = m rows)
@@ -1274,7 +1278,8 @@ mcnla_status mcnla_former_run(mcnla_stage s, const void* A, int64_t lda, int a_d
                   MCNLA_ERR_INVALID, "the symmetric former needs a square A");
     hipStream_t st = S(stream);
     f->timer.start(st);
-    f->core.run(f->prm, f->kind, f->jobv, A, lda, a_dtype, a_order, Qbar, st);
+    f->core.run(f->prm, f->kind, f->jobv, A, lda, a_dtype, a_order, Qbar, st, false,
+                f->precision != MCNLA_PREC_F64);
     f->timer.mark(st);
     f->computed = true;
     return MCNLA_OK;
@@ -1575,7 +1580,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
       gemm_f64(z, st);
     }
     s->form.run(p, c.former, c.jobv != 0, A, lda, a_dtype, a_order, s->Qbar.as<double>(), st,
-                via_a);
+                via_a, c.precision != MCNLA_PREC_F64);
     s->timer.mark(st);
     s->computed = true;
     int bad = 0;

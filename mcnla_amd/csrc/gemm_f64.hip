@@ -11,6 +11,8 @@
 // Tiles are staged through LDS as [k][mn] (mn contiguous) so that every MFMA operand fragment is
 // one LDS row segment; v_mfma_f64_16x16x4f64 does the arithmetic.  Source elements may be FP32
 // (mixed precision collections); they are widened to FP64 on the way into LDS.
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -456,15 +458,25 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
 // 32 x LP slab of B through LDS in FP64, k-major, with row strides of 128 bytes modulo 256 so
 // that the fragment reads of 32 lanes cover the 64 banks once.  The next step's slabs load into
 // registers while the MFMAs of the current one run.
-template <typename TA, int LP, bool VEC>
+//
+// TM = float is the mixed-precision former's variant: an FP32 A, B rounded to FP32, and
+// v_mfma_f32_16x16x4f32 (twice the FP64 rate) accumulating in FP32 over one split's K range;
+// the split-K partials are summed in FP64.  Its LDS rows are 16 floats apart modulo 64 banks.
+// (64-deep slabs, 256 contiguous bytes of each A row per step, were slower: 3 workgroups per CU
+// instead of 6.)
+template <typename TA, int LP, bool VEC, typename TM = double>
 __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
   constexpr int BM = 128, BK = 32;
-  constexpr int SA = BM + 16, SB = LP + 16;  // LDS row strides (doubles)
+  constexpr int KT = BK / 2;                 // consecutive k of one A row per thread
+  constexpr int SA = BM + 16, SB = LP + 16;  // LDS row strides (elements)
   constexpr int NB = BK * LP / 256;          // B elements per thread
   constexpr int VA = VEC ? 16 / static_cast<int>(sizeof(TA)) : 1;
+  constexpr bool F32 = sizeof(TM) == 4;
+  static_assert(!F32 || sizeof(TA) == 4, "FP32 MFMA needs an FP32 A");
   typedef typename VecOf<TA, VA>::type V;
-  __shared__ double sA[BK * SA];
-  __shared__ double sB[BK * SB];
+  typedef typename std::conditional<F32, f4, d4>::type Acc;
+  __shared__ TM sA[BK * SA];
+  __shared__ TM sB[BK * SB];
   if (p.gate != nullptr && *p.gate < p.gate_t) return;
   const int64_t tiles_m = ceil_div(p.M, BM);
   const int64_t tile = blockIdx.x % tiles_m, split = blockIdx.x / tiles_m;
@@ -474,39 +486,39 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
   const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const TA* A = static_cast<const TA*>(p.A);
   const double* B = static_cast<const double*>(p.B);
-  // A slab: thread -> row ra = tid / 2, k offset ka = 16 * (tid % 2)
-  const int ra = tid / 2, ka = 16 * (tid % 2);
+  // A slab: thread -> row ra = tid / 2, k offset ka = KT * (tid % 2)
+  const int ra = tid / 2, ka = KT * (tid % 2);
   const bool row_ok = i0 + ra < p.M;
   const TA* arow = A + (row_ok ? (i0 + ra) : 0) * p.lda;
-  double va[16], vb[NB];
+  TM va[KT], vb[NB];
   auto load = [&](int64_t k0) {
     const int64_t kk = k0 + ka;
     bool done = false;
     if constexpr (VEC) {
-      if (row_ok && kk + 16 <= kend) {
+      if (row_ok && kk + KT <= kend) {
 #pragma unroll
-        for (int v = 0; v < 16 / VA; ++v) {
+        for (int v = 0; v < KT / VA; ++v) {
           const V x = *reinterpret_cast<const V*>(arow + kk + v * VA);
 #pragma unroll
-          for (int e = 0; e < VA; ++e) va[v * VA + e] = static_cast<double>(x[e]);
+          for (int e = 0; e < VA; ++e) va[v * VA + e] = static_cast<TM>(x[e]);
         }
         done = true;
       }
     }
     if (!done) {
 #pragma unroll
-      for (int e = 0; e < 16; ++e)
-        va[e] = (row_ok && kk + e < kend) ? static_cast<double>(arow[kk + e]) : 0.0;
+      for (int e = 0; e < KT; ++e)
+        va[e] = (row_ok && kk + e < kend) ? static_cast<TM>(arow[kk + e]) : TM(0);
     }
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
       const int idx = e * 256 + tid, kb = idx / LP, jb = idx % LP;
-      vb[e] = (jb < p.N && k0 + kb < kend) ? B[(k0 + kb) * p.ldb + jb] : 0.0;
+      vb[e] = (jb < p.N && k0 + kb < kend) ? static_cast<TM>(B[(k0 + kb) * p.ldb + jb]) : TM(0);
     }
   };
   auto store = [&] {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) sA[(ka + e) * SA + ra] = va[e];
+    for (int e = 0; e < KT; ++e) sA[(ka + e) * SA + ra] = va[e];
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
       const int idx = e * 256 + tid;
@@ -514,11 +526,11 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
     }
   };
   constexpr int TJ = LP / 16;
-  d4 acc[2][TJ];
+  Acc acc[2][TJ];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < TJ; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < TJ; ++b) acc[a][b] = Acc{0, 0, 0, 0};
   const int fr = lane / 16, fc = lane % 16;
   if (kbeg < kend) load(kbeg);
   for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
@@ -529,7 +541,7 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
 #pragma unroll
     for (int s4 = 0; s4 < BK / 4; ++s4) {
       const int kr = 4 * s4 + fr;
-      double af[2], bf[TJ];
+      TM af[2], bf[TJ];
 #pragma unroll
       for (int a = 0; a < 2; ++a) af[a] = sA[kr * SA + wave * 32 + a * 16 + fc];
 #pragma unroll
@@ -537,11 +549,16 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < TJ; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < TJ; ++b) {
+          if constexpr (F32)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+          else
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+        }
     }
   }
-  // D element r of a 16 x 16 tile sits at row lane/16 + 4r, column lane%16
+  // D element r of a 16 x 16 tile sits at row lane/16 + 4r (FP64) or 4 (lane/16) + r (FP32),
+  // column lane%16
   const bool to_partial = p.partial != nullptr;
   double* P = to_partial ? p.partial + split * p.M * p.N : nullptr;
 #pragma unroll
@@ -550,10 +567,10 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
     for (int b = 0; b < TJ; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gi = i0 + wave * 32 + a * 16 + fr + 4 * r;
+        const int64_t gi = i0 + wave * 32 + a * 16 + (F32 ? 4 * fr + r : fr + 4 * r);
         const int64_t gj = b * 16 + fc;
         if (gi >= p.M || gj >= p.N) continue;
-        const double v = acc[a][b][r];
+        const double v = static_cast<double>(acc[a][b][r]);
         if (to_partial) {
           P[gi * p.N + gj] = v;
         } else {
@@ -563,7 +580,7 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
       }
 }
 
-template <typename TA, int LP>
+template <typename TA, int LP, typename TM = double>
 void launch_tall(const GemmF64Args& p, hipStream_t st) {
   const int64_t nblocks = ceil_div(p.M, 128) * p.splits;
   MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
@@ -571,10 +588,10 @@ void launch_tall(const GemmF64Args& p, hipStream_t st) {
   const bool vec = reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && p.lda % va == 0 &&
                    p.kchunk % va == 0;
   if (vec)
-    hipLaunchKernelGGL((tall_kernel<TA, LP, true>), dim3(static_cast<unsigned>(nblocks)),
+    hipLaunchKernelGGL((tall_kernel<TA, LP, true, TM>), dim3(static_cast<unsigned>(nblocks)),
                        dim3(256), 0, st, p);
   else
-    hipLaunchKernelGGL((tall_kernel<TA, LP, false>), dim3(static_cast<unsigned>(nblocks)),
+    hipLaunchKernelGGL((tall_kernel<TA, LP, false, TM>), dim3(static_cast<unsigned>(nblocks)),
                        dim3(256), 0, st, p);
   MCNLA_LAUNCHED();
 }
@@ -696,7 +713,10 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
                     (p.shape == GemmShape::kTallN32 || p.shape == GemmShape::kTallN64);
   if (p.ev_begin) MCNLA_HIP(hipEventRecord(p.ev_begin, st));
   if (tall) {
-    if (p.a_f32) {
+    if (p.a_f32 && p.f32_mfma) {
+      if (p.N <= 32) launch_tall<float, 32, float>(p, st);
+      else launch_tall<float, 64, float>(p, st);
+    } else if (p.a_f32) {
       if (p.N <= 32) launch_tall<float, 32>(p, st);
       else launch_tall<float, 64>(p, st);
     } else {

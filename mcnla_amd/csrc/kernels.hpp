@@ -22,6 +22,9 @@ struct GemmF64Args {
   int64_t ldb = 0;
   bool b_ncontig = true;
   bool b_f32 = false;
+  // Tall FP32-A products only: FP32 MFMA (B rounded to FP32, FP32 sums within a split, FP64
+  // across splits) instead of FP64 arithmetic.  Other shapes ignore it.
+  bool f32_mfma = false;
   double* C = nullptr;      // row-major, ldc; may be null when only partials are wanted
   int64_t ldc = 0;
   double alpha = 1.0, beta = 0.0;
