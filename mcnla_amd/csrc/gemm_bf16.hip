@@ -208,17 +208,18 @@ __global__ __launch_bounds__(kThreads) void sketch_bf16_kernel(
 // per operand.  Inside a block the 16-byte chunk c = k/8 of row r is stored at chunk position
 // c ^ ((r / 4) % 4): a linear copy into LDS then gives fragment reads (16 rows x one chunk per
 // 16 lanes) that hit 16 distinct 4-bank groups.  The k blocks are padded with zero blocks to a
-// multiple of kStages.
+// multiple of kUnroll.
 constexpr int kBR = 256;       // rows per block
 constexpr int kBKB = 32;       // k per block
 constexpr int kBlk = kBR * kBKB;  // elements per block
-constexpr int kStages = 4;
+constexpr int kStages = 5;   // LDS ring depth of the GEMM (5 x 32 KiB)
+constexpr int kUnroll = 10;  // steps per loop body: ring slots x fragment buffers
 
 __host__ __device__ constexpr int blk_off(int r, int k) {
   return r * kBKB + 8 * ((k >> 3) ^ ((r >> 2) & 3)) + (k & 7);
 }
 
-int64_t blk_nkt(int64_t n) { return ceil_div(ceil_div(n, kBKB), kStages) * kStages; }
+int64_t blk_nkt(int64_t n) { return ceil_div(ceil_div(n, kBKB), kUnroll) * kUnroll; }
 
 // src element (r, k) at src[r + k*ld] (R_CONTIG) or src[r*ld + k]; one workgroup per block.
 // VEC4: FP32 R_CONTIG source with ld % 4 == 0 and a 16-byte aligned base (float4 loads).
@@ -284,6 +285,13 @@ __global__ __launch_bounds__(256) void pack_blocks_kernel(const TS* __restrict__
 
 typedef __attribute__((address_space(3))) void lds_void;
 
+// vmcnt <= VM and lgkmcnt == 0 (every LDS read issued so far has completed).  Being a builtin,
+// the compiler's wait insertion sees it, so a later wait need not cover those reads again.
+template <int VM>
+__device__ __forceinline__ void wait_vmcnt_lgkm0() {
+  __builtin_amdgcn_s_waitcnt((VM & 0xF) | (0x7 << 4) | (((VM >> 4) & 3) << 14));
+}
+
 template <int VM>
 __device__ __forceinline__ void wait_vmcnt() {
   // gfx9 s_waitcnt fields: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt high bits [15:14]
@@ -302,8 +310,9 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
                                                         int64_t nkt, int64_t m, int64_t nl,
                                                         TY* __restrict__ Y, int64_t ldy) {
   constexpr int L = 8;  // DMA instructions per wave and stage (4 for A, 4 for B)
+  static_assert(kStages == 5 && kUnroll == 10, "the loop body below is written for 5 slots");
   __shared__ __attribute__((aligned(16))) uint16_t S0[2 * kBlk], S1[2 * kBlk], S2[2 * kBlk],
-      S3[2 * kBlk];
+      S3[2 * kBlk], S4[2 * kBlk];
   const int64_t tiles_n = ceil_div(nl, kBR), tiles_m = ceil_div(m, kBR);
   const int64_t logical = xcd_swizzle(blockIdx.x, tiles_m * tiles_n);
   const int64_t tm = logical / tiles_n, tn = logical % tiles_n;
@@ -327,33 +336,56 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
   for (int i = 0; i < 8; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  auto step = [&](int64_t kt, const uint16_t* ss, uint16_t* sn) {
-    asm volatile("" ::: "memory");
-    wait_vmcnt<2 * L>();           // stage kt landed (two younger stages may be in flight)
-    __builtin_amdgcn_s_barrier();  // ... in every wave; and the stage before it is fully read
-    asm volatile("" ::: "memory");
-    issue(kt + 3 < nkt ? kt + 3 : nkt - 1, sn);
-    bf16x8 af[8], bfr[8];
+  // Fragments are double-buffered: step kt reads stage kt + 1's fragments from LDS while its
+  // MFMAs consume stage kt's, which were read one step earlier.
+  bf16x8 fa0[8], fb0[8], fa1[8], fb1[8];
+  auto frags = [&](const uint16_t* ss, bf16x8* af, bf16x8* bfr) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
       af[i] = *reinterpret_cast<const bf16x8*>(&ss[blk_off(wm0 + i * 16 + lane % 16, 8 * kg)]);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
       bfr[j] = *reinterpret_cast<const bf16x8*>(&ss[kBlk + blk_off(wn0 + j * 16 + lane % 16, 8 * kg)]);
+  };
+  // Step kt: stage kt + 1 must have landed (stages up to kt + 3 are issued; two may still be in
+  // flight); the barrier makes that true in every wave.  The stage issued here, kt + 4 (clamped
+  // to the last block), overwrites stage kt - 1's slot, whose fragments every wave consumed in
+  // step kt - 1, before this barrier.  The last step's look-ahead reads a slot that holds the
+  // last block again and discards it.
+  auto step = [&](int64_t kt, const bf16x8* caf, const bf16x8* cbf, const uint16_t* snext,
+                  bf16x8* naf, bf16x8* nbf, uint16_t* sn) {
+    asm volatile("" ::: "memory");
+    wait_vmcnt_lgkm0<2 * L>();  // and this step's fragments (read last step) are in registers
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    issue(kt + 4 < nkt ? kt + 4 : nkt - 1, sn);
+    frags(snext, naf, nbf);
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) mfma_bf16_tied(acc[i][j], af[i], bfr[j]);
+      for (int j = 0; j < 8; ++j) mfma_bf16_tied(acc[i][j], caf[i], cbf[j]);
   };
   issue(0, S0);
   issue(1, S1);
   issue(2, S2);
+  issue(3, S3);
+  asm volatile("" ::: "memory");
+  wait_vmcnt<3 * L>();  // stage 0 landed in every wave
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+  frags(S0, fa0, fb0);
   asm volatile("s_nop 7");  // zeroed accumulators (VALU writes) before their first MFMA read
-  for (int64_t kt = 0; kt < nkt; kt += kStages) {
-    step(kt, S0, S3);
-    step(kt + 1, S1, S0);
-    step(kt + 2, S2, S1);
-    step(kt + 3, S3, S2);
+  for (int64_t kt = 0; kt < nkt; kt += kUnroll) {  // stage s sits in slot s % 5
+    step(kt, fa0, fb0, S1, fa1, fb1, S4);
+    step(kt + 1, fa1, fb1, S2, fa0, fb0, S0);
+    step(kt + 2, fa0, fb0, S3, fa1, fb1, S1);
+    step(kt + 3, fa1, fb1, S4, fa0, fb0, S2);
+    step(kt + 4, fa0, fb0, S0, fa1, fb1, S3);
+    step(kt + 5, fa1, fb1, S1, fa0, fb0, S4);
+    step(kt + 6, fa0, fb0, S2, fa1, fb1, S0);
+    step(kt + 7, fa1, fb1, S3, fa0, fb0, S1);
+    step(kt + 8, fa0, fb0, S4, fa1, fb1, S2);
+    step(kt + 9, fa1, fb1, S0, fa0, fb0, S3);
   }
   asm volatile("s_nop 7\n s_nop 7\n s_nop 7\n s_nop 7");  // last MFMAs drain before VALU reads
   wait_vmcnt<0>();  // no DMA may still target LDS when the workgroup ends
