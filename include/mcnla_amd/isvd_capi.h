@@ -280,6 +280,7 @@ typedef struct {
   int64_t h_omega;  /* P = H Omega (Gram-of-A route) */
   int64_t omega_p;  /* G = Omega'P (Gram-of-A route) */
   int64_t n_by_l;   /* X = Omega (T M) and Z = H X (Gram-of-A route) */
+  int64_t qbar;     /* Qbar = Y (T M) (sketch route) */
 } mcnla_tuning;
 
 /* The defaults (no table entry). */
@@ -302,7 +303,7 @@ mcnla_status mcnla_tune_get(const mcnla_params* params, const mcnla_solver_confi
                             mcnla_tuning* t, int* found);
 void mcnla_tune_clear(void);
 /* Text persistence, one entry a line: "route precision m n l N gram former h_omega omega_p
- * n_by_l".  Load merges into the table; a malformed line is MCNLA_ERR_INVALID and loads nothing. */
+ * n_by_l qbar".  Load merges into the table; a malformed line is MCNLA_ERR_INVALID and loads nothing. */
 mcnla_status mcnla_tune_save(const char* path);
 mcnla_status mcnla_tune_load(const char* path);
 

@@ -58,7 +58,7 @@ class SolverConfig(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """Split-K workgroup targets of a solve's FP64 GEMMs (mcnla_tuning)."""
     _fields_ = [("gram", ctypes.c_int64), ("former", ctypes.c_int64), ("h_omega", ctypes.c_int64),
-                ("omega_p", ctypes.c_int64), ("n_by_l", ctypes.c_int64)]
+                ("omega_p", ctypes.c_int64), ("n_by_l", ctypes.c_int64), ("qbar", ctypes.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

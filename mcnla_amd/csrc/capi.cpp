@@ -54,7 +54,7 @@ mcnla_status guard(F&& f) {
 }
 
 // ---- tuned GEMM plans: workgroup targets of the split-K FP64 GEMMs per solve shape ----
-constexpr mcnla_tuning kDefaultTuning = {2304, 4096, 2304, 1024, 512};
+constexpr mcnla_tuning kDefaultTuning = {2304, 4096, 2304, 1024, 512, 512};
 constexpr int64_t kMaxTarget = 1 << 20;
 
 struct TuneKey {
@@ -72,7 +72,7 @@ TuneKey tune_key(const mcnla_params& p, int route, int precision) {
   return TuneKey{{route, precision, p.nrow, p.ncol, p.rank + p.over_rank, p.num_sketch}};
 }
 bool valid_tuning(const mcnla_tuning& t) {
-  for (int64_t v : {t.gram, t.former, t.h_omega, t.omega_p, t.n_by_l})
+  for (int64_t v : {t.gram, t.former, t.h_omega, t.omega_p, t.n_by_l, t.qbar})
     if (v < 1 || v > kMaxTarget) return false;
   return true;
 }
@@ -774,6 +774,11 @@ int64_t n_by_l_splits(int64_t n, int64_t K, int64_t target) {
   return gemm_f64_splits_for(ceil_div(n, 128), K, target);
 }
 
+// Split-K factor of Qbar = Y (T M) (m x l, K = Nl) on the sketch route: m/128 row tiles.
+int64_t qbar_splits(int64_t m, int64_t nl, int64_t target) {
+  return gemm_f64_splits_for(ceil_div(m, 128), nl, target);
+}
+
 // G = Omega' P (Nl x Nl, symmetric): lower-triangle tiles.
 GemmF64Args plan_omega_p(int64_t n, int64_t nl, int64_t target) {
   GemmF64Args g;
@@ -811,6 +816,9 @@ void apply_tuning(mcnla_solver_s* s, const mcnla_tuning& t) {
                            gemm_f64_partial_elems(n, l, n_by_l_splits(n, n, t.n_by_l))}) *
                  sizeof(double));
   s->gram.prepare(false, m, via_a ? n : nl);
+  if (!via_a)  // the Qbar product's split-K partials reuse the Gram's (free by then)
+    s->gram.partial.ensure(gemm_f64_partial_elems(m, l, qbar_splits(m, nl, t.qbar)) *
+                           sizeof(double));
   s->form.prepare(s->prm);
 }
 
@@ -1540,6 +1548,13 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     q.C = s->Qbar.as<double>();
     q.ldc = l;
     q.shape = tall_shape(l);
+    if (!via_a) {
+      q.splits = qbar_splits(m, nl, s->tune.qbar);
+      if (q.splits > 1) {
+        q.kchunk = ceil_div(ceil_div(nl, q.splits), 16) * 16;
+        q.partial = s->gram.partial.as<double>();
+      }
+    }
     if (via_a) {  // X = Omega (T M) (n x l), then Qbar = A X
       GemmF64Args x = q;
       x.M = n;
@@ -1679,7 +1694,8 @@ mcnla_status mcnla_solver_autotune(mcnla_solver s, const void* A, int64_t lda, i
                {&mcnla_tuning::n_by_l, {128, 256, 512, 1024, 2048}, 12u}};
     } else {
       knobs = {{&mcnla_tuning::gram, {576, 1152, 2304, 4608, 9216}, 2u},
-               {&mcnla_tuning::former, {1024, 2048, 4096, 8192}, 8u}};
+               {&mcnla_tuning::former, {1024, 2048, 4096, 8192}, 8u},
+               {&mcnla_tuning::qbar, {512, 1024, 2048, 3072, 4096}, 4u}};
     }
     auto solve = [&] {
       const mcnla_status r = mcnla_solver_solve(s, A, lda, a_dtype, a_order, stream);
@@ -1763,11 +1779,11 @@ mcnla_status mcnla_tune_save(const char* path) {
         const mcnla_tuning& t = e.second;
         char line[256];
         std::snprintf(line, sizeof line,
-                      "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld\n",
+                      "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld\n",
                       (long long)k.v[0], (long long)k.v[1], (long long)k.v[2], (long long)k.v[3],
                       (long long)k.v[4], (long long)k.v[5], (long long)t.gram,
                       (long long)t.former, (long long)t.h_omega, (long long)t.omega_p,
-                      (long long)t.n_by_l);
+                      (long long)t.n_by_l, (long long)t.qbar);
         text += line;
       }
     }
@@ -1791,18 +1807,19 @@ mcnla_status mcnla_tune_load(const char* path) {
     std::string bad;
     while (bad.empty() && std::fgets(line, sizeof line, f)) {
       ++lineno;
-      long long v[11];
+      long long v[12];
       char extra;
-      const int got = std::sscanf(line, "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %c",
-                                  &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8],
-                                  &v[9], &v[10], &extra);
+      const int got =
+          std::sscanf(line, "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %c", &v[0],
+                      &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11],
+                      &extra);
       bool blank = true;
       for (const char* c = line; *c; ++c)
         if (*c != ' ' && *c != '\t' && *c != '\n' && *c != '\r') blank = false;
       if (blank) continue;
       TuneKey k{{v[0], v[1], v[2], v[3], v[4], v[5]}};
-      const mcnla_tuning t{v[6], v[7], v[8], v[9], v[10]};
-      const bool key_ok = got == 11 &&
+      const mcnla_tuning t{v[6], v[7], v[8], v[9], v[10], v[11]};
+      const bool key_ok = got == 12 &&
                           (k.v[0] == MCNLA_ROUTE_SKETCH || k.v[0] == MCNLA_ROUTE_GRAM_OF_A) &&
                           k.v[2] >= 1 && k.v[3] >= 1 && k.v[4] >= 1 && k.v[5] >= 1;
       if (!key_ok || !valid_tuning(t)) bad = path + std::string(":") + std::to_string(lineno);

@@ -205,7 +205,8 @@ def test_tune_table_round_trip(tmp_path):
     mc.tune_clear()
     P = _shape(100000, 10000, 20, 12, 64)
     d = mc.tuning_default()
-    assert d == {"gram": 2304, "former": 4096, "h_omega": 2304, "omega_p": 1024, "n_by_l": 512}
+    assert d == {"gram": 2304, "former": 4096, "h_omega": 2304, "omega_p": 1024, "n_by_l": 512,
+                 "qbar": 512}
     got, found = mc.tune_get(P, precision=_lib.PREC_MIXED_BF16)
     assert not found and got == d
     t = dict(d, gram=4608, former=2048)
@@ -233,7 +234,7 @@ def test_tune_table_round_trip(tmp_path):
     with pytest.raises(mc.McnlaError):
         mc.tune_set(P, dict(d, n_by_l=(1 << 20) + 1))
     bad = tmp_path / "bad.txt"
-    bad.write_text("1 0 100 100 8 4 2304 4096 2304 1024 512\n1 0 100 100 8 4 2304 4096\n")
+    bad.write_text("1 0 100 100 8 4 2304 4096 2304 1024 512 512\n1 0 100 100 8 4 2304 4096\n")
     mc.tune_clear()
     with pytest.raises(mc.McnlaError):
         mc.tune_load(bad)

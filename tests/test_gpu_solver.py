@@ -316,7 +316,8 @@ def test_solver_autotune(gpu, route):
     s = mc.Solver(P, route=route, seed=3)
     t = s.autotune(dA, reps=1)
     note("tuned %s" % t)
-    tuned = {"gram": {576, 1152, 2304, 4608, 9216}, "former": {1024, 2048, 4096, 8192}}
+    tuned = {"gram": {576, 1152, 2304, 4608, 9216}, "former": {1024, 2048, 4096, 8192},
+             "qbar": {512, 1024, 2048, 3072, 4096}}
     if route == mc._lib.ROUTE_GRAM_OF_A:
         tuned = {"gram": {1152, 2304, 4608}, "h_omega": {1152, 2304, 4608},
                  "omega_p": {256, 512, 1024, 2048}, "n_by_l": {128, 256, 512, 1024, 2048}}
