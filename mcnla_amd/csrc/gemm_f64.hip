@@ -580,6 +580,143 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
       }
 }
 
+// FP64 tall products with the A operand read from HBM straight into MFMA registers (no LDS slab
+// for A).  Lane L of a wave owns row L % 16 of each of its two 16-row groups; load q (0..3) of a
+// 32-deep step gives it k = 8q + 2 (L / 16) + {0, 1}, so the 4 lanes of a row read 64 contiguous
+// bytes per instruction; MFMA j contracts over k = 8 (j / 2) + 2 (L / 16) + j % 2, the same k for
+// the B rows its lanes read from LDS (any k order gives the sum; this one pairs A and B lanes).
+// Only the 32 x LP slab of B goes through LDS, so more workgroups fit on a CU than when A is
+// staged too.  The next step's A (other register set) and B load during the current MFMAs.  The
+// B slab's row stride is 4 mod 16 doubles, so the 4 k-rows a wave reads at once (2 apart) fall
+// on the 64 banks exactly twice.
+template <int LP, bool VEC>
+__global__ __launch_bounds__(256, 2) void tall_direct_kernel(GemmF64Args p) {
+  constexpr int BM = 128, BK = 32;
+  constexpr int SB = LP + 4;
+  constexpr int NB = BK * LP / 256;  // B elements per thread and step
+  constexpr int TJ = LP / 16;
+  typedef typename VecOf<double, 2>::type D2;
+  __shared__ double sB[BK * SB];
+  if (p.gate != nullptr && *p.gate < p.gate_t) return;
+  const int64_t tiles_m = ceil_div(p.M, BM);
+  const int64_t tile = blockIdx.x % tiles_m, split = blockIdx.x / tiles_m;
+  const int64_t i0 = tile * BM;
+  const int64_t kbeg = split * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const int fr = lane / 16, fc = lane % 16;
+  const double* A = static_cast<const double*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+  const double* arow[2];
+  bool rok[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int64_t row = i0 + wave * 32 + a * 16 + fc;
+    rok[a] = row < p.M;
+    arow[a] = A + (rok[a] ? row : 0) * p.lda;
+  }
+  // element j of a lane's 8 holds k = k0 + kof(j)
+  auto kof = [&](int j) { return 8 * (j / 2) + 2 * fr + (j % 2); };
+  auto load_a = [&](int64_t k0, double (&v)[2][8]) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      if (VEC && rok[a] && k0 + BK <= kend) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const D2 x = *reinterpret_cast<const D2*>(arow[a] + k0 + 8 * q + 2 * fr);
+          v[a][2 * q] = x[0];
+          v[a][2 * q + 1] = x[1];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          v[a][e] = (rok[a] && k0 + kof(e) < kend) ? arow[a][k0 + kof(e)] : 0.0;
+      }
+    }
+  };
+  double vb[NB];
+  auto load_b = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int idx = e * 256 + tid, kb = idx / LP, jb = idx % LP;
+      vb[e] = (jb < p.N && k0 + kb < kend) ? B[(k0 + kb) * p.ldb + jb] : 0.0;
+    }
+  };
+  d4 acc[2][TJ];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < TJ; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  auto step = [&](int64_t k0, const double (&cur)[2][8], double (&nxt)[2][8]) {
+    __syncthreads();  // the previous step's reads of sB are done
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int idx = e * 256 + tid;
+      sB[(idx / LP) * SB + idx % LP] = vb[e];
+    }
+    __syncthreads();
+    if (k0 + BK < kend) {
+      load_a(k0 + BK, nxt);
+      load_b(k0 + BK);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      double bf[TJ];
+#pragma unroll
+      for (int b = 0; b < TJ; ++b) bf[b] = sB[kof(j) * SB + b * 16 + fc];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < TJ; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[a][j], bf[b], acc[a][b], 0, 0, 0);
+    }
+  };
+  double va0[2][8], va1[2][8];
+  if (kbeg < kend) {
+    load_a(kbeg, va0);
+    load_b(kbeg);
+  }
+  for (int64_t k0 = kbeg; k0 < kend; k0 += 2 * BK) {
+    step(k0, va0, va1);
+    if (k0 + BK < kend) step(k0 + BK, va1, va0);
+  }
+  // D element r of a 16 x 16 tile sits at row lane/16 + 4r, column lane%16
+  const bool to_partial = p.partial != nullptr;
+  double* P = to_partial ? p.partial + split * p.M * p.N : nullptr;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < TJ; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = i0 + wave * 32 + a * 16 + fr + 4 * r;
+        const int64_t gj = b * 16 + fc;
+        if (gi >= p.M || gj >= p.N) continue;
+        const double v = acc[a][b][r];
+        if (to_partial) {
+          P[gi * p.N + gj] = v;
+        } else {
+          double* c = p.C + gi * p.ldc + gj;
+          *c = (p.beta == 0.0) ? p.alpha * v : p.alpha * v + p.beta * *c;
+        }
+      }
+}
+
+template <int LP>
+void launch_tall_direct(const GemmF64Args& p, hipStream_t st) {
+  const int64_t nblocks = ceil_div(p.M, 128) * p.splits;
+  MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
+  const bool vec = reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && p.lda % 2 == 0 &&
+                   p.kchunk % 2 == 0;
+  if (vec)
+    hipLaunchKernelGGL((tall_direct_kernel<LP, true>), dim3(static_cast<unsigned>(nblocks)),
+                       dim3(256), 0, st, p);
+  else
+    hipLaunchKernelGGL((tall_direct_kernel<LP, false>), dim3(static_cast<unsigned>(nblocks)),
+                       dim3(256), 0, st, p);
+  MCNLA_LAUNCHED();
+}
+
 template <typename TA, int LP, typename TM = double>
 void launch_tall(const GemmF64Args& p, hipStream_t st) {
   const int64_t nblocks = ceil_div(p.M, 128) * p.splits;
@@ -720,8 +857,8 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
       if (p.N <= 32) launch_tall<float, 32>(p, st);
       else launch_tall<float, 64>(p, st);
     } else {
-      if (p.N <= 32) launch_tall<double, 32>(p, st);
-      else launch_tall<double, 64>(p, st);
+      if (p.N <= 32) launch_tall_direct<32>(p, st);
+      else launch_tall_direct<64>(p, st);
     }
   } else if (dma) {
     // 256 x 128 tiles of 8 waves (64 x 64 each); a lower-triangle schedule takes the tiles that
