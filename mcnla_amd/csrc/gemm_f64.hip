@@ -450,8 +450,9 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
 // ------------------------------------------------------------------------------------------------
 // Tall-skinny products C (M x N, N <= 64) = A * B where A's rows run along K (element (i, k) at
 // A[i*lda + k]) and B's rows along N (element (k, j) at B[k*ldb + j]): the former's Z = A'Qbar
-// of a column-major A (FP32 or FP64), and Qbar = Y (T M) of the row-major collection Y.  A is
-// streamed once, so the kernel is bound by HBM or by FP64 MFMA, whichever is tighter.
+// of a column-major A, Qbar = A X of a row-major A.  A is streamed once, so the kernel is bound
+// by HBM or by MFMA, whichever is tighter.  This LDS-staged form serves FP32 A operands; FP64
+// ones (Qbar = Y (T M), the integrators' Bs products) go to tall_direct_kernel below.
 //
 // 128 x LP output tiles (LP = 32 or 64), four waves of 32 x LP.  Each step moves a 128 x 32 slab
 // of A (two threads per row, 16 consecutive k each: 128 or 256 contiguous bytes per row) and a
@@ -580,23 +581,34 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
       }
 }
 
-// FP64 tall products with the A operand read from HBM straight into MFMA registers (no LDS slab
-// for A).  Lane L of a wave owns row L % 16 of each of its two 16-row groups; load q (0..3) of a
-// 32-deep step gives it k = 8q + 2 (L / 16) + {0, 1}, so the 4 lanes of a row read 64 contiguous
-// bytes per instruction; MFMA j contracts over k = 8 (j / 2) + 2 (L / 16) + j % 2, the same k for
-// the B rows its lanes read from LDS (any k order gives the sum; this one pairs A and B lanes).
-// Only the 32 x LP slab of B goes through LDS, so more workgroups fit on a CU than when A is
-// staged too.  The next step's A (other register set) and B load during the current MFMAs.  The
-// B slab's row stride is 4 mod 16 doubles, so the 4 k-rows a wave reads at once (2 apart) fall
-// on the 64 banks exactly twice.
-template <int LP, bool VEC>
-__global__ __launch_bounds__(256, 2) void tall_direct_kernel(GemmF64Args p) {
+// Tall products with the A operand read from HBM straight into MFMA registers (no LDS slab for
+// A).  With VE = 16 / sizeof(TA) elements per 16-byte load, lane L of a wave owns row L % 16 of
+// each of its two 16-row groups, and load q of a 32-deep step gives it
+// k = 4 VE q + VE (L / 16) + e (e < VE): the 4 lanes of a row read 64 contiguous bytes per
+// instruction.  MFMA j contracts over k = kof(j) = 4 VE (j / VE) + VE (L / 16) + j % VE, the same
+// k for the B rows its lanes read from LDS (any k order gives the sum; this one pairs A and B
+// lanes).  Only the 32 x LP slab of B goes through LDS, so more workgroups fit on a CU and more
+// of A is in flight than when A is staged too; the next step's A (other register set) and B load
+// during the current MFMAs.  The B slab's row stride LP + 4 puts the 4 k-rows a wave reads at
+// once on the 64 banks at most twice each.
+//
+// TM is the arithmetic: double (v_mfma_f64_16x16x4f64; an FP32 A is widened on load) or float
+// (FP32 A, B rounded to FP32, v_mfma_f32_16x16x4f32 with FP32 sums within one split's K range;
+// the split-K partials are summed in FP64).  The GEMM dispatch uses it for FP64 A; for an FP32 A
+// the LDS-staged tall_kernel above measured as fast or faster (mixed former Z: 1.25 against
+// 1.31 ms; 500000 x 2000 FP32 A times X: equal).
+template <typename TA, typename TM, int LP, bool VEC>
+__global__ __launch_bounds__(256, LP == 32 ? 3 : 2) void tall_direct_kernel(GemmF64Args p) {
   constexpr int BM = 128, BK = 32;
   constexpr int SB = LP + 4;
   constexpr int NB = BK * LP / 256;  // B elements per thread and step
   constexpr int TJ = LP / 16;
-  typedef typename VecOf<double, 2>::type D2;
-  __shared__ double sB[BK * SB];
+  constexpr int VE = 16 / static_cast<int>(sizeof(TA));
+  constexpr bool F32 = sizeof(TM) == 4;
+  static_assert(!F32 || sizeof(TA) == 4, "FP32 MFMA needs an FP32 A");
+  typedef typename VecOf<TA, VE>::type V;
+  typedef typename std::conditional<F32, f4, d4>::type Acc;
+  __shared__ TM sB[BK * SB];
   if (p.gate != nullptr && *p.gate < p.gate_t) return;
   const int64_t tiles_m = ceil_div(p.M, BM);
   const int64_t tile = blockIdx.x % tiles_m, split = blockIdx.x / tiles_m;
@@ -605,9 +617,9 @@ __global__ __launch_bounds__(256, 2) void tall_direct_kernel(GemmF64Args p) {
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
   const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const int fr = lane / 16, fc = lane % 16;
-  const double* A = static_cast<const double*>(p.A);
+  const TA* A = static_cast<const TA*>(p.A);
   const double* B = static_cast<const double*>(p.B);
-  const double* arow[2];
+  const TA* arow[2];
   bool rok[2];
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
@@ -615,39 +627,38 @@ __global__ __launch_bounds__(256, 2) void tall_direct_kernel(GemmF64Args p) {
     rok[a] = row < p.M;
     arow[a] = A + (rok[a] ? row : 0) * p.lda;
   }
-  // element j of a lane's 8 holds k = k0 + kof(j)
-  auto kof = [&](int j) { return 8 * (j / 2) + 2 * fr + (j % 2); };
-  auto load_a = [&](int64_t k0, double (&v)[2][8]) {
+  auto kof = [&](int j) { return 4 * VE * (j / VE) + VE * fr + (j % VE); };
+  auto load_a = [&](int64_t k0, TM (&v)[2][8]) {
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       if (VEC && rok[a] && k0 + BK <= kend) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const D2 x = *reinterpret_cast<const D2*>(arow[a] + k0 + 8 * q + 2 * fr);
-          v[a][2 * q] = x[0];
-          v[a][2 * q + 1] = x[1];
+        for (int q = 0; q < 8 / VE; ++q) {
+          const V x = *reinterpret_cast<const V*>(arow[a] + k0 + 4 * VE * q + VE * fr);
+#pragma unroll
+          for (int e = 0; e < VE; ++e) v[a][VE * q + e] = static_cast<TM>(x[e]);
         }
       } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e)
-          v[a][e] = (rok[a] && k0 + kof(e) < kend) ? arow[a][k0 + kof(e)] : 0.0;
+          v[a][e] = (rok[a] && k0 + kof(e) < kend) ? static_cast<TM>(arow[a][k0 + kof(e)]) : TM(0);
       }
     }
   };
-  double vb[NB];
+  TM vb[NB];
   auto load_b = [&](int64_t k0) {
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
       const int idx = e * 256 + tid, kb = idx / LP, jb = idx % LP;
-      vb[e] = (jb < p.N && k0 + kb < kend) ? B[(k0 + kb) * p.ldb + jb] : 0.0;
+      vb[e] = (jb < p.N && k0 + kb < kend) ? static_cast<TM>(B[(k0 + kb) * p.ldb + jb]) : TM(0);
     }
   };
-  d4 acc[2][TJ];
+  Acc acc[2][TJ];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < TJ; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  auto step = [&](int64_t k0, const double (&cur)[2][8], double (&nxt)[2][8]) {
+    for (int b = 0; b < TJ; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+  auto step = [&](int64_t k0, const TM (&cur)[2][8], TM (&nxt)[2][8]) {
     __syncthreads();  // the previous step's reads of sB are done
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
@@ -661,17 +672,21 @@ __global__ __launch_bounds__(256, 2) void tall_direct_kernel(GemmF64Args p) {
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      double bf[TJ];
+      TM bf[TJ];
 #pragma unroll
       for (int b = 0; b < TJ; ++b) bf[b] = sB[kof(j) * SB + b * 16 + fc];
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < TJ; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[a][j], bf[b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < TJ; ++b) {
+          if constexpr (F32)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[a][j], bf[b], acc[a][b], 0, 0, 0);
+          else
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[a][j], bf[b], acc[a][b], 0, 0, 0);
+        }
     }
   };
-  double va0[2][8], va1[2][8];
+  TM va0[2][8], va1[2][8];
   if (kbeg < kend) {
     load_a(kbeg, va0);
     load_b(kbeg);
@@ -680,7 +695,8 @@ __global__ __launch_bounds__(256, 2) void tall_direct_kernel(GemmF64Args p) {
     step(k0, va0, va1);
     if (k0 + BK < kend) step(k0 + BK, va1, va0);
   }
-  // D element r of a 16 x 16 tile sits at row lane/16 + 4r, column lane%16
+  // D element r of a 16 x 16 tile sits at row lane/16 + 4r (FP64) or 4 (lane/16) + r (FP32),
+  // column lane%16
   const bool to_partial = p.partial != nullptr;
   double* P = to_partial ? p.partial + split * p.M * p.N : nullptr;
 #pragma unroll
@@ -689,10 +705,10 @@ __global__ __launch_bounds__(256, 2) void tall_direct_kernel(GemmF64Args p) {
     for (int b = 0; b < TJ; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gi = i0 + wave * 32 + a * 16 + fr + 4 * r;
+        const int64_t gi = i0 + wave * 32 + a * 16 + (F32 ? 4 * fr + r : fr + 4 * r);
         const int64_t gj = b * 16 + fc;
         if (gi >= p.M || gj >= p.N) continue;
-        const double v = acc[a][b][r];
+        const double v = static_cast<double>(acc[a][b][r]);
         if (to_partial) {
           P[gi * p.N + gj] = v;
         } else {
@@ -702,18 +718,19 @@ __global__ __launch_bounds__(256, 2) void tall_direct_kernel(GemmF64Args p) {
       }
 }
 
-template <int LP>
+template <typename TA, typename TM, int LP>
 void launch_tall_direct(const GemmF64Args& p, hipStream_t st) {
   const int64_t nblocks = ceil_div(p.M, 128) * p.splits;
   MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
-  const bool vec = reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && p.lda % 2 == 0 &&
-                   p.kchunk % 2 == 0;
+  constexpr int ve = 16 / static_cast<int>(sizeof(TA));
+  const bool vec = reinterpret_cast<uintptr_t>(p.A) % 16 == 0 && p.lda % ve == 0 &&
+                   p.kchunk % ve == 0;
   if (vec)
-    hipLaunchKernelGGL((tall_direct_kernel<LP, true>), dim3(static_cast<unsigned>(nblocks)),
-                       dim3(256), 0, st, p);
+    hipLaunchKernelGGL((tall_direct_kernel<TA, TM, LP, true>),
+                       dim3(static_cast<unsigned>(nblocks)), dim3(256), 0, st, p);
   else
-    hipLaunchKernelGGL((tall_direct_kernel<LP, false>), dim3(static_cast<unsigned>(nblocks)),
-                       dim3(256), 0, st, p);
+    hipLaunchKernelGGL((tall_direct_kernel<TA, TM, LP, false>),
+                       dim3(static_cast<unsigned>(nblocks)), dim3(256), 0, st, p);
   MCNLA_LAUNCHED();
 }
 
@@ -857,8 +874,8 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
       if (p.N <= 32) launch_tall<float, 32>(p, st);
       else launch_tall<float, 64>(p, st);
     } else {
-      if (p.N <= 32) launch_tall_direct<32>(p, st);
-      else launch_tall_direct<64>(p, st);
+      if (p.N <= 32) launch_tall_direct<double, double, 32>(p, st);
+      else launch_tall_direct<double, double, 64>(p, st);
     }
   } else if (dma) {
     // 256 x 128 tiles of 8 waves (64 x 64 each); a lower-triangle schedule takes the tiles that
