@@ -1568,6 +1568,9 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
       if (a_order == MCNLA_ROW_MAJOR) {  // stream the caller's A (FP32 reads half the bytes)
         q.A = A;
         q.a_f32 = a_dtype == MCNLA_F32;
+        // a mixed precision forms this basis of an FP32 A on FP32 MFMA (sigma come from Z = H X,
+        // which stays FP64)
+        q.f32_mfma = c.precision != MCNLA_PREC_F64;
         q.lda = lda;
       } else {
         q.A = Ar;

@@ -333,3 +333,29 @@ def test_solver_autotune(gpu, route):
     s2.solve(dA)
     assert np.array_equal(s2.vectorS(), S1)  # same plan, same bits
     mc.tune_clear()
+
+
+def test_gram_of_a_route_mixed_precision(gpu):
+    """Mixed precision on the Gram-of-A route: H, G, the integrator and Z = H X stay FP64, so the
+    singular values match the FP64 solve to rounding; Qbar = A X of the FP32 A runs on FP32 MFMA,
+    so U and Qbar match to FP32 accuracy (stated tolerance 1e-5 on the projectors)."""
+    note("gram-of-a mixed")
+    m, n, k, p, N = 20000, 300, 10, 6, 32
+    rng = np.random.default_rng(9)
+    U, _ = np.linalg.qr(rng.standard_normal((m, 60)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, 60)))
+    A = np.ascontiguousarray(((U * 0.8 ** np.arange(60)) @ V.T).astype(np.float32))
+    dA = mc.DeviceArray.from_numpy(A)
+    P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+    out = {}
+    for prec in (mc.PREC_F64, mc.PREC_MIXED_BF16):
+        s = mc.Solver(P, precision=prec, route=mc.ROUTE_GRAM_OF_A, seed=4)
+        s.solve(dA)
+        out[prec] = (s.vectorS(), s.matrixU(), s.matrixQbar(), s.iteration())
+    S64, U64, Q64, it64 = out[mc.PREC_F64]
+    Smx, Umx, Qmx, itmx = out[mc.PREC_MIXED_BF16]
+    assert it64 == itmx
+    assert np.max(np.abs(Smx - S64) / S64) < 1e-12
+    assert np.abs(proj(Umx) - proj(U64)).max() < 1e-5
+    assert np.abs(proj(Qmx) - proj(Q64)).max() < 1e-5
+    assert np.allclose(Qmx.T @ Qmx, np.eye(k + p), atol=1e-5)
