@@ -46,9 +46,10 @@ typedef enum {
 typedef enum { MCNLA_F64 = 0, MCNLA_F32 = 1 } mcnla_dtype;
 typedef enum { MCNLA_COL_MAJOR = 0, MCNLA_ROW_MAJOR = 1 } mcnla_order;
 
-/* Arithmetic of the sketch GEMM (and of the collection it writes), and of the former's Z = A'Qbar
- * for an FP32 A: FP32 MFMA under the mixed precisions (FP32 sums within each split-K range, FP64
- * across them), FP64 arithmetic under MCNLA_PREC_F64. */
+/* Arithmetic of the sketch GEMM (and of the collection it writes), and of the products that read
+ * an FP32 A again (the former's Z = A'Qbar; Qbar = A X on the Gram-of-A route): FP32 MFMA under
+ * the mixed precisions (FP32 sums within each split-K range, FP64 across them), FP64 arithmetic
+ * under MCNLA_PREC_F64. */
 typedef enum {
   MCNLA_PREC_F64 = 0,        /* everything FP64 (FP64 MFMA) */
   MCNLA_PREC_MIXED_BF16 = 1, /* sketch: BF16 MFMA, FP32 accumulate, FP32 Y; Gram/eig/KN/SVD FP64 */
