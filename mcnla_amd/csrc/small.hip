@@ -193,24 +193,6 @@ __global__ __launch_bounds__(256) void to_rowmajor_f64_kernel(const TS* __restri
   }
 }
 
-// Row-major FP32 -> row-major FP64 with 4 | n, 4 | lda and a 16-byte aligned source: each
-// thread widens 4 consecutive elements of a row (one 16-byte load, two 16-byte stores).
-__global__ __launch_bounds__(256) void widen_rows_f32_kernel(const float* __restrict__ A,
-                                                             int64_t lda, int64_t m, int64_t n,
-                                                             double* __restrict__ out) {
-  typedef float f4v __attribute__((ext_vector_type(4)));
-  typedef double d2v __attribute__((ext_vector_type(2)));
-  const int64_t per_row = n / 4, total = m * per_row;
-  for (int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; g < total;
-       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t r = g / per_row, c = 4 * (g % per_row);
-    const f4v x = *reinterpret_cast<const f4v*>(A + r * lda + c);
-    d2v* o = reinterpret_cast<d2v*>(out + r * n + c);
-    o[0] = d2v{static_cast<double>(x[0]), static_cast<double>(x[1])};
-    o[1] = d2v{static_cast<double>(x[2]), static_cast<double>(x[3])};
-  }
-}
-
 // Q_i = Y_i T_i in place; one thread per (row, sketch).
 template <typename T>
 __global__ void apply_blockdiag_kernel(T* __restrict__ Y, int64_t m, int64_t N, int l,
@@ -340,14 +322,6 @@ void congruence_blockdiag(const double* G, const double* Tst, double* Bs, int64_
 void to_rowmajor_f64(const void* A, int64_t lda, bool a_f32, bool colmajor, int64_t m, int64_t n,
                      double* out, hipStream_t st) {
   if (m == 0 || n == 0) return;
-  if (a_f32 && !colmajor && n % 4 == 0 && lda % 4 == 0 &&
-      reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0) {
-    const int64_t blocks = std::min<int64_t>(ceil_div(m * (n / 4), 256), 65536);
-    hipLaunchKernelGGL(widen_rows_f32_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
-                       st, static_cast<const float*>(A), lda, m, n, out);
-    MCNLA_LAUNCHED();
-    return;
-  }
   MCNLA_REQUIRE(ceil_div(m, 32) < 65536, MCNLA_ERR_INVALID, "too many rows for the converter");
   const dim3 grid(static_cast<unsigned>(ceil_div(n, 32)), static_cast<unsigned>(ceil_div(m, 32)));
   if (a_f32) {
