@@ -20,8 +20,13 @@ Parity pins (each enforced by a test under ``tests/``):
   * symmetric former: ``data/a_sym.mtx + data/q.mtx -> data/u_sym.mtx`` projectors at 1e-8,
     sigma vs the eigenvalues of ``Qbar' A Qbar``;
   * sketch: closed form ``A @ Omega``;
-  * ``omega_ref``: moments and determinism only -- the bit stream is *parity unpinned* because
-    no reference fixture stores Omega or its seed.
+  * ``mt19937_raw``: the C++ standard's required 10000th draw of the default seed, and an
+    independent restatement of MT19937 (``tests/stdrandom.py``) plus numpy's own MT19937;
+  * ``seed_seq_generate``: a restatement of the standard's seed_seq::generate algorithm;
+  * ``omega_ref``: bit-exact against a restatement of the Marsaglia-Bray polar method over
+    two-draw ``generate_canonical<double, 53>`` (libstdc++'s ``normal_distribution``), of the
+    thread chunking and of both seed chains.  The composed chain is still *parity unpinned*
+    against the reference itself: no reference fixture stores Omega or its seed.
 """
 import ctypes
 import os
@@ -48,6 +53,9 @@ def lib():
         L.orc_inverse.restype = ctypes.c_int
         L.orc_omega_ref.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _i64, _d]
+        L.orc_mt19937_raw.argtypes = [ctypes.c_uint32, _i64, ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_seed_seq_generate.argtypes = [ctypes.POINTER(ctypes.c_uint32), _i64,
+                                            ctypes.POINTER(ctypes.c_uint32), _i64]
         L.orc_sketch.argtypes = [_i64, _i64, _i64, _d, _i64, ctypes.c_int, _d, _d]
         L.orc_column_indices.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, _i64, _i64,
@@ -115,6 +123,23 @@ def omega_ref(seed, n, Nl, col_block=True, nprocs=1, rank=0, threads=1):
     out = np.empty((n, Nl))
     lib().orc_omega_ref(seed & 0xFFFFFFFF, 1 if col_block else 0, nprocs, rank, threads, n * Nl,
                         _p(out))
+    return out
+
+
+def mt19937_raw(seed, count):
+    """`count` raw draws of std::mt19937 seeded with `seed` (uint32 array)."""
+    out = np.empty(int(count), np.uint32)
+    lib().orc_mt19937_raw(seed & 0xFFFFFFFF, int(count),
+                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return out
+
+
+def seed_seq_generate(words, count):
+    """std::seed_seq{words...}.generate() into `count` uint32 words."""
+    w = np.ascontiguousarray(np.asarray(words, dtype=np.uint32))
+    out = np.empty(int(count), np.uint32)
+    lib().orc_seed_seq_generate(w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), w.size,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(count))
     return out
 
 

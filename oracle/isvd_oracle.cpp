@@ -16,10 +16,18 @@
 //   orc_svd_former / orc_gramian_former    -> data/a.mtx + data/q.mtx -> data/u.mtx projectors
 //   orc_reduction / orc_extrinsic_mean     -> data/qi.mtx -> data/qb_re.mtx / qb_em.mtx projectors
 //   orc_symmetric_former                   -> data/a_sym.mtx + data/q.mtx -> data/u_sym.mtx
-//   orc_omega_ref                          -> moments + determinism only ("parity unpinned" for
-//                                             the bit stream: no reference fixture stores Omega)
-//   orc_column_indices                     -> range, uniformity and the 32-bit wrap only
-//                                             ("parity unpinned" for the bit stream, as Omega)
+//   orc_mt19937_raw                        -> the C++ standard's required value (10000th draw
+//                                             of the default seed is 4123659995) and an
+//                                             independent numpy restatement of MT19937
+//   orc_seed_seq_generate                  -> numpy restatement of [rand.util.seedseq]'s generate
+//   orc_omega_ref                          -> bit-exact vs a restatement of the Marsaglia-Bray
+//                                             polar method over two-draw generate_canonical
+//                                             <double,53> (libstdc++'s normal_distribution), the
+//                                             thread chunking and both seed chains; the composed
+//                                             chain itself is "parity unpinned" (no reference
+//                                             fixture stores Omega or its seed)
+//   orc_column_indices                     -> range, uniformity, the 32-bit wrap, and the same
+//                                             raw draws as orc_mt19937_raw on the same seed chain
 //   orc_sketch                             -> closed form (numpy A @ Omega)
 
 #include <cmath>
@@ -271,8 +279,11 @@ int orc_inverse(int64_t n, double* a) { return inverse(n, a); }
 //   (row_block_gaussian_projection_sketcher.hpp:87-89).
 // Each thread i runs std::mt19937 (32-bit index_t build) over chunk i of len/threads values, the
 // last chunk taking the remainder.
-void orc_omega_ref(uint32_t seed, int col_block, int nprocs, int rank, int threads, int64_t len,
-                   double* out) {
+// The per-thread seeds of one rank: the col-block chain runs seed_seq{seed} over nprocs outputs
+// and takes this rank's (streams.hpp:136-150), the row-block chain starts from seed itself; then
+// seed_seq{s0} makes one seed per OpenMP thread (streams.hpp:118-131).
+static std::vector<uint32_t> thread_seeds(uint32_t seed, int col_block, int nprocs, int rank,
+                                   int threads) {
   uint32_t s0 = seed;
   if (col_block) {
     std::seed_seq seq{seed};
@@ -283,14 +294,41 @@ void orc_omega_ref(uint32_t seed, int col_block, int nprocs, int rank, int threa
   std::seed_seq seq2{s0};
   std::vector<uint32_t> per_thread(threads);
   seq2.generate(per_thread.begin(), per_thread.end());
+  return per_thread;
+}
+
+// Thread t's chunk of a length-len vector: len/threads values from t*(len/threads), the last
+// thread taking the remainder (gaussian.hpp:45-55).
+static void thread_chunk(int64_t len, int threads, int t, int64_t* start, int64_t* count) {
+  int64_t chunk = len / threads;
+  *start = chunk * t;
+  if (t == threads - 1) chunk = len - *start;
+  *count = chunk;
+}
+
+void orc_omega_ref(uint32_t seed, int col_block, int nprocs, int rank, int threads, int64_t len,
+                   double* out) {
+  const std::vector<uint32_t> per_thread = thread_seeds(seed, col_block, nprocs, rank, threads);
   for (int t = 0; t < threads; ++t) {
-    int64_t chunk = len / threads;
-    const int64_t start = chunk * t;
-    if (t == threads - 1) chunk = len - start;
+    int64_t start = 0, chunk = 0;
+    thread_chunk(len, threads, t, &start, &chunk);
     std::mt19937 gen(per_thread[t]);
     std::normal_distribution<> dis(0.0, 1.0);
     for (int64_t i = 0; i < chunk; ++i) out[start + i] = dis(gen);
   }
+}
+
+// The standard-library primitives the chain is built of, exported so that each can be pinned on
+// its own: `count` raw draws of std::mt19937 seeded with `seed`, and std::seed_seq{in...}'s
+// generate() into `nout` words.
+void orc_mt19937_raw(uint32_t seed, int64_t count, uint32_t* out) {
+  std::mt19937 gen(seed);
+  for (int64_t i = 0; i < count; ++i) out[i] = static_cast<uint32_t>(gen());
+}
+
+void orc_seed_seq_generate(const uint32_t* in, int64_t nin, uint32_t* out, int64_t nout) {
+  std::seed_seq seq(in, in + nin);
+  seq.generate(out, out + nout);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -300,20 +338,10 @@ void orc_omega_ref(uint32_t seed, int col_block, int nprocs, int rank, int threa
 // default build (def.hpp:34-38: draws of 2^31 and above wrap to negative), then abs(v) % ncol.
 void orc_column_indices(uint32_t seed, int col_block, int nprocs, int rank, int threads,
                         int64_t len, int64_t ncol, int64_t* out) {
-  uint32_t s0 = seed;
-  if (col_block) {
-    std::seed_seq seq{seed};
-    std::vector<uint32_t> per_rank(nprocs);
-    seq.generate(per_rank.begin(), per_rank.end());
-    s0 = per_rank[rank];
-  }
-  std::seed_seq seq2{s0};
-  std::vector<uint32_t> per_thread(threads);
-  seq2.generate(per_thread.begin(), per_thread.end());
+  const std::vector<uint32_t> per_thread = thread_seeds(seed, col_block, nprocs, rank, threads);
   for (int t = 0; t < threads; ++t) {
-    int64_t chunk = len / threads;
-    const int64_t start = chunk * t;
-    if (t == threads - 1) chunk = len - start;
+    int64_t start = 0, chunk = 0;
+    thread_chunk(len, threads, t, &start, &chunk);
     std::mt19937 gen(per_thread[t]);
     for (int64_t i = 0; i < chunk; ++i) {
       const uint32_t raw = static_cast<uint32_t>(gen());

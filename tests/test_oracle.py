@@ -161,9 +161,8 @@ def test_sketch_closed_form():
 
 
 def test_omega_ref_properties():
-    # No reference fixture stores Omega: the stream itself is parity unpinned.  What is pinned:
-    # determinism, N(0,1) moments, and the chunked seeding (thread t's chunk changes with the
-    # thread count while the first chunk's seed chain is shared by every count).
+    # Determinism and N(0,1) moments; the bit stream's pieces (mt19937, seed_seq, the polar
+    # method, chunking, both seed chains) are pinned in test_omega_pins.py.
     a = oracle.omega_ref(123, 500, 40)
     b = oracle.omega_ref(123, 500, 40)
     assert np.array_equal(a, b)
