@@ -138,8 +138,18 @@ struct DevBuf {
 struct Timer {
   std::vector<hipEvent_t> ev;
   int used = 0;
+  bool own = true;  // false: `ev` is a borrowed, fixed set (SolveRes::marks)
   ~Timer() {
-    for (auto e : ev) (void)hipEventDestroy(e);
+    if (own)
+      for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  // Records into `n` events owned elsewhere from now on (at most n marks per run).
+  void borrow(const hipEvent_t* evs, int n) {
+    if (own)
+      for (auto e : ev) (void)hipEventDestroy(e);
+    ev.assign(evs, evs + n);
+    own = false;
+    used = 0;
   }
   void start(hipStream_t st) {
     used = 0;
@@ -147,6 +157,7 @@ struct Timer {
   }
   void mark(hipStream_t st) {
     if (used == static_cast<int>(ev.size())) {
+      MCNLA_REQUIRE(own, MCNLA_ERR_STATE, "more stage marks than borrowed events");
       hipEvent_t e;
       MCNLA_HIP(hipEventCreate(&e));
       ev.push_back(e);
@@ -207,9 +218,10 @@ struct SketchCore {
   DevBuf omega, ws;
   std::vector<double> host;
   hipEvent_t gemm_beg = nullptr, gemm_end = nullptr;
+  bool own_ev = true;  // false: the events belong to a SolveRes
   ~SketchCore() {
-    if (gemm_beg) (void)hipEventDestroy(gemm_beg);
-    if (gemm_end) (void)hipEventDestroy(gemm_end);
+    if (own_ev && gemm_beg) (void)hipEventDestroy(gemm_beg);
+    if (own_ev && gemm_end) (void)hipEventDestroy(gemm_end);
   }
   static SketchBf16Args bf16_args(const mcnla_params& p, int precision) {
     SketchBf16Args s;
@@ -294,9 +306,10 @@ struct SketchCore {
 struct GramCore {
   DevBuf partial;
   hipEvent_t beg = nullptr, end = nullptr;
+  bool own_ev = true;  // false: the events belong to a SolveRes
   ~GramCore() {
-    if (beg) (void)hipEventDestroy(beg);
-    if (end) (void)hipEventDestroy(end);
+    if (own_ev && beg) (void)hipEventDestroy(beg);
+    if (own_ev && end) (void)hipEventDestroy(end);
   }
   double gemm_seconds() {
     if (!beg) return 0.0;
@@ -719,17 +732,30 @@ struct mcnla_solver_s {
   int route = MCNLA_ROUTE_SKETCH;
   mcnla_tuning tune{};  // split-K workgroup targets of the FP64 GEMMs
   mcnla::Timer timer;
-  // second stream for the first block's eigen-decomposition, which runs beside the integrator
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // HIP objects of a solve (the side stream of the first block's eigen-decomposition, which runs
+  // beside the integrator, and the timing events): the solver's own set, created on its first
+  // solve, or a set lent by the caller of solver_create_in
+  mcnla::SolveRes own_res;
+  mcnla::SolveRes* res = &own_res;
   std::unique_ptr<mcnla::IterationGraphs> graphs;  // cfg.graph: recorded iteration chunks
   int64_t iteration = 0;
   bool computed = false;
   bool converged = false;
   ~mcnla_solver_s() {
-    if (ev_fork) (void)hipEventDestroy(ev_fork);
-    if (ev_join) (void)hipEventDestroy(ev_join);
-    if (side) (void)hipStreamDestroy(side);
+    // the cores' and timer's events are the SolveRes's; drop them before the set goes
+    timer.ev.clear();
+    own_res.destroy();
+  }
+  // Creates the solver's own set if it has none, and points the timer and cores at it.
+  void bind_res() {
+    if (!res->ready()) res->create();
+    timer.borrow(res->marks, mcnla::SolveRes::kMarks);
+    sketch.gemm_beg = res->sketch_beg;
+    sketch.gemm_end = res->sketch_end;
+    sketch.own_ev = false;
+    gram.beg = res->gram_beg;
+    gram.end = res->gram_end;
+    gram.own_ev = false;
   }
 };
 
@@ -890,14 +916,40 @@ size_t solver_required_bytes(const mcnla_params* params, const mcnla_solver_conf
 }
 
 mcnla_solver solver_create_in(const mcnla_params* params, const mcnla_solver_config* cfg,
-                              void* base, size_t bytes) {
+                              void* base, size_t bytes, SolveRes* res) {
   MCNLA_REQUIRE(base != nullptr && reinterpret_cast<uintptr_t>(base) % kCarveAlign == 0,
                 MCNLA_ERR_INVALID, "region base must be 256-byte aligned");
+  MCNLA_REQUIRE(res == nullptr || res->ready(), MCNLA_ERR_STATE, "lent SolveRes is not created");
   Carve c;
   c.base = static_cast<char*>(base);
   c.cap = bytes;
   CarveScope scope(&c);
-  return new_solver(params, cfg).release();
+  auto s = new_solver(params, cfg);
+  if (res) s->res = res;
+  return s.release();
+}
+
+void SolveRes::create() {
+  if (ready()) return;
+  try {
+    MCNLA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    MCNLA_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    MCNLA_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    for (auto& e : marks) MCNLA_HIP(hipEventCreate(&e));
+    for (hipEvent_t* e : {&sketch_beg, &sketch_end, &gram_beg, &gram_end})
+      MCNLA_HIP(hipEventCreate(e));
+  } catch (...) {
+    destroy();
+    throw;
+  }
+}
+
+void SolveRes::destroy() {
+  for (hipEvent_t* e : {&fork, &join, &sketch_beg, &sketch_end, &gram_beg, &gram_end})
+    if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+  for (auto& e : marks)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
+  if (side) (void)hipStreamDestroy(side), side = nullptr;
 }
 
 }  // namespace mcnla
@@ -1412,6 +1464,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     const int l = static_cast<int>(p.rank + p.over_rank);
     const int64_t N = p.num_sketch, nl = l * N, m = p.nrow;
     hipStream_t st = S(stream);
+    s->bind_res();
     s->timer.start(st);
     const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
     const int64_t n = p.ncol;
@@ -1461,17 +1514,13 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     MCNLA_HIP(hipMemsetAsync(s->bad.p, 0, sizeof(int), st));
     const bool rotate = !is_combining(c.integrator);
     if (rotate) {
-      if (!s->side) {
-        MCNLA_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
-        MCNLA_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
-        MCNLA_HIP(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
-      }
-      MCNLA_HIP(hipEventRecord(s->ev_fork, st));
-      MCNLA_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-      batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), 1, l, s->side);
+      hipStream_t side = s->res->side;
+      MCNLA_HIP(hipEventRecord(s->res->fork, st));
+      MCNLA_HIP(hipStreamWaitEvent(side, s->res->fork, 0));
+      batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), 1, l, side);
       orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T0.as<double>(), 1, l,
-                      s->bad.as<int>(), s->side);
-      MCNLA_HIP(hipEventRecord(s->ev_join, s->side));
+                      s->bad.as<int>(), side);
+      MCNLA_HIP(hipEventRecord(s->res->join, side));
       cholesky_transforms(s->W.as<double>(), N, l, s->T.as<double>(), s->L0.as<double>(),
                           s->bad.as<int>(), st);
     } else {
@@ -1489,7 +1538,7 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     s->converged = conv;
     const double* cf = s->coef.as<double>();
     if (rotate) {
-      MCNLA_HIP(hipStreamWaitEvent(st, s->ev_join, 0));
+      MCNLA_HIP(hipStreamWaitEvent(st, s->res->join, 0));
       GemmF64Args wr;  // W_0 = L_0' T_0^ref
       wr.M = l;
       wr.N = l;

@@ -178,11 +178,15 @@ struct mcnla_server_s {
     }
     return nullptr;
   }
-  void run(Req* q, hipStream_t st);
+  // client threads blocked in mcnla_server_wait / next_done; destroy waits until none is left
+  int waiters = 0;
+  std::condition_variable cv_left;
+
+  void run(Req* q, hipStream_t st, SolveRes* res);
   void worker();
 };
 
-void mcnla_server_s::run(Req* q, hipStream_t st) {
+void mcnla_server_s::run(Req* q, hipStream_t st, SolveRes* res) {
   char* ws = base + q->ws_off;
   char* rs = base + q->res_off;
   mcnla_solver s = nullptr;
@@ -201,7 +205,7 @@ void mcnla_server_s::run(Req* q, hipStream_t st) {
       A = dA;
       lda = len;
     }
-    s = solver_create_in(&p, &q->cfg, ws, q->solver_bytes);
+    s = solver_create_in(&p, &q->cfg, ws, q->solver_bytes, res);
     q->status = mcnla_solver_solve(s, A, lda, q->r.a_dtype, q->r.a_order, st);
     if (q->status != MCNLA_OK) q->error = mcnla_last_error();
     if (q->status == MCNLA_OK || q->status == MCNLA_ERR_NOT_CONVERGED) {
@@ -227,15 +231,30 @@ void mcnla_server_s::run(Req* q, hipStream_t st) {
     q->status = MCNLA_ERR_INVALID;
     q->error = e.what();
   }
-  if (s) mcnla_solver_destroy(s);
-  // Whatever happened, the stream is idle before the workspace region is handed back.
+  // Whatever happened, both streams are idle before the solver goes and the workspace region is
+  // handed back (a failed solve may have left work queued behind its error).
   (void)hipStreamSynchronize(st);
+  if (res->side) (void)hipStreamSynchronize(res->side);
+  if (s) mcnla_solver_destroy(s);
 }
 
+// A worker owns its stream and the HIP objects its solves record into (SolveRes) for its whole
+// life, so that admitting, running and finishing a request creates and destroys no HIP stream or
+// event.
 void mcnla_server_s::worker() {
   (void)hipSetDevice(device);
   hipStream_t st = nullptr;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+  SolveRes res;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+    st = nullptr;
+  } else {
+    try {
+      res.create();
+    } catch (const Error&) {
+      (void)hipStreamDestroy(st);
+      st = nullptr;
+    }
+  }
   for (;;) {
     Req* q = nullptr;
     {
@@ -247,7 +266,7 @@ void mcnla_server_s::worker() {
       ++running;
       max_running = std::max(max_running, running);
     }
-    run(q, st);
+    run(q, st, &res);
     {
       std::lock_guard<std::mutex> lk(mu);
       q->t_done = Clock::now();
@@ -267,6 +286,7 @@ void mcnla_server_s::worker() {
     cv_done.notify_all();
     cv_work.notify_all();
   }
+  res.destroy();
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -350,6 +370,13 @@ mcnla_status mcnla_server_destroy(mcnla_server s) {
     }
     s->cv_work.notify_all();
     for (auto& t : s->workers) t.join();
+    {
+      // client threads still waiting on a request fail with MCNLA_ERR_STATE; none may touch the
+      // server after this block
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_done.notify_all();
+      s->cv_left.wait(lk, [&] { return s->waiters == 0; });
+    }
     (void)hipFree(s->base);
     delete s;
     return MCNLA_OK;
@@ -396,15 +423,19 @@ mcnla_status mcnla_server_wait(mcnla_server s, uint64_t id, double timeout_s, in
   return serve_guard([&] {
     MCNLA_REQUIRE(s && done, MCNLA_ERR_INVALID, "null argument");
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    MCNLA_REQUIRE(it != s->reqs.end(), MCNLA_ERR_INVALID, "unknown request id");
-    Req* q = it->second.get();
-    auto pred = [&] { return q->state == State::kDone; };
+    MCNLA_REQUIRE(s->reqs.count(id), MCNLA_ERR_INVALID, "unknown request id");
+    // The request is looked up again on every wake-up: another thread may release it meanwhile.
+    auto gone = [&] { return !s->reqs.count(id); };
+    auto pred = [&] { return s->stop || gone() || s->reqs.at(id)->state == State::kDone; };
+    ++s->waiters;
     if (timeout_s < 0)
       s->cv_done.wait(lk, pred);
     else
       s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
-    *done = pred() ? 1 : 0;
+    if (--s->waiters == 0) s->cv_left.notify_all();
+    MCNLA_REQUIRE(!gone(), MCNLA_ERR_INVALID, "request was released while waiting");
+    *done = s->reqs.at(id)->state == State::kDone ? 1 : 0;
+    MCNLA_REQUIRE(*done || !s->stop, MCNLA_ERR_STATE, "server is shutting down");
     return MCNLA_OK;
   });
 }
@@ -413,19 +444,24 @@ mcnla_status mcnla_server_next_done(mcnla_server s, double timeout_s, uint64_t* 
   return serve_guard([&] {
     MCNLA_REQUIRE(s && id, MCNLA_ERR_INVALID, "null argument");
     std::unique_lock<std::mutex> lk(s->mu);
-    auto pred = [&] {
+    auto ready = [&] {
       while (!s->done_q.empty() && !s->reqs.count(s->done_q.front())) s->done_q.pop_front();
       return !s->done_q.empty();
     };
+    auto pred = [&] { return ready() || s->stop; };
+    ++s->waiters;
     if (timeout_s < 0)
       s->cv_done.wait(lk, pred);
     else
       s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+    if (--s->waiters == 0) s->cv_left.notify_all();
     *id = 0;
-    if (pred()) {
+    if (ready()) {
       *id = s->done_q.front();
       s->done_q.pop_front();
+      return MCNLA_OK;
     }
+    MCNLA_REQUIRE(!s->stop, MCNLA_ERR_STATE, "server is shutting down");
     return MCNLA_OK;
   });
 }
