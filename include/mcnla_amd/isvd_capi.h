@@ -267,6 +267,25 @@ mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* s
 mcnla_status mcnla_solver_kernel_times(mcnla_solver s, double times[2]);
 
 /* ---------------------------------------------------------------------------------------------
+ * Solve without allocating.  No counterpart in the reference (its stages allocate their
+ * workspaces in initialize(), stage_wrapper.hpp:37-46); this is for callers that cannot allocate
+ * device memory: code that must not call hipMalloc/hipFree (tight serving loops, harnesses that
+ * forbid it).  Runs the fused solve of mcnla_solver_create + mcnla_solver_solve with every
+ * workspace carved from a static device array of MCNLA_STATIC_WORKSPACE_BYTES per GPU (held from
+ * the library's first use on that GPU until the process ends), and copies S (k), U (m x k), V
+ * (n x k, with jobv) and Qbar (m x (k+p)) into the caller's device buffers (a NULL output is
+ * skipped); layouts as mcnla_solver_results.  Synchronous on `stream`.  A problem whose
+ * workspace (mcnla_solver_required_bytes) exceeds the array fails with MCNLA_ERR_NOMEM;
+ * cfg->graph must be 0.  Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU race, sequential calls do not (each solve initializes what it reads).  Results are those of
+ * mcnla_solver_solve with the same configuration, bit for bit. */
+#define MCNLA_STATIC_WORKSPACE_BYTES ((size_t)256 << 20)
+mcnla_status mcnla_solve_static(const mcnla_params* params, const mcnla_solver_config* cfg,
+                                const void* A, int64_t lda, int a_dtype, int a_order, double* S,
+                                double* U, double* V, double* Qbar, int64_t* iterations,
+                                void* stream);
+
+/* ---------------------------------------------------------------------------------------------
  * Autotuner.  No reference counterpart (the reference has no device plans to tune).  The FP64
  * GEMMs of a solve split K so that their grid holds about `target` workgroups; the defaults are
  * the values measured best at the 100000 x 10000 (Nl = 2048) and 500000 x 2000 (Nl = 8192)

@@ -27,6 +27,9 @@ ORTH_GRAMIAN, ORTH_ROW_BLOCK_GRAMIAN = 0, 1
 (FORM_SVD, FORM_GRAMIAN, FORM_COL_BLOCK_GRAMIAN, FORM_ROW_BLOCK_GRAMIAN,
  FORM_ROW_BLOCK_SYMMETRIC) = range(5)
 
+# MCNLA_STATIC_WORKSPACE_BYTES: the static device array mcnla_solve_static carves its workspace from
+STATIC_WORKSPACE_BYTES = 256 << 20
+
 STATUS_NAMES = {
     OK: "OK", ERR_INVALID: "INVALID", ERR_NOMEM: "NOMEM", ERR_HIP: "HIP",
     ERR_NOT_CONVERGED: "NOT_CONVERGED", ERR_NUMERIC: "NUMERIC", ERR_STATE: "STATE",
@@ -161,6 +164,8 @@ SIGNATURES = {
     "mcnla_tune_load": (_st, [ctypes.c_char_p]),
     "mcnla_solver_required_bytes": (_st, [ctypes.POINTER(Params), ctypes.POINTER(SolverConfig),
                                           ctypes.POINTER(ctypes.c_size_t)]),
+    "mcnla_solve_static": (_st, [ctypes.POINTER(Params), ctypes.POINTER(SolverConfig), _vp, _i64,
+                                 _i, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int64), _vp]),
     "mcnla_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
     "mcnla_server_create": (_st, [ctypes.POINTER(ServerConfig), _pp]),
     "mcnla_server_destroy": (_st, [_vp]),
