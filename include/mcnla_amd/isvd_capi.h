@@ -53,7 +53,14 @@ typedef enum { MCNLA_COL_MAJOR = 0, MCNLA_ROW_MAJOR = 1 } mcnla_order;
 typedef enum {
   MCNLA_PREC_F64 = 0,        /* everything FP64 (FP64 MFMA) */
   MCNLA_PREC_MIXED_BF16 = 1, /* sketch: BF16 MFMA, FP32 accumulate, FP32 Y; Gram/eig/KN/SVD FP64 */
-  MCNLA_PREC_MIXED_BF16X3 = 2/* as MIXED_BF16 with a 3-product hi/lo BF16 split of A and Omega */
+  MCNLA_PREC_MIXED_BF16X3 = 2,/* as MIXED_BF16 with a 3-product hi/lo BF16 split of A and Omega */
+  MCNLA_PREC_MIXED_F16 = 3,  /* sketch: FP16 MFMA with each row of A scaled by a power of two
+                                (undone exactly in the output), FP32 accumulate */
+  MCNLA_PREC_MIXED_F16X2 = 4,/* as MIXED_F16 with A split hi + lo (2 products, ~22 bits of A);
+                                Omega enters rounded to FP16, so pair it with
+                                MCNLA_OMEGA_PHILOX_F16, whose Omega is exactly FP16 */
+  MCNLA_PREC_MIXED_F16X3 = 5 /* as MIXED_F16X2 plus the product A_hi Omega_lo (3 products; any
+                                Omega) */
 } mcnla_precision;
 
 /* isvd::Parameters (include/mcnla/isvd/core/parameters.hh:31-119): m, n, k, p, N. */
@@ -109,7 +116,12 @@ typedef enum {
  * chunks; include/mcnla/core/random/streams/streams.hpp:118-150, routine/gaussian.hpp:39-56),
  * generated on the host and uploaded, as the reference's GPU sketcher does
  * (col_block_gaussian_projection_sketcher_gpu.hpp:110). */
-typedef enum { MCNLA_OMEGA_PHILOX = 0, MCNLA_OMEGA_REFERENCE = 1 } mcnla_omega_mode;
+typedef enum {
+  MCNLA_OMEGA_PHILOX = 0,
+  MCNLA_OMEGA_REFERENCE = 1,
+  MCNLA_OMEGA_PHILOX_F16 = 2  /* the PHILOX stream with each value rounded to the nearest FP16
+                                 number: an N(0,1) sketch that the FP16 sketch reads exactly */
+} mcnla_omega_mode;
 
 mcnla_status mcnla_sketcher_create(const mcnla_params* params, int kind, int precision,
                                    mcnla_stage* out);
@@ -408,6 +420,15 @@ mcnla_status mcnla_server_reset_stats(mcnla_server s);
 mcnla_status mcnla_synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
                                  const double* sigma, double noise, uint64_t seed, void* A,
                                  int64_t lda, int a_dtype, int a_order, void* stream);
+/* A = (I - 2uu') [diag(sigma); 0] (I - 2vv') (m x n, n <= m) with unit device vectors u (m) and v
+ * (n) and device sigma (n): a dense matrix whose singular values are exactly sigma, written in
+ * O(mn) (the serving stream's per-request matrices).  Orthogonal factors do not change what the
+ * method computes in distribution (Omega is rotation-invariant and the method commutes with an
+ * orthogonal transform from the left), so these behave like Haar factors.  No reference
+ * counterpart. */
+mcnla_status mcnla_synth_reflected(int64_t m, int64_t n, const double* sigma, const double* u,
+                                   const double* v, void* A, int64_t lda, int a_dtype,
+                                   int a_order, void* stream);
 /* out[e] = scale * g_e with g the Philox N(0,1) stream of `seed` (device buffer of `count`
  * doubles): random factors for the synthetic matrices.  No reference counterpart (the reference
  * demo draws its factors on the host, demo/demo.cpp:250-293). */

@@ -21,12 +21,17 @@ from .isvd import (
     tune_load, tune_save, tune_set, tuning_default,
 )
 from .serve import Served, Server, required_bytes
+from . import workloads
 
 PREC_F64 = _lib.PREC_F64
 PREC_MIXED_BF16 = _lib.PREC_MIXED_BF16
 PREC_MIXED_BF16X3 = _lib.PREC_MIXED_BF16X3
+PREC_MIXED_F16 = _lib.PREC_MIXED_F16
+PREC_MIXED_F16X2 = _lib.PREC_MIXED_F16X2
+PREC_MIXED_F16X3 = _lib.PREC_MIXED_F16X3
 OMEGA_PHILOX = _lib.OMEGA_PHILOX
 OMEGA_REFERENCE = _lib.OMEGA_REFERENCE
+OMEGA_PHILOX_F16 = _lib.OMEGA_PHILOX_F16
 ROUTE_AUTO = _lib.ROUTE_AUTO
 ROUTE_SKETCH = _lib.ROUTE_SKETCH
 ROUTE_GRAM_OF_A = _lib.ROUTE_GRAM_OF_A

@@ -15,11 +15,12 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_NOT_CONVERGED, ERR_NUMERIC, ERR_STATE, 
 F64, F32 = 0, 1
 COL_MAJOR, ROW_MAJOR = 0, 1
 # mcnla_precision
-PREC_F64, PREC_MIXED_BF16, PREC_MIXED_BF16X3 = 0, 1, 2
+(PREC_F64, PREC_MIXED_BF16, PREC_MIXED_BF16X3, PREC_MIXED_F16, PREC_MIXED_F16X2,
+ PREC_MIXED_F16X3) = range(6)
 # kinds
 (SKETCHER_COL_BLOCK_GAUSSIAN, SKETCHER_ROW_BLOCK_GAUSSIAN, SKETCHER_GAUSSIAN,
  SKETCHER_COLUMN_SAMPLING, SKETCHER_ROW_BLOCK_COLUMN_SAMPLING) = range(5)
-OMEGA_PHILOX, OMEGA_REFERENCE = 0, 1
+OMEGA_PHILOX, OMEGA_REFERENCE, OMEGA_PHILOX_F16 = 0, 1, 2
 ROUTE_AUTO, ROUTE_SKETCH, ROUTE_GRAM_OF_A = 0, 1, 2
 ORTH_GRAMIAN, ORTH_ROW_BLOCK_GRAMIAN = 0, 1
 (INT_KN, INT_ROW_BLOCK_KN, INT_ROW_BLOCK_GRAMIAN_KN, INT_ROW_BLOCK_WY, INT_ROW_BLOCK_GRAMIAN_WY,
@@ -179,6 +180,7 @@ SIGNATURES = {
     "mcnla_server_get_stats": (_st, [_vp, ctypes.POINTER(ServerStats)]),
     "mcnla_server_reset_stats": (_st, [_vp]),
     "mcnla_fill_gaussian": (_st, [_vp, _i64, _u64, _d, _vp]),
+    "mcnla_synth_reflected": (_st, [_i64, _i64, _vp, _vp, _vp, _vp, _i64, _i, _i, _vp]),
     "mcnla_synth_lowrank": (_st, [_i64, _i64, _i64, _vp, _vp, _vp, _d, _u64, _vp, _i64, _i, _i,
                                   _vp]),
 }

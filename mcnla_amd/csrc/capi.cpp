@@ -190,7 +190,25 @@ void check_params(const mcnla_params* p) {
 }
 
 bool is_mixed(int precision) {
-  return precision == MCNLA_PREC_MIXED_BF16 || precision == MCNLA_PREC_MIXED_BF16X3;
+  return precision >= MCNLA_PREC_MIXED_BF16 && precision <= MCNLA_PREC_MIXED_F16X3;
+}
+
+bool valid_omega_mode(int mode) {
+  return mode == MCNLA_OMEGA_PHILOX || mode == MCNLA_OMEGA_REFERENCE ||
+         mode == MCNLA_OMEGA_PHILOX_F16;
+}
+
+// 16-bit format (0 BF16, 1 FP16) and product count of a mixed precision's sketch.
+int sketch_fmt(int precision) {
+  return precision >= MCNLA_PREC_MIXED_F16 ? 1 : 0;
+}
+int sketch_terms(int precision) {
+  switch (precision) {
+    case MCNLA_PREC_MIXED_BF16X3:
+    case MCNLA_PREC_MIXED_F16X3: return 3;
+    case MCNLA_PREC_MIXED_F16X2: return 2;
+    default: return 1;
+  }
 }
 
 void check_precision(int precision) {
@@ -228,7 +246,8 @@ struct SketchCore {
     s.m = p.nrow;
     s.n = p.ncol;
     s.nl = (p.rank + p.over_rank) * p.num_sketch;
-    s.terms = (precision == MCNLA_PREC_MIXED_BF16X3) ? 3 : 1;
+    s.terms = sketch_terms(precision);
+    s.fmt = sketch_fmt(precision);
     return s;
   }
   // Allocates Omega and, for a mixed precision, the BF16 panels of A and Omega.
@@ -247,7 +266,7 @@ struct SketchCore {
       MCNLA_HIP(hipMemcpyAsync(omega.p, host.data(), count * sizeof(double), hipMemcpyHostToDevice, st));
       MCNLA_HIP(hipStreamSynchronize(st));  // `host` is reused by the next call
     } else {
-      omega_philox(omega.as<double>(), count, seed, st);
+      omega_philox(omega.as<double>(), count, seed, st, mode == MCNLA_OMEGA_PHILOX_F16);
     }
   }
   // y_f64: a mixed-precision sketch writes its FP32-accumulated Y as FP64 (the fused solver's
@@ -770,7 +789,8 @@ int resolve_route(const mcnla_params& p, const mcnla_solver_config& c) {
   if (c.route == MCNLA_ROUTE_SKETCH || c.route == MCNLA_ROUTE_GRAM_OF_A) return c.route;
   const double m = static_cast<double>(p.nrow), n = static_cast<double>(p.ncol);
   const double nl = static_cast<double>((p.rank + p.over_rank) * p.num_sketch);
-  const double sketch = 2.0 * m * n * nl / (c.precision == MCNLA_PREC_F64 ? 1.0 : 16.0);
+  const double sketch = 2.0 * m * n * nl *
+                        (c.precision == MCNLA_PREC_F64 ? 1.0 : sketch_terms(c.precision) / 16.0);
   const double via_y = m * nl * nl + sketch;
   const double via_a = m * n * n + 2.0 * n * n * nl + n * nl * nl;
   return via_a < via_y ? MCNLA_ROUTE_GRAM_OF_A : MCNLA_ROUTE_SKETCH;
@@ -865,8 +885,7 @@ std::unique_ptr<mcnla_solver_s> new_solver(const mcnla_params* params,
                 MCNLA_ERR_INVALID, "unknown former kind");
   MCNLA_REQUIRE(cfg->former != MCNLA_FORM_ROW_BLOCK_SYMMETRIC || params->nrow == params->ncol,
                 MCNLA_ERR_INVALID, "the symmetric former needs a square A");
-  MCNLA_REQUIRE(cfg->omega_mode == MCNLA_OMEGA_PHILOX || cfg->omega_mode == MCNLA_OMEGA_REFERENCE,
-                MCNLA_ERR_INVALID, "unknown omega mode");
+  MCNLA_REQUIRE(valid_omega_mode(cfg->omega_mode), MCNLA_ERR_INVALID, "unknown omega mode");
   MCNLA_REQUIRE(cfg->maxiter >= 0 && cfg->tol >= 0.0, MCNLA_ERR_INVALID, "bad maxiter/tol");
   auto s = std::make_unique<mcnla_solver_s>();
   s->prm = *params;
@@ -1099,8 +1118,7 @@ mcnla_status mcnla_sketcher_create(const mcnla_params* params, int kind, int pre
 mcnla_status mcnla_sketcher_set_seed(mcnla_stage s, uint64_t seed, int omega_mode, int threads) {
   return guard([&] {
     MCNLA_REQUIRE(s && s->type == mcnla_stage_s::kSketcher, MCNLA_ERR_INVALID, "not a sketcher");
-    MCNLA_REQUIRE(omega_mode == MCNLA_OMEGA_PHILOX || omega_mode == MCNLA_OMEGA_REFERENCE,
-                  MCNLA_ERR_INVALID, "unknown omega mode");
+    MCNLA_REQUIRE(valid_omega_mode(omega_mode), MCNLA_ERR_INVALID, "unknown omega mode");
     auto* k = static_cast<Sketcher*>(s);
     k->seed = seed;
     k->mode = omega_mode;
@@ -1921,6 +1939,22 @@ mcnla_status mcnla_synth_lowrank(int64_t m, int64_t n, int64_t r, const double* 
     synth_finish(m, n, Tm.as<double>(), noise, seed, A, lda, a_dtype == MCNLA_F32,
                  a_order == MCNLA_COL_MAJOR, st);
     MCNLA_HIP(hipStreamSynchronize(st));  // the temporaries are freed on return
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_synth_reflected(int64_t m, int64_t n, const double* sigma, const double* u,
+                                   const double* v, void* A, int64_t lda, int a_dtype,
+                                   int a_order, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(m >= 1 && n >= 1 && n <= m && A && sigma && u && v, MCNLA_ERR_INVALID,
+                  "bad synth arguments (need 1 <= n <= m and non-null vectors)");
+    MCNLA_REQUIRE(a_dtype == MCNLA_F64 || a_dtype == MCNLA_F32, MCNLA_ERR_INVALID, "A dtype");
+    MCNLA_REQUIRE(a_order == MCNLA_COL_MAJOR || a_order == MCNLA_ROW_MAJOR, MCNLA_ERR_INVALID,
+                  "A order");
+    MCNLA_REQUIRE(lda >= (a_order == MCNLA_COL_MAJOR ? m : n), MCNLA_ERR_INVALID, "lda too small");
+    synth_reflected(m, n, sigma, u, v, A, lda, a_dtype == MCNLA_F32, a_order == MCNLA_COL_MAJOR,
+                    S(stream));
     return MCNLA_OK;
   });
 }

@@ -1,13 +1,25 @@
-// Mixed-precision sketch Y = A * Omega on BF16 MFMA with FP32 accumulation (gfx950).
+// Mixed-precision sketch Y = A * Omega on 16-bit MFMA with FP32 accumulation (gfx950).
 //
-// One term (MIXED_BF16): A (m x n, FP32/FP64, either order) and Omega (n x Nl, FP64) are rounded
-// to BF16 and packed into swizzled 256 x 32 blocks; the GEMM streams the blocks into LDS with
-// direct-to-LDS loads (256 x 256 tiles, 4 waves of 128 x 128).
-// Three terms (MIXED_BF16X3): each operand is split x = hi + lo (both BF16, lo = bf16(x - hi))
-// into K-contiguous zero-padded panels Ap[Mp][Kp], Wp[Np][Kp], and the GEMM (128 x 128 tiles,
-// 4 waves of 64 x 64) accumulates hi*hi + lo*hi + hi*lo, which carries ~16 mantissa bits of each
-// operand.  Both use v_mfma_f32_16x16x32_bf16 with each lane reading 8 consecutive k of a row
-// from LDS (one 16-byte read per fragment).
+// A (m x n, FP32/FP64, either order) and Omega (n x Nl, FP64) are rounded to BF16 or FP16 and
+// packed into swizzled 256 x 32 blocks; one GEMM streams the blocks into LDS with direct-to-LDS
+// loads (256 x 256 tiles, 4 waves of 128 x 128, v_mfma_f32_16x16x32_bf16 / _f16).
+//
+// The more accurate forms add products along K rather than adding kernels: an operand x is split
+// into x = hi + lo (lo = round(x - hi)), and the packed K range is a sequence of segments whose
+// products sum in the same FP32 accumulators:
+//   1 term:  [A] [Omega]
+//   2 terms: [A_hi, A_lo] [Omega; Omega]            (A carries about twice the mantissa bits;
+//                                                     exact in Omega when Omega is representable
+//                                                     in the 16-bit format)
+//   3 terms: [A_hi, A_lo, A_hi] [Omega_hi; Omega_hi; Omega_lo]
+// FP16 has a 5-bit exponent, so every row i of A is scaled by 2^-e_i before rounding, with e_i
+// chosen from the row's largest magnitude so that it lands in [2^13, 2^14); the scale is a power
+// of two (exact), and the epilogue multiplies row i of Y by 2^e_i.  A row scale commutes with the
+// sum over K, which a per-column scale of A would not.  BF16 shares FP32's exponent range and is
+// not scaled.
+#include <algorithm>
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -15,8 +27,9 @@ namespace mcnla {
 
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kFmtBf16 = 0, kFmtF16 = 1;
 
 // acc += a * b on the matrix cores with the accumulator tied to the destination.  Through the
 // builtin, the register allocator may give each MFMA a fresh destination (the untied form's
@@ -24,191 +37,81 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // registers that is more AGPR moves than MFMAs, and every move waits for the MFMA it reads.
 // Inline asm is outside the compiler's hazard tracking, so a caller keeps consecutive MFMAs on
 // one accumulator far apart and pads with s_nop before the first MFMA and before VALU reads of
-// the results.
-__device__ __forceinline__ void mfma_bf16_tied(f4& acc, const bf16x8& a, const bf16x8& b) {
-  asm volatile("v_mfma_f32_16x16x32_bf16 %0, %1, %2, %0" : "+a"(acc) : "v"(a), "v"(b));
+// the results.  a and b hold 8 packed 16-bit values each.
+template <int FMT>
+__device__ __forceinline__ void mfma_tied(f4& acc, const u32x4& a, const u32x4& b) {
+  if constexpr (FMT == kFmtBf16)
+    asm volatile("v_mfma_f32_16x16x32_bf16 %0, %1, %2, %0" : "+a"(acc) : "v"(a), "v"(b));
+  else
+    asm volatile("v_mfma_f32_16x16x32_f16 %0, %1, %2, %0" : "+a"(acc) : "v"(a), "v"(b));
 }
 
-constexpr int kBM = 128, kBN = 128;
-constexpr int kThreads = 256;
-constexpr int kTile = 64;  // pack transpose tile
-
-__host__ __device__ constexpr int bk_for(int terms) { return terms == 1 ? 64 : 32; }
-
-__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7F800000u) == 0x7F800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return static_cast<uint16_t>(u >> 16);
+// Round to nearest even into the 16-bit format, and back.
+template <int FMT>
+__device__ __forceinline__ uint16_t to16(float x) {
+  if constexpr (FMT == kFmtBf16)
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(x));
+  else
+    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x));
 }
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
-  return __uint_as_float(static_cast<uint32_t>(h) << 16);
-}
-
-// Packs a matrix whose element (r, k) lives at src[r + k*ld] (R_CONTIG) or src[r*ld + k] into
-// dst_hi/dst_lo [rows_pad][kpad] (k contiguous), zero outside (rows, cols).
-template <typename TS, bool R_CONTIG>
-__global__ __launch_bounds__(kThreads) void pack_kernel(const TS* __restrict__ src, int64_t ld,
-                                                        int64_t rows, int64_t cols,
-                                                        int64_t kpad, uint16_t* __restrict__ hi,
-                                                        uint16_t* __restrict__ lo) {
-  __shared__ float tile[kTile][kTile + 1];
-  const int64_t r0 = blockIdx.y * (int64_t)kTile, k0 = blockIdx.x * (int64_t)kTile;
-  for (int e = threadIdx.x; e < kTile * kTile; e += kThreads) {
-    int rr, kk;
-    if (R_CONTIG) {
-      kk = e / kTile;
-      rr = e % kTile;
-    } else {
-      rr = e / kTile;
-      kk = e % kTile;
-    }
-    const int64_t gr = r0 + rr, gk = k0 + kk;
-    float v = 0.0f;
-    if (gr < rows && gk < cols) v = static_cast<float>(R_CONTIG ? src[gr + gk * ld] : src[gr * ld + gk]);
-    tile[rr][kk] = v;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kTile * kTile; e += kThreads) {
-    const int rr = e / kTile, kk = e % kTile;
-    const int64_t o = (r0 + rr) * kpad + k0 + kk;
-    const float v = tile[rr][kk];
-    const uint16_t h = f32_to_bf16_rne(v);
-    hi[o] = h;
-    if (lo) lo[o] = f32_to_bf16_rne(v - bf16_to_f32(h));
-  }
+template <int FMT>
+__device__ __forceinline__ float from16(uint16_t h) {
+  if constexpr (FMT == kFmtBf16)
+    return static_cast<float>(__builtin_bit_cast(__bf16, h));
+  else
+    return static_cast<float>(__builtin_bit_cast(_Float16, h));
 }
 
-template <int NOPS, int LOADS>
-__device__ __forceinline__ void tile_gload(u32x4 (&ra)[NOPS][LOADS], u32x4 (&rb)[NOPS][LOADS],
-                                           const uint16_t* __restrict__ a_hi,
-                                           const uint16_t* __restrict__ a_lo,
-                                           const uint16_t* __restrict__ b_hi,
-                                           const uint16_t* __restrict__ b_lo,
-                                           const int64_t (&goff)[LOADS], int64_t k0) {
-#pragma unroll
-  for (int it = 0; it < LOADS; ++it) {
-    ra[0][it] = *reinterpret_cast<const u32x4*>(a_hi + goff[it] + k0);
-    rb[0][it] = *reinterpret_cast<const u32x4*>(b_hi + goff[it] + k0);
-    if (NOPS == 2) {
-      ra[NOPS - 1][it] = *reinterpret_cast<const u32x4*>(a_lo + goff[it] + k0);
-      rb[NOPS - 1][it] = *reinterpret_cast<const u32x4*>(b_lo + goff[it] + k0);
-    }
-  }
+// The exponent e of a row whose largest magnitude has IEEE bits `maxbits`: |a| 2^-e <= 2^14 for
+// every entry, the largest in [2^13, 2^14).  Zero rows, and rows holding Inf, keep e = 0 (their
+// Inf and NaN values reach Y unchanged).
+__device__ __forceinline__ int row_exponent(uint32_t maxbits) {
+  const float mx = __uint_as_float(maxbits);
+  if (!(mx > 0.f) || !(mx <= 3.4028235e38f)) return 0;
+  int e2 = 0;
+  (void)frexpf(mx, &e2);  // mx = f 2^e2, f in [0.5, 1)
+  return e2 - 14;
 }
 
-template <int NOPS, int LOADS, int PANEL>
-__device__ __forceinline__ void tile_lstore_impl(uint16_t (*sa)[PANEL], uint16_t (*sb)[PANEL],
-                                                 const u32x4 (&ra)[NOPS][LOADS],
-                                                 const u32x4 (&rb)[NOPS][LOADS],
-                                                 const int (&loff)[LOADS]) {
-#pragma unroll
-  for (int o = 0; o < NOPS; ++o)
-#pragma unroll
-    for (int it = 0; it < LOADS; ++it) {
-      *reinterpret_cast<u32x4*>(&sa[o][loff[it]]) = ra[o][it];
-      *reinterpret_cast<u32x4*>(&sb[o][loff[it]]) = rb[o][it];
-    }
+// rowmax[i] = bits of max_k |A(i, k)| (FP32).  Column-major A: thread per row, K split over
+// grid.y, combined with atomicMax on the bit patterns (order-independent; the bits of a
+// non-negative float order like the values).  rowmax is zeroed first.
+template <typename TS>
+__global__ __launch_bounds__(256) void rowmax_colmajor_kernel(const TS* __restrict__ A,
+                                                              int64_t lda, int64_t m, int64_t n,
+                                                              int64_t kchunk,
+                                                              uint32_t* __restrict__ rowmax) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= m) return;
+  const int64_t k0 = blockIdx.y * kchunk;
+  const int64_t k1 = k0 + kchunk < n ? k0 + kchunk : n;
+  float mx = 0.f;
+#pragma unroll 8
+  for (int64_t k = k0; k < k1; ++k) mx = fmaxf(mx, fabsf(static_cast<float>(A[i + k * lda])));
+  atomicMax(&rowmax[i], __float_as_uint(mx));
 }
 
-
-template <int TERMS, typename TY>
-__global__ __launch_bounds__(kThreads) void sketch_bf16_kernel(
-    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Alo,
-    const uint16_t* __restrict__ Whi, const uint16_t* __restrict__ Wlo, int64_t kpad, int64_t m,
-    int64_t nl, TY* __restrict__ Y, int64_t ldy) {
-  constexpr int BK = bk_for(TERMS);
-  constexpr int NOPS = TERMS == 1 ? 1 : 2;   // hi (+ lo) panels per operand
-  constexpr int LDK = BK + 8;                // bf16 per LDS row (16-byte pad)
-  constexpr int CH = BK / 8;                 // 16-byte chunks per row
-  constexpr int LOADS = kBM * CH / kThreads; // 16-byte loads per thread per panel
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][NOPS][kBM * LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][NOPS][kBN * LDK];
-
-  const int64_t tiles_n = ceil_div(nl, kBN);
-  const int64_t tiles_m = ceil_div(m, kBM);
-  const int64_t nblocks = tiles_m * tiles_n;
-  const int64_t logical = xcd_swizzle(blockIdx.x, nblocks);
-  const int64_t tm = logical / tiles_n, tn = logical % tiles_n;
-  const int64_t m0 = tm * kBM, n0 = tn * kBN;
-  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-  const int wm0 = (wave / 2) * 64, wn0 = (wave % 2) * 64;
-
-  // Per-thread global offsets of its 16-byte chunks (row r, chunk c) inside a K tile.
-  const uint16_t* __restrict__ a_hi = Ahi + m0 * kpad;
-  const uint16_t* __restrict__ b_hi = Whi + n0 * kpad;
-  const uint16_t* __restrict__ a_lo = (NOPS == 2) ? Alo + m0 * kpad : a_hi;
-  const uint16_t* __restrict__ b_lo = (NOPS == 2) ? Wlo + n0 * kpad : b_hi;
-  int64_t goff[LOADS];
-  int loff[LOADS];
-#pragma unroll
-  for (int it = 0; it < LOADS; ++it) {
-    const int idx = it * kThreads + threadIdx.x;
-    const int r = idx / CH, c = idx % CH;
-    goff[it] = r * kpad + c * 8;
-    loff[it] = r * LDK + c * 8;
-  }
-
-  u32x4 ra[NOPS][LOADS], rb[NOPS][LOADS];
-  tile_gload<NOPS, LOADS>(ra, rb, a_hi, a_lo, b_hi, b_lo, goff, 0);
-  tile_lstore_impl<NOPS, LOADS>(sA[0], sB[0], ra, rb, loff);
-
-  f4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-
-  const int64_t ntk = kpad / BK;
-  __syncthreads();
-  for (int64_t kt = 0; kt < ntk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < ntk;
-    if (more) tile_gload<NOPS, LOADS>(ra, rb, a_hi, a_lo, b_hi, b_lo, goff, (kt + 1) * BK);
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 32) {
-      const int kof = kk + 8 * (lane / 16);
-      bf16x8 af[NOPS][4], bfr[NOPS][4];
-#pragma unroll
-      for (int o = 0; o < NOPS; ++o)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          af[o][i] = *reinterpret_cast<const bf16x8*>(&sA[cur][o][(wm0 + i * 16 + lane % 16) * LDK + kof]);
-          bfr[o][i] = *reinterpret_cast<const bf16x8*>(&sB[cur][o][(wn0 + i * 16 + lane % 16) * LDK + kof]);
-        }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[0][j], acc[i][j], 0, 0, 0);
-          if (TERMS == 3) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[NOPS - 1][i], bfr[0][j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[NOPS - 1][j], acc[i][j], 0, 0, 0);
-          }
-        }
-    }
-    if (more) tile_lstore_impl<NOPS, LOADS>(sA[cur ^ 1], sB[cur ^ 1], ra, rb, loff);
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t gi = m0 + wm0 + i * 16 + 4 * (lane / 16) + r;
-        const int64_t gj = n0 + wn0 + j * 16 + lane % 16;
-        if (gi < m && gj < nl) Y[gi * ldy + gj] = static_cast<TY>(acc[i][j][r]);
-      }
+// Row-major A: one wave per row (4 rows per workgroup), written directly.
+template <typename TS>
+__global__ __launch_bounds__(256) void rowmax_rowmajor_kernel(const TS* __restrict__ A,
+                                                              int64_t lda, int64_t m, int64_t n,
+                                                              uint32_t* __restrict__ rowmax) {
+  const int64_t i = blockIdx.x * 4ll + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (i >= m) return;
+  const TS* row = A + i * lda;
+  float mx = 0.f;
+  for (int64_t k = lane; k < n; k += kWave) mx = fmaxf(mx, fabsf(static_cast<float>(row[k])));
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+  if (lane == 0) rowmax[i] = __float_as_uint(mx);
 }
 
-// ------------------------------------------------------------------------------------------------
-// One-term path.  Both operands are packed into 256-row x 32-k BF16 blocks of 16 KiB, laid out
-// [row tile][k block][256][32] so that every pipeline stage of the GEMM is one contiguous block
-// per operand.  Inside a block the 16-byte chunk c = k/8 of row r is stored at chunk position
-// c ^ ((r / 4) % 4): a linear copy into LDS then gives fragment reads (16 rows x one chunk per
-// 16 lanes) that hit 16 distinct 4-bank groups.  The k blocks are padded with zero blocks to a
-// multiple of kUnroll.
+// Operands are packed into 256-row x 32-k blocks of 16 KiB, laid out
+// [row tile][segment][k block][256][32] so that every pipeline stage of the GEMM is one contiguous
+// block per operand.  Inside a block the 16-byte chunk c = k/8 of row r is stored at chunk
+// position c ^ ((r / 4) % 4): a linear copy into LDS then gives fragment reads (16 rows x one
+// chunk per 16 lanes) that hit 16 distinct 4-bank groups.  Each segment's k blocks are padded
+// with zero blocks to a multiple of kUnroll.
 constexpr int kBR = 256;       // rows per block
 constexpr int kBKB = 32;       // k per block
 constexpr int kBlk = kBR * kBKB;  // elements per block
@@ -221,11 +124,15 @@ __host__ __device__ constexpr int blk_off(int r, int k) {
 
 int64_t blk_nkt(int64_t n) { return ceil_div(ceil_div(n, kBKB), kUnroll) * kUnroll; }
 
-// src element (r, k) at src[r + k*ld] (R_CONTIG) or src[r*ld + k]; one workgroup per block.
+// Packs src element (r, k) (at src[r + k*ld] when R_CONTIG, else src[r*ld + k]) into segment
+// `seg_hi` of `nseg` (its rounded value) and, when seg_lo >= 0, into segment seg_lo (the rounded
+// remainder).  rowmax (FP16): row r is scaled by 2^-e_r first.  One workgroup per block.
 // VEC4: FP32 R_CONTIG source with ld % 4 == 0 and a 16-byte aligned base (float4 loads).
-template <typename TS, bool R_CONTIG, bool VEC4>
+template <typename TS, bool R_CONTIG, bool VEC4, int FMT>
 __global__ __launch_bounds__(256) void pack_blocks_kernel(const TS* __restrict__ src, int64_t ld,
                                                           int64_t rows, int64_t cols, int64_t nkt,
+                                                          int nseg, int seg_hi, int seg_lo,
+                                                          const uint32_t* __restrict__ rowmax,
                                                           uint16_t* __restrict__ dst) {
   constexpr int LDT = kBR + 4;  // floats per LDS row (rows stay 16-byte aligned)
   __shared__ __attribute__((aligned(16))) float tile[kBKB][LDT];
@@ -270,16 +177,29 @@ __global__ __launch_bounds__(256) void pack_blocks_kernel(const TS* __restrict__
   }
   __syncthreads();
   // every store instruction writes 1 KiB contiguous: chunk g = q*256 + t is (row g/4, slot g%4)
-  uint16_t* blk = dst + (tr * nkt + kt) * kBlk;
+  uint16_t* hi = dst + ((tr * nseg + seg_hi) * nkt + kt) * kBlk;
+  uint16_t* lo = dst + ((tr * nseg + (seg_lo < 0 ? 0 : seg_lo)) * nkt + kt) * kBlk;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int g = q * 256 + t;
     const int r = g >> 2;
     const int c = (g & 3) ^ ((r >> 2) & 3);
-    bf16x8 h;
+    const int e = (rowmax && r0 + r < rows) ? row_exponent(rowmax[r0 + r]) : 0;
+    uint16_t h[8], l[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) h[i] = static_cast<__bf16>(tile[8 * c + i][r]);  // RNE
-    *reinterpret_cast<u32x4*>(blk + g * 8) = __builtin_bit_cast(u32x4, h);
+    for (int i = 0; i < 8; ++i) {
+      const float v = e ? ldexpf(tile[8 * c + i][r], -e) : tile[8 * c + i][r];
+      h[i] = to16<FMT>(v);
+      l[i] = to16<FMT>(v - from16<FMT>(h[i]));  // exact difference in FP32
+    }
+    u32x4 ph, pl;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ph[i] = static_cast<uint32_t>(h[2 * i]) | (static_cast<uint32_t>(h[2 * i + 1]) << 16);
+      pl[i] = static_cast<uint32_t>(l[2 * i]) | (static_cast<uint32_t>(l[2 * i + 1]) << 16);
+    }
+    *reinterpret_cast<u32x4*>(hi + g * 8) = ph;
+    if (seg_lo >= 0) *reinterpret_cast<u32x4*>(lo + g * 8) = pl;
   }
 }
 
@@ -298,16 +218,22 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((VM & 0xF) | (0x7 << 4) | (0xF << 8) | (((VM >> 4) & 3) << 14));
 }
 
-// Y (m x nl) = A * Omega from blocks Ab[tm][kt] and Wb[tn][kt]: 256 x 256 tiles, 4 waves of
-// 128 x 128 (v_mfma_f32_16x16x32_bf16, 256 accumulators per lane).  Blocks stream from HBM into
-// LDS with global_load_lds_dwordx4 through a four-stage ring (three stages in flight).  The ring
-// buffers are distinct LDS arrays with static indices and the loop body has no branches (the
-// stage three ahead is issued every step, clamped to the last block; nkt is a multiple of 4), so
-// the compiler keeps per-buffer wait tracking and the only waits are the explicit ones.
-template <typename TY>
+// Y (m x nl) = sum over segments s of A_seg(s) * B_seg(s), from blocks
+// Ab[tm][a segment][kt] and Wb[tn][b segment][kt]: step kt of the K loop (0 <= kt < nseg * nkt)
+// reads k block kt - s*nkt of A segment amap[s] and B segment bmap[s], s = kt / nkt (2 bits per
+// segment in amap/bmap).  256 x 256 tiles, 4 waves of 128 x 128 (v_mfma_f32_16x16x32_bf16/_f16,
+// 256 accumulators per lane).  Blocks stream from HBM into LDS with global_load_lds_dwordx4
+// through a five-slot ring.  The ring buffers are distinct LDS arrays with static indices and the
+// loop body has no branches (the stage four ahead is issued every step, clamped to the last
+// block; the K step count is a multiple of 10), so the compiler keeps per-buffer wait tracking
+// and the only waits are the explicit ones.  rowmax (FP16): row i of Y is multiplied by 2^e_i.
+template <typename TY, int FMT>
 __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restrict__ Ab,
                                                         const uint16_t* __restrict__ Wb,
-                                                        int64_t nkt, int64_t m, int64_t nl,
+                                                        int64_t nkt, int nseg, int nseg_a,
+                                                        int nseg_b, uint32_t amap, uint32_t bmap,
+                                                        const uint32_t* __restrict__ rowmax,
+                                                        int64_t m, int64_t nl,
                                                         TY* __restrict__ Y, int64_t ldy) {
   constexpr int L = 8;  // DMA instructions per wave and stage (4 for A, 4 for B)
   static_assert(kStages == 5 && kUnroll == 10, "the loop body below is written for 5 slots");
@@ -319,16 +245,20 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const int wm0 = (wave / 2) * 128, wn0 = (wave % 2) * 128;
   const int kg = lane / 16;
-  const uint16_t* abase = Ab + tm * nkt * kBlk;
-  const uint16_t* bbase = Wb + tn * nkt * kBlk;
+  const int64_t ktot = nseg * nkt;
+  const uint16_t* abase = Ab + tm * nseg_a * nkt * kBlk;
+  const uint16_t* bbase = Wb + tn * nseg_b * nkt * kBlk;
   auto issue = [&](int64_t kt, uint16_t* sd) {
+    // segment of step kt without a division: at most three segments
+    const int s = static_cast<int>(kt >= nkt) + static_cast<int>(kt >= 2 * nkt);
+    const int64_t kk = kt - s * nkt;
+    const uint16_t* ak = abase + (((amap >> (2 * s)) & 3) * nkt + kk) * kBlk;
+    const uint16_t* bk = bbase + (((bmap >> (2 * s)) & 3) * nkt + kk) * kBlk;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int off = (q * 4 + wave) * 512;  // 1 KiB per wave instruction
-      __builtin_amdgcn_global_load_lds(abase + kt * kBlk + off + lane * 8, (lds_void*)&sd[off], 16,
-                                       0, 0);
-      __builtin_amdgcn_global_load_lds(bbase + kt * kBlk + off + lane * 8,
-                                       (lds_void*)&sd[kBlk + off], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(ak + off + lane * 8, (lds_void*)&sd[off], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(bk + off + lane * 8, (lds_void*)&sd[kBlk + off], 16, 0, 0);
     }
   };
   f4 acc[8][8];
@@ -338,32 +268,32 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
     for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   // Fragments are double-buffered: step kt reads stage kt + 1's fragments from LDS while its
   // MFMAs consume stage kt's, which were read one step earlier.
-  bf16x8 fa0[8], fb0[8], fa1[8], fb1[8];
-  auto frags = [&](const uint16_t* ss, bf16x8* af, bf16x8* bfr) {
+  u32x4 fa0[8], fb0[8], fa1[8], fb1[8];
+  auto frags = [&](const uint16_t* ss, u32x4* af, u32x4* bfr) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      af[i] = *reinterpret_cast<const bf16x8*>(&ss[blk_off(wm0 + i * 16 + lane % 16, 8 * kg)]);
+      af[i] = *reinterpret_cast<const u32x4*>(&ss[blk_off(wm0 + i * 16 + lane % 16, 8 * kg)]);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-      bfr[j] = *reinterpret_cast<const bf16x8*>(&ss[kBlk + blk_off(wn0 + j * 16 + lane % 16, 8 * kg)]);
+      bfr[j] = *reinterpret_cast<const u32x4*>(&ss[kBlk + blk_off(wn0 + j * 16 + lane % 16, 8 * kg)]);
   };
   // Step kt: stage kt + 1 must have landed (stages up to kt + 3 are issued; two may still be in
   // flight); the barrier makes that true in every wave.  The stage issued here, kt + 4 (clamped
   // to the last block), overwrites stage kt - 1's slot, whose fragments every wave consumed in
   // step kt - 1, before this barrier.  The last step's look-ahead reads a slot that holds the
   // last block again and discards it.
-  auto step = [&](int64_t kt, const bf16x8* caf, const bf16x8* cbf, const uint16_t* snext,
-                  bf16x8* naf, bf16x8* nbf, uint16_t* sn) {
+  auto step = [&](int64_t kt, const u32x4* caf, const u32x4* cbf, const uint16_t* snext,
+                  u32x4* naf, u32x4* nbf, uint16_t* sn) {
     asm volatile("" ::: "memory");
     wait_vmcnt_lgkm0<2 * L>();  // and this step's fragments (read last step) are in registers
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    issue(kt + 4 < nkt ? kt + 4 : nkt - 1, sn);
+    issue(kt + 4 < ktot ? kt + 4 : ktot - 1, sn);
     frags(snext, naf, nbf);
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) mfma_bf16_tied(acc[i][j], caf[i], cbf[j]);
+      for (int j = 0; j < 8; ++j) mfma_tied<FMT>(acc[i][j], caf[i], cbf[j]);
   };
   issue(0, S0);
   issue(1, S1);
@@ -375,7 +305,7 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
   asm volatile("" ::: "memory");
   frags(S0, fa0, fb0);
   asm volatile("s_nop 7");  // zeroed accumulators (VALU writes) before their first MFMA read
-  for (int64_t kt = 0; kt < nkt; kt += kUnroll) {  // stage s sits in slot s % 5
+  for (int64_t kt = 0; kt < ktot; kt += kUnroll) {  // stage s sits in slot s % 5
     step(kt, fa0, fb0, S1, fa1, fb1, S4);
     step(kt + 1, fa1, fb1, S2, fa0, fb0, S0);
     step(kt + 2, fa0, fb0, S3, fa1, fb1, S1);
@@ -393,13 +323,34 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int r = 0; r < 4; ++r) {
+      const int64_t gi = m0 + wm0 + i * 16 + 4 * (lane / 16) + r;
+      if (gi >= m) continue;
+      const int e = rowmax ? row_exponent(rowmax[gi]) : 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t gi = m0 + wm0 + i * 16 + 4 * (lane / 16) + r;
+      for (int j = 0; j < 8; ++j) {
         const int64_t gj = n0 + wn0 + j * 16 + lane % 16;
-        if (gi < m && gj < nl) Y[gi * ldy + gj] = static_cast<TY>(acc[i][j][r]);
+        if (gj < nl) {
+          if constexpr (sizeof(TY) == 8)
+            Y[gi * ldy + gj] = ldexp(static_cast<double>(acc[i][j][r]), e);
+          else
+            Y[gi * ldy + gj] = ldexpf(acc[i][j][r], e);
+        }
       }
+    }
+}
+
+// Segment layout of a term count: A segments, B segments, and the (A, B) segment of each step.
+struct Terms {
+  int nseg, nseg_a, nseg_b;
+  uint32_t amap, bmap;
+};
+Terms terms_of(int terms) {
+  switch (terms) {
+    case 2: return Terms{2, 2, 1, 0u | (1u << 2), 0u};                    // hi*W + lo*W
+    case 3: return Terms{3, 2, 2, 0u | (1u << 2), 1u << 4};               // + hi*W_lo
+    default: return Terms{1, 1, 1, 0u, 0u};
+  }
 }
 
 struct BlkDims {
@@ -409,112 +360,126 @@ BlkDims blk_dims(const SketchBf16Args& a) {
   return BlkDims{ceil_div(a.m, kBR), ceil_div(a.nl, kBR), blk_nkt(a.n)};
 }
 
-template <typename TS>
-void launch_pack_blocks(const TS* src, int64_t ld, bool r_contig, int64_t rows, int64_t cols,
-                        int64_t tiles, int64_t nkt, uint16_t* dst, hipStream_t st) {
+template <typename TS, int FMT>
+void launch_pack_fmt(const TS* src, int64_t ld, bool r_contig, int64_t rows, int64_t cols,
+                     int64_t tiles, int64_t nkt, int nseg, int seg_hi, int seg_lo,
+                     const uint32_t* rowmax, uint16_t* dst, hipStream_t st) {
   const dim3 grid(static_cast<unsigned>(nkt), static_cast<unsigned>(tiles));
   const bool vec4 = r_contig && ld % 4 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0;
   if constexpr (sizeof(TS) == 4) {
     if (vec4) {
-      hipLaunchKernelGGL((pack_blocks_kernel<TS, true, true>), grid, dim3(256), 0, st, src, ld,
-                         rows, cols, nkt, dst);
+      hipLaunchKernelGGL((pack_blocks_kernel<TS, true, true, FMT>), grid, dim3(256), 0, st, src,
+                         ld, rows, cols, nkt, nseg, seg_hi, seg_lo, rowmax, dst);
       MCNLA_LAUNCHED();
       return;
     }
   }
   if (r_contig)
-    hipLaunchKernelGGL((pack_blocks_kernel<TS, true, false>), grid, dim3(256), 0, st, src, ld, rows,
-                       cols, nkt, dst);
+    hipLaunchKernelGGL((pack_blocks_kernel<TS, true, false, FMT>), grid, dim3(256), 0, st, src, ld,
+                       rows, cols, nkt, nseg, seg_hi, seg_lo, rowmax, dst);
   else
-    hipLaunchKernelGGL((pack_blocks_kernel<TS, false, false>), grid, dim3(256), 0, st, src, ld,
-                       rows, cols, nkt, dst);
+    hipLaunchKernelGGL((pack_blocks_kernel<TS, false, false, FMT>), grid, dim3(256), 0, st, src,
+                       ld, rows, cols, nkt, nseg, seg_hi, seg_lo, rowmax, dst);
   MCNLA_LAUNCHED();
-}
-
-struct PackDims {
-  int64_t mp, np, kp;
-};
-PackDims pack_dims(const SketchBf16Args& a) {
-  const int bk = 64;  // pad K to the larger K tile so both term counts fit
-  return PackDims{ceil_div(a.m, kBM) * kBM, ceil_div(a.nl, kBN) * kBN, ceil_div(a.n, bk) * bk};
 }
 
 template <typename TS>
-void launch_pack(const TS* src, int64_t ld, bool r_contig, int64_t rows, int64_t cols,
-                 int64_t rows_pad, int64_t kpad, uint16_t* hi, uint16_t* lo, hipStream_t st) {
-  dim3 grid(static_cast<unsigned>(ceil_div(kpad, kTile)), static_cast<unsigned>(ceil_div(rows_pad, kTile)));
-  if (r_contig)
-    hipLaunchKernelGGL((pack_kernel<TS, true>), grid, dim3(kThreads), 0, st, src, ld, rows, cols,
-                       kpad, hi, lo);
+void launch_pack(int fmt, const TS* src, int64_t ld, bool r_contig, int64_t rows, int64_t cols,
+                 int64_t tiles, int64_t nkt, int nseg, int seg_hi, int seg_lo,
+                 const uint32_t* rowmax, uint16_t* dst, hipStream_t st) {
+  if (fmt == kFmtF16)
+    launch_pack_fmt<TS, kFmtF16>(src, ld, r_contig, rows, cols, tiles, nkt, nseg, seg_hi, seg_lo,
+                                 rowmax, dst, st);
   else
-    hipLaunchKernelGGL((pack_kernel<TS, false>), grid, dim3(kThreads), 0, st, src, ld, rows, cols,
-                       kpad, hi, lo);
+    launch_pack_fmt<TS, kFmtBf16>(src, ld, r_contig, rows, cols, tiles, nkt, nseg, seg_hi, seg_lo,
+                                  rowmax, dst, st);
+}
+
+template <typename TS>
+void launch_rowmax(const TS* A, int64_t lda, bool colmajor, int64_t m, int64_t n,
+                   uint32_t* rowmax, hipStream_t st) {
+  if (colmajor) {
+    MCNLA_HIP(hipMemsetAsync(rowmax, 0, static_cast<size_t>(m) * sizeof(uint32_t), st));
+    // K in chunks of at least 256, so that a CU holds enough rows in flight
+    const int64_t rows_blocks = ceil_div(m, 256);
+    const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, rows_blocks),
+                                                                  ceil_div(n, 256)));
+    const int64_t kchunk = ceil_div(n, chunks);
+    const dim3 grid(static_cast<unsigned>(rows_blocks), static_cast<unsigned>(ceil_div(n, kchunk)));
+    hipLaunchKernelGGL(rowmax_colmajor_kernel<TS>, grid, dim3(256), 0, st, A, lda, m, n, kchunk,
+                       rowmax);
+  } else {
+    hipLaunchKernelGGL(rowmax_rowmajor_kernel<TS>, dim3(static_cast<unsigned>(ceil_div(m, 4))),
+                       dim3(256), 0, st, A, lda, m, n, rowmax);
+  }
   MCNLA_LAUNCHED();
+}
+
+struct Workspace {
+  uint16_t* Ab;
+  uint16_t* Wb;
+  uint32_t* rowmax;
+  size_t bytes;
+};
+Workspace carve(const SketchBf16Args& a, void* base) {
+  const BlkDims d = blk_dims(a);
+  const Terms t = terms_of(a.terms);
+  const size_t ab = static_cast<size_t>(t.nseg_a * d.tm * d.nkt) * kBlk * sizeof(uint16_t);
+  const size_t wb = static_cast<size_t>(t.nseg_b * d.tn * d.nkt) * kBlk * sizeof(uint16_t);
+  const size_t rb = a.fmt == kFmtF16 ? (static_cast<size_t>(a.m) * 4 + 255) / 256 * 256 : 0;
+  char* p = static_cast<char*>(base);
+  return Workspace{reinterpret_cast<uint16_t*>(p), reinterpret_cast<uint16_t*>(p + ab),
+                   rb ? reinterpret_cast<uint32_t*>(p + ab + wb) : nullptr, ab + wb + rb};
 }
 
 }  // namespace
 
-size_t sketch_bf16_workspace(const SketchBf16Args& a) {
-  if (a.terms == 1) {
-    const BlkDims d = blk_dims(a);
-    return static_cast<size_t>(d.tm + d.tn) * static_cast<size_t>(d.nkt) * kBlk * sizeof(uint16_t);
-  }
-  const PackDims d = pack_dims(a);
-  // Mp and Np are multiples of 128 and Kp of 64, so each panel is a multiple of 16 KiB.
-  return 2 * static_cast<size_t>(d.kp) * static_cast<size_t>(d.mp + d.np) * sizeof(uint16_t);
-}
+size_t sketch_bf16_workspace(const SketchBf16Args& a) { return carve(a, nullptr).bytes; }
 
 void sketch_bf16(const SketchBf16Args& a, void* workspace, size_t workspace_bytes, hipStream_t st,
                  hipEvent_t gemm_begin, hipEvent_t gemm_end) {
-  MCNLA_REQUIRE(a.terms == 1 || a.terms == 3, MCNLA_ERR_INVALID, "terms must be 1 or 3");
+  MCNLA_REQUIRE(a.terms >= 1 && a.terms <= 3, MCNLA_ERR_INVALID, "terms must be 1, 2 or 3");
+  MCNLA_REQUIRE(a.fmt == kFmtBf16 || a.fmt == kFmtF16, MCNLA_ERR_INVALID, "unknown 16-bit format");
   MCNLA_REQUIRE(workspace_bytes >= sketch_bf16_workspace(a), MCNLA_ERR_INVALID,
-                "bf16 sketch workspace too small");
+                "16-bit sketch workspace too small");
   if (a.m == 0 || a.nl == 0) return;
-  if (a.terms == 1) {
-    const BlkDims d = blk_dims(a);
-    uint16_t* Ab = static_cast<uint16_t*>(workspace);
-    uint16_t* Wb = Ab + d.tm * d.nkt * kBlk;
-    // A: element (i, k) at A[i + k*lda] (col-major) or A[i*lda + k] (row-major).
-    if (a.a_f32)
-      launch_pack_blocks(static_cast<const float*>(a.A), a.lda, a.a_colmajor, a.m, a.n, d.tm, d.nkt,
-                         Ab, st);
-    else
-      launch_pack_blocks(static_cast<const double*>(a.A), a.lda, a.a_colmajor, a.m, a.n, d.tm, d.nkt,
-                         Ab, st);
-    // Omega^T: element (j, k) = Omega[k*nl + j], i.e. row index j contiguous.
-    launch_pack_blocks(a.omega, a.nl, true, a.nl, a.n, d.tn, d.nkt, Wb, st);
-    const dim3 grid(static_cast<unsigned>(d.tm * d.tn));
-    if (gemm_begin) MCNLA_HIP(hipEventRecord(gemm_begin, st));
-    if (a.Yd)
-      hipLaunchKernelGGL((sketch_blk_kernel<double>), grid, dim3(256), 0, st, Ab, Wb, d.nkt, a.m,
-                         a.nl, a.Yd, a.ldy);
-    else
-      hipLaunchKernelGGL((sketch_blk_kernel<float>), grid, dim3(256), 0, st, Ab, Wb, d.nkt, a.m,
-                         a.nl, a.Y, a.ldy);
-    MCNLA_LAUNCHED();
-    if (gemm_end) MCNLA_HIP(hipEventRecord(gemm_end, st));
-    return;
+  const BlkDims d = blk_dims(a);
+  const Terms t = terms_of(a.terms);
+  const Workspace w = carve(a, workspace);
+  // A: element (i, k) at A[i + k*lda] (col-major) or A[i*lda + k] (row-major); hi in segment 0,
+  // lo in segment 1 for two or three terms.
+  const int a_lo = t.nseg_a == 2 ? 1 : -1;
+  if (a.a_f32) {
+    const float* A = static_cast<const float*>(a.A);
+    if (w.rowmax) launch_rowmax(A, a.lda, a.a_colmajor, a.m, a.n, w.rowmax, st);
+    launch_pack(a.fmt, A, a.lda, a.a_colmajor, a.m, a.n, d.tm, d.nkt, t.nseg_a, 0, a_lo, w.rowmax,
+                w.Ab, st);
+  } else {
+    const double* A = static_cast<const double*>(a.A);
+    if (w.rowmax) launch_rowmax(A, a.lda, a.a_colmajor, a.m, a.n, w.rowmax, st);
+    launch_pack(a.fmt, A, a.lda, a.a_colmajor, a.m, a.n, d.tm, d.nkt, t.nseg_a, 0, a_lo, w.rowmax,
+                w.Ab, st);
   }
-  const PackDims d = pack_dims(a);
-  uint16_t* base = static_cast<uint16_t*>(workspace);
-  uint16_t* Ahi = base;
-  uint16_t* Whi = Ahi + d.mp * d.kp;
-  uint16_t* Alo = Whi + d.np * d.kp;
-  uint16_t* Wlo = Alo + d.mp * d.kp;
-  if (a.a_f32)
-    launch_pack(static_cast<const float*>(a.A), a.lda, a.a_colmajor, a.m, a.n, d.mp, d.kp, Ahi, Alo, st);
-  else
-    launch_pack(static_cast<const double*>(a.A), a.lda, a.a_colmajor, a.m, a.n, d.mp, d.kp, Ahi, Alo, st);
-  launch_pack(a.omega, a.nl, true, a.nl, a.n, d.np, d.kp, Whi, Wlo, st);
-  const int64_t nblocks = ceil_div(a.m, kBM) * ceil_div(a.nl, kBN);
+  // Omega^T: element (j, k) = Omega[k*nl + j], i.e. row index j contiguous; N(0,1) values need
+  // no scaling in either format.
+  launch_pack(a.fmt, a.omega, a.nl, true, a.nl, a.n, d.tn, d.nkt, t.nseg_b, 0,
+              t.nseg_b == 2 ? 1 : -1, static_cast<const uint32_t*>(nullptr), w.Wb, st);
+  const dim3 grid(static_cast<unsigned>(d.tm * d.tn));
   if (gemm_begin) MCNLA_HIP(hipEventRecord(gemm_begin, st));
-  const dim3 grid(static_cast<unsigned>(nblocks));
-  if (a.Yd)
-    hipLaunchKernelGGL((sketch_bf16_kernel<3, double>), grid, dim3(kThreads), 0, st, Ahi, Alo, Whi,
-                       Wlo, d.kp, a.m, a.nl, a.Yd, a.ldy);
-  else
-    hipLaunchKernelGGL((sketch_bf16_kernel<3, float>), grid, dim3(kThreads), 0, st, Ahi, Alo, Whi,
-                       Wlo, d.kp, a.m, a.nl, a.Y, a.ldy);
+  auto launch = [&](auto* Y, auto fmt_tag) {
+    using TY = std::remove_pointer_t<decltype(Y)>;
+    constexpr int FMT = decltype(fmt_tag)::value;
+    hipLaunchKernelGGL((sketch_blk_kernel<TY, FMT>), grid, dim3(256), 0, st, w.Ab, w.Wb, d.nkt,
+                       t.nseg, t.nseg_a, t.nseg_b, t.amap, t.bmap,
+                       static_cast<const uint32_t*>(w.rowmax), a.m, a.nl, Y, a.ldy);
+  };
+  using Bf = std::integral_constant<int, kFmtBf16>;
+  using F16 = std::integral_constant<int, kFmtF16>;
+  if (a.Yd) {
+    if (a.fmt == kFmtF16) launch(a.Yd, F16{}); else launch(a.Yd, Bf{});
+  } else {
+    if (a.fmt == kFmtF16) launch(a.Y, F16{}); else launch(a.Y, Bf{});
+  }
   MCNLA_LAUNCHED();
   if (gemm_end) MCNLA_HIP(hipEventRecord(gemm_end, st));
 }

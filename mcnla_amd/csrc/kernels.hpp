@@ -50,7 +50,8 @@ int64_t gemm_f64_splits_for(int64_t tiles, int64_t K, int64_t target_blocks);
 
 // ------------------------------------------------------------------------------------------------
 // Mixed-precision sketch (gemm_bf16.hip): Y (FP32, row-major m x nl) = A * Omega with A and
-// Omega rounded to BF16 (one term) or split hi+lo (three products), FP32 accumulation.
+// Omega rounded to BF16 or FP16 (FP16: rows of A scaled by powers of two), FP32 accumulation;
+// terms 1: A Omega; 2: A_hi Omega + A_lo Omega; 3: A_hi Omega_hi + A_lo Omega_hi + A_hi Omega_lo.
 struct SketchBf16Args {
   int64_t m = 0, n = 0, nl = 0;
   const void* A = nullptr;  // FP32 or FP64 source
@@ -61,7 +62,8 @@ struct SketchBf16Args {
   float* Y = nullptr;
   double* Yd = nullptr;  // when set, Y is written in FP64 here instead (values are the same)
   int64_t ldy = 0;
-  int terms = 1;  // 1 or 3
+  int terms = 1;  // 1, 2 or 3
+  int fmt = 0;    // 0: BF16, 1: FP16
 };
 // Packs A and Omega, then runs the GEMM; the optional events bracket the GEMM launch alone.
 void sketch_bf16(const SketchBf16Args& a, void* workspace, size_t workspace_bytes, hipStream_t st,
@@ -70,7 +72,9 @@ size_t sketch_bf16_workspace(const SketchBf16Args& a);
 
 // ------------------------------------------------------------------------------------------------
 // Random numbers and synthetic data (rng.hip).
-void omega_philox(double* out, int64_t count, uint64_t seed, hipStream_t st);
+// f16_grid: each value rounded to the nearest FP16 number (kept in FP64).
+void omega_philox(double* out, int64_t count, uint64_t seed, hipStream_t st,
+                  bool f16_grid = false);
 // A = Tm + noise * g (Tm row-major m x n FP64), in A's dtype and storage order.
 void synth_finish(int64_t m, int64_t n, const double* Tm, double noise, uint64_t seed, void* A,
                   int64_t lda, bool a_f32, bool a_colmajor, hipStream_t st);
@@ -82,6 +86,10 @@ void gather_columns(const void* A, int64_t lda, bool a_f32, bool a_colmajor, int
 void synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const double* V,
                    const double* sigma, double noise, uint64_t seed, void* A, int64_t lda,
                    bool a_f32, bool a_colmajor, hipStream_t st);
+// A = (I - 2uu') [diag(sigma); 0] (I - 2vv') (m x n, n <= m; u, v unit device vectors of m and
+// n doubles): singular values exactly sigma, written in A's dtype and order.
+void synth_reflected(int64_t m, int64_t n, const double* sigma, const double* u, const double* v,
+                     void* A, int64_t lda, bool a_f32, bool a_colmajor, hipStream_t st);
 
 // ------------------------------------------------------------------------------------------------
 // Small dense kernels (small.hip).  All FP64, l <= kMaxL.

@@ -6,6 +6,8 @@
 // u2 in [0,1), and z = sqrt(-2 ln u1) * (cos, sin)(2 pi u2) gives elements 2p and 2p+1.  The
 // result is independent of the launch shape, so every precision and every shape of a solve sees
 // the same Omega.
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -46,12 +48,23 @@ __device__ __forceinline__ void gaussian_pair(uint64_t seed, uint64_t pair, doub
   z1 = rad * s;
 }
 
+// The nearest FP16 value of z (through FP32; stored as FP64): an N(0,1) sample that the FP16
+// sketch packs without rounding.
+__device__ __forceinline__ double to_f16_grid(double z) {
+  return static_cast<double>(static_cast<float>(static_cast<_Float16>(static_cast<float>(z))));
+}
+
+template <bool F16_GRID>
 __global__ void omega_philox_kernel(double* __restrict__ out, int64_t count, uint64_t seed) {
   const int64_t npairs = (count + 1) / 2;
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npairs;
        p += (int64_t)gridDim.x * blockDim.x) {
     double z0, z1;
     gaussian_pair(seed, static_cast<uint64_t>(p), z0, z1);
+    if (F16_GRID) {
+      z0 = to_f16_grid(z0);
+      z1 = to_f16_grid(z1);
+    }
     out[2 * p] = z0;
     if (2 * p + 1 < count) out[2 * p + 1] = z1;
   }
@@ -120,6 +133,32 @@ __global__ void gather_columns_kernel(const T* __restrict__ A, int64_t lda, bool
   }
 }
 
+// A = (I - 2uu') Dt (I - 2vv'), Dt = [diag(sigma); 0] (m x n, n <= m), u (m) and v (n) unit
+// vectors: A(i, j) = [i == j] sigma_j - 2 (sigma_i v_i) v_j - 2 u_i (sigma_j u_j) + 4 c u_i v_j
+// with sigma_i v_i = 0 for i >= n and c = u(0:n)' diag(sigma) v.  Every block forms c itself with
+// the same reduction order (256 threads), so all blocks use the same value.
+template <typename T>
+__global__ void __launch_bounds__(256) synth_reflected_kernel(
+    int64_t m, int64_t n, const double* __restrict__ sigma, const double* __restrict__ u,
+    const double* __restrict__ v, T* __restrict__ A, int64_t lda, bool colmajor) {
+  __shared__ double red[4];
+  double part = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 256) part += u[j] * sigma[j] * v[j];
+  part = wave_sum(part);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  const double c = (red[0] + red[1]) + (red[2] + red[3]);
+  const int64_t total = m * n;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int64_t i = colmajor ? e % m : e / n;
+    const int64_t j = colmajor ? e / m : e % n;
+    const double ai = i < n ? sigma[i] * v[i] : 0.0;
+    double x = -2.0 * ai * v[j] - 2.0 * u[i] * (sigma[j] * u[j]) + 4.0 * c * u[i] * v[j];
+    if (i == j) x += sigma[j];
+    A[colmajor ? (i + j * lda) : (i * lda + j)] = static_cast<T>(x);
+  }
+}
+
 __global__ void scale_stream_kernel(double* __restrict__ out, int64_t count, double scale) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
        e += (int64_t)gridDim.x * blockDim.x)
@@ -167,13 +206,17 @@ void gaussian_fill(double* out, int64_t count, uint64_t seed, double scale, hipS
   }
 }
 
-void omega_philox(double* out, int64_t count, uint64_t seed, hipStream_t st) {
+void omega_philox(double* out, int64_t count, uint64_t seed, hipStream_t st, bool f16_grid) {
   if (count <= 0) return;
   const int threads = 256;
   int64_t blocks = ceil_div((count + 1) / 2, threads);
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(omega_philox_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0,
-                     st, out, count, seed);
+  if (f16_grid)
+    hipLaunchKernelGGL(omega_philox_kernel<true>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(threads), 0, st, out, count, seed);
+  else
+    hipLaunchKernelGGL(omega_philox_kernel<false>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(threads), 0, st, out, count, seed);
   MCNLA_LAUNCHED();
 }
 
@@ -193,6 +236,22 @@ void synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const doubl
                        st, m, n, r, U, V, sigma, noise, seed, static_cast<double*>(A), lda,
                        a_colmajor);
   }
+  MCNLA_LAUNCHED();
+}
+
+void synth_reflected(int64_t m, int64_t n, const double* sigma, const double* u, const double* v,
+                     void* A, int64_t lda, bool a_f32, bool a_colmajor, hipStream_t st) {
+  if (m <= 0 || n <= 0) return;
+  int64_t blocks = ceil_div(m * n, 256 * 16);  // about 16 elements per thread
+  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 4096));
+  if (a_f32)
+    hipLaunchKernelGGL(synth_reflected_kernel<float>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(256), 0, st, m, n, sigma, u, v, static_cast<float*>(A), lda,
+                       a_colmajor);
+  else
+    hipLaunchKernelGGL(synth_reflected_kernel<double>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(256), 0, st, m, n, sigma, u, v, static_cast<double*>(A), lda,
+                       a_colmajor);
   MCNLA_LAUNCHED();
 }
 

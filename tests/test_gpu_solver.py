@@ -129,17 +129,23 @@ def test_solver_rowmajor_and_gramian_former(gpu, problem):
     assert np.abs(proj(s.matrixU()) - proj(U_o)).max() < 1e-8
 
 
-@pytest.mark.parametrize("prec,bound", [(mc.PREC_MIXED_BF16, 1e-3), (mc.PREC_MIXED_BF16X3, 2e-6)])
-def test_solver_mixed_vs_f64_slow_decay(gpu, problem, prec, bound):
-    """Mixed precision on the same Omega (Philox, seed 5) as the FP64 solve at the CPU-runnable
-    workload.  Its spectrum decays slowly (0.95^i, k=10 of l=20, N=4), so the Ritz values are
-    sensitive to the sketch: measured sigma errors 2.9e-4 (bf16) and 4.9e-7 (bf16x3)."""
-    note("solver mixed precision %d (slow decay)" % prec)
+@pytest.mark.parametrize("prec,omode,bound", [
+    (mc.PREC_MIXED_BF16, mc.OMEGA_PHILOX, 1e-3), (mc.PREC_MIXED_BF16X3, mc.OMEGA_PHILOX, 2e-6),
+    (mc.PREC_MIXED_F16, mc.OMEGA_PHILOX, 2e-4), (mc.PREC_MIXED_F16X2, mc.OMEGA_PHILOX_F16, 1e-5),
+    (mc.PREC_MIXED_F16X3, mc.OMEGA_PHILOX, 1e-5)])
+def test_solver_mixed_vs_f64_slow_decay(gpu, problem, prec, omode, bound):
+    """Mixed precision on the same Omega (seed 5) as the FP64 solve at the CPU-runnable workload.
+    Its spectrum decays slowly (0.95^i, k=10 of l=20, N=4), so the Ritz values are sensitive to the
+    sketch: one BF16 or FP16 product misses the 1e-5 target here (bounds 1e-3 and 2e-4 are what
+    they hold, measured 2.9e-4 for BF16), while the split forms carry about twice the mantissa of
+    A and meet it: FP16 A_hi + A_lo against an Omega that is exactly FP16 (two products), or
+    three products against any Omega."""
+    note("solver mixed precision %d omega %d (slow decay)" % (prec, omode))
     A = problem["A"]
     P = params()
-    ref = mc.Solver(P, seed=5)
+    ref = mc.Solver(P, seed=5, omega_mode=omode)
     ref.solve(mc.DeviceArray.from_numpy(np.asfortranarray(A)))
-    mix = mc.Solver(P, seed=5, precision=prec)
+    mix = mc.Solver(P, seed=5, precision=prec, omega_mode=omode)
     mix.solve(mc.DeviceArray.from_numpy(np.asfortranarray(A.astype(np.float32))))
     S64, Smx = ref.vectorS(), mix.vectorS()
     rel = np.max(np.abs(Smx - S64) / S64)
@@ -150,8 +156,10 @@ def test_solver_mixed_vs_f64_slow_decay(gpu, problem, prec, bound):
     assert abs(mix.iteration() - ref.iteration()) <= 1
 
 
-@pytest.mark.parametrize("prec,bound", [(mc.PREC_MIXED_BF16, 1e-5), (mc.PREC_MIXED_BF16X3, 1e-7)])
-def test_solver_mixed_vs_f64_headline_spectrum(gpu, prec, bound):
+@pytest.mark.parametrize("prec,omode,bound", [
+    (mc.PREC_MIXED_BF16, mc.OMEGA_PHILOX, 1e-5), (mc.PREC_MIXED_BF16X3, mc.OMEGA_PHILOX, 1e-7),
+    (mc.PREC_MIXED_F16, mc.OMEGA_PHILOX, 1e-5), (mc.PREC_MIXED_F16X2, mc.OMEGA_PHILOX_F16, 1e-7)])
+def test_solver_mixed_vs_f64_headline_spectrum(gpu, prec, omode, bound):
     """The headline workload's spectrum and sketch shape at reduced m, n: A = U50 diag(0.9^i)
     V50' + 1e-3 noise floor, k=20, p=12, N=64.  The mixed path's sigma is within 1e-5 of the
     FP64 path's on the same Omega (the benchmark checks the same at 100000 x 10000)."""
@@ -166,9 +174,9 @@ def test_solver_mixed_vs_f64_headline_spectrum(gpu, prec, bound):
     A64 = mcm.synth_lowrank(m, n, U, V, sig, noise, seed=1, dtype=np.float64, order="F")
     A32 = mcm.synth_lowrank(m, n, U, V, sig, noise, seed=1, dtype=np.float32, order="F")
     P = mc.Parameters().setSize(m, n).setRank(20).setOverRank(12).setNumSketch(64)
-    ref = mc.Solver(P, seed=3)
+    ref = mc.Solver(P, seed=3, omega_mode=omode)
     ref.solve(A64)
-    mix = mc.Solver(P, seed=3, precision=prec)
+    mix = mc.Solver(P, seed=3, precision=prec, omega_mode=omode)
     mix.solve(A32)
     S64, Smx = ref.vectorS(), mix.vectorS()
     rel = np.max(np.abs(Smx - S64) / S64)

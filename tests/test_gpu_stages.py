@@ -101,6 +101,63 @@ def test_sketch_mixed(gpu, order):
     assert np.linalg.norm(Y3 - exact) / np.linalg.norm(exact) < 5e-5
 
 
+def f16_rows(A):
+    """Per-row power-of-two exponents e (largest |a| 2^-e in [2^13, 2^14); 0 for a zero row) and
+    the scaled rows A 2^-e in FP32."""
+    mx = np.abs(A).max(axis=1).astype(np.float32)
+    e = np.where(mx > 0, np.frexp(mx)[1] - 14, 0)
+    return e, np.ldexp(A.astype(np.float32), -e[:, None]).astype(np.float32)
+
+
+def f16_split(x):
+    hi = x.astype(np.float16)
+    lo = (x - hi.astype(np.float32)).astype(np.float16)
+    return hi.astype(np.float64), lo.astype(np.float64)
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_sketch_mixed_f16(gpu, order):
+    """FP16 sketches on rows spanning 10^-30 .. 10^30 (and a zero row): each row of A is scaled by
+    a power of two into FP16 range and the output row scaled back.  One term equals the product of
+    the rounded scaled operands up to FP32 accumulation; the two-term split of A against an Omega
+    that is exactly FP16 (OMEGA_PHILOX_F16), and the three-term split against any Omega, match
+    the FP64 product row by row to FP32 accumulation accuracy."""
+    m, n, k, p, N = 333, 517, 10, 6, 5
+    note("sketch mixed fp16 order=%s" % order)
+    rng = np.random.default_rng(4)
+    A = (rng.standard_normal((m, n)) * 10.0 ** rng.uniform(-30, 30, (m, 1))).astype(np.float32)
+    A[7] = 0.0
+    P = params(m, n, k, p, N)
+    dA = mc.DeviceArray.from_numpy(np.asarray(A, order=order))
+    out = {}
+    for prec, omode in ((mc.PREC_MIXED_F16, mc.OMEGA_PHILOX), (mc.PREC_MIXED_F16X2, mc.OMEGA_PHILOX_F16),
+                        (mc.PREC_MIXED_F16X3, mc.OMEGA_PHILOX)):
+        sk = mc.ColBlockGaussianProjectionSketcher(P, seed=9, precision=prec, omega_mode=omode)
+        Y = P.createCollectionQ(np.float32)
+        sk(dA, Y)
+        out[prec] = (Y.numpy().astype(np.float64), sk.omega())
+    e, As = f16_rows(A)
+    hi, lo = f16_split(As)
+
+    def rowrel(Y, ref):
+        scale = np.abs(ref).max(axis=1)
+        scale[scale == 0] = 1.0
+        return (np.abs(Y - ref).max(axis=1) / scale).max()
+
+    Y1, om = out[mc.PREC_MIXED_F16]
+    pinned = np.ldexp(hi @ om.astype(np.float32).astype(np.float16).astype(np.float64), e[:, None])
+    assert np.all(Y1[7] == 0.0)
+    assert rowrel(Y1, pinned) < 2e-5
+    assert rowrel(Y1, A.astype(np.float64) @ om) < 3e-3
+    Y2, om2 = out[mc.PREC_MIXED_F16X2]
+    assert np.array_equal(om2.astype(np.float16).astype(np.float64), om2)  # Omega is FP16-exact
+    assert np.abs(om2 - om).max() <= 2.0 ** -11 * np.abs(om).max()  # the same stream, rounded
+    assert rowrel(Y2, A.astype(np.float64) @ om2) < 2e-6
+    Y3, om3 = out[mc.PREC_MIXED_F16X3]
+    assert np.array_equal(om3, om)
+    assert rowrel(Y3, A.astype(np.float64) @ om3) < 2e-6
+
+
 @pytest.mark.parametrize("kind,cb", [(mc.ColumnSamplingSketcher, True),
                                      (mc.RowBlockColumnSamplingSketcher, False)])
 @pytest.mark.parametrize("dtype,order", [(np.float64, "F"), (np.float32, "C")])
