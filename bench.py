@@ -2,7 +2,9 @@
 1e-4, SVD former.  The plain FP64 path (A FP64) and the mixed path (the same A stored FP32, BF16
 MFMA sketch with FP32 accumulation, FP64 Gram/eig/KN/SVD) run on the same Philox Omega; `value`
 is the mixed path's solves/s, reported beside the FP64 path's and the sigma relative error
-between them.  Prints one JSON line.
+between them.  The FP16 modes (one product with per-row scaling; two products of a hi/lo split of
+A against an Omega that is exactly FP16) are reported beside it, each against an FP64 solve on
+its own Omega.  Prints one JSON line.
 
 A is synthetic: A = U50 diag(0.9^i) V50' + E with Haar U50 (m x 50), V50 (n x 50) and
 E_ij = 1e-3 g_ij / (sqrt(m) + sqrt(n)), g ~ N(0,1) (||E||_2 ~ 1e-3); it is resident in HBM
@@ -22,7 +24,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MI355X_PEAK = {"bf16": 2.5e15, "f64": 78.6e12, "hbm": 8e12}
+MI355X_PEAK = {"bf16": 2.5e15, "f16": 2.5e15, "f64": 78.6e12, "hbm": 8e12}
+
+# mixed mode -> (precision name, Omega mode name, sketch products, 16-bit format)
+MIXED = {"bf16": ("PREC_MIXED_BF16", "OMEGA_PHILOX", 1, "bf16"),
+         "bf16x3": ("PREC_MIXED_BF16X3", "OMEGA_PHILOX", 3, "bf16"),
+         "f16": ("PREC_MIXED_F16", "OMEGA_PHILOX", 1, "f16"),
+         "f16x2": ("PREC_MIXED_F16X2", "OMEGA_PHILOX_F16", 2, "f16"),
+         "f16x3": ("PREC_MIXED_F16X3", "OMEGA_PHILOX", 3, "f16")}
 
 
 def log(msg):
@@ -64,99 +73,138 @@ def load_traffic():
         return json.load(f)
 
 
-def cpu_baseline(m, n, k, p, N, iters, budget_rows):
-    """The oracle (single-threaded C restatement) on a bounded row sample of the same workload:
-    every stage is linear in m except l x l work, so solve time ~ sample time * m / m_s, with the
-    integrator scaled to the GPU's iteration count."""
-    import oracle
-    l = k + p
-    ms = budget_rows
+def cpu_threads():
+    """Threads the host BLAS runs with (and the CPUs this process may use)."""
+    try:
+        import threadpoolctl
+        info = threadpoolctl.threadpool_info()
+        blas = [d["num_threads"] for d in info if d.get("user_api") == "blas"]
+        if blas:
+            return int(blas[0])
+    except Exception:
+        pass
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(m, n, k, p, N, rows):
+    """The reference's CPU pipeline on the host cores: every m-sized product through numpy's
+    multithreaded BLAS, the l x l and l x n factorizations through the oracle's small kernels
+    (oracle/blas_pipeline.py, pinned against the reference's fixtures and the C oracle).  It runs
+    the same workload family at `rows` of the m rows (A = U50 diag(0.9^i) V50' + noise floor, same
+    n, k, p, N, KN to convergence); every stage is linear in m apart from the l x l steps, so the
+    solve time is scaled by m / rows."""
+    from oracle import blas_pipeline
+    ms = rows
     rng = np.random.default_rng(7)
     U, V = make_factors(ms, n, 50, 8)
-    A = (U * 0.9 ** np.arange(50)) @ V.T + 1e-3 * rng.standard_normal((ms, n)) / (np.sqrt(m) + np.sqrt(n))
-    om = rng.standard_normal((n, N * l))
-    t = time.perf_counter()
-    Y = oracle.sketch(A, om, a_colmajor=True)
-    t_sk = time.perf_counter() - t
-    t = time.perf_counter()
-    Q, _ = oracle.gramian_orthogonalize(Y, l, N)
-    t_or = time.perf_counter() - t
-    t = time.perf_counter()
-    Qb, it_s = oracle.kn_rowblock(Q, l, N, 8, 0.0)  # 8 fixed iterations
-    t_it = (time.perf_counter() - t) / it_s
-    t = time.perf_counter()
-    oracle.svd_former(A, Qb, k, a_colmajor=True)
-    t_fo = time.perf_counter() - t
+    A = (U * 0.9 ** np.arange(50)) @ V.T + 1e-3 * rng.standard_normal((ms, n)) / (np.sqrt(ms) + np.sqrt(n))
+    A = np.asfortranarray(A)
+    om = rng.standard_normal((n, N * (k + p)))
+    blas_pipeline.solve(A[:2000], om, k, p, N, maxiter=2)  # warm the BLAS threads
+    t0 = time.perf_counter()
+    S, U_, it, stages = blas_pipeline.solve(A, om, k, p, N)
+    sample_s = time.perf_counter() - t0
     scale = m / ms
-    total = (t_sk + t_or + t_fo) * scale + t_it * scale * iters
-    sample_s = t_sk + t_or + t_it * it_s + t_fo
-    return {"value": 1.0 / total, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "oracle pipeline (sketch, Gramian orthogonalizer, 8 KN iterations, SVD "
-                      "former) on %d of %d rows (%.1f s of CPU), scaled linearly in m, KN to %d "
-                      "iterations" % (ms, m, sample_s, iters),
-            "seconds_per_solve": total}
+    total = sample_s * scale
+    threads = cpu_threads()
+    return {"value": 1.0 / total, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": "reference pipeline (sketch, Gramian orthogonalizer, row-block KN to "
+                      "convergence: %d iterations, SVD former) with numpy/OpenBLAS GEMMs on %d "
+                      "threads and the oracle's small kernels, at %d of %d rows (%.0f%%): %.2f s, "
+                      "stages %s s, scaled by m" % (it, threads, ms, m, 100.0 * ms / m, sample_s,
+                                                    [round(x, 3) for x in stages]),
+            "seconds_per_solve": total, "stage_seconds_sample": [round(x, 4) for x in stages]}
 
 
-SERVE_M, SERVE_N, SERVE_R, SERVE_BASES = 50000, 5000, 128, 8
+class MatrixRing:
+    """Device memory for the client's per-request matrices: regions are taken front to back
+    (wrapping to the start) and each is given back when its request is released; a region that
+    is still in use blocks the next allocation until it is released."""
 
+    def __init__(self, mc, nbytes):
+        import threading
+        self.buf = mc.DeviceArray((nbytes // 4,), np.float32)
+        self.cap = nbytes
+        self.head = 0
+        self.live = {}  # key -> (offset, size)
+        self.cv = threading.Condition()
 
-def serve_stream_requests(count, seed=3):
-    """The serving stream: m log-uniform in [1000, 50000], n log-uniform in [500, 5000] resampled
-    until n <= m, k uniform in {5..50}, p = 10, N uniform in {8..32}, rho ~ U(0.8, 0.98) (mapped
-    to the nearest of the resident bases), Omega seed per request."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        m = int(round(np.exp(rng.uniform(np.log(1000), np.log(SERVE_M)))))
-        while True:
-            n = int(round(np.exp(rng.uniform(np.log(500), np.log(SERVE_N)))))
-            if n <= m:
-                break
-        k = int(rng.integers(5, 51))
-        N = int(rng.integers(8, 33))
-        rho = float(rng.uniform(0.8, 0.98))
-        base = min(SERVE_BASES - 1, int((rho - 0.8) / 0.18 * SERVE_BASES))
-        out.append((m, n, k, 10, N, base, 5000 + i))
-    return out
+    def _free_at(self, off, size):
+        return all(off + size <= o or o + s <= off for o, s in self.live.values())
+
+    def take(self, size, key):
+        size = (size + 255) // 256 * 256
+        assert size <= self.cap
+        with self.cv:
+            while True:
+                off = self.head if self.head + size <= self.cap else 0
+                if self._free_at(off, size):
+                    break
+                self.cv.wait(1.0)
+            self.live[key] = (off, size)
+            self.head = off + size
+        return self.buf.ptr + off
+
+    def give(self, key):
+        with self.cv:
+            self.live.pop(key, None)
+            self.cv.notify_all()
+
+    def free(self):
+        self.buf.free()
 
 
 def serve_bench(mc, args):
-    """Continuous-batching serving of the request stream: capacity from a closed-loop burst, then
-    Poisson arrivals at 75% of it; p50/p99 latency from submit to results ready."""
+    """Continuous-batching serving of the request stream (mcnla_amd.workloads.serving_stream):
+    every request brings its own FP32 m x n matrix with sigma_i = rho^i, written into device
+    memory by the client on its own stream ahead of the arrival clock; the request carries the
+    event that completes when its matrix is written, and its solve waits for that event on the
+    device (so a late matrix shows up in the latency, which runs from submit to results).
+    Capacity from a closed-loop burst, then Poisson arrivals at 75% of it; p50/p99 latency from
+    submit to results ready."""
     import threading
-    rhos = 0.8 + 0.18 * (np.arange(SERVE_BASES) + 0.5) / SERVE_BASES
-    bases = []
-    for b in range(SERVE_BASES):
-        U, V = make_factors(SERVE_M, SERVE_N, SERVE_R, 100 + b)
-        bases.append(mc.synth_lowrank(SERVE_M, SERVE_N, U, V, rhos[b] ** np.arange(SERVE_R), 0.0,
-                                      seed=200 + b, dtype=np.float32, order="F"))
-    log("serve: %d resident FP32 bases %d x %d (rank %d, sigma_i = rho^i)"
-        % (SERVE_BASES, SERVE_M, SERVE_N, SERVE_R))
-    reqs = serve_stream_requests(args.serve_requests + args.serve_burst)
+    from mcnla_amd import workloads
+    reqs = workloads.serving_stream(args.serve_requests + args.serve_burst)
     burst, stream = reqs[:args.serve_burst], reqs[args.serve_burst:]
     kw = dict(hbm_budget=int(args.serve_budget_gb * (1 << 30)), precision=mc.PREC_MIXED_BF16)
+    gen = workloads.ReflectedGenerator(50000, 5000)
+    ring = MatrixRing(mc, int(args.serve_ring_gb * (1 << 30)))
 
-    def submit(srv, r):
-        m, n, k, p, N, b, seed = r
-        return srv.submit(bases[b], m, n, k, p, N, seed=seed, lda=SERVE_M)
+    def make(r, key):
+        """The request's matrix, enqueued on the generator's stream: (A, event when written)."""
+        ptr = ring.take(r["m"] * r["n"] * 4, key)
+        A = mc.DeviceArray.view(ptr, (r["m"], r["n"]), np.float32, "F")
+        return A, gen.write_async(A, workloads.stream_sigma(r), r["a_seed"])
+
+    def submit(srv, r, A_ev):
+        A, ev = A_ev
+        return srv.submit(A, r["m"], r["n"], r["k"], r["p"], r["N"], seed=r["omega_seed"],
+                          ready_event=ev)
 
     # the same requests alone (one slot, one at a time), to compare with their results in the mix
     check_idx = list(range(0, min(64, len(stream)), 4))  # kept however the stream is cut
     alone = {}
     with mc.Server(slots=1, **kw) as srv:
         for i in check_idx:
-            rid = submit(srv, stream[i])
+            A = make(stream[i], ("alone", i))
+            rid = submit(srv, stream[i], A)
             srv.wait(rid)
             r = srv.result(rid)
             alone[i] = (r.S.numpy(), r.Qbar.numpy(), r.iterations)
             srv.release(rid)
+            ring.give(("alone", i))
+    log("serve: %d requests solved alone for the comparison" % len(check_idx))
 
     with mc.Server(slots=args.serve_slots, **kw) as srv:
+        mats = [make(r, ("burst", i)) for i, r in enumerate(burst)]
+        mc.lib().mcnla_stream_synchronize(gen.stream)  # every burst matrix written
         t0 = time.perf_counter()
-        ids = [submit(srv, r) for r in burst]
+        ids = [submit(srv, r, A) for r, A in zip(burst, mats)]
         for _ in ids:
             srv.release(srv.next_done())
         cap = len(burst) / (time.perf_counter() - t0)
+        for i in range(len(burst)):
+            ring.give(("burst", i))
         log("serve: capacity %.1f solves/s (closed-loop burst of %d)" % (cap, len(burst)))
         rate = 0.75 * cap
         if len(stream) > rate * args.serve_seconds:  # keep the open-loop phase bounded
@@ -181,25 +229,49 @@ def serve_bench(mc, args):
                     ok = ok and got[2] == alone[i][2]
                     (matched if ok else mismatched)[0] += 1
                 srv.release(rid)
+                ring.give(("stream", i))
+
+        # A producer writes the request matrices ahead of the arrival clock (up to `ahead` of
+        # them), so that a request's matrix is resident when it arrives; `late` counts arrivals
+        # whose matrix was not ready yet (submitted as soon as it was).
+        import queue
+        ready = queue.Queue(maxsize=args.serve_ahead)
+
+        def produce():
+            for i, r in enumerate(stream):
+                ready.put((i, make(r, ("stream", i))))
 
         th = threading.Thread(target=collect)
+        pr = threading.Thread(target=produce)
+        pr.start()
+        while ready.qsize() < min(args.serve_ahead, len(stream)) and pr.is_alive():
+            time.sleep(0.001)
         th.start()
+        late = 0
         t0 = time.perf_counter()
         for i, r in enumerate(stream):
             delay = t0 + arrivals[i] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
+            if ready.empty():
+                late += 1
+            j, A = ready.get()  # (matrix, event): its solve waits on the device for the event
+            assert j == i
             with lock:
-                id_to_idx[submit(srv, r)] = i
+                id_to_idx[submit(srv, r, A)] = i
             if i % 512 == 511:
                 log("serve: %d/%d submitted" % (i + 1, len(stream)))
+        submit_span = time.perf_counter() - t0
+        pr.join()
         th.join()
         st = srv.stats()
-    for b in bases:
-        b.free()
-    log("serve: p50 %.2f ms, p99 %.2f ms, %.1f solves/s, max running %d"
-        % (st["p50_s"] * 1e3, st["p99_s"] * 1e3, st["completed"] / st["span_s"], st["max_running"]))
+    gen.free()
+    ring.free()
+    log("serve: p50 %.2f ms, p99 %.2f ms, %.1f solves/s, max running %d, %d late arrivals"
+        % (st["p50_s"] * 1e3, st["p99_s"] * 1e3, st["completed"] / st["span_s"], st["max_running"],
+           late))
     return {"requests": len(stream), "arrivals": "Poisson", "rate_per_s": round(rate, 2),
+            "offered_per_s": round(len(stream) / submit_span, 2), "late_arrivals": late,
             "capacity_solves_per_s": round(cap, 2), "solves_per_s": round(st["completed"] / st["span_s"], 2),
             "p50_ms": round(st["p50_s"] * 1e3, 3), "p99_ms": round(st["p99_s"] * 1e3, 3),
             "mean_ms": round(st["mean_s"] * 1e3, 3), "max_ms": round(st["max_s"] * 1e3, 3),
@@ -208,9 +280,10 @@ def serve_bench(mc, args):
             "peak_gb": round(st["peak_bytes"] / 2 ** 30, 2), "precision": "bf16 sketch, f64 rest",
             "alone_vs_mix_identical": "%d/%d" % (matched[0], matched[0] + mismatched[0]),
             "workload": "m log-U[1000,50000], n log-U[500,5000] (n<=m), k U{5..50}, p=10, "
-                        "N U{8..32}, rho U(0.8,0.98) over %d resident FP32 %dx%d bases "
-                        "(sigma_i=rho^i, rank %d; a request is the leading m x n block), KN tol "
-                        "1e-4, SVD former, stream seed 3" % (SERVE_BASES, SERVE_M, SERVE_N, SERVE_R)}
+                        "N U{8..32}, rho U(0.8,0.98); each request's A is a fresh FP32 m x n "
+                        "matrix (I - 2uu')[diag(rho^i); 0](I - 2vv') with singular values "
+                        "exactly rho^i, written on the device before submission; KN tol 1e-4, "
+                        "SVD former, stream seed 3"}
 
 
 def c4_bench(mc, args):
@@ -219,11 +292,16 @@ def c4_bench(mc, args):
     forms G = Omega'(A'A)Omega instead of the 500000 x 8192 collection; one solve on the sketch
     route (FP64 sketch and Gram) is the reference for the sigma error."""
     m, n, k, p, N = 500000, 2000, 50, 14, 128
-    V, _ = np.linalg.qr(np.random.default_rng(41).standard_normal((n, n)))
-    dU = mc.gaussian((m, n), seed=40, scale=1.0 / np.sqrt(m))  # columns orthonormal to ~sqrt(n/m)
+    from mcnla_amd import workloads
+    blocks = m // n
     sig = 1.0 / (1.0 + np.arange(n)) ** 2
-    A = mc.synth_lowrank(m, n, dU, V, sig, 0.0, seed=1, dtype=np.float32, order="C")
-    dU.free()
+    Q, _ = np.linalg.qr(np.random.default_rng(42).standard_normal((n, n)))
+    V, _ = np.linalg.qr(np.random.default_rng(43).standard_normal((n, n)))
+    M = ((Q * sig) @ V.T / np.sqrt(blocks)).astype(np.float32)
+    host, _ = workloads.sign_block_stack(blocks, M, seed=44)
+    A = mc.DeviceArray.from_numpy(host)
+    del host
+    sig = np.sqrt(blocks) * np.linalg.svd(M.astype(np.float64), compute_uv=False)  # exact
     log("c4: A %d x %d FP32 row-major resident (sigma_i = 1/(1+i)^2)" % (m, n))
     P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
     kw = dict(sketcher=mc.RowBlockGaussianProjectionSketcher,
@@ -250,8 +328,10 @@ def c4_bench(mc, args):
            "sketch_route_iterations": ref.iteration(),
            "sigma_rel_err_vs_sketch_route": float(np.max(np.abs(S - Sr) / Sr)),
            "sigma_vs_generator_max_rel": float(np.max(np.abs(S - sig[:k]) / sig[:k])),
-           "data": "A = U diag(1/(1+i)^2) V', V Haar 2000x2000, U = N(0,1)/sqrt(m) (columns "
-                   "orthonormal to ~6%), Omega Philox seed 2"}
+           "data": "A = [diag(s_t) M]_t, 250 row blocks of 2000 with random signs s_t, "
+                   "M = Q diag(1/(1+i)^2) V' / sqrt(250) (Q, V Haar 2000x2000) in FP32: the "
+                   "block stack has orthonormal columns, so sigma(A) is 1/(1+i)^2 up to the FP32 "
+                   "rounding of M and is known exactly; Omega Philox seed 2"}
     del s, ref
     A.free()
     return out
@@ -277,14 +357,21 @@ def main():
     ap.add_argument("--p", type=int, default=12)
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--cpu-rows", type=int, default=2000)
+    ap.add_argument("--cpu-rows", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mixed", default="bf16", choices=["bf16", "bf16x3"])
+    ap.add_argument("--mixed", default="bf16", choices=sorted(MIXED))
+    ap.add_argument("--accurate", default="f16,f16x2", help="comma-separated extra mixed modes "
+                    "reported beside the headline one, each against an FP64 solve on its Omega "
+                    "('' for none)")
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-burst", type=int, default=256)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-budget-gb", type=float, default=48.0)
     ap.add_argument("--serve-seconds", type=float, default=90.0)
+    ap.add_argument("--serve-ring-gb", type=float, default=32.0,
+                    help="device memory for the client's per-request matrices")
+    ap.add_argument("--serve-ahead", type=int, default=64,
+                    help="request matrices the client writes ahead of the arrival clock")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0)
     ap.add_argument("--c4-steps", type=int, default=3)
@@ -329,9 +416,10 @@ def main():
     A32 = mc.synth_lowrank(m, n, U, V, sig, noise, seed=1, dtype=np.float32, order="F")
     log("bench: A resident (FP64 %.1f GB, FP32 %.1f GB)" % (A64.nbytes / 1e9, A32.nbytes / 1e9))
     P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
-    prec = mc.PREC_MIXED_BF16 if args.mixed == "bf16" else mc.PREC_MIXED_BF16X3
-    s64 = mc.Solver(P, seed=2, graph=args.graph)
-    smx = mc.Solver(P, seed=2, precision=prec, graph=args.graph)
+    pname, oname, terms, fmt = MIXED[args.mixed]
+    prec, omode = getattr(mc, pname), getattr(mc, oname)
+    s64 = mc.Solver(P, seed=2, graph=args.graph, omega_mode=omode)
+    smx = mc.Solver(P, seed=2, precision=prec, graph=args.graph, omega_mode=omode)
 
     tune64 = tune(s64, A64, args, "fp64")
     tunemx = tune(smx, A32, args, "mixed")
@@ -342,8 +430,9 @@ def main():
         % (dt64 / fp64_steps * 1e3, g64.mean() * 1e3, k64[:, 1].mean() * 1e3, s64.iteration()))
     dtmx, kmx, stmx = timed_solves(mc, smx, A32, args.steps, args.warmup)
     gmx, grmx = kmx[:, 0], kmx[:, 1]
-    log("bench: mixed %.2f ms/solve (sketch gemm %.2f ms, gram gemm %.2f ms), iterations %d"
-        % (dtmx / args.steps * 1e3, gmx.mean() * 1e3, grmx.mean() * 1e3, smx.iteration()))
+    log("bench: mixed %s %.2f ms/solve (sketch gemm %.2f ms, gram gemm %.2f ms), iterations %d"
+        % (args.mixed, dtmx / args.steps * 1e3, gmx.mean() * 1e3, grmx.mean() * 1e3,
+           smx.iteration()))
     S64, Smx = s64.vectorS(), smx.vectorS()
     rel = float(np.max(np.abs(Smx - S64) / S64))
     Q64, Qmx = s64.matrixQbar(), smx.matrixQbar()
@@ -351,22 +440,44 @@ def main():
     c = np.linalg.svd(Q64.T @ Qmx, compute_uv=False)
     dist = float(np.sqrt(max(0.0, 1.0 - c.min() ** 2)))
 
+    # the other mixed modes, each beside an FP64 solve on the same Omega
+    accurate = {}
+    for name in [x for x in args.accurate.split(",") if x and x != args.mixed]:
+        pn, on, _, _ = MIXED[name]
+        ref = s64 if getattr(mc, on) == omode else mc.Solver(P, seed=2, omega_mode=getattr(mc, on))
+        if ref is not s64:
+            ref.solve(A64)
+        sx = mc.Solver(P, seed=2, precision=getattr(mc, pn), omega_mode=getattr(mc, on))
+        tune(sx, A32, args, name)
+        dtx, kx, stx = timed_solves(mc, sx, A32, args.steps, args.warmup)
+        Sr, Sx = ref.vectorS(), sx.vectorS()
+        accurate[name] = {
+            "value": round(args.steps / dtx, 3), "unit": "solves/s",
+            "ms_per_step": round(dtx / args.steps * 1e3, 3),
+            "omega": on, "iterations": sx.iteration(), "fp64_iterations": ref.iteration(),
+            "sigma_rel_err_vs_fp64": float(np.max(np.abs(Sx - Sr) / Sr)),
+            "speedup_vs_fp64": round((args.steps / dtx) / (fp64_steps / dt64), 2),
+            "stage_ms": [round(x * 1e3, 3) for x in stx.mean(axis=0)],
+            "sketch_gemm_ms": round(kx[:, 0].mean() * 1e3, 3)}
+        log("bench: mixed %s %.2f ms/solve, sigma rel err %.2e vs fp64 on its Omega"
+            % (name, dtx / args.steps * 1e3, accurate[name]["sigma_rel_err_vs_fp64"]))
+        del sx, ref
+
     nl = N * l
-    flops_bf16 = 2.0 * m * n * nl * (1 if args.mixed == "bf16" else 3)
+    flops_mx = 2.0 * m * n * nl * terms
     flops_f64 = 2.0 * m * n * nl
     flops_gram = 1.0 * m * nl * (nl + 1)  # lower triangle with diagonal, 2 flops per product
-    ach_mx = flops_bf16 / gmx.mean()
+    ach_mx = flops_mx / gmx.mean()
     ach_64 = flops_f64 / g64.mean()
     traffic = load_traffic()
     kernels = {
-        "sketch_gemm": {"kernel": "sketch_blk_kernel (Y = A*Omega, BF16 MFMA, FP32 accumulate)"
-                        if args.mixed == "bf16" else
-                        "sketch_bf16_kernel<3> (Y = A*Omega, BF16x3 MFMA, FP32 accumulate)",
+        "sketch_gemm": {"kernel": "sketch_blk_kernel (Y = A*Omega, %s MFMA x%d products, FP32 "
+                                  "accumulate)" % (fmt.upper(), terms),
                         "ms": round(gmx.mean() * 1e3, 3), "bound": "mfma",
-                        "achieved": round(ach_mx / 1e12, 2), "peak": MI355X_PEAK["bf16"] / 1e12,
-                        "unit": "TFLOP/s", "frac": round(ach_mx / MI355X_PEAK["bf16"], 4),
+                        "achieved": round(ach_mx / 1e12, 2), "peak": MI355X_PEAK[fmt] / 1e12,
+                        "unit": "TFLOP/s", "frac": round(ach_mx / MI355X_PEAK[fmt], 4),
                         "traffic": traffic.get("sketch_blk_kernel"),
-                        "algorithmic": "2*m*n*N*l = %.4e FLOP per launch" % flops_bf16},
+                        "algorithmic": "2*m*n*N*l*products = %.4e FLOP per launch" % flops_mx},
         "gram_gemm": {"kernel": "gemm_f64_dma_kernel (G = Y'Y, FP64 MFMA, lower-triangle tiles)",
                       "ms": round(grmx.mean() * 1e3, 3), "bound": "mfma",
                       "achieved": round(flops_gram / grmx.mean() / 1e12, 2),
@@ -389,9 +500,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 sketch (fp32 acc), f64 gram/eig/kn/svd",
+        "dtype": "%s sketch (fp32 acc), f64 gram/eig/kn/svd" % fmt,
         "data": "synthetic: A = U50 diag(0.9^i) V50' + 1e-3 noise floor, random Haar factors; "
-                "Omega Philox seed 2",
+                "Omega %s seed 2" % oname,
         "config": {"workload": "iSVD 100000x10000 k=20 p=12 N=64, KN tol 1e-4, SVD former; "
                                "mixed (A FP32) vs plain FP64 (A FP64) on the same Omega",
                    "m": m, "n": n, "k": k, "p": p, "N": N, "precision": args.mixed},
@@ -408,18 +519,18 @@ def main():
                   "sigma_rel_err_vs_fp64": rel, "subspace_dist_vs_fp64": dist,
                   "speedup_vs_fp64": round(value / fp64_value, 2),
                   "kernels": kernels},
+        "mixed_modes": accurate,
         "stage_order": ["sketch", "orthogonalize", "integrate", "form"],
         "roofline": {k: v for k, v in kernels[dominant].items() if k != "ms"},
     }
-    iters64 = s64.iteration()
     del s64, smx, A64, A32  # the other workloads' matrices and workspaces take their place
     if not args.no_c4:
         out["c4"] = c4_bench(mc, args)
     if not args.no_serve:
         out["serve"] = serve_bench(mc, args)
     if not args.no_cpu:
-        log("bench: cpu baseline (oracle, %d-row sample)" % args.cpu_rows)
-        out["cpu_baseline"] = cpu_baseline(m, n, k, p, N, iters64, args.cpu_rows)
+        log("bench: cpu baseline (BLAS pipeline, %d-row sample)" % args.cpu_rows)
+        out["cpu_baseline"] = cpu_baseline(m, n, k, p, N, min(args.cpu_rows, m))
     print(json.dumps(out), flush=True)
 
 

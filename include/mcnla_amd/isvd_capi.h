@@ -88,6 +88,18 @@ mcnla_status mcnla_stream_create(void** stream);
 mcnla_status mcnla_stream_destroy(void* stream);
 mcnla_status mcnla_stream_synchronize(void* stream);
 mcnla_status mcnla_device_synchronize(void);
+/* Events (hipEvent_t as void*, created without timing), pinned host memory and a stream-ordered
+ * host-to-device copy: a client that produces a request's matrix on its own stream records an
+ * event after it and hands the event to the server (mcnla_request.ready_event).  The copy is
+ * asynchronous from pinned memory (mcnla_host_malloc); its source must stay untouched until the
+ * stream has passed it. */
+mcnla_status mcnla_event_create(void** event);
+mcnla_status mcnla_event_destroy(void* event);
+mcnla_status mcnla_event_record(void* event, void* stream);
+mcnla_status mcnla_event_synchronize(void* event);
+mcnla_status mcnla_host_malloc(void** ptr, size_t bytes);
+mcnla_status mcnla_host_free(void* ptr);
+mcnla_status mcnla_memcpy_htod_async(void* dst, const void* src, size_t bytes, void* stream);
 mcnla_status mcnla_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ---------------------------------------------------------------------------------------------
@@ -370,6 +382,8 @@ typedef struct {
   int a_order;     /* mcnla_order */
   int a_on_host;
   uint64_t seed;   /* Omega seed */
+  void* ready_event; /* hipEvent_t (or NULL): the solve waits for it on the device before it reads
+                        A, so A may still be in production on another stream at submission */
 } mcnla_request;
 
 typedef struct {

@@ -76,7 +76,7 @@ class ServerConfig(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("params", Params), ("A", ctypes.c_void_p), ("lda", ctypes.c_int64),
                 ("a_dtype", ctypes.c_int), ("a_order", ctypes.c_int), ("a_on_host", ctypes.c_int),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("ready_event", ctypes.c_void_p)]
 
 
 class Served(ctypes.Structure):
@@ -122,6 +122,13 @@ SIGNATURES = {
     "mcnla_stream_destroy": (_st, [_vp]),
     "mcnla_stream_synchronize": (_st, [_vp]),
     "mcnla_device_synchronize": (_st, []),
+    "mcnla_event_create": (_st, [_pp]),
+    "mcnla_event_destroy": (_st, [_vp]),
+    "mcnla_event_record": (_st, [_vp, _vp]),
+    "mcnla_event_synchronize": (_st, [_vp]),
+    "mcnla_memcpy_htod_async": (_st, [_vp, _vp, _sz, _vp]),
+    "mcnla_host_malloc": (_st, [_pp, _sz]),
+    "mcnla_host_free": (_st, [_vp]),
     "mcnla_mem_info": (_st, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "mcnla_sketcher_create": (_st, [ctypes.POINTER(Params), _i, _i, _pp]),
     "mcnla_sketcher_set_seed": (_st, [_vp, _u64, _i, _i]),

@@ -236,7 +236,8 @@ struct SketchCore {
   DevBuf omega, ws;
   std::vector<double> host;
   hipEvent_t gemm_beg = nullptr, gemm_end = nullptr;
-  bool own_ev = true;  // false: the events belong to a SolveRes
+  bool own_ev = true;     // false: the events belong to a SolveRes
+  bool recorded = false;  // the events bracket a GEMM launched since the last reset
   ~SketchCore() {
     if (own_ev && gemm_beg) (void)hipEventDestroy(gemm_beg);
     if (own_ev && gemm_end) (void)hipEventDestroy(gemm_end);
@@ -296,6 +297,7 @@ struct SketchCore {
       MCNLA_HIP(hipEventRecord(gemm_beg, st));
       gemm_f64(g, st);
       MCNLA_HIP(hipEventRecord(gemm_end, st));
+      recorded = true;
     } else {
       SketchBf16Args s = bf16_args(p, precision);
       s.A = A;
@@ -310,10 +312,11 @@ struct SketchCore {
       s.ldy = nl;
       ws.ensure(sketch_bf16_workspace(s));
       sketch_bf16(s, ws.p, ws.bytes, st, gemm_beg, gemm_end);
+      recorded = true;
     }
   }
   double gemm_seconds() {
-    if (!gemm_beg) return 0.0;
+    if (!gemm_beg || !recorded) return 0.0;
     MCNLA_HIP(hipEventSynchronize(gemm_end));
     float ms = 0.f;
     MCNLA_HIP(hipEventElapsedTime(&ms, gemm_beg, gemm_end));
@@ -325,13 +328,14 @@ struct SketchCore {
 struct GramCore {
   DevBuf partial;
   hipEvent_t beg = nullptr, end = nullptr;
-  bool own_ev = true;  // false: the events belong to a SolveRes
+  bool own_ev = true;     // false: the events belong to a SolveRes
+  bool recorded = false;  // the events bracket a GEMM launched since the last reset
   ~GramCore() {
     if (own_ev && beg) (void)hipEventDestroy(beg);
     if (own_ev && end) (void)hipEventDestroy(end);
   }
   double gemm_seconds() {
-    if (!beg) return 0.0;
+    if (!beg || !recorded) return 0.0;
     MCNLA_HIP(hipEventSynchronize(end));
     float ms = 0.f;
     MCNLA_HIP(hipEventElapsedTime(&ms, beg, end));
@@ -381,6 +385,7 @@ struct GramCore {
     prepare(y_f32, m, nl);
     g.partial = partial.as<double>();
     gemm_f64(g, st);
+    recorded = true;
   }
 };
 
@@ -772,9 +777,11 @@ struct mcnla_solver_s {
     sketch.gemm_beg = res->sketch_beg;
     sketch.gemm_end = res->sketch_end;
     sketch.own_ev = false;
+    sketch.recorded = false;
     gram.beg = res->gram_beg;
     gram.end = res->gram_end;
     gram.own_ev = false;
+    gram.recorded = false;
   }
 };
 
@@ -1069,6 +1076,67 @@ mcnla_status mcnla_stream_destroy(void* stream) {
 mcnla_status mcnla_stream_synchronize(void* stream) {
   return guard([&] {
     MCNLA_HIP(hipStreamSynchronize(S(stream)));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_event_create(void** event) {
+  return guard([&] {
+    MCNLA_REQUIRE(event, MCNLA_ERR_INVALID, "event is null");
+    hipEvent_t e;
+    MCNLA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *event = e;
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_event_destroy(void* event) {
+  return guard([&] {
+    if (event) MCNLA_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_event_record(void* event, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(event, MCNLA_ERR_INVALID, "event is null");
+    MCNLA_HIP(hipEventRecord(static_cast<hipEvent_t>(event), S(stream)));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_event_synchronize(void* event) {
+  return guard([&] {
+    MCNLA_REQUIRE(event, MCNLA_ERR_INVALID, "event is null");
+    MCNLA_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_host_malloc(void** ptr, size_t bytes) {
+  return guard([&] {
+    MCNLA_REQUIRE(ptr, MCNLA_ERR_INVALID, "ptr is null");
+    *ptr = nullptr;
+    if (bytes == 0) return MCNLA_OK;
+    if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      *ptr = nullptr;
+      throw Error{MCNLA_ERR_NOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed"};
+    }
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_host_free(void* ptr) {
+  return guard([&] {
+    if (ptr) MCNLA_HIP(hipHostFree(ptr));
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_memcpy_htod_async(void* dst, const void* src, size_t bytes, void* stream) {
+  return guard([&] {
+    if (bytes) MCNLA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)));
     return MCNLA_OK;
   });
 }

@@ -135,12 +135,14 @@ __global__ void gather_columns_kernel(const T* __restrict__ A, int64_t lda, bool
 
 // A = (I - 2uu') Dt (I - 2vv'), Dt = [diag(sigma); 0] (m x n, n <= m), u (m) and v (n) unit
 // vectors: A(i, j) = [i == j] sigma_j - 2 (sigma_i v_i) v_j - 2 u_i (sigma_j u_j) + 4 c u_i v_j
-// with sigma_i v_i = 0 for i >= n and c = u(0:n)' diag(sigma) v.  Every block forms c itself with
-// the same reduction order (256 threads), so all blocks use the same value.
+// with sigma_i v_i = 0 for i >= n and c = u(0:n)' diag(sigma) v.  A block writes `lines`
+// consecutive storage lines (columns of a column-major A, rows of a row-major one) with its
+// threads along each line, so writes are coalesced and no element index is divided; every
+// block forms c itself with the same reduction order (256 threads), so all use the same value.
 template <typename T>
 __global__ void __launch_bounds__(256) synth_reflected_kernel(
     int64_t m, int64_t n, const double* __restrict__ sigma, const double* __restrict__ u,
-    const double* __restrict__ v, T* __restrict__ A, int64_t lda, bool colmajor) {
+    const double* __restrict__ v, T* __restrict__ A, int64_t lda, bool colmajor, int64_t lines) {
   __shared__ double red[4];
   double part = 0.0;
   for (int64_t j = threadIdx.x; j < n; j += 256) part += u[j] * sigma[j] * v[j];
@@ -148,14 +150,29 @@ __global__ void __launch_bounds__(256) synth_reflected_kernel(
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
   __syncthreads();
   const double c = (red[0] + red[1]) + (red[2] + red[3]);
-  const int64_t total = m * n;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
-    const int64_t i = colmajor ? e % m : e / n;
-    const int64_t j = colmajor ? e / m : e % n;
-    const double ai = i < n ? sigma[i] * v[i] : 0.0;
-    double x = -2.0 * ai * v[j] - 2.0 * u[i] * (sigma[j] * u[j]) + 4.0 * c * u[i] * v[j];
-    if (i == j) x += sigma[j];
-    A[colmajor ? (i + j * lda) : (i * lda + j)] = static_cast<T>(x);
+  const int64_t nlines = colmajor ? n : m;
+  const int64_t l0 = blockIdx.x * lines;
+  const int64_t l1 = l0 + lines < nlines ? l0 + lines : nlines;
+  for (int64_t line = l0; line < l1; ++line) {
+    T* out = A + line * lda;
+    if (colmajor) {  // column j = line; i runs along it
+      const int64_t j = line;
+      const double vj = v[j], bj = sigma[j] * u[j], sj = sigma[j];
+      for (int64_t i = threadIdx.x; i < m; i += 256) {
+        const double ai = i < n ? sigma[i] * v[i] : 0.0;
+        double x = -2.0 * ai * vj - 2.0 * u[i] * bj + 4.0 * c * u[i] * vj;
+        if (i == j) x += sj;
+        out[i] = static_cast<T>(x);
+      }
+    } else {  // row i = line; j runs along it
+      const int64_t i = line;
+      const double ai = i < n ? sigma[i] * v[i] : 0.0, ui = u[i];
+      for (int64_t j = threadIdx.x; j < n; j += 256) {
+        double x = -2.0 * ai * v[j] - 2.0 * ui * (sigma[j] * u[j]) + 4.0 * c * ui * v[j];
+        if (i == j) x += sigma[j];
+        out[j] = static_cast<T>(x);
+      }
+    }
   }
 }
 
@@ -242,16 +259,15 @@ void synth_lowrank(int64_t m, int64_t n, int64_t r, const double* U, const doubl
 void synth_reflected(int64_t m, int64_t n, const double* sigma, const double* u, const double* v,
                      void* A, int64_t lda, bool a_f32, bool a_colmajor, hipStream_t st) {
   if (m <= 0 || n <= 0) return;
-  int64_t blocks = ceil_div(m * n, 256 * 16);  // about 16 elements per thread
-  blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 4096));
+  const int64_t nlines = a_colmajor ? n : m;
+  const int64_t lines = ceil_div(nlines, 1024);  // about 1024 blocks
+  const unsigned blocks = static_cast<unsigned>(ceil_div(nlines, lines));
   if (a_f32)
-    hipLaunchKernelGGL(synth_reflected_kernel<float>, dim3(static_cast<unsigned>(blocks)),
-                       dim3(256), 0, st, m, n, sigma, u, v, static_cast<float*>(A), lda,
-                       a_colmajor);
+    hipLaunchKernelGGL(synth_reflected_kernel<float>, dim3(blocks), dim3(256), 0, st, m, n, sigma,
+                       u, v, static_cast<float*>(A), lda, a_colmajor, lines);
   else
-    hipLaunchKernelGGL(synth_reflected_kernel<double>, dim3(static_cast<unsigned>(blocks)),
-                       dim3(256), 0, st, m, n, sigma, u, v, static_cast<double*>(A), lda,
-                       a_colmajor);
+    hipLaunchKernelGGL(synth_reflected_kernel<double>, dim3(blocks), dim3(256), 0, st, m, n,
+                       sigma, u, v, static_cast<double*>(A), lda, a_colmajor, lines);
   MCNLA_LAUNCHED();
 }
 

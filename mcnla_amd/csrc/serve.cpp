@@ -194,6 +194,8 @@ void mcnla_server_s::run(Req* q, hipStream_t st, SolveRes* res) {
     const mcnla_params& p = q->r.params;
     const void* A = q->r.A;
     int64_t lda = q->r.lda;
+    if (q->r.ready_event)
+      MCNLA_HIP(hipStreamWaitEvent(st, static_cast<hipEvent_t>(q->r.ready_event), 0));
     if (q->r.a_on_host) {
       // Dense copy of A after the solver's workspaces: lines of `len` elements, `count` of them.
       const size_t esz = q->r.a_dtype == MCNLA_F32 ? sizeof(float) : sizeof(double);

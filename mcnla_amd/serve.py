@@ -66,10 +66,11 @@ class Server:
         self._params = {}
         self._hold = {}
 
-    def _request(self, A, m, n, k, p, N, seed, lda=None, order=None):
+    def _request(self, A, m, n, k, p, N, seed, lda=None, order=None, ready_event=None):
         r = _lib.Request()
         r.params = _lib.Params(int(m), int(n), int(k), int(p), int(N))
         r.seed = int(seed)
+        r.ready_event = ready_event
         if isinstance(A, DeviceArray):
             r.A = A.ptr
             r.lda = int(lda if lda is not None else A.ld)
@@ -98,10 +99,12 @@ class Server:
         _lib.check(_lib.lib().mcnla_server_request_bytes(self._h, ctypes.byref(r), ctypes.byref(b)))
         return b.value
 
-    def submit(self, A, m, n, k, p, N, seed=0, lda=None):
+    def submit(self, A, m, n, k, p, N, seed=0, lda=None, ready_event=None):
         """Queues an m x n iSVD of A (a DeviceArray, of which the leading m x n block is used with
-        leading dimension ``lda``, or a host array that is copied into the arena); returns an id."""
-        r, host = self._request(A, m, n, k, p, N, seed, lda)
+        leading dimension ``lda``, or a host array that is copied into the arena); returns an id.
+        ``ready_event`` (an event handle): the solve waits for it on the device before reading
+        A, so A may still be being written on another stream."""
+        r, host = self._request(A, m, n, k, p, N, seed, lda, ready_event=ready_event)
         rid = ctypes.c_uint64()
         _lib.check(_lib.lib().mcnla_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
         self._params[rid.value] = r.params

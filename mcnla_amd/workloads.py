@@ -22,12 +22,33 @@ from .device import DeviceArray
 
 
 class ReflectedGenerator:
-    """Writes reflected matrices into caller-provided device buffers; one staging buffer for the
-    vectors (up to ``max_m + 2 max_n`` doubles) is reused by every call."""
+    """Writes reflected matrices into caller-provided device buffers on its own stream.  The
+    vectors (up to ``max_m + 2 max_n`` doubles per matrix) go through a ring of ``slots`` device
+    staging buffers; a slot is reused only after the kernel that read it has finished (an event
+    per slot), so ``write_async`` never waits unless the ring has wrapped onto unfinished work."""
 
-    def __init__(self, max_m, max_n):
+    def __init__(self, max_m, max_n, slots=128):
         self.max_m, self.max_n = int(max_m), int(max_n)
-        self._stage = DeviceArray((self.max_m + 2 * self.max_n,), np.float64)
+        self.width = self.max_m + 2 * self.max_n
+        self.slots = int(slots)
+        self._stage = DeviceArray((self.slots, self.width), np.float64)
+        L = _lib.lib()
+        hp = ctypes.c_void_p()
+        _lib.check(L.mcnla_host_malloc(ctypes.byref(hp), self.slots * self.width * 8))
+        self._pinned_ptr = hp.value
+        self._pinned = np.ctypeslib.as_array(
+            ctypes.cast(hp.value, ctypes.POINTER(ctypes.c_double)),
+            shape=(self.slots, self.width))
+        h = ctypes.c_void_p()
+        _lib.check(L.mcnla_stream_create(ctypes.byref(h)))
+        self.stream = h.value
+        self._events = []
+        for _ in range(self.slots):
+            e = ctypes.c_void_p()
+            _lib.check(L.mcnla_event_create(ctypes.byref(e)))
+            self._events.append(e.value)
+        self._used = [False] * self.slots
+        self._next = 0
 
     @staticmethod
     def vectors(m, n, seed):
@@ -37,18 +58,34 @@ class ReflectedGenerator:
         v = rng.standard_normal(n)
         return u / np.linalg.norm(u), v / np.linalg.norm(v)
 
-    def write(self, out, sigma, seed, stream=None):
-        """A into ``out`` (a DeviceArray, m x n with n <= m, any dtype/order/ld); synchronous."""
+    def write_async(self, out, sigma, seed):
+        """Enqueues A into ``out`` (a DeviceArray, m x n with n <= m, any dtype/order/ld) on the
+        generator's stream; returns an event that completes when A is written."""
         m, n = out.shape
         assert n <= m <= self.max_m and n <= self.max_n and len(sigma) == n
-        u, v = self.vectors(m, n, seed)
-        host = np.concatenate([np.asarray(sigma, np.float64), u, v])
         L = _lib.lib()
-        _lib.check(L.mcnla_memcpy_htod(self._stage.ptr, host.ctypes.data, host.nbytes))
-        base = self._stage.ptr
+        slot = self._next
+        self._next = (slot + 1) % self.slots
+        ev = self._events[slot]
+        if self._used[slot]:
+            _lib.check(L.mcnla_event_synchronize(ev))
+        u, v = self.vectors(m, n, seed)
+        host = self._pinned[slot]  # free: the slot's last copy and kernel have finished
+        host[:n] = sigma
+        host[n:n + m] = u
+        host[n + m:2 * n + m] = v
+        base = self._stage.ptr + slot * self.width * 8
+        _lib.check(L.mcnla_memcpy_htod_async(base, host.ctypes.data, (2 * n + m) * 8,
+                                             self.stream))
         _lib.check(L.mcnla_synth_reflected(m, n, base, base + 8 * n, base + 8 * (n + m), out.ptr,
-                                           out.ld, out.dtype_code, out.order_code, stream))
-        _lib.check(L.mcnla_stream_synchronize(stream))
+                                           out.ld, out.dtype_code, out.order_code, self.stream))
+        _lib.check(L.mcnla_event_record(ev, self.stream))
+        self._used[slot] = True
+        return ev
+
+    def write(self, out, sigma, seed):
+        """As write_async, and waits for it."""
+        _lib.check(_lib.lib().mcnla_event_synchronize(self.write_async(out, sigma, seed)))
         return out
 
     def make(self, m, n, sigma, seed, dtype=np.float32, order="F"):
@@ -65,6 +102,16 @@ class ReflectedGenerator:
         return Hu @ D @ Hv
 
     def free(self):
+        L = _lib.lib()
+        if self.stream:
+            _lib.check(L.mcnla_stream_synchronize(self.stream))
+            for e in self._events:
+                L.mcnla_event_destroy(e)
+            L.mcnla_stream_destroy(self.stream)
+            self.stream = None
+            self._events = []
+            self._pinned = None
+            L.mcnla_host_free(self._pinned_ptr)
         self._stage.free()
 
 

@@ -268,6 +268,10 @@ def test_solver_gram_of_a_route(gpu, dtype, order, shape):
         s = mc.Solver(P, seed=SEED, omega_mode=mc.OMEGA_REFERENCE, route=route)
         s.solve(dA)
         assert s.converged
+        times, sketch_s = s.stage_times()  # on either route, whether or not a sketch GEMM ran
+        assert all(t > 0 for t in times) and sketch_s >= 0
+        kern = s.kernel_times()
+        assert (kern[0] > 0) == (route == mc.ROUTE_SKETCH) and kern[1] > 0
         outs[route] = (s.vectorS(), s.matrixU(), s.matrixV(), s.matrixQbar(), s.iteration())
     S, Uu, Vv, Qb, it = outs[mc.ROUTE_GRAM_OF_A]
     assert abs(it - it_o) <= 1

@@ -326,6 +326,39 @@ def test_serving_stream_alone_vs_mix(gpu, stream_mats):
         assert a[4] == b[4]
 
 
+def test_serving_ready_event(gpu, stream_mats):
+    """Requests submitted while their matrices are still being written on the generator's
+    stream, each carrying the generator's event: the solve waits for it on the device and gives
+    the same bits as the request whose matrix was written before submission."""
+    reqs, mats = stream_mats
+    picked = list(range(0, 64, 8))
+    gen = workloads.ReflectedGenerator(50000, 5000, slots=4)  # slots wrap: write_async waits too
+    fresh = [mc.DeviceArray((reqs[i]["m"], reqs[i]["n"]), np.float32, "F") for i in picked]
+    with mc.Server(slots=4, hbm_budget=16 << 30, precision=mc.PREC_MIXED_BF16) as srv:
+        ids = []
+        for i, A in zip(picked, fresh):
+            ev = gen.write_async(A, workloads.stream_sigma(reqs[i]), reqs[i]["a_seed"])
+            ids.append(srv.submit(A, reqs[i]["m"], reqs[i]["n"], reqs[i]["k"], reqs[i]["p"],
+                                  reqs[i]["N"], seed=reqs[i]["omega_seed"], ready_event=ev))
+        got = []
+        for rid in ids:
+            assert srv.wait(rid, 60.0)
+            got.append(collect(srv, rid))
+        ref = []
+        for i in picked:
+            r = reqs[i]
+            rid = submit(srv, r, mats[i])
+            assert srv.wait(rid, 60.0)
+            ref.append(collect(srv, rid))
+    gen.free()
+    for a, b in zip(got, ref):
+        for x, y in zip(a[:4], b[:4]):
+            assert np.array_equal(x, y)
+        assert a[4] == b[4]
+    for A in fresh:
+        A.free()
+
+
 def test_serving_stream_vs_oracle(gpu, stream_mats):
     """The four cheapest of those requests, served in FP64 alongside eight others, against the
     oracle pipeline on the same A (the FP32 matrix) and the same Philox Omega: sigma to 1e-9,
