@@ -170,11 +170,14 @@ def serve_bench(mc, args):
     gen = workloads.ReflectedGenerator(50000, 5000)
     ring = MatrixRing(mc, int(args.serve_ring_gb * (1 << 30)))
 
-    def make(r, key):
-        """The request's matrix, enqueued on the generator's stream: (A, event when written)."""
+    def make(r, key, pre=None):
+        """The request's matrix, enqueued on the generator's stream: (A, event when written).
+        pre: its (sigma, (u, v)) computed beforehand."""
         ptr = ring.take(r["m"] * r["n"] * 4, key)
         A = mc.DeviceArray.view(ptr, (r["m"], r["n"]), np.float32, "F")
-        return A, gen.write_async(A, workloads.stream_sigma(r), r["a_seed"])
+        if pre is None:
+            return A, gen.write_async(A, workloads.stream_sigma(r), r["a_seed"])
+        return A, gen.write_async(A, pre[0], r["a_seed"], vectors=pre[1])
 
     def submit(srv, r, A_ev):
         A, ev = A_ev
@@ -234,12 +237,16 @@ def serve_bench(mc, args):
         # A producer writes the request matrices ahead of the arrival clock (up to `ahead` of
         # them), so that a request's matrix is resident when it arrives; `late` counts arrivals
         # whose matrix was not ready yet (submitted as soon as it was).
+        # the requests' sigma and random vectors, drawn before the arrival clock starts
+        pre = [(workloads.stream_sigma(r), workloads.ReflectedGenerator.vectors(r["m"], r["n"],
+                                                                              r["a_seed"]))
+               for r in stream]
         import queue
         ready = queue.Queue(maxsize=args.serve_ahead)
 
         def produce():
             for i, r in enumerate(stream):
-                ready.put((i, make(r, ("stream", i))))
+                ready.put((i, make(r, ("stream", i), pre[i])))
 
         th = threading.Thread(target=collect)
         pr = threading.Thread(target=produce)
@@ -383,6 +390,12 @@ def main():
     ap.add_argument("--only-c4", action="store_true", help="run the row-block Wen-Yin workload "
                     "alone and print its record (for profiling)")
     args = ap.parse_args()
+    # The serving phase runs 2 streams per slot (a solve and its side stream); HIP maps streams
+    # onto GPU_MAX_HW_QUEUES hardware queues per process (4 by default), and streams that share a
+    # queue run their kernels one after another.  Before the runtime starts, give the process one
+    # queue per stream of the default 8 slots.
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 2 * args.serve_slots:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, 2 * args.serve_slots))
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the iSVD path here is built for one MI355X "
               "(multi-GPU sharding is a design note, not built); rerun with --gpus 1",

@@ -58,9 +58,10 @@ class ReflectedGenerator:
         v = rng.standard_normal(n)
         return u / np.linalg.norm(u), v / np.linalg.norm(v)
 
-    def write_async(self, out, sigma, seed):
+    def write_async(self, out, sigma, seed, vectors=None):
         """Enqueues A into ``out`` (a DeviceArray, m x n with n <= m, any dtype/order/ld) on the
-        generator's stream; returns an event that completes when A is written."""
+        generator's stream; returns an event that completes when A is written.  ``vectors``:
+        the (u, v) of ``vectors(m, n, seed)``, if the caller computed them already."""
         m, n = out.shape
         assert n <= m <= self.max_m and n <= self.max_n and len(sigma) == n
         L = _lib.lib()
@@ -69,7 +70,7 @@ class ReflectedGenerator:
         ev = self._events[slot]
         if self._used[slot]:
             _lib.check(L.mcnla_event_synchronize(ev))
-        u, v = self.vectors(m, n, seed)
+        u, v = self.vectors(m, n, seed) if vectors is None else vectors
         host = self._pinned[slot]  # free: the slot's last copy and kernel have finished
         host[:n] = sigma
         host[n:n + m] = u
