@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <utility>
 #include <iomanip>
 #include <limits>
 #include <sstream>
@@ -22,6 +23,14 @@
 #include "isvd.hpp"
 
 namespace mcnla {
+
+// A half-open index range [begin, end) (the reference's IdxRange, core/utility/index_range.hpp),
+// as the block loaders take it.
+struct IdxRange {
+  index_t begin = 0, end = 0;
+  index_t len() const { return end - begin; }
+};
+
 namespace io {
 
 inline std::ifstream open_in(const char* file) {
@@ -41,6 +50,67 @@ inline void loadMatrixMarketSize(index_t& nrow, index_t& ncol, const char* file)
   skip_comments(fin);
   fin >> nrow >> ncol;
   if (!fin) throw Error(MCNLA_ERR_INVALID, std::string("bad MatrixMarket size line in ") + file);
+}
+
+// The size as the matrix of trans (io::loadMatrixMarketSize<Trans::NORMAL/TRANS>,
+// matrix_market/dense_load.hpp): TRANS gives (n, m).
+template <Trans trans>
+void loadMatrixMarketSize(index_t& nrow, index_t& ncol, const char* file) {
+  loadMatrixMarketSize(nrow, ncol, file);
+  if (trans == Trans::TRANS) std::swap(nrow, ncol);
+}
+
+// Columns [cols.begin, cols.end) of a column-major MatrixMarket file
+// (loadMatrixMarketColBlock, matrix_market/dense_load_block.hpp:62-103): the rank's column block
+// of isvd_dense_cb.  An empty block is allocated m x len.
+template <typename T, bool ROW>
+void loadMatrixMarketColBlock(matrix::DenseMatrix<T, ROW>& block, const char* file,
+                              const IdxRange& cols) {
+  auto fin = open_in(file);
+  skip_comments(fin);
+  index_t m = 0, n = 0;
+  fin >> m >> n;
+  if (cols.begin < 0 || cols.end > n || cols.len() < 0)
+    throw Error(MCNLA_ERR_INVALID, "column range outside the MatrixMarket matrix");
+  if (block.isEmpty()) block.reconstruct(m, cols.len());
+  if (block.nrow() != m || block.ncol() != cols.len())
+    throw Error(MCNLA_ERR_INVALID, "column block size mismatch");
+  double v;
+  for (index_t e = 0; e < cols.begin * m; ++e) fin >> v;
+  for (index_t j = 0; j < cols.len(); ++j)
+    for (index_t i = 0; i < m; ++i) {
+      fin >> v;
+      block(i, j) = static_cast<T>(v);
+    }
+  if (!fin) throw Error(MCNLA_ERR_INVALID, std::string("truncated MatrixMarket file ") + file);
+}
+
+// Rows [rows.begin, rows.end) of a column-major MatrixMarket file
+// (loadMatrixMarketRowBlock, dense_load_block.hpp:127-179): the rank's row block of
+// isvd_dense_rb.  Each column's values outside the range are skipped (m - end of them after the
+// block; the reference skips n - end, which agrees for square matrices).
+template <typename T, bool ROW>
+void loadMatrixMarketRowBlock(matrix::DenseMatrix<T, ROW>& block, const char* file,
+                              const IdxRange& rows) {
+  auto fin = open_in(file);
+  skip_comments(fin);
+  index_t m = 0, n = 0;
+  fin >> m >> n;
+  if (rows.begin < 0 || rows.end > m || rows.len() < 0)
+    throw Error(MCNLA_ERR_INVALID, "row range outside the MatrixMarket matrix");
+  if (block.isEmpty()) block.reconstruct(rows.len(), n);
+  if (block.nrow() != rows.len() || block.ncol() != n)
+    throw Error(MCNLA_ERR_INVALID, "row block size mismatch");
+  double v;
+  for (index_t j = 0; j < n; ++j) {
+    for (index_t i = 0; i < rows.begin; ++i) fin >> v;
+    for (index_t i = 0; i < rows.len(); ++i) {
+      fin >> v;
+      block(i, j) = static_cast<T>(v);
+    }
+    for (index_t i = rows.end; i < m; ++i) fin >> v;
+  }
+  if (!fin) throw Error(MCNLA_ERR_INVALID, std::string("truncated MatrixMarket file ") + file);
 }
 
 template <typename T, bool ROW>
@@ -148,6 +218,61 @@ void loadBinarySize(index_t& nrow, index_t& ncol, const char* file) {
   if (nd != 2) throw Error(MCNLA_ERR_INVALID, "binary file is not a matrix");
   nrow = ROW ? d1 : d0;
   ncol = ROW ? d0 : d1;
+}
+
+// The size as the matrix of trans (io::loadBinarySize<Trans::NORMAL/TRANS>, binary/dense_load.hpp):
+// NORMAL reads the storage dims as (m, n) of a column-major matrix, TRANS as (n, m).
+template <Trans trans>
+void loadBinarySize(index_t& nrow, index_t& ncol, const char* file) {
+  loadBinarySize<trans == Trans::TRANS>(nrow, ncol, file);
+}
+
+namespace detail {
+// Opens a dense binary file of value type T; returns its storage dims.
+template <typename T>
+std::ifstream open_binary(const char* file, int64_t& d0, int64_t& d1) {
+  std::ifstream in(file, std::ios::binary);
+  if (!in) throw Error(MCNLA_ERR_INVALID, std::string("cannot open ") + file);
+  if (read_header(in) != val_code<T>()) throw Error(MCNLA_ERR_INVALID, "binary value type mismatch");
+  int64_t nd = 0;
+  in.read(reinterpret_cast<char*>(&nd), 8);
+  in.read(reinterpret_cast<char*>(&d0), 8);
+  in.read(reinterpret_cast<char*>(&d1), 8);
+  if (!in || nd != 2) throw Error(MCNLA_ERR_INVALID, "binary file is not a matrix");
+  return in;
+}
+}  // namespace detail
+
+// Storage lines [range.begin, range.end) of a dense binary file: the columns of a column-major
+// matrix (loadBinaryColBlock, binary/dense_load_block.hpp:68-113) or the rows of a row-major one
+// (loadBinaryRowBlock, :122-142, which reads the transpose's column block).  Seeks past the lines
+// before the block, so a rank reads only its own bytes.
+template <typename T, bool ROW>
+void loadBinaryLines(matrix::DenseMatrix<T, ROW>& block, const char* file, const IdxRange& range) {
+  int64_t d0 = 0, d1 = 0;
+  auto in = detail::open_binary<T>(file, d0, d1);
+  if (range.begin < 0 || range.end > d1 || range.len() < 0)
+    throw Error(MCNLA_ERR_INVALID, "block range outside the binary matrix");
+  const index_t m = ROW ? range.len() : d0, n = ROW ? d0 : range.len();
+  if (block.isEmpty()) block.reconstruct(m, n);
+  if (block.nrow() != m || block.ncol() != n) throw Error(MCNLA_ERR_INVALID, "block size mismatch");
+  in.seekg(static_cast<std::streamoff>(d0 * range.begin * static_cast<int64_t>(sizeof(T))),
+           std::ios_base::cur);
+  in.read(reinterpret_cast<char*>(block.valPtr()),
+          static_cast<std::streamsize>(d0 * range.len() * static_cast<int64_t>(sizeof(T))));
+  if (!in) throw Error(MCNLA_ERR_INVALID, std::string("truncated binary file ") + file);
+}
+
+template <typename T>
+void loadBinaryColBlock(matrix::DenseMatrix<T, false>& block, const char* file,
+                        const IdxRange& cols) {
+  loadBinaryLines(block, file, cols);
+}
+
+template <typename T>
+void loadBinaryRowBlock(matrix::DenseMatrix<T, true>& block, const char* file,
+                        const IdxRange& rows) {
+  loadBinaryLines(block, file, rows);
 }
 
 template <typename T, bool ROW>
