@@ -43,7 +43,13 @@ __global__ void orth_transforms_kernel(const double* __restrict__ w, const doubl
     const int64_t b = e / (l * l);
     const int r = (e / l) % l, c = e % l;
     const double ev = w[b * l + c];
-    if (r == 0 && !(ev > 0.0)) atomicAdd(bad, 1);
+    if (r == 0) {
+      // an eigenvalue at or below l eps of the block's largest is zero at working precision:
+      // the sketch does not have full column rank (or holds NaN / Inf)
+      double wmax = 0.0;
+      for (int q = 0; q < l; ++q) wmax = fmax(wmax, w[b * l + q]);
+      if (!(ev > l * 0x1p-52 * wmax)) atomicAdd(bad, 1);
+    }
     T[e] = V[e] / sqrt(ev);
   }
 }
@@ -114,7 +120,8 @@ __global__ __launch_bounds__(256) void congruence_kernel(const double* __restric
 
 // T_i = L_i^{-T} from the Cholesky factorization W_i = L_i L_i' (one workgroup per block), so
 // that Y_i T_i has orthonormal columns when W_i = Y_i' Y_i.  L_0 is also written to L0 (lower
-// triangle, zeros above).  A pivot that is not positive flags *bad.
+// triangle, zeros above).  A pivot that is not above l eps of W_i's largest diagonal entry flags
+// *bad.
 __global__ __launch_bounds__(256) void cholesky_transforms_kernel(const double* __restrict__ W,
                                                                  double* __restrict__ T,
                                                                  double* __restrict__ L0, int l,
@@ -131,11 +138,16 @@ __global__ __launch_bounds__(256) void cholesky_transforms_kernel(const double* 
     x[i * ld + j] = 0.0;
   }
   __syncthreads();
+  // A pivot at or below l eps of the largest diagonal entry is zero at working precision: the
+  // sketch does not have full column rank (or W holds NaN / Inf, which fails the test too).
+  double wmax = 0.0;
+  for (int i = 0; i < l; ++i) wmax = fmax(wmax, Wb[i * l + i]);
+  const double floor_piv = l * 0x1p-52 * wmax;
   bool fail = false;
   for (int j = 0; j < l; ++j) {
     const double d = a[j * ld + j];
     const double piv = (d > 0.0) ? sqrt(d) : 1.0;
-    fail = fail || !(d > 0.0);
+    fail = fail || !(d > floor_piv);
     __syncthreads();  // every thread has read the pivot before it is overwritten
     if (tid == 0) a[j * ld + j] = piv;
     for (int r = j + 1 + tid; r < l; r += blockDim.x) a[r * ld + j] /= piv;

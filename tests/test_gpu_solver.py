@@ -371,3 +371,65 @@ def test_gram_of_a_route_mixed_precision(gpu):
     assert np.abs(proj(Umx) - proj(U64)).max() < 1e-5
     assert np.abs(proj(Qmx) - proj(Q64)).max() < 1e-5
     assert np.allclose(Qmx.T @ Qmx, np.eye(k + p), atol=1e-5)
+
+
+def test_rank_deficient_sketch_is_numeric(gpu):
+    """A of rank 8 < l = 20: every FP64 sketch Y_i = A Omega_i has rank 8, so its Gram matrix is
+    singular and the orthogonalizer cannot make Y_i orthonormal.  The solver reports
+    MCNLA_ERR_NUMERIC (the reference would divide by a zero singular value); the stage
+    orthogonalizer does the same.  A full-rank A of the same size solves."""
+    note("rank-deficient A")
+    m, n, k, p, N = 600, 300, 10, 10, 4
+    rng = np.random.default_rng(12)
+    U, _ = np.linalg.qr(rng.standard_normal((m, 8)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, 8)))
+    A = mc.DeviceArray.from_numpy(np.asfortranarray((U * 0.9 ** np.arange(8)) @ V.T))
+    P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+    with pytest.raises(mc.McnlaError) as e:
+        mc.Solver(P, seed=3, route=mc.ROUTE_SKETCH).solve(A)
+    assert e.value.status == mc._lib.ERR_NUMERIC
+    sk = mc.ColBlockGaussianProjectionSketcher(P, seed=3)
+    orth = mc.RowBlockGramianOrthogonalizer(P)
+    for st in (sk, orth):
+        st.initialize()
+    Q = P.createCollectionQ()
+    sk(A, Q)
+    with pytest.raises(mc.McnlaError) as e:
+        orth(Q)
+    assert e.value.status == mc._lib.ERR_NUMERIC
+    B = mc.DeviceArray.from_numpy(np.asfortranarray(rng.standard_normal((m, n))))
+    s = mc.Solver(P, seed=3).solve(B)
+    assert np.all(np.isfinite(s.vectorS()))
+
+
+@pytest.mark.parametrize("prec", [mc.PREC_F64, mc.PREC_MIXED_BF16, mc.PREC_MIXED_F16])
+def test_nan_in_a_is_numeric(gpu, prec):
+    """One NaN in A reaches every sketch through the product and the solve reports
+    MCNLA_ERR_NUMERIC in every precision (FP16's row scaling passes it through unchanged)."""
+    note("NaN in A, precision %d" % prec)
+    m, n = 500, 300
+    A = np.random.default_rng(13).standard_normal((m, n))
+    A[17, 5] = np.nan
+    dA = mc.DeviceArray.from_numpy(np.asfortranarray(A.astype(np.float64 if prec == mc.PREC_F64
+                                                              else np.float32)))
+    P = mc.Parameters().setSize(m, n).setRank(8).setOverRank(8).setNumSketch(4)
+    with pytest.raises(mc.McnlaError) as e:
+        mc.Solver(P, seed=3, precision=prec).solve(dA)
+    assert e.value.status == mc._lib.ERR_NUMERIC
+
+
+def test_allocation_failure_is_nomem(gpu):
+    """A solver whose workspace cannot fit the device (the 4e7 x 2048 collection alone is
+    655 GB) fails to create with MCNLA_ERR_NOMEM, frees what it had taken, and the library keeps
+    working."""
+    note("allocation failure")
+    P = mc.Parameters().setSize(40_000_000, 20000).setRank(20).setOverRank(12).setNumSketch(64)
+    before, _ = mc.mem_info()
+    with pytest.raises(mc.McnlaError) as e:
+        mc.Solver(P, route=mc.ROUTE_SKETCH)
+    assert e.value.status == mc._lib.ERR_NOMEM
+    after, _ = mc.mem_info()
+    assert after >= before - (64 << 20)
+    A = mc.DeviceArray.from_numpy(np.asfortranarray(np.random.default_rng(1).standard_normal((400, 200))))
+    s = mc.Solver(mc.Parameters().setSize(400, 200).setRank(5).setOverRank(5).setNumSketch(2)).solve(A)
+    assert np.all(np.isfinite(s.vectorS()))

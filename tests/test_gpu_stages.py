@@ -330,6 +330,30 @@ def test_former_fixture(gpu, kind, order):
     assert np.allclose((q.T @ a) @ V, (q.T @ U) * S, atol=1e-9 * S[0])
 
 
+def test_svd_former_wide_spectrum(gpu):
+    """The SVD former on W = Qbar'A with singular values spanning 1 .. 1e-6 (known exactly:
+    A = U diag(s) V', Qbar = U R with R orthogonal, so W = R' diag(s) V').  It takes the right
+    vectors from eig(Z'Z), Z = A'Qbar, and sigma_j = ||Z w_j||, which keeps sigma to 1e-9 here
+    (measured 4e-11 on the host with LAPACK's eigensolver; 4e-7 at 1e-7, and the squared form
+    breaks down below about 1e-8, where a QR-based SVD of W would still give ~eps sigma_1)."""
+    note("svd former wide spectrum")
+    m, n, k, p = 2000, 900, 24, 8
+    l = k + p
+    rng = np.random.default_rng(14)
+    s = np.logspace(0, -6, l)
+    U, _ = np.linalg.qr(rng.standard_normal((m, l)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, l)))
+    R, _ = np.linalg.qr(rng.standard_normal((l, l)))
+    A = (U * s) @ V.T
+    f = mc.SvdFormer(params(m, n, k, p, 1), True)
+    f.initialize()
+    f(mc.DeviceArray.from_numpy(np.asfortranarray(A)), mc.DeviceArray.from_numpy(U @ R))
+    S = f.vectorS()
+    note("  max rel sigma err %.3e" % np.max(np.abs(S - s[:k]) / s[:k]))
+    assert np.max(np.abs(S - s[:k]) / s[:k]) < 1e-9
+    assert np.abs(proj(f.matrixU()) - proj(U[:, :k])).max() < 1e-8
+
+
 # ------------------------------------------------------------------------------------------------
 def test_small_eigensolver_and_l64(gpu):
     """l = 64 (the largest supported): a pipeline through the stage API on a small problem,
