@@ -344,6 +344,43 @@ def c4_bench(mc, args):
     return out
 
 
+def solo_requests(mc, args):
+    """Serving-stream requests solved one at a time on one thread (no server), each with its own
+    solver, after one untimed solve of its shape: per-request wall time, stage times and
+    iterations (the latency floor of a request, and a profile target without concurrent
+    threads)."""
+    from mcnla_amd import workloads
+    reqs = workloads.serving_stream(args.solo_requests)
+    gen = workloads.ReflectedGenerator(50000, 5000)
+    out = []
+    for r in reqs:
+        A = gen.make(r["m"], r["n"], workloads.stream_sigma(r), r["a_seed"])
+        P = mc.Parameters().setSize(r["m"], r["n"]).setRank(r["k"]).setOverRank(r["p"]).setNumSketch(r["N"])
+        s = mc.Solver(P, seed=r["omega_seed"], precision=mc.PREC_MIXED_BF16)
+        s.solve(A)
+        mc.synchronize()
+        t0 = time.perf_counter()
+        s.solve(A)
+        mc.synchronize()
+        dt = time.perf_counter() - t0
+        st, _ = s.stage_times()
+        out.append({"m": r["m"], "n": r["n"], "l": r["k"] + r["p"], "N": r["N"],
+                    "iterations": s.iteration(), "ms": round(dt * 1e3, 3),
+                    "stage_ms": [round(x * 1e3, 3) for x in st]})
+        del s
+        A.free()
+    gen.free()
+    ms = np.array([o["ms"] for o in out])
+    log("solo: %d requests, mean %.2f ms, p50 %.2f ms, p99 %.2f ms, mean iterations %.1f"
+        % (len(out), ms.mean(), np.percentile(ms, 50), np.percentile(ms, 99),
+           np.mean([o["iterations"] for o in out])))
+    return {"requests": len(out), "mean_ms": round(float(ms.mean()), 3),
+            "p50_ms": round(float(np.percentile(ms, 50)), 3),
+            "stage_ms_mean": [round(float(x), 3) for x in np.mean([o["stage_ms"] for o in out], axis=0)],
+            "mean_iterations": round(float(np.mean([o["iterations"] for o in out])), 2),
+            "per_request": out}
+
+
 def tune(solver, A, args, name):
     """Runs the autotuner on this solver (untimed set-up) unless --autotune 0; returns its plan."""
     if args.autotune:
@@ -387,6 +424,8 @@ def main():
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--autotune", type=int, default=1, help="tune each solver's split-K plans "
                     "before its warm-up (untimed); 0 keeps the default plans")
+    ap.add_argument("--solo-requests", type=int, default=0, help="solve this many serving-stream "
+                    "requests one at a time (no server) and print their record alone")
     ap.add_argument("--only-c4", action="store_true", help="run the row-block Wen-Yin workload "
                     "alone and print its record (for profiling)")
     args = ap.parse_args()
@@ -418,6 +457,9 @@ def main():
         return
     if args.only_c4:
         print(json.dumps({"c4": c4_bench(mc, args)}), flush=True)
+        return
+    if args.solo_requests:
+        print(json.dumps({"solo": solo_requests(mc, args)}), flush=True)
         return
     m, n, k, p, N = args.m, args.n, args.k, args.p, args.N
     l = k + p
