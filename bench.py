@@ -198,7 +198,7 @@ def serve_bench(mc, args):
             ring.give(("alone", i))
     log("serve: %d requests solved alone for the comparison" % len(check_idx))
 
-    with mc.Server(slots=args.serve_slots, **kw) as srv:
+    with mc.Server(slots=args.serve_slots, batch_max=args.serve_batch, **kw) as srv:
         mats = [make(r, ("burst", i)) for i, r in enumerate(burst)]
         mc.lib().mcnla_stream_synchronize(gen.stream)  # every burst matrix written
         t0 = time.perf_counter()
@@ -209,7 +209,16 @@ def serve_bench(mc, args):
         for i in range(len(burst)):
             ring.give(("burst", i))
         log("serve: capacity %.1f solves/s (closed-loop burst of %d)" % (cap, len(burst)))
-        rate = 0.75 * cap
+        if not stream:
+            st = srv.stats()
+            gen.free()
+            ring.free()
+            return {"capacity_solves_per_s": round(cap, 2), "slots": args.serve_slots,
+                    "batch_max": args.serve_batch, "admissions": st["batches"],
+            "kn_group_mean": round(st["kn_group_mean"], 2),
+            "kn_chunk_ms": round(st["kn_chunk_mean_s"] * 1e3, 3),
+                    "max_running": st["max_running"]}
+        rate = args.serve_rate if args.serve_rate > 0 else 0.75 * cap
         if len(stream) > rate * args.serve_seconds:  # keep the open-loop phase bounded
             stream = stream[:max(64, int(rate * args.serve_seconds))]
             log("serve: stream cut to %d requests (%.0f s at %.1f/s)"
@@ -220,13 +229,17 @@ def serve_bench(mc, args):
         id_to_idx = {}
         lock = threading.Lock()
 
+        queue_s, run_s = [], []
+
         def collect():
             for _ in range(len(stream)):
                 rid = srv.next_done()
                 with lock:
                     i = id_to_idx.get(rid)
+                r = srv.result(rid)
+                queue_s.append(r.queue_s)
+                run_s.append(r.run_s)
                 if i in alone:
-                    r = srv.result(rid)
                     got = (r.S.numpy(), r.Qbar.numpy(), r.iterations)
                     ok = all(np.array_equal(x, y) for x, y in zip(got[:2], alone[i][:2]))
                     ok = ok and got[2] == alone[i][2]
@@ -281,8 +294,14 @@ def serve_bench(mc, args):
             "offered_per_s": round(len(stream) / submit_span, 2), "late_arrivals": late,
             "capacity_solves_per_s": round(cap, 2), "solves_per_s": round(st["completed"] / st["span_s"], 2),
             "p50_ms": round(st["p50_s"] * 1e3, 3), "p99_ms": round(st["p99_s"] * 1e3, 3),
+            "queue_p50_ms": round(float(np.percentile(queue_s, 50)) * 1e3, 3),
+            "run_p50_ms": round(float(np.percentile(run_s, 50)) * 1e3, 3),
+            "run_p99_ms": round(float(np.percentile(run_s, 99)) * 1e3, 3),
             "mean_ms": round(st["mean_s"] * 1e3, 3), "max_ms": round(st["max_s"] * 1e3, 3),
             "completed": st["completed"], "failed": st["failed"], "slots": args.serve_slots,
+            "batch_max": args.serve_batch, "admissions": st["batches"],
+            "kn_group_mean": round(st["kn_group_mean"], 2),
+            "kn_chunk_ms": round(st["kn_chunk_mean_s"] * 1e3, 3),
             "max_running": st["max_running"], "hbm_budget_gb": args.serve_budget_gb,
             "peak_gb": round(st["peak_bytes"] / 2 ** 30, 2), "precision": "bf16 sketch, f64 rest",
             "alone_vs_mix_identical": "%d/%d" % (matched[0], matched[0] + mismatched[0]),
@@ -416,7 +435,13 @@ def main():
                     help="device memory for the client's per-request matrices")
     ap.add_argument("--serve-ahead", type=int, default=64,
                     help="request matrices the client writes ahead of the arrival clock")
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="Poisson arrival rate of the "
+                    "open-loop phase (requests/s); 0: 75%% of the measured capacity")
+    ap.add_argument("--serve-batch", type=int, default=1, help="KN requests a serving worker "
+                    "admits at once and solves as one batch (their iterations in lockstep)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--only-serve", action="store_true", help="run the serving workload alone "
+                    "and print its record")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0)
     ap.add_argument("--c4-steps", type=int, default=3)
     ap.add_argument("--graph", type=int, default=0, help="replay the integrator iterations as "
@@ -429,12 +454,14 @@ def main():
     ap.add_argument("--only-c4", action="store_true", help="run the row-block Wen-Yin workload "
                     "alone and print its record (for profiling)")
     args = ap.parse_args()
-    # The serving phase runs 2 streams per slot (a solve and its side stream); HIP maps streams
+    # The serving phase runs 2 streams per slot (a solve and its side stream; 3 with a KN group:
+    # the group's iterations, the members' other stages and the side stream); HIP maps streams
     # onto GPU_MAX_HW_QUEUES hardware queues per process (4 by default), and streams that share a
     # queue run their kernels one after another.  Before the runtime starts, give the process one
-    # queue per stream of the default 8 slots.
-    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 2 * args.serve_slots:
-        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, 2 * args.serve_slots))
+    # queue per stream of the serving slots (at most 32).
+    queues = (3 if args.serve_batch > 1 else 2) * args.serve_slots
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < queues:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, queues))
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the iSVD path here is built for one MI355X "
               "(multi-GPU sharding is a design note, not built); rerun with --gpus 1",
@@ -460,6 +487,9 @@ def main():
         return
     if args.solo_requests:
         print(json.dumps({"solo": solo_requests(mc, args)}), flush=True)
+        return
+    if args.only_serve:
+        print(json.dumps({"serve": serve_bench(mc, args)}), flush=True)
         return
     m, n, k, p, N = args.m, args.n, args.k, args.p, args.N
     l = k + p

@@ -278,6 +278,22 @@ mcnla_status mcnla_solver_required_bytes(const mcnla_params* params,
 mcnla_status mcnla_solver_set_seed(mcnla_solver s, uint64_t seed);
 mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
                                 int a_order, void* stream);
+/* Up to MCNLA_MAX_BATCH fused solves on one stream, the KN iterations of all of them in lockstep:
+ * every iteration is one launch per kernel for the whole batch (not one per solve), and the host
+ * reads the batch's convergence flags once per chunk of iterations.  Solver i solves A[i] (lda,
+ * dtype, order as in mcnla_solver_solve); its results, iteration count and status[i] are
+ * bit-identical to mcnla_solver_solve(solvers[i], ...) alone.  The solvers may differ in shape,
+ * precision and configuration; Wen-Yin and combining integrators run one after another inside
+ * the batch.  The call returns MCNLA_OK once every member has finished (each member's outcome is
+ * in status[]; the error text is the first failing member's) or an error for invalid arguments,
+ * before anything is enqueued.  Stage and kernel times of batched solves read zero; iteration
+ * graphs (cfg.graph) are not used.  No reference counterpart: the reference runs one solve per
+ * MPI job (src/isvd_dense_cb.cpp:46-302); this is for servers that hold many small requests. */
+#define MCNLA_MAX_BATCH 16
+mcnla_status mcnla_solver_solve_batch(int count, const mcnla_solver* solvers,
+                                      const void* const* A, const int64_t* lda,
+                                      const int* a_dtype, const int* a_order,
+                                      mcnla_status* status, void* stream);
 mcnla_status mcnla_solver_results(mcnla_solver s, const double** S, const double** U,
                                   const double** V, const double** Qbar);
 mcnla_status mcnla_solver_iteration(mcnla_solver s, int64_t* iteration);
@@ -362,8 +378,10 @@ mcnla_status mcnla_tune_load(const char* path);
  * and memory free up; up to `max_bypass` later requests may be admitted ahead of a head request
  * that does not fit yet.  Each worker owns one HIP stream, so admitted solves overlap on the
  * device.  A finished request releases its workspaces at once and keeps only its results (S, U,
- * V, Qbar) in the arena until mcnla_server_release.  A solve is deterministic: its results do
- * not depend on what else runs. */
+ * V, Qbar) in the arena until mcnla_server_release.  With batch_max > 1 a free worker admits the
+ * queue head together with up to batch_max - 1 requests behind it (under the same bypass rule)
+ * and runs their KN iterations in lockstep, one launch per kernel per iteration for all of them.
+ * A solve is deterministic: its results do not depend on what else runs, alone or batched. */
 typedef struct mcnla_server_s* mcnla_server;
 
 typedef struct {
@@ -372,6 +390,9 @@ typedef struct {
   int slots;                  /* concurrent solves, each with a host thread and a stream; 1..16 */
   size_t hbm_budget;          /* bytes of the device arena */
   int max_bypass;             /* later requests admitted ahead of a blocked head, at most */
+  int batch_max;              /* KN requests a worker admits at once and solves as one batch
+                                 (mcnla_solver_solve_batch), 1..MCNLA_MAX_BATCH; 1 = one at a
+                                 time.  Results do not depend on it. */
 } mcnla_server_config;
 
 typedef struct {
@@ -406,6 +427,10 @@ typedef struct {
   size_t budget_bytes;
   double p50_s, p99_s, mean_s, max_s; /* latency (submit -> results ready) of completed requests */
   double span_s;         /* first submit -> last completion */
+  uint64_t batches;      /* admissions: each admits one request or a batch of them */
+  uint64_t kn_chunks;    /* KN-group chunks run (batch_max > 1) */
+  double kn_group_mean;  /* members per KN-group chunk, on average */
+  double kn_chunk_mean_s;/* wall time of a KN-group chunk (its iterations and read-back) */
 } mcnla_server_stats;
 
 void mcnla_server_config_default(mcnla_server_config* cfg);

@@ -17,8 +17,8 @@ from .isvd import (
     RowBlockExtrinsicMeanIntegrator, RowBlockGramianFormer, RowBlockGramianKolmogorovNagumoIntegrator,
     RowBlockGramianOrthogonalizer, RowBlockGramianWenYinIntegrator,
     RowBlockKolmogorovNagumoIntegrator, RowBlockReductionIntegrator, RowBlockSymmetricFormer,
-    RowBlockWenYinIntegrator, Solver, SvdFormer, gaussian, synth_lowrank, tune_clear, tune_get,
-    tune_load, tune_save, tune_set, tuning_default,
+    RowBlockWenYinIntegrator, Solver, SvdFormer, gaussian, solve_batch, synth_lowrank, tune_clear,
+    tune_get, tune_load, tune_save, tune_set, tuning_default,
 )
 from .serve import Served, Server, required_bytes
 from . import workloads

@@ -30,6 +30,7 @@ ORTH_GRAMIAN, ORTH_ROW_BLOCK_GRAMIAN = 0, 1
 
 # MCNLA_STATIC_WORKSPACE_BYTES: the static device array mcnla_solve_static carves its workspace from
 STATIC_WORKSPACE_BYTES = 256 << 20
+MAX_BATCH = 16
 
 STATUS_NAMES = {
     OK: "OK", ERR_INVALID: "INVALID", ERR_NOMEM: "NOMEM", ERR_HIP: "HIP",
@@ -70,7 +71,7 @@ class Tuning(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("solver", SolverConfig), ("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
-                ("max_bypass", ctypes.c_int)]
+                ("max_bypass", ctypes.c_int), ("batch_max", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -93,7 +94,9 @@ class ServerStats(ctypes.Structure):
                 ("peak_bytes", ctypes.c_size_t), ("budget_bytes", ctypes.c_size_t),
                 ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double),
                 ("mean_s", ctypes.c_double), ("max_s", ctypes.c_double),
-                ("span_s", ctypes.c_double)]
+                ("span_s", ctypes.c_double), ("batches", ctypes.c_uint64),
+                ("kn_chunks", ctypes.c_uint64), ("kn_group_mean", ctypes.c_double),
+                ("kn_chunk_mean_s", ctypes.c_double)]
 
 
 # Every exported symbol and its signature; tests check the library exports exactly these.
@@ -156,6 +159,11 @@ SIGNATURES = {
     "mcnla_solver_workspace_bytes": (_st, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "mcnla_solver_set_seed": (_st, [_vp, _u64]),
     "mcnla_solver_solve": (_st, [_vp, _vp, _i64, _i, _i, _vp]),
+    "mcnla_solver_solve_batch": (_st, [_i, ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       _vp]),
     "mcnla_solver_results": (_st, [_vp, _pp, _pp, _pp, _pp]),
     "mcnla_solver_iteration": (_st, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "mcnla_solver_stage_times": (_st, [_vp, _dp, _dp]),

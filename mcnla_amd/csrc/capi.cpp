@@ -982,6 +982,333 @@ void SolveRes::destroy() {
 
 using namespace mcnla;
 
+namespace mcnla {
+
+void solve_check(mcnla_solver s, const void* A, int64_t lda, int a_dtype, int a_order) {
+  MCNLA_REQUIRE(s, MCNLA_ERR_INVALID, "solver is null");
+  check_a(s->prm, A, lda, a_dtype, a_order);
+}
+
+void solve_begin(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, int a_order,
+                 hipStream_t st, SolveCtx& cx, bool timed) {
+  const auto& p = s->prm;
+  const auto& c = s->cfg;
+  const int l = static_cast<int>(p.rank + p.over_rank);
+  const int64_t N = p.num_sketch, nl = l * N, m = p.nrow;
+  s->bind_res();
+  if (timed) s->timer.start(st);
+  cx.ldar = p.ncol;
+  cx.rotate = !is_combining(c.integrator);
+  const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
+  const int64_t n = p.ncol;
+  // 1. sketch: Y = A * Omega; on the Gram-of-A route, H = A'A (A as row-major FP64) instead
+  s->sketch.generate(p, c.seed, c.omega_mode, c.omega_threads,
+                     col_block_chain(c.sketcher), st);
+  if (!via_a) {
+    s->sketch.project(p, c.precision, A, lda, a_dtype, a_order, s->Y.p, st, true);
+  } else {
+    if (a_dtype == MCNLA_F64 && a_order == MCNLA_ROW_MAJOR && lda == n) {
+      cx.Ar = static_cast<const double*>(A);
+    } else {
+      to_rowmajor_f64(A, lda, a_dtype == MCNLA_F32, a_order == MCNLA_COL_MAJOR, m, n,
+                      s->A64.as<double>(), st);
+      cx.Ar = s->A64.as<double>();
+    }
+    s->gram.run(cx.Ar, false, m, n, s->H.as<double>(), st);
+  }
+  if (timed) s->timer.mark(st);
+  // 2. orthogonalize through the Gram matrix: G = Y'Y, T_i from G_ii, Bs = T' G T.  The
+  // reference's T_i = V_i S_i^{-1/2} (eig of G_ii) needs a Jacobi eigen-solve per block; for
+  // the KN and WY integrators, which are equivariant in the basis of the first sketch (Qbar
+  // turns into Qbar R when Q_0 turns into Q_0 R) and depend on the other sketches only through
+  // their spans, T_i = L_i^{-T} (Cholesky, G_ii = L_i L_i') gives the same subspaces, and the
+  // reference's basis is restored at the end by W_0 = L_0' V_0 S_0^{-1/2} = Q_0' Q_0^ref, with
+  // the one eigen-solve it needs running on a second stream beside the integrator.  The
+  // combining integrators depend on every basis and keep the eigen transforms.
+  if (!via_a) {
+    s->gram.run(s->Y.p, false, m, nl, s->G.as<double>(), st);
+  } else {  // G = Omega' (H Omega)
+    GemmF64Args hp = plan_h_omega(n, nl, s->tune.h_omega);
+    hp.A = s->H.p;
+    hp.B = s->sketch.omega.p;
+    hp.C = s->P.as<double>();
+    hp.partial = s->gp.as<double>();
+    gemm_f64(hp, st);
+    GemmF64Args pg = plan_omega_p(n, nl, s->tune.omega_p);
+    pg.A = s->sketch.omega.p;
+    pg.B = s->P.p;
+    pg.C = s->G.as<double>();
+    pg.partial = s->gp.as<double>();
+    gemm_f64(pg, st);
+  }
+  gram_diag_blocks(s->G.as<double>(), nl, s->W.as<double>(), N, l, st);
+  MCNLA_HIP(hipMemsetAsync(s->bad.p, 0, sizeof(int), st));
+  if (cx.rotate) {
+    hipStream_t side = s->res->side;
+    MCNLA_HIP(hipEventRecord(s->res->fork, st));
+    MCNLA_HIP(hipStreamWaitEvent(side, s->res->fork, 0));
+    batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), 1, l, side);
+    orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T0.as<double>(), 1, l,
+                    s->bad.as<int>(), side);
+    MCNLA_HIP(hipEventRecord(s->res->join, side));
+    cholesky_transforms(s->W.as<double>(), N, l, s->T.as<double>(), s->L0.as<double>(),
+                        s->bad.as<int>(), st);
+  } else {
+    batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), N, l, st);
+    orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T.as<double>(), N, l,
+                    s->bad.as<int>(), st);
+  }
+  congruence_blockdiag(s->G.as<double>(), s->T.as<double>(), s->Bs.as<double>(), N, l, st);
+  if (timed) s->timer.mark(st);
+}
+
+void solve_finish(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, int a_order,
+                  hipStream_t st, const SolveCtx& cx, bool timed) {
+  const auto& p = s->prm;
+  const auto& c = s->cfg;
+  const int l = static_cast<int>(p.rank + p.over_rank);
+  const int64_t N = p.num_sketch, nl = l * N, m = p.nrow, n = p.ncol;
+  const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
+  (void)N;
+  const double* cf = s->coef.as<double>();
+  if (cx.rotate) {
+    MCNLA_HIP(hipStreamWaitEvent(st, s->res->join, 0));
+    GemmF64Args wr;  // W_0 = L_0' T_0^ref
+    wr.M = l;
+    wr.N = l;
+    wr.K = l;
+    wr.A = s->L0.p;
+    wr.lda = l;
+    wr.a_mcontig = true;
+    wr.B = s->T0.p;
+    wr.ldb = l;
+    wr.C = s->W0.as<double>();
+    wr.ldc = l;
+    wr.shape = GemmShape::kSmall;
+    gemm_f64(wr, st);
+    GemmF64Args cr;  // coef W_0 (Nl x l) into the scratch
+    cr.M = nl;
+    cr.N = l;
+    cr.K = l;
+    cr.A = s->coef.p;
+    cr.lda = l;
+    cr.a_mcontig = false;
+    cr.B = s->W0.p;
+    cr.ldb = l;
+    cr.C = s->tmp.as<double>();
+    cr.ldc = l;
+    cr.shape = tall_shape(l);
+    gemm_f64(cr, st);
+    cf = s->tmp.as<double>();
+  }
+  GemmF64Args tm;  // M2_i = T_i * coef_i  (batched l x l)
+  tm.M = l;
+  tm.N = l;
+  tm.K = l;
+  tm.A = s->T.p;
+  tm.lda = l;
+  tm.a_mcontig = false;
+  tm.B = cf;
+  tm.ldb = l;
+  tm.C = s->M2.as<double>();
+  tm.ldc = l;
+  tm.shape = GemmShape::kSmall;
+  tm.batch = N;
+  tm.stride_a = static_cast<int64_t>(l) * l;
+  tm.stride_b = static_cast<int64_t>(l) * l;
+  tm.stride_c = static_cast<int64_t>(l) * l;
+  gemm_f64(tm, st);
+  GemmF64Args q;
+  q.M = m;
+  q.N = l;
+  q.K = nl;
+  q.A = s->Y.p;
+  q.lda = nl;
+  q.a_mcontig = false;
+  q.a_f32 = false;
+  q.B = s->M2.p;
+  q.ldb = l;
+  q.C = s->Qbar.as<double>();
+  q.ldc = l;
+  q.shape = tall_shape(l);
+  if (!via_a) {
+    q.splits = qbar_splits(m, nl, s->tune.qbar);
+    if (q.splits > 1) {
+      q.kchunk = ceil_div(ceil_div(nl, q.splits), 16) * 16;
+      q.partial = s->gram.partial.as<double>();
+    }
+  }
+  if (via_a) {  // X = Omega (T M) (n x l), then Qbar = A X
+    GemmF64Args x = q;
+    x.M = n;
+    x.A = s->sketch.omega.p;
+    x.C = s->X.as<double>();
+    x.splits = n_by_l_splits(n, nl, s->tune.n_by_l);
+    x.partial = s->gp.as<double>();
+    gemm_f64(x, st);
+    q.K = n;
+    q.B = s->X.p;
+    if (a_order == MCNLA_ROW_MAJOR) {  // stream the caller's A (FP32 reads half the bytes)
+      q.A = A;
+      q.a_f32 = a_dtype == MCNLA_F32;
+      // a mixed precision forms this basis of an FP32 A on FP32 MFMA (sigma come from Z = H X,
+      // which stays FP64)
+      q.f32_mfma = c.precision != MCNLA_PREC_F64;
+      q.lda = lda;
+    } else {
+      q.A = cx.Ar;
+      q.lda = cx.ldar;
+    }
+  }
+  gemm_f64(q, st);
+  if (timed) s->timer.mark(st);
+  // 4. form; on the Gram-of-A route Z = A'Qbar = H X without another pass over A
+  if (via_a) {
+    GemmF64Args z;
+    z.M = n;
+    z.N = l;
+    z.K = n;
+    z.A = s->H.p;
+    z.lda = n;
+    z.a_mcontig = false;
+    z.B = s->X.p;
+    z.ldb = l;
+    z.C = s->form.Z.as<double>();
+    z.ldc = l;
+    z.shape = tall_shape(l);
+    z.splits = n_by_l_splits(n, n, s->tune.n_by_l);
+    z.partial = s->gp.as<double>();
+    gemm_f64(z, st);
+  }
+  s->form.run(p, c.former, c.jobv != 0, A, lda, a_dtype, a_order, s->Qbar.as<double>(), st,
+              via_a, c.precision != MCNLA_PREC_F64);
+  if (timed) s->timer.mark(st);
+  s->computed = true;
+}
+
+void read_bad(mcnla_solver_s* s, int* bad, hipStream_t st) {
+  MCNLA_HIP(hipMemcpyAsync(bad, s->bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+}
+
+mcnla_status solve_status(const mcnla_solver_s* s, const SolveCtx& cx, int bad) {
+  if (bad || !cx.ok) {
+    set_error(bad ? "a sketch Gram matrix is not positive definite"
+                  : "non-finite value in the integrator's l x l step");
+    return MCNLA_ERR_NUMERIC;
+  }
+  if (!cx.conv && s->cfg.maxiter > 0) {
+    set_error("integrator reached maxiter without meeting the tolerance");
+    return MCNLA_ERR_NOT_CONVERGED;
+  }
+  return MCNLA_OK;
+}
+
+namespace {
+bool is_kn(int kind) { return !is_combining(kind) && !is_wen_yin(kind); }
+static_assert(MCNLA_MAX_BATCH <= kMaxKnGroup, "a batch's KN members form one group");
+}  // namespace
+
+bool solver_is_kn(mcnla_solver s) { return is_kn(s->cfg.integrator); }
+
+GramIntegrator solver_kn_integrator(mcnla_solver s) {
+  const auto& p = s->prm;
+  const auto& c = s->cfg;
+  const int l = static_cast<int>(p.rank + p.over_rank);
+  s->ws.ensure(integration_workspace(c.integrator, p.num_sketch, l));
+  GramIntegrator gi;
+  gi.N = p.num_sketch;
+  gi.l = l;
+  gi.maxiter = c.maxiter;
+  gi.tol = c.tol;
+  gi.Bs = s->Bs.as<double>();
+  gi.ws = s->ws.as<char>();
+  gi.ws_bytes = s->ws.bytes;
+  return gi;
+}
+
+double* solver_coef(mcnla_solver s) { return s->coef.as<double>(); }
+
+void solver_set_integration(mcnla_solver s, int64_t iterations, bool converged) {
+  s->iteration = iterations;
+  s->converged = converged;
+}
+
+}  // namespace mcnla
+
+namespace mcnla {
+
+void solve_batch(int count, const mcnla_solver* solvers, const void* const* A, const int64_t* lda,
+                 const int* a_dtype, const int* a_order, mcnla_status* status,
+                 std::vector<std::string>* errors, hipStream_t st) {
+  MCNLA_REQUIRE(count >= 1 && count <= MCNLA_MAX_BATCH, MCNLA_ERR_INVALID,
+                "a batch holds 1 to 16 solves");
+  MCNLA_REQUIRE(solvers && A && lda && a_dtype && a_order && status, MCNLA_ERR_INVALID,
+                "null argument");
+  for (int i = 0; i < count; ++i) {
+    MCNLA_REQUIRE(solvers[i], MCNLA_ERR_INVALID, "solver is null");
+    for (int j = 0; j < i; ++j)
+      MCNLA_REQUIRE(solvers[j] != solvers[i], MCNLA_ERR_INVALID,
+                    "a solver appears twice in the batch");
+    check_a(solvers[i]->prm, A[i], lda[i], a_dtype[i], a_order[i]);
+  }
+  std::vector<SolveCtx> cx(count);
+  for (int i = 0; i < count; ++i)
+    solve_begin(solvers[i], A[i], lda[i], a_dtype[i], a_order[i], st, cx[i], false);
+  // KN members in one group; the others one after another
+  std::vector<GramIntegrator> group;
+  std::vector<double*> coefs;
+  std::vector<int> who;
+  for (int i = 0; i < count; ++i) {
+    mcnla_solver_s* s = solvers[i];
+    const auto& p = s->prm;
+    const auto& c = s->cfg;
+    const int l = static_cast<int>(p.rank + p.over_rank);
+    if (is_kn(c.integrator)) {
+      group.push_back(solver_kn_integrator(s));
+      coefs.push_back(s->coef.as<double>());
+      who.push_back(i);
+    } else {
+      s->iteration = run_integration(c.integrator, s->Bs.as<double>(), p.num_sketch, l,
+                                     c.maxiter, c.tol, s->coef.as<double>(), s->ws, st,
+                                     &cx[i].conv, &cx[i].ok);
+      s->converged = cx[i].conv;
+    }
+  }
+  if (!group.empty()) {
+    const int g = static_cast<int>(group.size());
+    std::vector<int64_t> iters(g);
+    std::unique_ptr<bool[]> conv(new bool[g]), ok(new bool[g]);
+    integrate_gram_group(group.data(), coefs.data(), g, st, iters.data(), conv.get(), ok.get());
+    for (int k = 0; k < g; ++k) {
+      mcnla_solver_s* s = solvers[who[k]];
+      s->iteration = iters[k];
+      s->converged = conv[k];
+      cx[who[k]].conv = conv[k];
+      cx[who[k]].ok = ok[k];
+    }
+  }
+  for (int i = 0; i < count; ++i)
+    solve_finish(solvers[i], A[i], lda[i], a_dtype[i], a_order[i], st, cx[i], false);
+  std::vector<int> bad(count, 0);
+  for (int i = 0; i < count; ++i) read_bad(solvers[i], &bad[i], st);
+  MCNLA_HIP(hipStreamSynchronize(st));
+  if (errors) errors->assign(count, std::string());
+  std::string first_error;
+  for (int i = 0; i < count; ++i) {
+    // the members' GEMM timing events may have been re-recorded by later members
+    solvers[i]->sketch.recorded = false;
+    solvers[i]->gram.recorded = false;
+    status[i] = solve_status(solvers[i], cx[i], bad[i]);
+    if (status[i] != MCNLA_OK) {
+      if (errors) (*errors)[i] = get_error();
+      if (first_error.empty()) first_error = get_error();
+    }
+  }
+  set_error(first_error);
+}
+
+}  // namespace mcnla
+
 extern "C" {
 
 const char* mcnla_last_error(void) { return get_error(); }
@@ -1548,206 +1875,29 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
     const auto& p = s->prm;
     const auto& c = s->cfg;
     const int l = static_cast<int>(p.rank + p.over_rank);
-    const int64_t N = p.num_sketch, nl = l * N, m = p.nrow;
     hipStream_t st = S(stream);
-    s->bind_res();
-    s->timer.start(st);
-    const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
-    const int64_t n = p.ncol;
-    // 1. sketch: Y = A * Omega; on the Gram-of-A route, H = A'A (A as row-major FP64) instead
-    s->sketch.generate(p, c.seed, c.omega_mode, c.omega_threads,
-                       col_block_chain(c.sketcher), st);
-    const double* Ar = nullptr;
-    int64_t ldar = n;
-    if (!via_a) {
-      s->sketch.project(p, c.precision, A, lda, a_dtype, a_order, s->Y.p, st, true);
-    } else {
-      if (a_dtype == MCNLA_F64 && a_order == MCNLA_ROW_MAJOR && lda == n) {
-        Ar = static_cast<const double*>(A);
-      } else {
-        to_rowmajor_f64(A, lda, a_dtype == MCNLA_F32, a_order == MCNLA_COL_MAJOR, m, n,
-                        s->A64.as<double>(), st);
-        Ar = s->A64.as<double>();
-      }
-      s->gram.run(Ar, false, m, n, s->H.as<double>(), st);
-    }
-    s->timer.mark(st);
-    // 2. orthogonalize through the Gram matrix: G = Y'Y, T_i from G_ii, Bs = T' G T.  The
-    // reference's T_i = V_i S_i^{-1/2} (eig of G_ii) needs a Jacobi eigen-solve per block; for
-    // the KN and WY integrators, which are equivariant in the basis of the first sketch (Qbar
-    // turns into Qbar R when Q_0 turns into Q_0 R) and depend on the other sketches only through
-    // their spans, T_i = L_i^{-T} (Cholesky, G_ii = L_i L_i') gives the same subspaces, and the
-    // reference's basis is restored at the end by W_0 = L_0' V_0 S_0^{-1/2} = Q_0' Q_0^ref, with
-    // the one eigen-solve it needs running on a second stream beside the integrator.  The
-    // combining integrators depend on every basis and keep the eigen transforms.
-    if (!via_a) {
-      s->gram.run(s->Y.p, false, m, nl, s->G.as<double>(), st);
-    } else {  // G = Omega' (H Omega)
-      GemmF64Args hp = plan_h_omega(n, nl, s->tune.h_omega);
-      hp.A = s->H.p;
-      hp.B = s->sketch.omega.p;
-      hp.C = s->P.as<double>();
-      hp.partial = s->gp.as<double>();
-      gemm_f64(hp, st);
-      GemmF64Args pg = plan_omega_p(n, nl, s->tune.omega_p);
-      pg.A = s->sketch.omega.p;
-      pg.B = s->P.p;
-      pg.C = s->G.as<double>();
-      pg.partial = s->gp.as<double>();
-      gemm_f64(pg, st);
-    }
-    gram_diag_blocks(s->G.as<double>(), nl, s->W.as<double>(), N, l, st);
-    MCNLA_HIP(hipMemsetAsync(s->bad.p, 0, sizeof(int), st));
-    const bool rotate = !is_combining(c.integrator);
-    if (rotate) {
-      hipStream_t side = s->res->side;
-      MCNLA_HIP(hipEventRecord(s->res->fork, st));
-      MCNLA_HIP(hipStreamWaitEvent(side, s->res->fork, 0));
-      batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), 1, l, side);
-      orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T0.as<double>(), 1, l,
-                      s->bad.as<int>(), side);
-      MCNLA_HIP(hipEventRecord(s->res->join, side));
-      cholesky_transforms(s->W.as<double>(), N, l, s->T.as<double>(), s->L0.as<double>(),
-                          s->bad.as<int>(), st);
-    } else {
-      batched_syev(s->W.as<double>(), s->w.as<double>(), s->V.as<double>(), N, l, st);
-      orth_transforms(s->w.as<double>(), s->V.as<double>(), s->T.as<double>(), N, l,
-                      s->bad.as<int>(), st);
-    }
-    congruence_blockdiag(s->G.as<double>(), s->T.as<double>(), s->Bs.as<double>(), N, l, st);
-    s->timer.mark(st);
+    SolveCtx cx;
+    solve_begin(s, A, lda, a_dtype, a_order, st, cx, true);
     // 3. integrate on Bs, then Qbar = Y * (T * coef) in one pass over Y
-    bool conv = false, ok = true;
     if (c.graph && !s->graphs) s->graphs = std::make_unique<IterationGraphs>();
-    s->iteration = run_integration(c.integrator, s->Bs.as<double>(), N, l, c.maxiter, c.tol,
-                                   s->coef.as<double>(), s->ws, st, &conv, &ok, s->graphs.get());
-    s->converged = conv;
-    const double* cf = s->coef.as<double>();
-    if (rotate) {
-      MCNLA_HIP(hipStreamWaitEvent(st, s->res->join, 0));
-      GemmF64Args wr;  // W_0 = L_0' T_0^ref
-      wr.M = l;
-      wr.N = l;
-      wr.K = l;
-      wr.A = s->L0.p;
-      wr.lda = l;
-      wr.a_mcontig = true;
-      wr.B = s->T0.p;
-      wr.ldb = l;
-      wr.C = s->W0.as<double>();
-      wr.ldc = l;
-      wr.shape = GemmShape::kSmall;
-      gemm_f64(wr, st);
-      GemmF64Args cr;  // coef W_0 (Nl x l) into the scratch
-      cr.M = nl;
-      cr.N = l;
-      cr.K = l;
-      cr.A = s->coef.p;
-      cr.lda = l;
-      cr.a_mcontig = false;
-      cr.B = s->W0.p;
-      cr.ldb = l;
-      cr.C = s->tmp.as<double>();
-      cr.ldc = l;
-      cr.shape = tall_shape(l);
-      gemm_f64(cr, st);
-      cf = s->tmp.as<double>();
-    }
-    GemmF64Args tm;  // M2_i = T_i * coef_i  (batched l x l)
-    tm.M = l;
-    tm.N = l;
-    tm.K = l;
-    tm.A = s->T.p;
-    tm.lda = l;
-    tm.a_mcontig = false;
-    tm.B = cf;
-    tm.ldb = l;
-    tm.C = s->M2.as<double>();
-    tm.ldc = l;
-    tm.shape = GemmShape::kSmall;
-    tm.batch = N;
-    tm.stride_a = static_cast<int64_t>(l) * l;
-    tm.stride_b = static_cast<int64_t>(l) * l;
-    tm.stride_c = static_cast<int64_t>(l) * l;
-    gemm_f64(tm, st);
-    GemmF64Args q;
-    q.M = m;
-    q.N = l;
-    q.K = nl;
-    q.A = s->Y.p;
-    q.lda = nl;
-    q.a_mcontig = false;
-    q.a_f32 = false;
-    q.B = s->M2.p;
-    q.ldb = l;
-    q.C = s->Qbar.as<double>();
-    q.ldc = l;
-    q.shape = tall_shape(l);
-    if (!via_a) {
-      q.splits = qbar_splits(m, nl, s->tune.qbar);
-      if (q.splits > 1) {
-        q.kchunk = ceil_div(ceil_div(nl, q.splits), 16) * 16;
-        q.partial = s->gram.partial.as<double>();
-      }
-    }
-    if (via_a) {  // X = Omega (T M) (n x l), then Qbar = A X
-      GemmF64Args x = q;
-      x.M = n;
-      x.A = s->sketch.omega.p;
-      x.C = s->X.as<double>();
-      x.splits = n_by_l_splits(n, nl, s->tune.n_by_l);
-      x.partial = s->gp.as<double>();
-      gemm_f64(x, st);
-      q.K = n;
-      q.B = s->X.p;
-      if (a_order == MCNLA_ROW_MAJOR) {  // stream the caller's A (FP32 reads half the bytes)
-        q.A = A;
-        q.a_f32 = a_dtype == MCNLA_F32;
-        // a mixed precision forms this basis of an FP32 A on FP32 MFMA (sigma come from Z = H X,
-        // which stays FP64)
-        q.f32_mfma = c.precision != MCNLA_PREC_F64;
-        q.lda = lda;
-      } else {
-        q.A = Ar;
-        q.lda = ldar;
-      }
-    }
-    gemm_f64(q, st);
-    s->timer.mark(st);
-    // 4. form; on the Gram-of-A route Z = A'Qbar = H X without another pass over A
-    if (via_a) {
-      GemmF64Args z;
-      z.M = n;
-      z.N = l;
-      z.K = n;
-      z.A = s->H.p;
-      z.lda = n;
-      z.a_mcontig = false;
-      z.B = s->X.p;
-      z.ldb = l;
-      z.C = s->form.Z.as<double>();
-      z.ldc = l;
-      z.shape = tall_shape(l);
-      z.splits = n_by_l_splits(n, n, s->tune.n_by_l);
-      z.partial = s->gp.as<double>();
-      gemm_f64(z, st);
-    }
-    s->form.run(p, c.former, c.jobv != 0, A, lda, a_dtype, a_order, s->Qbar.as<double>(), st,
-                via_a, c.precision != MCNLA_PREC_F64);
-    s->timer.mark(st);
-    s->computed = true;
+    s->iteration = run_integration(c.integrator, s->Bs.as<double>(), p.num_sketch, l, c.maxiter,
+                                   c.tol, s->coef.as<double>(), s->ws, st, &cx.conv, &cx.ok,
+                                   s->graphs.get());
+    s->converged = cx.conv;
+    solve_finish(s, A, lda, a_dtype, a_order, st, cx, true);
     int bad = 0;
-    MCNLA_HIP(hipMemcpyAsync(&bad, s->bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    read_bad(s, &bad, st);
     MCNLA_HIP(hipStreamSynchronize(st));
-    if (bad || !ok) {
-      set_error(bad ? "a sketch Gram matrix is not positive definite"
-                    : "non-finite value in the integrator's l x l step");
-      return MCNLA_ERR_NUMERIC;
-    }
-    if (!conv && c.maxiter > 0) {
-      set_error("integrator reached maxiter without meeting the tolerance");
-      return MCNLA_ERR_NOT_CONVERGED;
-    }
+    return solve_status(s, cx, bad);
+  });
+}
+
+mcnla_status mcnla_solver_solve_batch(int count, const mcnla_solver* solvers,
+                                      const void* const* A, const int64_t* lda,
+                                      const int* a_dtype, const int* a_order,
+                                      mcnla_status* status, void* stream) {
+  return guard([&] {
+    solve_batch(count, solvers, A, lda, a_dtype, a_order, status, nullptr, S(stream));
     return MCNLA_OK;
   });
 }

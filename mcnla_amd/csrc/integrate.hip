@@ -13,7 +13,9 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "small.hpp"
+#include "tall_direct.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <vector>
 
@@ -283,11 +285,12 @@ __device__ void kn_step_ns(int t, IntegratorState* st, int l, double invN, doubl
 constexpr int kStepThreads = 512;
 constexpr int kStepRows = 64;
 
-__global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
+// Workgroup `blk` of nblk (kStepThreads threads).
+__device__ __forceinline__ void kn_reduce_step_body(
     int t, IntegratorState* st, int64_t nl, int l, double invN, double tol,
     const double* __restrict__ Pbg, int64_t splits, const double* __restrict__ B,
     double* __restrict__ Bg, double* __restrict__ P, int64_t nblk, double* __restrict__ F,
-    double* __restrict__ Cinv) {
+    double* __restrict__ Cinv, int64_t blk) {
   if (skip(st, t)) return;
   constexpr int kBuf = kMaxL * (kMaxL + 1);
   static_assert(2 * kStepRows * (kMaxL + 1) <= 4 * kBuf, "row blocks fit the step buffers");
@@ -298,7 +301,7 @@ __global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
   const int ldr = kMaxL + 1;
   double* sB = smem;
   double* sG = smem + kStepRows * ldr;
-  const int64_t r0 = blockIdx.x * static_cast<int64_t>(kStepRows);
+  const int64_t r0 = blk * kStepRows;
   const int64_t per = nl * l;
   // elements e and e + kStepThreads together, so two sums' loads are in flight at once
   for (int e = tid; e < kStepRows * l; e += 2 * kStepThreads) {
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
   }
   __syncthreads();
   const int ll = l * l;
-  double* Pb = P + blockIdx.x * static_cast<int64_t>(2 * ll);
+  double* Pb = P + blk * (2 * ll);
   {  // [B'B, B'G] over the block's rows on f64 MFMA tiles: 2 (np/16)^2 tiles over the waves
     constexpr int NW = kStepThreads / kWave;
     const int wave = tid / kWave, lane = tid % kWave;
@@ -357,12 +360,15 @@ __global__ __launch_bounds__(kStepThreads) void kn_reduce_step_kernel(
   kn_step_ns<kStepThreads>(t, st, l, invN, tol, P, nblk, F, Cinv, smem, red);
 }
 
-// B+ = Bc F + Bg Cinv; E+ = Ec F + invN Bc Cinv; FF+ = FFc F (block 0).
-__global__ __launch_bounds__(kThreads) void kn_update_kernel(
+// B+ = Bc F + Bg Cinv; E+ = Ec F + invN Bc Cinv; FF+ = FFc F (block 0).  Workgroup `blk` of
+// nblocks (kThreads threads), a grid-stride loop over the elements: every element is one dot
+// product in a fixed order, whatever the grid.
+__device__ __forceinline__ void kn_update_body(
     int t, const IntegratorState* st, int64_t nl, int l, double invN,
     const double* __restrict__ Bc, const double* __restrict__ Bg, double* __restrict__ Bp,
     const double* __restrict__ Ec, double* __restrict__ Ep, const double* __restrict__ FFc,
-    double* __restrict__ FFp, const double* __restrict__ F, const double* __restrict__ Cinv) {
+    double* __restrict__ FFp, const double* __restrict__ F, const double* __restrict__ Cinv,
+    int64_t blk, int64_t nblocks) {
   if (skip(st, t)) return;
   __shared__ double sF[kMaxL * kMaxL];
   __shared__ double sC[kMaxL * kMaxL];
@@ -372,8 +378,7 @@ __global__ __launch_bounds__(kThreads) void kn_update_kernel(
   }
   __syncthreads();
   const int64_t total = nl * l;
-  for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * kThreads) {
+  for (int64_t e = blk * kThreads + threadIdx.x; e < total; e += nblocks * kThreads) {
     const int64_t r = e / l;
     const int j = e % l;
     const double* bc = Bc + r * l;
@@ -388,7 +393,7 @@ __global__ __launch_bounds__(kThreads) void kn_update_kernel(
     Bp[e] = b;
     Ep[e] = ee;
   }
-  if (blockIdx.x == 0) {
+  if (blk == 0) {
     for (int e = threadIdx.x; e < l * l; e += kThreads) {
       const int i = e / l, j = e % l;
       double acc = 0.0;
@@ -396,6 +401,88 @@ __global__ __launch_bounds__(kThreads) void kn_update_kernel(
       FFp[e] = acc;
     }
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// KN iterations of several solves per launch.  A group's launch runs workgroup x of member y
+// (blockIdx.y) on that member's own buffers, shapes, state and iteration index, with the
+// workgroup -> rows/tiles map the member would have alone, so a member's arithmetic (and result,
+// bit for bit) does not depend on the group it runs in or on when it joined.  A solve alone is a
+// group of one.  The member table is a
+// by-value kernel argument (read through the kernel-argument segment), so a launch needs no
+// device memory of its own and can be recorded into a HIP graph as is.
+
+struct KnMember {
+  const double* Bs;   // Nl x Nl
+  double* Pbg;        // bs_splits split-K partials of Bs * B
+  double* B[2];
+  double* Bg;
+  double* E[2];
+  double* FF[2];
+  double* P;          // block partials of the step
+  double* F;
+  double* C;          // Cinv
+  IntegratorState* st;
+  int64_t nl, kchunk, bs_splits, bs_blocks, nblk_s, upd_blocks;
+  double invN, tol;
+  int l, vec;
+  int t;              // the member's iteration index in this launch
+};
+
+struct KnGroup {
+  int count;
+  KnMember m[kMaxKnGroup];
+};
+static_assert(sizeof(KnGroup) <= 4096, "the member table fits the kernel arguments");
+
+// Bg partials of iteration t: Pbg[s] = Bs(:, K range s) * Bc(K range s, :) on the tall-skinny
+// FP64-MFMA tiles (the alpha = 1/N scaling is applied where the partials are summed).
+template <int LP>
+__global__ __launch_bounds__(256, LP == 32 ? 3 : 2) void kn_bs_group_kernel(KnGroup g) {
+  const KnMember& m = g.m[blockIdx.y];
+  const int t = m.t;
+  if (blockIdx.x >= m.bs_blocks || skip(m.st, t)) return;
+  GemmF64Args p;
+  p.M = m.nl;
+  p.N = m.l;
+  p.K = m.nl;
+  p.A = m.Bs;
+  p.lda = m.nl;
+  p.a_mcontig = false;
+  p.B = m.B[t & 1];
+  p.ldb = m.l;
+  p.splits = m.bs_splits;
+  p.kchunk = m.kchunk;
+  p.partial = m.Pbg;
+  if (m.vec)
+    tall_direct_tile<double, double, LP, true>(p, blockIdx.x);
+  else
+    tall_direct_tile<double, double, LP, false>(p, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kStepThreads) void kn_reduce_step_group_kernel(KnGroup g) {
+  const KnMember& m = g.m[blockIdx.y];
+  const int t = m.t;
+  if (blockIdx.x >= m.nblk_s) return;
+  kn_reduce_step_body(t, m.st, m.nl, m.l, m.invN, m.tol, m.Pbg, m.bs_splits, m.B[t & 1], m.Bg,
+                      m.P, m.nblk_s, m.F, m.C, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kThreads) void kn_update_group_kernel(KnGroup g) {
+  const KnMember& m = g.m[blockIdx.y];
+  const int t = m.t;
+  if (blockIdx.x >= m.upd_blocks) return;
+  const int c = t & 1, p = c ^ 1;
+  kn_update_body(t, m.st, m.nl, m.l, m.invN, m.B[c], m.Bg, m.B[p], m.E[c], m.E[p], m.FF[c],
+                 m.FF[p], m.F, m.C, blockIdx.x, m.upd_blocks);
+}
+
+// out[2i] = converged_at, out[2i + 1] = nan_seen of member i (one read-back for the group).
+__global__ void kn_flags_kernel(KnGroup g, int* out) {
+  const int i = threadIdx.x;
+  if (i >= g.count) return;
+  out[2 * i] = g.m[i].st->converged_at;
+  out[2 * i + 1] = g.m[i].st->nan_seen;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1021,6 +1108,7 @@ struct Layout {
   double* P;       // partial dot products
   double* Pbg;     // split-K partials of the Bs products
   double* T;       // reduced WY terms, 3 l^2 + 2
+  int* flags;      // a KN group's read-back of [converged_at, nan_seen] per member
   IntegratorState* st;
   size_t bytes;
 };
@@ -1061,6 +1149,7 @@ Layout layout(char* base, int64_t N, int l) {
   L.P = take(pp > pt ? pp : pt);
   L.Pbg = take(static_cast<size_t>(L.splits + 1) * nll);
   L.T = take(3 * ll + 2);
+  L.flags = reinterpret_cast<int*>(take(kMaxKnGroup));  // 2 kMaxKnGroup ints
   L.st = reinterpret_cast<IntegratorState*>(base ? base + off : nullptr);
   off += 256;
   L.bytes = off;
@@ -1076,73 +1165,379 @@ IterationGraphs::~IterationGraphs() {
   if (cap) (void)hipStreamDestroy(cap);
 }
 
+namespace {
+
+// Bout = Bs * Bin / N (Bout null: only the split-K partials into L.Pbg), gated on t, through the
+// GEMM dispatch (the Wen-Yin iterations and the final polish).
+void bs_product(const GramIntegrator& g, const Layout& L, const double* Bin, double* Bout, int t,
+                hipStream_t st) {
+  const int64_t nl = L.nl;
+  GemmF64Args p;
+  p.M = nl;
+  p.N = g.l;
+  p.K = nl;
+  p.A = g.Bs;
+  p.lda = nl;
+  p.a_mcontig = false;
+  p.B = Bin;
+  p.ldb = g.l;
+  p.C = Bout;
+  p.ldc = g.l;
+  p.alpha = 1.0 / static_cast<double>(g.N);
+  p.shape = g.l <= 32 ? GemmShape::kTallN32 : GemmShape::kTallN64;
+  p.splits = L.splits;
+  p.kchunk = ceil_div(ceil_div(nl, L.splits), 16) * 16;
+  p.partial = L.Pbg;
+  p.gate = &L.st->converged_at;
+  p.gate_t = t;
+  gemm_f64(p, st);
+}
+
+void check_integrator(const GramIntegrator& g, const Layout& L) {
+  MCNLA_REQUIRE(g.l >= 1 && g.l <= kMaxL, MCNLA_ERR_INVALID, "integrator needs 1 <= l <= 64");
+  MCNLA_REQUIRE(g.N >= 1, MCNLA_ERR_INVALID, "integrator needs N >= 1");
+  MCNLA_REQUIRE(g.ws_bytes >= L.bytes, MCNLA_ERR_INVALID, "integrator workspace too small");
+}
+
+// State reset, B0 = Bs(:, 0:l), E0 = 0, FF0 = I.
+void init_iteration(const GramIntegrator& g, const Layout& L, hipStream_t st) {
+  hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(1), 0, st, L.st);
+  MCNLA_LAUNCHED();
+  hipLaunchKernelGGL(init_coef_kernel, dim3(grid_elems(L.nl * g.l)), dim3(kThreads), 0, st, g.Bs,
+                     L.nl, g.l, L.B[0], L.E[0], L.FF[0]);
+  MCNLA_LAUNCHED();
+}
+
+// The coefficients after `iters` iterations, re-orthonormalized: coef = M W with
+// M = E + [FF; 0] and W = (M' Bs M)^{-1/2}.
+void finish_iteration(const GramIntegrator& g, const Layout& L, int64_t iters, double* coef,
+                      hipStream_t st) {
+  const int64_t nl = L.nl;
+  const int l = g.l;
+  const int fin = static_cast<int>(iters & 1);
+  hipLaunchKernelGGL(coef_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, L.E[fin],
+                     L.FF[fin], nl, l, L.U1);
+  MCNLA_LAUNCHED();
+  bs_product(g, L, L.U1, L.BsU1, -1, st);
+  hipLaunchKernelGGL(gram_partials_kernel, dim3(static_cast<unsigned>(L.nblk_p)), dim3(kThreads),
+                     0, st, -1, L.st, nl, l, L.U1, L.BsU1, L.P, 2);
+  MCNLA_LAUNCHED();
+  hipLaunchKernelGGL(polish_kernel, dim3(1), dim3(kWyThreads), 0, st, l, static_cast<double>(g.N),
+                     L.P, L.nblk_p, L.F);
+  MCNLA_LAUNCHED();
+  hipLaunchKernelGGL(right_mul_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, nl, l, L.U1,
+                     L.F, coef);
+  MCNLA_LAUNCHED();
+}
+
+KnMember kn_member(const GramIntegrator& g, const Layout& L) {
+  KnMember m{};
+  const int64_t nl = L.nl;
+  m.Bs = g.Bs;
+  m.Pbg = L.Pbg;
+  m.B[0] = L.B[0];
+  m.B[1] = L.B[1];
+  m.Bg = L.Bg[0];
+  m.E[0] = L.E[0];
+  m.E[1] = L.E[1];
+  m.FF[0] = L.FF[0];
+  m.FF[1] = L.FF[1];
+  m.P = L.P;
+  m.F = L.F;
+  m.C = L.C;
+  m.st = L.st;
+  m.nl = nl;
+  m.l = g.l;
+  m.kchunk = ceil_div(ceil_div(nl, L.splits), 16) * 16;  // whole 16-deep K tiles per split
+  m.bs_splits = ceil_div(nl, m.kchunk);
+  m.bs_blocks = ceil_div(nl, 128) * m.bs_splits;
+  m.nblk_s = ceil_div(nl, kStepRows);
+  m.upd_blocks = grid_elems(nl * g.l);
+  m.invN = 1.0 / static_cast<double>(g.N);
+  m.tol = g.tol;
+  m.vec = (reinterpret_cast<uintptr_t>(g.Bs) % 16 == 0 && nl % 2 == 0) ? 1 : 0;
+  return m;
+}
+
+// One KN iteration of the members in `act` (member i at its iteration act.m[i].t): the split-K
+// Bs products (a launch per tile-width class present), then the reductions + l x l steps, then
+// the updates.
+void launch_kn_iteration(const KnGroup& act, hipStream_t st) {
+  KnGroup narrow, wide;
+  narrow.count = wide.count = 0;
+  int64_t bn = 0, bw = 0, bs = 0, bu = 0;
+  for (int i = 0; i < act.count; ++i) {
+    const KnMember& m = act.m[i];
+    if (m.l <= 32) {
+      narrow.m[narrow.count++] = m;
+      bn = std::max(bn, m.bs_blocks);
+    } else {
+      wide.m[wide.count++] = m;
+      bw = std::max(bw, m.bs_blocks);
+    }
+    bs = std::max(bs, m.nblk_s);
+    bu = std::max(bu, m.upd_blocks);
+  }
+  MCNLA_REQUIRE(bn < (1LL << 31) && bw < (1LL << 31) && bs < (1LL << 31), MCNLA_ERR_INVALID,
+                "integrator grid too large");
+  if (narrow.count) {
+    hipLaunchKernelGGL(kn_bs_group_kernel<32>, dim3(static_cast<unsigned>(bn), narrow.count),
+                       dim3(256), 0, st, narrow);
+    MCNLA_LAUNCHED();
+  }
+  if (wide.count) {
+    hipLaunchKernelGGL(kn_bs_group_kernel<64>, dim3(static_cast<unsigned>(bw), wide.count),
+                       dim3(256), 0, st, wide);
+    MCNLA_LAUNCHED();
+  }
+  hipLaunchKernelGGL(kn_reduce_step_group_kernel, dim3(static_cast<unsigned>(bs), act.count),
+                     dim3(kStepThreads), 0, st, act);
+  MCNLA_LAUNCHED();
+  hipLaunchKernelGGL(kn_update_group_kernel, dim3(static_cast<unsigned>(bu), act.count),
+                     dim3(kThreads), 0, st, act);
+  MCNLA_LAUNCHED();
+}
+
+// The KN loop of a group.  `graphs` (a group of one only) records each chunk once and replays it.
+void run_kn(const GramIntegrator* gs, double* const* coefs, int count, hipStream_t st,
+            IterationGraphs* graphs, int64_t* iterations, bool* converged, bool* numeric_ok) {
+  MCNLA_REQUIRE(count >= 1 && count <= kMaxKnGroup, MCNLA_ERR_INVALID,
+                "a KN group holds 1 to 16 solves");
+  MCNLA_REQUIRE(graphs == nullptr || count == 1, MCNLA_ERR_INVALID,
+                "iteration graphs are recorded for a single solve only");
+  std::vector<Layout> Ls(count);
+  KnGroup all;
+  all.count = count;
+  for (int i = 0; i < count; ++i) {
+    MCNLA_REQUIRE(!gs[i].wen_yin, MCNLA_ERR_INVALID, "a KN group holds KN integrations only");
+    Ls[i] = layout(gs[i].ws, gs[i].N, gs[i].l);
+    check_integrator(gs[i], Ls[i]);
+    all.m[i] = kn_member(gs[i], Ls[i]);
+  }
+  for (int i = 0; i < count; ++i) init_iteration(gs[i], Ls[i], st);
+
+  std::vector<int> conv_at(count, INT_MAX), nan_seen(count, 0), flags(2 * count);
+  int64_t t = 0;
+  int64_t chunk = 8;
+  hipStream_t ls = st;  // where the iteration launches go: a capture stream while recording
+  for (;;) {
+    int64_t tmax = 0;  // the furthest iteration any unconverged member may still run
+    for (int i = 0; i < count; ++i)
+      if (conv_at[i] == INT_MAX) tmax = std::max(tmax, gs[i].maxiter);
+    if (t >= tmax) break;
+    const int64_t tend = (t + chunk < tmax) ? t + chunk : tmax;
+    hipGraphExec_t exec = graphs ? graphs->find(t, tend) : nullptr;
+    const bool recording = exec == nullptr && graphs != nullptr;
+    if (recording) {
+      if (!graphs->cap) MCNLA_HIP(hipStreamCreateWithFlags(&graphs->cap, hipStreamNonBlocking));
+      ls = graphs->cap;
+      MCNLA_HIP(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+    }
+    const int64_t t0 = t;
+    if (exec != nullptr) {
+      t = tend;
+    } else try {
+      for (; t < tend; ++t) {
+        KnGroup act;  // members not known to have converged and below their maxiter
+        act.count = 0;
+        for (int i = 0; i < count; ++i)
+          if (conv_at[i] == INT_MAX && t < gs[i].maxiter) {
+            act.m[act.count] = all.m[i];
+            act.m[act.count++].t = static_cast<int>(t);
+          }
+        launch_kn_iteration(act, ls);
+      }
+    } catch (...) {
+      if (recording) {  // leave the capture stream usable
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(ls, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+      }
+      throw;
+    }
+    if (recording) {
+      hipGraph_t graph = nullptr;
+      MCNLA_HIP(hipStreamEndCapture(ls, &graph));
+      ls = st;
+      hipGraphExec_t made = nullptr;
+      const hipError_t e = hipGraphInstantiate(&made, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      MCNLA_HIP(e);
+      graphs->execs.push_back({{t0, tend}, made});
+      exec = made;
+    }
+    if (exec != nullptr) MCNLA_HIP(hipGraphLaunch(exec, st));
+    hipLaunchKernelGGL(kn_flags_kernel, dim3(1), dim3(kWave), 0, st, all, Ls[0].flags);
+    MCNLA_LAUNCHED();
+    MCNLA_HIP(hipMemcpyAsync(flags.data(), Ls[0].flags, 2 * count * sizeof(int),
+                             hipMemcpyDeviceToHost, st));
+    MCNLA_HIP(hipStreamSynchronize(st));
+    bool running = false;
+    for (int i = 0; i < count; ++i) {
+      conv_at[i] = flags[2 * i];
+      nan_seen[i] = flags[2 * i + 1];
+      if (conv_at[i] == INT_MAX && t < gs[i].maxiter) running = true;
+    }
+    if (!running) break;
+    if (chunk < 32) chunk *= 2;
+  }
+  for (int i = 0; i < count; ++i) {
+    const int64_t iters =
+        (conv_at[i] != INT_MAX) ? static_cast<int64_t>(conv_at[i]) + 1 : gs[i].maxiter;
+    finish_iteration(gs[i], Ls[i], iters, coefs[i], st);
+    if (iterations) iterations[i] = iters;
+    if (converged) converged[i] = conv_at[i] != INT_MAX;
+    if (numeric_ok) numeric_ok[i] = nan_seen[i] == 0;
+  }
+}
+
+}  // namespace
+
+void integrate_gram_group(const GramIntegrator* gs, double* const* coefs, int count,
+                          hipStream_t st, int64_t* iterations, bool* converged, bool* numeric_ok) {
+  run_kn(gs, coefs, count, st, nullptr, iterations, converged, numeric_ok);
+}
+
+struct KnBatch::Impl {
+  struct Slot {
+    bool used = false;
+    GramIntegrator g;
+    Layout L;
+    KnMember m;
+    int64_t t = 0;
+    int conv_at = INT_MAX;
+    int nan_seen = 0;
+  };
+  int* flags = nullptr;
+  Slot slot[kMaxKnGroup];
+  int host[2 * kMaxKnGroup];
+  bool live(const Slot& x) const {
+    return x.used && x.conv_at == INT_MAX && x.t < x.g.maxiter;
+  }
+};
+
+KnBatch::KnBatch(int* flags) : impl_(new Impl) { impl_->flags = flags; }
+KnBatch::~KnBatch() { delete impl_; }
+
+int KnBatch::free_slots() const {
+  int n = 0;
+  for (const auto& x : impl_->slot) n += x.used ? 0 : 1;
+  return n;
+}
+
+int KnBatch::add(const GramIntegrator& g, hipStream_t st) {
+  MCNLA_REQUIRE(!g.wen_yin, MCNLA_ERR_INVALID, "a KN batch holds KN integrations only");
+  for (int i = 0; i < kMaxKnGroup; ++i) {
+    Impl::Slot& x = impl_->slot[i];
+    if (x.used) continue;
+    x.g = g;
+    x.L = layout(g.ws, g.N, g.l);
+    check_integrator(g, x.L);
+    x.m = kn_member(g, x.L);
+    x.t = 0;
+    x.conv_at = INT_MAX;
+    x.nan_seen = 0;
+    init_iteration(g, x.L, st);
+    x.used = true;
+    return i;
+  }
+  throw Error{MCNLA_ERR_STATE, "KN batch is full"};
+}
+
+void KnBatch::advance(int iters, hipStream_t st) {
+  for (int k = 0; k < iters; ++k) {
+    KnGroup act;
+    act.count = 0;
+    for (auto& x : impl_->slot)
+      if (impl_->live(x)) {
+        act.m[act.count] = x.m;
+        act.m[act.count++].t = static_cast<int>(x.t);
+        ++x.t;
+      }
+    if (act.count == 0) break;
+    launch_kn_iteration(act, st);
+  }
+  KnGroup used;
+  used.count = 0;
+  int who[kMaxKnGroup];
+  for (int i = 0; i < kMaxKnGroup; ++i)
+    if (impl_->slot[i].used) {
+      who[used.count] = i;
+      used.m[used.count++] = impl_->slot[i].m;
+    }
+  if (used.count == 0) return;
+  hipLaunchKernelGGL(kn_flags_kernel, dim3(1), dim3(kWave), 0, st, used, impl_->flags);
+  MCNLA_LAUNCHED();
+  MCNLA_HIP(hipMemcpyAsync(impl_->host, impl_->flags, 2 * used.count * sizeof(int),
+                           hipMemcpyDeviceToHost, st));
+  MCNLA_HIP(hipStreamSynchronize(st));
+  for (int j = 0; j < used.count; ++j) {
+    Impl::Slot& x = impl_->slot[who[j]];
+    x.conv_at = impl_->host[2 * j];
+    x.nan_seen = impl_->host[2 * j + 1];
+  }
+}
+
+bool KnBatch::running(int slot) const { return impl_->live(impl_->slot[slot]); }
+
+int64_t KnBatch::iterations(int slot) const {
+  const Impl::Slot& x = impl_->slot[slot];
+  return x.conv_at != INT_MAX ? static_cast<int64_t>(x.conv_at) + 1 : x.t;
+}
+
+bool KnBatch::converged(int slot) const { return impl_->slot[slot].conv_at != INT_MAX; }
+
+bool KnBatch::numeric_ok(int slot) const { return impl_->slot[slot].nan_seen == 0; }
+
+void KnBatch::finish(int slot, double* coef, hipStream_t st) {
+  Impl::Slot& x = impl_->slot[slot];
+  MCNLA_REQUIRE(x.used && !impl_->live(x), MCNLA_ERR_STATE, "KN batch member still running");
+  finish_iteration(x.g, x.L, iterations(slot), coef, st);
+  x.used = false;
+}
+
+void KnBatch::drop(int slot) { impl_->slot[slot].used = false; }
+
 int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bool* converged,
                        bool* numeric_ok) {
-  MCNLA_REQUIRE(g.l >= 1 && g.l <= kMaxL, MCNLA_ERR_INVALID, "integrator needs 1 <= l <= 64");
+  if (!g.wen_yin) {
+    int64_t iters = 0;
+    bool conv = false, ok = true;
+    run_kn(&g, &coef, 1, st, g.graphs, &iters, &conv, &ok);
+    if (converged) *converged = conv;
+    if (numeric_ok) *numeric_ok = ok;
+    return iters;
+  }
   Layout L = layout(g.ws, g.N, g.l);
-  MCNLA_REQUIRE(g.ws_bytes >= L.bytes, MCNLA_ERR_INVALID, "integrator workspace too small");
+  check_integrator(g, L);
   const int l = g.l;
   const int64_t nl = L.nl;
   const double invN = 1.0 / static_cast<double>(g.N);
   const int ll = l * l;
-
-  // Bout = Bs * Bin / N on the MFMA GEMM (split-K, deterministic reduction), gated on t; with
-  // Bout null only the bs_splits partial products are left in L.Pbg ([split][Nl][l]).  Its rows
-  // run along K, which takes the tall-skinny kernel (row slabs through LDS).
-  const int64_t bs_kchunk = ceil_div(ceil_div(nl, L.splits), 16) * 16;
-  const int64_t bs_splits = ceil_div(nl, bs_kchunk);
-  const int64_t nblk_s = ceil_div(nl, kStepRows);
-  hipStream_t ls = st;  // where the iteration launches go: a capture stream while recording
-  auto bs_product = [&](const double* Bin, double* Bout, int t) {
-    GemmF64Args p;
-    p.M = nl;
-    p.N = l;
-    p.K = nl;
-    p.A = g.Bs;
-    p.lda = nl;
-    p.a_mcontig = false;
-    p.B = Bin;
-    p.ldb = l;
-    p.C = Bout;
-    p.ldc = l;
-    p.alpha = invN;
-    p.shape = l <= 32 ? GemmShape::kTallN32 : GemmShape::kTallN64;
-    p.splits = L.splits;
-    p.kchunk = bs_kchunk;
-    p.partial = L.Pbg;
-    p.gate = &L.st->converged_at;
-    p.gate_t = t;
-    gemm_f64(p, ls);
-  };
   const unsigned gp = static_cast<unsigned>(L.nblk_p);
+  hipStream_t ls = st;  // where the iteration launches go: a capture stream while recording
 
-  hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(1), 0, st, L.st);
-  MCNLA_LAUNCHED();
-  hipLaunchKernelGGL(init_coef_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, g.Bs, nl, l,
-                     L.B[0], L.E[0], L.FF[0]);
-  MCNLA_LAUNCHED();
-
+  init_iteration(g, L, st);
   const size_t wy_lds = wy_lds_bytes(l);
-  if (g.wen_yin) {
-    for (const void* f : {reinterpret_cast<const void*>(wy_search_kernel),
-                          reinterpret_cast<const void*>(wy_tau_kernel)})
-      MCNLA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    static_cast<int>(wy_lds)));
-    // Bg0 = Bs B0 / N and [Dc; Dgc; Hgg] of parity 0.
-    bs_product(L.B[0], L.Bg[0], -1);
-    hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, -1, L.st, nl, l,
-                       L.B[0], L.Bg[0], L.P, 3);
-    MCNLA_LAUNCHED();
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll, kThreads)), dim3(kThreads), 0,
-                       st, -1, L.st, L.P, L.nblk_p, 3 * ll, invN, L.D3[0]);
-    MCNLA_LAUNCHED();
-    hipLaunchKernelGGL(wy_init_scalars_kernel, dim3(1), dim3(64), 0, st, L.st, l, L.D3[0]);
-    MCNLA_LAUNCHED();
-  }
+  for (const void* f : {reinterpret_cast<const void*>(wy_search_kernel),
+                        reinterpret_cast<const void*>(wy_tau_kernel)})
+    MCNLA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(wy_lds)));
+  // Bg0 = Bs B0 / N and [Dc; Dgc; Hgg] of parity 0.
+  bs_product(g, L, L.B[0], L.Bg[0], -1, st);
+  hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, -1, L.st, nl, l,
+                     L.B[0], L.Bg[0], L.P, 3);
+  MCNLA_LAUNCHED();
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll, kThreads)), dim3(kThreads), 0,
+                     st, -1, L.st, L.P, L.nblk_p, 3 * ll, invN, L.D3[0]);
+  MCNLA_LAUNCHED();
+  hipLaunchKernelGGL(wy_init_scalars_kernel, dim3(1), dim3(64), 0, st, L.st, l, L.D3[0]);
+  MCNLA_LAUNCHED();
 
   int64_t t = 0;
   int64_t chunk = 8;
   int host_conv = INT_MAX;
+  int nan_seen = 0;
   while (t < g.maxiter) {
     const int64_t tend = (t + chunk < g.maxiter) ? t + chunk : g.maxiter;
     hipGraphExec_t exec = g.graphs ? g.graphs->find(t, tend) : nullptr;
@@ -1159,46 +1554,34 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
       for (; t < tend; ++t) {
         const int c = static_cast<int>(t & 1), p = c ^ 1;
         const int ti = static_cast<int>(t);
-        if (!g.wen_yin) {
-          bs_product(L.B[c], nullptr, ti);  // split-K partials only
-          hipLaunchKernelGGL(kn_reduce_step_kernel, dim3(static_cast<unsigned>(nblk_s)),
-                             dim3(kStepThreads), 0, ls, ti, L.st, nl, l, invN, g.tol, L.Pbg, bs_splits,
-                             L.B[c], L.Bg[0], L.P, nblk_s, L.F, L.C);
-          MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(kn_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
-                             L.st, nl, l, invN, L.B[c], L.Bg[0], L.B[p], L.E[c], L.E[p], L.FF[c],
-                             L.FF[p], L.F, L.C);
-          MCNLA_LAUNCHED();
-        } else {
-          const int odd = p;  // the reference toggles is_odd before using it: odd = !(t & 1)
-          hipLaunchKernelGGL(wy_search_kernel, dim3(1), dim3(kWyThreads), wy_lds, ls, ti, L.st, l,
-                             L.D3[c], L.F, L.C);
-          MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(wy_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
-                             L.st, nl, l, invN, L.B[c], L.Bg[c], L.B[p], L.U1, L.BsU1, L.E[c],
-                             L.E[p], L.FF[c], L.FF[p], L.F, L.C);
-          MCNLA_LAUNCHED();
-          bs_product(L.B[p], L.Bg[p], ti);
-          hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, ls, ti, L.st, nl, l,
-                             L.B[p], L.Bg[p], L.P, 3);
-          MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll, kThreads)), dim3(kThreads),
-                             0, ls, ti, L.st, L.P, L.nblk_p, 3 * ll, invN, L.D3[p]);
-          MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(wy_check_kernel, dim3(1), dim3(kThreads), 0, ls, ti, L.st, l, g.tol,
-                             L.D3[p]);
-          MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(wy_terms_kernel, dim3(static_cast<unsigned>(L.nblk_t)), dim3(kThreads), 0,
-                             ls, ti, L.st, nl, l, invN, odd, L.B[c], L.B[p], L.Bg[c], L.Bg[p], L.U1,
-                             L.BsU1, L.D3[p], L.E[c], L.FF[c], L.P);
-          MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll + 2, kThreads)),
-                             dim3(kThreads), 0, ls, ti, L.st, L.P, L.nblk_t, 3 * ll + 2, 1.0, L.T);
-          MCNLA_LAUNCHED();
-          hipLaunchKernelGGL(wy_tau_kernel, dim3(1), dim3(kWyThreads), wy_lds, ls, ti, L.st, l, odd,
-                             L.D3[c], L.D3[p], L.F, L.T);
-          MCNLA_LAUNCHED();
-        }
+        const int odd = p;  // the reference toggles is_odd before using it: odd = !(t & 1)
+        hipLaunchKernelGGL(wy_search_kernel, dim3(1), dim3(kWyThreads), wy_lds, ls, ti, L.st, l,
+                           L.D3[c], L.F, L.C);
+        MCNLA_LAUNCHED();
+        hipLaunchKernelGGL(wy_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
+                           L.st, nl, l, invN, L.B[c], L.Bg[c], L.B[p], L.U1, L.BsU1, L.E[c],
+                           L.E[p], L.FF[c], L.FF[p], L.F, L.C);
+        MCNLA_LAUNCHED();
+        bs_product(g, L, L.B[p], L.Bg[p], ti, ls);
+        hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, ls, ti, L.st, nl, l,
+                           L.B[p], L.Bg[p], L.P, 3);
+        MCNLA_LAUNCHED();
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll, kThreads)), dim3(kThreads),
+                           0, ls, ti, L.st, L.P, L.nblk_p, 3 * ll, invN, L.D3[p]);
+        MCNLA_LAUNCHED();
+        hipLaunchKernelGGL(wy_check_kernel, dim3(1), dim3(kThreads), 0, ls, ti, L.st, l, g.tol,
+                           L.D3[p]);
+        MCNLA_LAUNCHED();
+        hipLaunchKernelGGL(wy_terms_kernel, dim3(static_cast<unsigned>(L.nblk_t)), dim3(kThreads), 0,
+                           ls, ti, L.st, nl, l, invN, odd, L.B[c], L.B[p], L.Bg[c], L.Bg[p], L.U1,
+                           L.BsU1, L.D3[p], L.E[c], L.FF[c], L.P);
+        MCNLA_LAUNCHED();
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(3 * ll + 2, kThreads)),
+                           dim3(kThreads), 0, ls, ti, L.st, L.P, L.nblk_t, 3 * ll + 2, 1.0, L.T);
+        MCNLA_LAUNCHED();
+        hipLaunchKernelGGL(wy_tau_kernel, dim3(1), dim3(kWyThreads), wy_lds, ls, ti, L.st, l, odd,
+                           L.D3[c], L.D3[p], L.F, L.T);
+        MCNLA_LAUNCHED();
       }
     } catch (...) {
       if (recording) {  // leave the capture stream usable
@@ -1221,29 +1604,16 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
       exec = made;
     }
     if (exec != nullptr) MCNLA_HIP(hipGraphLaunch(exec, st));
-    MCNLA_HIP(hipMemcpyAsync(&host_conv, &L.st->converged_at, sizeof(int), hipMemcpyDeviceToHost, st));
+    int fl[2] = {INT_MAX, 0};
+    MCNLA_HIP(hipMemcpyAsync(fl, &L.st->converged_at, sizeof(fl), hipMemcpyDeviceToHost, st));
     MCNLA_HIP(hipStreamSynchronize(st));
+    host_conv = fl[0];
+    nan_seen = fl[1];
     if (host_conv != INT_MAX) break;
     if (chunk < 32) chunk *= 2;
   }
   const int64_t iters = (host_conv != INT_MAX) ? static_cast<int64_t>(host_conv) + 1 : g.maxiter;
-  const int fin = static_cast<int>(iters & 1);
-  hipLaunchKernelGGL(coef_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, L.E[fin],
-                     L.FF[fin], nl, l, L.U1);
-  MCNLA_LAUNCHED();
-  bs_product(L.U1, L.BsU1, -1);
-  hipLaunchKernelGGL(gram_partials_kernel, dim3(gp), dim3(kThreads), 0, st, -1, L.st, nl, l, L.U1,
-                     L.BsU1, L.P, 2);
-  MCNLA_LAUNCHED();
-  hipLaunchKernelGGL(polish_kernel, dim3(1), dim3(kWyThreads), 0, st, l, static_cast<double>(g.N),
-                     L.P, L.nblk_p, L.F);
-  MCNLA_LAUNCHED();
-  hipLaunchKernelGGL(right_mul_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, st, nl, l, L.U1,
-                     L.F, coef);
-  MCNLA_LAUNCHED();
-  int nan_seen = 0;
-  MCNLA_HIP(hipMemcpyAsync(&nan_seen, &L.st->nan_seen, sizeof(int), hipMemcpyDeviceToHost, st));
-  MCNLA_HIP(hipStreamSynchronize(st));
+  finish_iteration(g, L, iters, coef, st);
   if (converged) *converged = (host_conv != INT_MAX);
   if (numeric_ok) *numeric_ok = (nan_seen == 0);
   return iters;

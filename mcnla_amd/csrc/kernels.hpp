@@ -168,6 +168,47 @@ size_t integrator_workspace_bytes(int64_t N, int l);
 int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bool* converged,
                        bool* numeric_ok);
 
+// KN integrations of up to kMaxKnGroup solves (wen_yin false; any N, l, tol, maxiter each) run in
+// lockstep on one stream: each iteration is one launch per kernel for the whole group (the
+// Bs product: one per tile-width class, l <= 32 and l > 32), and the host reads the group's
+// convergence flags once per chunk of iterations.  Member i's coefficients, iteration count and
+// flags are bit-identical to integrate_gram(gs[i]) alone.  `graphs` of the members are not used.
+constexpr int kMaxKnGroup = 16;
+void integrate_gram_group(const GramIntegrator* gs, double* const* coefs, int count,
+                          hipStream_t st, int64_t* iterations, bool* converged, bool* numeric_ok);
+
+// KN integrations that join and leave a running group between chunks of iterations (the serving
+// loop's iteration-level batching).  Each member keeps its own iteration index and state, so its
+// coefficients, iteration count and flags are bit-identical to integrate_gram alone, whatever it
+// shares launches with and whenever it joined.  Not thread-safe; one stream at a time.
+class KnBatch {
+ public:
+  // `flags`: device memory for 2 kMaxKnGroup ints (the per-chunk read-back of the flags).
+  explicit KnBatch(int* flags);
+  ~KnBatch();
+  KnBatch(const KnBatch&) = delete;
+  KnBatch& operator=(const KnBatch&) = delete;
+  int free_slots() const;
+  // Takes a KN integration (its Bs complete in stream order on `st`) and enqueues its initial
+  // state; returns its slot.
+  int add(const GramIntegrator& g, hipStream_t st);
+  // Enqueues up to `iters` iterations of each running member, then the read-back of every
+  // member's flags, and synchronizes `st`.
+  void advance(int iters, hipStream_t st);
+  bool running(int slot) const;  // neither converged nor at its maxiter
+  int64_t iterations(int slot) const;
+  bool converged(int slot) const;
+  bool numeric_ok(int slot) const;
+  // Enqueues the member's final coefficients into `coef` (Nl x l) and releases its slot.
+  void finish(int slot, double* coef, hipStream_t st);
+  // Releases a slot without finishing.
+  void drop(int slot);
+
+ private:
+  struct Impl;
+  Impl* impl_;
+};
+
 // ------------------------------------------------------------------------------------------------
 // Non-iterative integrators on Bs (combine.hip): the pairwise reduction tree and the extrinsic
 // mean.  Both leave the Nl x l coefficients with Qbar = Qs * coef in `coef`.

@@ -82,6 +82,13 @@ class Arena {
 
 enum class State { kPending, kRunning, kDone };
 
+// Device bytes of a worker's KN-group flags (2 kMaxKnGroup ints, one 256-byte line).
+constexpr size_t kFlagBytes = 256;
+static_assert(2 * kMaxKnGroup * sizeof(int) <= kFlagBytes, "flag words fit");
+
+// Iterations a worker's KN group runs between admissions and completions.
+constexpr int kChunk = 8;
+
 // Offsets of the results inside a request's result region.
 struct ResultLayout {
   size_t s = 0, u = 0, v = 0, q = 0, bytes = 0;
@@ -117,6 +124,7 @@ struct Req {
   mcnla_status status = MCNLA_OK;
   std::string error;
   int64_t iterations = 0;
+  int bad = 0;  // the solve's hazard flag, read back on its stream
   Clock::time_point t_submit, t_admit, t_done;
 };
 
@@ -136,6 +144,7 @@ struct mcnla_server_s {
   mcnla_server_config cfg{};
   int device = 0;
   char* base = nullptr;
+  int* flags = nullptr;  // per worker: 2 kMaxKnGroup ints of device memory for its KN group
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   Arena arena{0};
@@ -150,6 +159,10 @@ struct mcnla_server_s {
   // statistics since the last reset
   uint64_t submitted = 0, completed = 0, failed = 0;
   int max_running = 0;
+  uint64_t batches = 0;  // admissions (a batch of one or more requests each)
+  // KN groups: chunks run, members summed over chunks, wall seconds of the chunks
+  uint64_t chunks = 0, chunk_members = 0;
+  double chunk_seconds = 0.0;
   std::vector<double> latency;
   bool any_submit = false;
   Clock::time_point t_first, t_last;
@@ -182,47 +195,67 @@ struct mcnla_server_s {
   int waiters = 0;
   std::condition_variable cv_left;
 
+  mcnla_solver prepare(Req* q, hipStream_t st, SolveRes* res, const void** A, int64_t* lda);
+  void collect(Req* q, mcnla_solver s, hipStream_t st);
   void run(Req* q, hipStream_t st, SolveRes* res);
-  void worker();
+  void finish_locked(Req* q);
+  void worker(int index);
+  void worker_kn(hipStream_t st, SolveRes* res, int* flags, int cap);
 };
 
-void mcnla_server_s::run(Req* q, hipStream_t st, SolveRes* res) {
+// Makes q's A visible to a solve on `st` (after its ready event; a host A is copied behind the
+// solver's workspaces) and creates its solver in its workspace region.
+mcnla_solver mcnla_server_s::prepare(Req* q, hipStream_t st, SolveRes* res, const void** A,
+                                     int64_t* lda) {
   char* ws = base + q->ws_off;
+  const mcnla_params& p = q->r.params;
+  *A = q->r.A;
+  *lda = q->r.lda;
+  if (q->r.ready_event)
+    MCNLA_HIP(hipStreamWaitEvent(st, static_cast<hipEvent_t>(q->r.ready_event), 0));
+  if (q->r.a_on_host) {
+    // Dense copy of A after the solver's workspaces: lines of `len` elements, `count` of them.
+    const size_t esz = q->r.a_dtype == MCNLA_F32 ? sizeof(float) : sizeof(double);
+    const bool cm = q->r.a_order == MCNLA_COL_MAJOR;
+    const int64_t len = cm ? p.nrow : p.ncol, count = cm ? p.ncol : p.nrow;
+    char* dA = ws + q->solver_bytes;
+    MCNLA_HIP(hipMemcpy2DAsync(dA, len * esz, q->r.A, q->r.lda * esz, len * esz, count,
+                               hipMemcpyHostToDevice, st));
+    *A = dA;
+    *lda = len;
+  }
+  return solver_create_in(&p, &q->cfg, ws, q->solver_bytes, res);
+}
+
+// Enqueues the copies of a solved request's results into its result region.
+void mcnla_server_s::collect(Req* q, mcnla_solver s, hipStream_t st) {
   char* rs = base + q->res_off;
+  const mcnla_params& p = q->r.params;
+  const double *S = nullptr, *U = nullptr, *V = nullptr, *Qb = nullptr;
+  if (mcnla_solver_results(s, &S, &U, &V, &Qb) != MCNLA_OK)
+    throw Error{MCNLA_ERR_STATE, mcnla_last_error()};
+  const size_t k = static_cast<size_t>(p.rank), l = static_cast<size_t>(p.rank + p.over_rank);
+  MCNLA_HIP(hipMemcpyAsync(rs + q->res.s, S, k * sizeof(double), hipMemcpyDeviceToDevice, st));
+  MCNLA_HIP(hipMemcpyAsync(rs + q->res.u, U, p.nrow * k * sizeof(double),
+                           hipMemcpyDeviceToDevice, st));
+  if (q->res.has_v && V)
+    MCNLA_HIP(hipMemcpyAsync(rs + q->res.v, V, p.ncol * k * sizeof(double),
+                             hipMemcpyDeviceToDevice, st));
+  MCNLA_HIP(hipMemcpyAsync(rs + q->res.q, Qb, p.nrow * l * sizeof(double),
+                           hipMemcpyDeviceToDevice, st));
+}
+
+// Solves one admitted request on `st` (mcnla_solver_solve) and copies its results.
+void mcnla_server_s::run(Req* q, hipStream_t st, SolveRes* res) {
   mcnla_solver s = nullptr;
   try {
-    const mcnla_params& p = q->r.params;
-    const void* A = q->r.A;
-    int64_t lda = q->r.lda;
-    if (q->r.ready_event)
-      MCNLA_HIP(hipStreamWaitEvent(st, static_cast<hipEvent_t>(q->r.ready_event), 0));
-    if (q->r.a_on_host) {
-      // Dense copy of A after the solver's workspaces: lines of `len` elements, `count` of them.
-      const size_t esz = q->r.a_dtype == MCNLA_F32 ? sizeof(float) : sizeof(double);
-      const bool cm = q->r.a_order == MCNLA_COL_MAJOR;
-      const int64_t len = cm ? p.nrow : p.ncol, count = cm ? p.ncol : p.nrow;
-      char* dA = ws + q->solver_bytes;
-      MCNLA_HIP(hipMemcpy2DAsync(dA, len * esz, A, lda * esz, len * esz, count,
-                                 hipMemcpyHostToDevice, st));
-      A = dA;
-      lda = len;
-    }
-    s = solver_create_in(&p, &q->cfg, ws, q->solver_bytes, res);
+    const void* A = nullptr;
+    int64_t lda = 0;
+    s = prepare(q, st, res, &A, &lda);
     q->status = mcnla_solver_solve(s, A, lda, q->r.a_dtype, q->r.a_order, st);
     if (q->status != MCNLA_OK) q->error = mcnla_last_error();
     if (q->status == MCNLA_OK || q->status == MCNLA_ERR_NOT_CONVERGED) {
-      const double *S = nullptr, *U = nullptr, *V = nullptr, *Qb = nullptr;
-      if (mcnla_solver_results(s, &S, &U, &V, &Qb) != MCNLA_OK)
-        throw Error{MCNLA_ERR_STATE, mcnla_last_error()};
-      const size_t k = static_cast<size_t>(p.rank), l = static_cast<size_t>(p.rank + p.over_rank);
-      MCNLA_HIP(hipMemcpyAsync(rs + q->res.s, S, k * sizeof(double), hipMemcpyDeviceToDevice, st));
-      MCNLA_HIP(hipMemcpyAsync(rs + q->res.u, U, p.nrow * k * sizeof(double),
-                               hipMemcpyDeviceToDevice, st));
-      if (q->res.has_v && V)
-        MCNLA_HIP(hipMemcpyAsync(rs + q->res.v, V, p.ncol * k * sizeof(double),
-                                 hipMemcpyDeviceToDevice, st));
-      MCNLA_HIP(hipMemcpyAsync(rs + q->res.q, Qb, p.nrow * l * sizeof(double),
-                               hipMemcpyDeviceToDevice, st));
+      collect(q, s, st);
       MCNLA_HIP(hipStreamSynchronize(st));
       mcnla_solver_iteration(s, &q->iterations);
     }
@@ -240,10 +273,256 @@ void mcnla_server_s::run(Req* q, hipStream_t st, SolveRes* res) {
   if (s) mcnla_solver_destroy(s);
 }
 
+// Bookkeeping of a finished request (its workspace region goes back to the arena).
+void mcnla_server_s::finish_locked(Req* q) {
+  q->t_done = Clock::now();
+  q->state = State::kDone;
+  arena.give(q->ws_off, q->ws_bytes);
+  --running;
+  const bool ok = q->status == MCNLA_OK || q->status == MCNLA_ERR_NOT_CONVERGED;
+  if (ok) {
+    ++completed;
+    latency.push_back(seconds(q->t_submit, q->t_done));
+  } else {
+    ++failed;
+  }
+  t_last = q->t_done;
+  done_q.push_back(q->id);
+}
+
+// Iteration-level batching of KN solves.  The worker keeps a group of up to `cap` solves whose KN
+// iterations run together on its stream `st` (KnBatch: one launch per kernel per iteration for
+// the whole group) and cycles:
+//   - admit requests into free places of the group; each runs its sketch and orthogonalizer
+//     stages on the worker's second stream `aux` (beside the group's iterations) and joins the
+//     group at its iteration 0 once those have finished;
+//   - run kChunk iterations of the group and read its flags (synchronizes `st`);
+//   - hand the members that stopped to `aux` for the rest of their solve (coefficients, basis,
+//     former, result copies); each completes when `aux` has passed its work.
+// Nothing waits on the host while the group has members to iterate.  Every member computes
+// exactly what it computes alone.
+void mcnla_server_s::worker_kn(hipStream_t st, SolveRes* res, int* flags, int cap) {
+  struct Live {
+    Req* q = nullptr;
+    mcnla_solver s = nullptr;
+    const void* A = nullptr;
+    int64_t lda = 0;
+    SolveCtx cx;
+    int slot = -1;  // place in the KN group
+    int ev = -1;    // event: stages 1-2 done (joining) / results copied (finishing)
+    bool failed = false;
+  };
+  hipStream_t aux = nullptr;
+  hipEvent_t chunk_done = nullptr;
+  std::vector<hipEvent_t> evs;  // one per member in flight: at most cap joining or live + cap
+                                // finishing
+  std::vector<int> free_ev;
+  try {
+    MCNLA_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    MCNLA_HIP(hipEventCreateWithFlags(&chunk_done, hipEventDisableTiming));
+    for (int i = 0; i < 2 * cap; ++i) {
+      hipEvent_t e = nullptr;
+      MCNLA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      evs.push_back(e);
+      free_ev.push_back(i);
+    }
+  } catch (const Error&) {
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    if (chunk_done) (void)hipEventDestroy(chunk_done);
+    if (aux) (void)hipStreamDestroy(aux);
+    aux = nullptr;
+  }
+  KnBatch kb(flags);
+  uint64_t my_chunks = 0, my_members = 0;  // flushed into the server's totals under its lock
+  double my_seconds = 0.0;
+  std::vector<Live> joining, live, finishing;
+  std::vector<Req*> fresh;
+  auto done_event = [&](const Live& L) {
+    const hipError_t e = hipEventQuery(evs[L.ev]);
+    if (e == hipErrorNotReady) return false;
+    (void)hipGetLastError();
+    return true;
+  };
+  // completes a finishing member (its work on `aux` has passed, or it failed)
+  auto complete = [&](Live& L) {
+    if (!L.failed) {
+      L.q->status = solve_status(L.s, L.cx, L.q->bad);
+      if (L.q->status != MCNLA_OK) L.q->error = mcnla_last_error();
+    }
+    if (L.s) mcnla_solver_destroy(L.s);
+    if (L.ev >= 0) free_ev.push_back(L.ev);
+    std::lock_guard<std::mutex> lk(mu);
+    finish_locked(L.q);
+  };
+  auto fail = [&](Live& L, mcnla_status code, const std::string& msg) {
+    L.q->status = code;
+    L.q->error = msg;
+    L.failed = true;
+    if (L.slot >= 0) kb.drop(L.slot);
+    L.slot = -1;
+  };
+  for (;;) {
+    // 1. admission
+    fresh.clear();
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      chunks += my_chunks;
+      chunk_members += my_members;
+      chunk_seconds += my_seconds;
+      my_chunks = my_members = 0;
+      my_seconds = 0.0;
+      const bool idle = joining.empty() && live.empty() && finishing.empty();
+      if (idle) {
+        cv_work.wait(lk, [&] {
+          if (stop) return true;
+          Req* q = (aux && !free_ev.empty()) ? pick_locked() : nullptr;
+          if (q) fresh.push_back(q);
+          return q != nullptr;
+        });
+        if (fresh.empty()) break;  // stopping, nothing in flight
+      }
+      while (!stop && aux && static_cast<int>(joining.size() + live.size() + fresh.size()) < cap &&
+             free_ev.size() > fresh.size()) {
+        Req* q = pick_locked();
+        if (!q) break;
+        fresh.push_back(q);
+      }
+      if (!fresh.empty()) {
+        const Clock::time_point now = Clock::now();
+        for (Req* q : fresh) {
+          q->state = State::kRunning;
+          q->t_admit = now;
+        }
+        running += static_cast<int>(fresh.size());
+        max_running = std::max(max_running, running);
+        ++batches;
+      }
+    }
+    bool progressed = !fresh.empty();
+    for (Req* q : fresh) {  // stages 1-2 on aux
+      Live L;
+      L.q = q;
+      L.ev = free_ev.back();
+      free_ev.pop_back();
+      try {
+        L.s = prepare(q, aux, res, &L.A, &L.lda);
+        solve_check(L.s, L.A, L.lda, q->r.a_dtype, q->r.a_order);
+        solve_begin(L.s, L.A, L.lda, q->r.a_dtype, q->r.a_order, aux, L.cx, false);
+      } catch (const Error& e) {
+        fail(L, e.code, e.msg);
+      } catch (const std::exception& e) {
+        fail(L, MCNLA_ERR_INVALID, e.what());
+      }
+      (void)hipEventRecord(evs[L.ev], aux);
+      joining.push_back(L);
+    }
+    // 2. members whose stages 1-2 are done join the group (wait for the oldest when nothing else
+    // can run)
+    if (live.empty() && !joining.empty()) (void)hipEventSynchronize(evs[joining.front().ev]);
+    for (size_t i = 0; i < joining.size();) {
+      Live& L = joining[i];
+      if (!done_event(L)) {
+        ++i;
+        continue;
+      }
+      if (!L.failed) {
+        try {
+          MCNLA_HIP(hipStreamWaitEvent(st, evs[L.ev], 0));
+          L.slot = kb.add(solver_kn_integrator(L.s), st);
+        } catch (const Error& e) {
+          fail(L, e.code, e.msg);
+        }
+      }
+      if (L.failed) {
+        (void)hipStreamSynchronize(aux);
+        if (res->side) (void)hipStreamSynchronize(res->side);
+        complete(L);
+      } else {
+        live.push_back(L);
+      }
+      joining[i] = joining.back();
+      joining.pop_back();
+      progressed = true;
+    }
+    // 3. a chunk of the group's iterations and its flags (synchronizes st)
+    if (!live.empty()) {
+      progressed = true;
+      try {
+        const Clock::time_point c0 = Clock::now();
+        kb.advance(kChunk, st);
+        ++my_chunks;
+        my_members += live.size();
+        my_seconds += seconds(c0, Clock::now());
+        MCNLA_HIP(hipEventRecord(chunk_done, st));
+        MCNLA_HIP(hipStreamWaitEvent(aux, chunk_done, 0));
+      } catch (const Error& e) {
+        for (Live& L : live) fail(L, e.code, e.msg);
+      }
+    }
+    // 4. members that stopped: the rest of their solve on aux
+    for (size_t i = 0; i < live.size();) {
+      Live& L = live[i];
+      if (!L.failed && kb.running(L.slot)) {
+        ++i;
+        continue;
+      }
+      if (!L.failed) {
+        try {
+          const int dt = L.q->r.a_dtype, od = L.q->r.a_order;
+          L.cx.conv = kb.converged(L.slot);
+          L.cx.ok = kb.numeric_ok(L.slot);
+          L.q->iterations = kb.iterations(L.slot);
+          solver_set_integration(L.s, L.q->iterations, L.cx.conv);
+          kb.finish(L.slot, solver_coef(L.s), aux);
+          L.slot = -1;
+          solve_finish(L.s, L.A, L.lda, dt, od, aux, L.cx, false);
+          read_bad(L.s, &L.q->bad, aux);
+          collect(L.q, L.s, aux);  // a failed solve's copies are never handed out
+        } catch (const Error& e) {
+          fail(L, e.code, e.msg);
+        }
+      }
+      (void)hipEventRecord(evs[L.ev], aux);
+      finishing.push_back(L);
+      live[i] = live.back();
+      live.pop_back();
+    }
+    // 5. complete the finishing members that aux has passed (wait for the oldest when nothing
+    // else can run)
+    if (live.empty() && joining.empty() && !finishing.empty())
+      (void)hipEventSynchronize(evs[finishing.front().ev]);
+    for (size_t i = 0; i < finishing.size();) {
+      Live& L = finishing[i];
+      if (!done_event(L)) {
+        ++i;
+        continue;
+      }
+      if (L.failed) {
+        (void)hipStreamSynchronize(aux);
+        if (res->side) (void)hipStreamSynchronize(res->side);
+      }
+      complete(L);
+      finishing[i] = finishing.back();
+      finishing.pop_back();
+      progressed = true;
+    }
+    if (progressed) {
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+  if (aux) {
+    (void)hipStreamSynchronize(aux);
+    (void)hipStreamDestroy(aux);
+  }
+  for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+  if (chunk_done) (void)hipEventDestroy(chunk_done);
+}
+
 // A worker owns its stream and the HIP objects its solves record into (SolveRes) for its whole
 // life, so that admitting, running and finishing a request creates and destroys no HIP stream or
 // event.
-void mcnla_server_s::worker() {
+void mcnla_server_s::worker(int index) {
   (void)hipSetDevice(device);
   hipStream_t st = nullptr;
   SolveRes res;
@@ -257,36 +536,33 @@ void mcnla_server_s::worker() {
       st = nullptr;
     }
   }
-  for (;;) {
-    Req* q = nullptr;
-    {
-      std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stop || (st && (q = pick_locked()) != nullptr); });
-      if (!q) break;
-      q->state = State::kRunning;
-      q->t_admit = Clock::now();
-      ++running;
-      max_running = std::max(max_running, running);
-    }
-    run(q, st, &res);
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      q->t_done = Clock::now();
-      q->state = State::kDone;
-      arena.give(q->ws_off, q->ws_bytes);
-      --running;
-      const bool ok = q->status == MCNLA_OK || q->status == MCNLA_ERR_NOT_CONVERGED;
-      if (ok) {
-        ++completed;
-        latency.push_back(seconds(q->t_submit, q->t_done));
-      } else {
-        ++failed;
+  // KN solves with batch_max > 1 run in the worker's KN group; everything else one at a time.
+  const int kind = cfg.solver.integrator;
+  const bool kn = kind == MCNLA_INT_KN || kind == MCNLA_INT_ROW_BLOCK_KN ||
+                  kind == MCNLA_INT_ROW_BLOCK_GRAMIAN_KN;
+  if (st && kn && cfg.batch_max > 1) {
+    worker_kn(st, &res, flags + index * (kFlagBytes / sizeof(int)), cfg.batch_max);
+  } else {
+    for (;;) {
+      Req* q = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] { return stop || (st && (q = pick_locked()) != nullptr); });
+        if (!q) break;
+        q->state = State::kRunning;
+        q->t_admit = Clock::now();
+        ++running;
+        max_running = std::max(max_running, running);
+        ++batches;
       }
-      t_last = q->t_done;
-      done_q.push_back(q->id);
+      run(q, st, &res);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        finish_locked(q);
+      }
+      cv_done.notify_all();
+      cv_work.notify_all();
     }
-    cv_done.notify_all();
-    cv_work.notify_all();
   }
   res.destroy();
   if (st) (void)hipStreamDestroy(st);
@@ -337,6 +613,7 @@ void mcnla_server_config_default(mcnla_server_config* cfg) {
   cfg->slots = 8;
   cfg->hbm_budget = static_cast<size_t>(16) << 30;
   cfg->max_bypass = 8;
+  cfg->batch_max = 1;
 }
 
 mcnla_status mcnla_server_create(const mcnla_server_config* cfg, mcnla_server* out) {
@@ -345,6 +622,8 @@ mcnla_status mcnla_server_create(const mcnla_server_config* cfg, mcnla_server* o
     MCNLA_REQUIRE(cfg->slots >= 1 && cfg->slots <= 16, MCNLA_ERR_INVALID, "slots must be 1..16");
     MCNLA_REQUIRE(cfg->hbm_budget >= kCarveAlign, MCNLA_ERR_INVALID, "hbm_budget too small");
     MCNLA_REQUIRE(cfg->max_bypass >= 0, MCNLA_ERR_INVALID, "max_bypass must be >= 0");
+    MCNLA_REQUIRE(cfg->batch_max >= 1 && cfg->batch_max <= MCNLA_MAX_BATCH, MCNLA_ERR_INVALID,
+                  "batch_max must be 1..16");
     auto s = std::make_unique<mcnla_server_s>();
     s->cfg = *cfg;
     s->cfg.hbm_budget = align_up(cfg->hbm_budget, kCarveAlign);
@@ -357,7 +636,12 @@ mcnla_status mcnla_server_create(const mcnla_server_config* cfg, mcnla_server* o
     }
     s->base = static_cast<char*>(p);
     s->arena = Arena(s->cfg.hbm_budget);
-    for (int i = 0; i < s->cfg.slots; ++i) s->workers.emplace_back([srv = s.get()] { srv->worker(); });
+    size_t foff = 0;  // the workers' flag words, at the front of the arena
+    MCNLA_REQUIRE(s->arena.take(static_cast<size_t>(s->cfg.slots) * kFlagBytes, &foff),
+                  MCNLA_ERR_NOMEM, "hbm_budget too small");
+    s->flags = reinterpret_cast<int*>(s->base + foff);
+    for (int i = 0; i < s->cfg.slots; ++i)
+      s->workers.emplace_back([srv = s.get(), i] { srv->worker(i); });
     *out = s.release();
     return MCNLA_OK;
   });
@@ -516,6 +800,10 @@ mcnla_status mcnla_server_get_stats(mcnla_server s, mcnla_server_stats* out) {
     out->completed = s->completed;
     out->failed = s->failed;
     out->max_running = s->max_running;
+    out->batches = s->batches;
+    out->kn_chunks = s->chunks;
+    out->kn_group_mean = s->chunks ? static_cast<double>(s->chunk_members) / s->chunks : 0.0;
+    out->kn_chunk_mean_s = s->chunks ? s->chunk_seconds / s->chunks : 0.0;
     out->peak_bytes = s->arena.peak();
     out->budget_bytes = s->arena.cap();
     out->p50_s = out->p99_s = out->mean_s = out->max_s = out->span_s = 0.0;
@@ -544,6 +832,9 @@ mcnla_status mcnla_server_reset_stats(mcnla_server s) {
     MCNLA_REQUIRE(s, MCNLA_ERR_INVALID, "server is null");
     std::lock_guard<std::mutex> lk(s->mu);
     s->submitted = s->completed = s->failed = 0;
+    s->batches = 0;
+    s->chunks = s->chunk_members = 0;
+    s->chunk_seconds = 0.0;
     s->max_running = s->running;
     s->latency.clear();
     s->any_submit = false;

@@ -598,6 +598,29 @@ class Solver:
             pass
 
 
+def solve_batch(solvers, matrices, stream=None, allow_not_converged=True):
+    """Runs ``solvers[i].solve(matrices[i])`` for up to ``_lib.MAX_BATCH`` solvers as one batch
+    (``mcnla_solver_solve_batch``: the KN iterations of all members in lockstep, one launch per
+    kernel per iteration for the whole batch).  Each member's results are bit-identical to its
+    solve alone.  Returns the members' status codes; raises on the first one that is an error
+    (NOT_CONVERGED is allowed unless ``allow_not_converged`` is false)."""
+    n = len(solvers)
+    assert n == len(matrices) and 1 <= n <= _lib.MAX_BATCH
+    hs = (ctypes.c_void_p * n)(*[s._h for s in solvers])
+    As = (ctypes.c_void_p * n)(*[a.ptr for a in matrices])
+    lds = (ctypes.c_int64 * n)(*[a.ld for a in matrices])
+    dts = (ctypes.c_int * n)(*[a.dtype_code for a in matrices])
+    ods = (ctypes.c_int * n)(*[a.order_code for a in matrices])
+    sts = (ctypes.c_int * n)()
+    _lib.check(_lib.lib().mcnla_solver_solve_batch(n, hs, As, lds, dts, ods, sts, stream))
+    allow = (_lib.ERR_NOT_CONVERGED,) if allow_not_converged else ()
+    for s, st in zip(solvers, sts):
+        s.converged = st == _lib.OK
+    for st in sts:
+        _lib.check(st, allow=allow)
+    return list(sts)
+
+
 def _tune_args(parameters, precision, route):
     cfg = _lib.SolverConfig()
     _lib.lib().mcnla_solver_config_default(ctypes.byref(cfg))

@@ -39,7 +39,7 @@ class Served:
 
 
 class Server:
-    def __init__(self, slots=8, hbm_budget=16 << 30, max_bypass=8,
+    def __init__(self, slots=8, hbm_budget=16 << 30, max_bypass=8, batch_max=None,
                  sketcher=ColBlockGaussianProjectionSketcher,
                  integrator=RowBlockKolmogorovNagumoIntegrator, former=SvdFormer,
                  precision=_lib.PREC_F64, max_iteration=256, tolerance=None, jobv=True,
@@ -58,6 +58,8 @@ class Server:
         cfg.slots = int(slots)
         cfg.hbm_budget = int(hbm_budget)
         cfg.max_bypass = int(max_bypass)
+        if batch_max is not None:
+            cfg.batch_max = int(batch_max)
         self.config = cfg
         self.former_kind = former._kind
         h = ctypes.c_void_p()
