@@ -439,6 +439,10 @@ def main():
                     "open-loop phase (requests/s); 0: 75%% of the measured capacity")
     ap.add_argument("--serve-batch", type=int, default=1, help="KN requests a serving worker "
                     "admits at once and solves as one batch (their iterations in lockstep)")
+    ap.add_argument("--serve-group-slots", type=int, default=4, help="workers of the second "
+                    "serving run, through KN groups at the first run's arrival rate (0: skip it)")
+    ap.add_argument("--serve-group-max", type=int, default=16, help="members per KN group of the "
+                    "second serving run")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--only-serve", action="store_true", help="run the serving workload alone "
                     "and print its record")
@@ -459,7 +463,8 @@ def main():
     # onto GPU_MAX_HW_QUEUES hardware queues per process (4 by default), and streams that share a
     # queue run their kernels one after another.  Before the runtime starts, give the process one
     # queue per stream of the serving slots (at most 32).
-    queues = (3 if args.serve_batch > 1 else 2) * args.serve_slots
+    queues = max((3 if args.serve_batch > 1 else 2) * args.serve_slots,
+                 3 * args.serve_group_slots)
     if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < queues:
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, queues))
     if args.gpus != 1:
@@ -613,6 +618,12 @@ def main():
         out["c4"] = c4_bench(mc, args)
     if not args.no_serve:
         out["serve"] = serve_bench(mc, args)
+        if args.serve_group_slots > 0:
+            # the same stream at the same arrival rate through KN groups (iteration-level batching)
+            g = argparse.Namespace(**vars(args))
+            g.serve_slots, g.serve_batch = args.serve_group_slots, args.serve_group_max
+            g.serve_rate = out["serve"]["rate_per_s"]
+            out["serve_grouped"] = serve_bench(mc, g)
     if not args.no_cpu:
         log("bench: cpu baseline (BLAS pipeline, %d-row sample)" % args.cpu_rows)
         out["cpu_baseline"] = cpu_baseline(m, n, k, p, N, min(args.cpu_rows, m))
