@@ -312,6 +312,47 @@ def serve_bench(mc, args):
                         "SVD former, stream seed 3"}
 
 
+def host_stream_bench(mc, P, A64, A32, s64, smx, args):
+    """The same c2/c3 solves on a host A in pinned memory (mcnla_solver_solve_host): A streamed
+    over PCIe in chunks beside the sketch of the chunks already on the device, against the plain
+    H2D copy of A alone and the device-resident solve.  PCIe-inclusive rates; not the headline."""
+    import ctypes
+    L = mc.lib()
+    out = {}
+    for name, dA, s, ctype in (("fp64", A64, s64, ctypes.c_double), ("mixed", A32, smx,
+                                                                     ctypes.c_float)):
+        m, n = dA.shape
+        hp = ctypes.c_void_p()
+        mc._lib.check(L.mcnla_host_malloc(ctypes.byref(hp), dA.nbytes))
+        try:
+            flat = np.ctypeslib.as_array(ctypes.cast(hp.value, ctypes.POINTER(ctype)),
+                                         shape=(m * n,))
+            hA = flat.reshape((m, n), order="F")
+            mc._lib.check(L.mcnla_memcpy_dtoh(hp.value, dA.ptr, dA.nbytes))
+            t0 = time.perf_counter()
+            mc._lib.check(L.mcnla_memcpy_htod(dA.ptr, hp.value, dA.nbytes))
+            h2d = time.perf_counter() - t0
+            s.solve(dA)
+            S_dev = s.vectorS()
+            s.solve_host(hA)  # warm-up: the solver's device copy of A is allocated here
+            t0 = time.perf_counter()
+            for _ in range(args.host_steps):
+                s.solve_host(hA)
+            dt = (time.perf_counter() - t0) / args.host_steps
+            S_host = s.vectorS()
+            out[name] = {"ms_per_solve": round(dt * 1e3, 2), "solves_per_s": round(1.0 / dt, 3),
+                         "h2d_copy_ms": round(h2d * 1e3, 2),
+                         "h2d_GBps": round(dA.nbytes / h2d / 1e9, 1),
+                         "sigma_rel_vs_device_solve": float(np.max(np.abs(S_host - S_dev) / S_dev)),
+                         "a_bytes": int(dA.nbytes)}
+            log("bench: host-streamed %s %.1f ms/solve (H2D of A alone %.1f ms, %.1f GB/s)"
+                % (name, dt * 1e3, h2d * 1e3, dA.nbytes / h2d / 1e9))
+            del flat, hA
+        finally:
+            L.mcnla_host_free(hp.value)
+    return out
+
+
 def c4_bench(mc, args):
     """The tall row-block workload with the Wen-Yin integrator: A 500000 x 2000 FP32 row-major
     with sigma_i = 1/(1+i)^2, k=50, p=14, N=128 (Nl = 8192 > n).  The solver's automatic route
@@ -443,6 +484,8 @@ def main():
                     "serving run, through KN groups at the first run's arrival rate (0: skip it)")
     ap.add_argument("--serve-group-max", type=int, default=16, help="members per KN group of the "
                     "second serving run")
+    ap.add_argument("--host-steps", type=int, default=1, help="c2/c3 solves of a host A streamed "
+                    "over PCIe (mcnla_solver_solve_host), per precision; 0 skips them")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--only-serve", action="store_true", help="run the serving workload alone "
                     "and print its record")
@@ -553,6 +596,8 @@ def main():
             % (name, dtx / args.steps * 1e3, accurate[name]["sigma_rel_err_vs_fp64"]))
         del sx, ref
 
+    host = host_stream_bench(mc, P, A64, A32, s64, smx, args) if args.host_steps > 0 else None
+
     nl = N * l
     flops_mx = 2.0 * m * n * nl * terms
     flops_f64 = 2.0 * m * n * nl
@@ -610,6 +655,7 @@ def main():
                   "speedup_vs_fp64": round(value / fp64_value, 2),
                   "kernels": kernels},
         "mixed_modes": accurate,
+        "host_stream": host,
         "stage_order": ["sketch", "orthogonalize", "integrate", "form"],
         "roofline": {k: v for k, v in kernels[dominant].items() if k != "ms"},
     }

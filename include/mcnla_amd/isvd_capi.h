@@ -278,6 +278,21 @@ mcnla_status mcnla_solver_required_bytes(const mcnla_params* params,
 mcnla_status mcnla_solver_set_seed(mcnla_solver s, uint64_t seed);
 mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
                                 int a_order, void* stream);
+/* The fused solve of a HOST matrix A (pageable or pinned host memory; lda, dtype, order as in
+ * mcnla_solver_solve), streamed to the device in chunks of `chunk` columns (column-major A) or
+ * rows (row-major A) on the solver's own copy stream (0: chunks of about 1 GiB, at most 64 of
+ * them).  On the sketch route in FP64, each chunk's share of the sketch Y = A Omega runs as soon
+ * as the chunk has landed, beside the copy of the next (column chunks: Y += A(:, K_k) Omega(K_k,
+ * :), the reference GPU sketchers' chunked la::mm with beta = 1,
+ * include/mcnla/isvd_gpu/sketcher/col_block_gaussian_projection_sketcher_gpu.hpp:108-134 and
+ * row_block_gaussian_projection_sketcher_gpu.hpp:111-137; row chunks of a row-major A: their
+ * rows of Y); the other routes and precisions take the copy first.
+ * The solver keeps the device copy of A (m n elements) for the rest of the solve and for later
+ * calls.  Results equal mcnla_solver_solve's on the same A to rounding (the chunked sums only
+ * reorder the K sum); with a mixed precision or on the Gram-of-A route they are bit-identical.
+ * A solver created by a server cannot grow this copy (MCNLA_ERR_STATE). */
+mcnla_status mcnla_solver_solve_host(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
+                                     int a_order, int64_t chunk, void* stream);
 /* Up to MCNLA_MAX_BATCH fused solves on one stream, the KN iterations of all of them in lockstep:
  * every iteration is one launch per kernel for the whole batch (not one per solve), and the host
  * reads the batch's convergence flags once per chunk of iterations.  Solver i solves A[i] (lda,

@@ -159,6 +159,7 @@ SIGNATURES = {
     "mcnla_solver_workspace_bytes": (_st, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "mcnla_solver_set_seed": (_st, [_vp, _u64]),
     "mcnla_solver_solve": (_st, [_vp, _vp, _i64, _i, _i, _vp]),
+    "mcnla_solver_solve_host": (_st, [_vp, _vp, _i64, _i, _i, _i64, _vp]),
     "mcnla_solver_solve_batch": (_st, [_i, ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int),

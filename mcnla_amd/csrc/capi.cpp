@@ -751,6 +751,10 @@ struct mcnla_solver_s {
   mcnla::FormCore form;
   mcnla::DevBuf Y, G, W, w, V, T, Bs, tmp, ws, coef, M2, Qbar, bad;
   mcnla::DevBuf L0, T0, W0;  // first block's Cholesky factor, eigen transform, and rotation
+  // mcnla_solver_solve_host: the device copy of a host A, the copy stream and one event per chunk
+  mcnla::DevBuf Acopy;
+  hipStream_t copy_stream = nullptr;
+  std::vector<hipEvent_t> chunk_ev;
   // Gram-of-A route: A as row-major FP64, H = A'A, P = H Omega, X = Omega (T M), split-K scratch
   mcnla::DevBuf A64, H, P, X, gp;
   int route = MCNLA_ROUTE_SKETCH;
@@ -769,6 +773,8 @@ struct mcnla_solver_s {
     // the cores' and timer's events are the SolveRes's; drop them before the set goes
     timer.ev.clear();
     own_res.destroy();
+    for (hipEvent_t e : chunk_ev) (void)hipEventDestroy(e);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
   }
   // Creates the solver's own set if it has none, and points the timer and cores at it.
   void bind_res() {
@@ -989,14 +995,13 @@ void solve_check(mcnla_solver s, const void* A, int64_t lda, int a_dtype, int a_
   check_a(s->prm, A, lda, a_dtype, a_order);
 }
 
-void solve_begin(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, int a_order,
-                 hipStream_t st, SolveCtx& cx, bool timed) {
+// Stage 1 of a fused solve on `st` (after bind_res and the timer's start): Omega, then the
+// sketch Y = A Omega, or H = A'A on the Gram-of-A route.
+void solve_sketch(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, int a_order,
+                  hipStream_t st, SolveCtx& cx) {
   const auto& p = s->prm;
   const auto& c = s->cfg;
-  const int l = static_cast<int>(p.rank + p.over_rank);
-  const int64_t N = p.num_sketch, nl = l * N, m = p.nrow;
-  s->bind_res();
-  if (timed) s->timer.start(st);
+  const int64_t m = p.nrow;
   cx.ldar = p.ncol;
   cx.rotate = !is_combining(c.integrator);
   const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
@@ -1016,7 +1021,16 @@ void solve_begin(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, int
     }
     s->gram.run(cx.Ar, false, m, n, s->H.as<double>(), st);
   }
-  if (timed) s->timer.mark(st);
+}
+
+// Stage 2: G = Y'Y (or Omega'(H Omega)), the per-block transforms T and Bs = T'GT.
+void solve_orthogonalize(mcnla_solver_s* s, hipStream_t st, SolveCtx& cx) {
+  const auto& p = s->prm;
+  const auto& c = s->cfg;
+  const int l = static_cast<int>(p.rank + p.over_rank);
+  const int64_t N = p.num_sketch, nl = l * N, m = p.nrow, n = p.ncol;
+  const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
+  (void)c;
   // 2. orthogonalize through the Gram matrix: G = Y'Y, T_i from G_ii, Bs = T' G T.  The
   // reference's T_i = V_i S_i^{-1/2} (eig of G_ii) needs a Jacobi eigen-solve per block; for
   // the KN and WY integrators, which are equivariant in the basis of the first sketch (Qbar
@@ -1059,6 +1073,15 @@ void solve_begin(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, int
                     s->bad.as<int>(), st);
   }
   congruence_blockdiag(s->G.as<double>(), s->T.as<double>(), s->Bs.as<double>(), N, l, st);
+}
+
+void solve_begin(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, int a_order,
+                 hipStream_t st, SolveCtx& cx, bool timed) {
+  s->bind_res();
+  if (timed) s->timer.start(st);
+  solve_sketch(s, A, lda, a_dtype, a_order, st, cx);
+  if (timed) s->timer.mark(st);
+  solve_orthogonalize(s, st, cx);
   if (timed) s->timer.mark(st);
 }
 
@@ -1885,6 +1908,108 @@ mcnla_status mcnla_solver_solve(mcnla_solver s, const void* A, int64_t lda, int 
                                    s->graphs.get());
     s->converged = cx.conv;
     solve_finish(s, A, lda, a_dtype, a_order, st, cx, true);
+    int bad = 0;
+    read_bad(s, &bad, st);
+    MCNLA_HIP(hipStreamSynchronize(st));
+    return solve_status(s, cx, bad);
+  });
+}
+
+mcnla_status mcnla_solver_solve_host(mcnla_solver s, const void* A, int64_t lda, int a_dtype,
+                                     int a_order, int64_t chunk, void* stream) {
+  return guard([&] {
+    MCNLA_REQUIRE(s, MCNLA_ERR_INVALID, "solver is null");
+    check_a(s->prm, A, lda, a_dtype, a_order);
+    MCNLA_REQUIRE(chunk >= 0, MCNLA_ERR_INVALID, "chunk must be >= 0");
+    const auto& p = s->prm;
+    const auto& c = s->cfg;
+    const int l = static_cast<int>(p.rank + p.over_rank);
+    const int64_t nl = l * p.num_sketch;
+    const bool cm = a_order == MCNLA_COL_MAJOR;
+    const size_t esz = a_dtype == MCNLA_F32 ? sizeof(float) : sizeof(double);
+    // A travels as `lines` lines of `len` elements: columns of a column-major A, rows of a
+    // row-major one; its device copy is dense (leading dimension len)
+    const int64_t len = cm ? p.nrow : p.ncol, lines = cm ? p.ncol : p.nrow;
+    s->Acopy.ensure(static_cast<size_t>(len) * static_cast<size_t>(lines) * esz);
+    if (!s->copy_stream) MCNLA_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    constexpr int64_t kMaxChunks = 64;
+    const int64_t line_bytes = len * static_cast<int64_t>(esz);
+    int64_t w = chunk > 0 ? chunk : std::max<int64_t>(1, (int64_t(1) << 30) / line_bytes);
+    w = std::max(w, ceil_div(lines, kMaxChunks));
+    const int64_t nch = ceil_div(lines, w);
+    while (static_cast<int64_t>(s->chunk_ev.size()) < nch) {
+      hipEvent_t e = nullptr;
+      MCNLA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      s->chunk_ev.push_back(e);
+    }
+    hipStream_t st = S(stream);
+    char* dA = s->Acopy.as<char>();
+    SolveCtx cx;
+    s->bind_res();
+    s->timer.start(st);
+    // The copy stream starts after what the caller enqueued on `st` before this call.
+    MCNLA_HIP(hipEventRecord(s->chunk_ev[0], st));
+    MCNLA_HIP(hipStreamWaitEvent(s->copy_stream, s->chunk_ev[0], 0));
+    // On the sketch route in FP64, chunk k's share of Y = A Omega runs as soon as the chunk has
+    // landed, beside the copy of the next one: column chunks add A(:, K_k) Omega(K_k, :) into Y
+    // (the chunked la::mm with beta = 1 of col_block_gaussian_projection_sketcher_gpu.hpp:108-134),
+    // row chunks write their rows of Y.  Otherwise the copy completes first and the solve runs
+    // on the copy.
+    const bool chunked = s->route == MCNLA_ROUTE_SKETCH && c.precision == MCNLA_PREC_F64;
+    if (chunked)
+      s->sketch.generate(p, c.seed, c.omega_mode, c.omega_threads, col_block_chain(c.sketcher),
+                         st);
+    for (int64_t k = 0; k < nch; ++k) {
+      const int64_t l0 = k * w, l1 = std::min(lines, l0 + w);
+      MCNLA_HIP(hipMemcpy2DAsync(dA + l0 * line_bytes, line_bytes,
+                                 static_cast<const char*>(A) + l0 * lda * esz, lda * esz,
+                                 line_bytes, l1 - l0, hipMemcpyHostToDevice, s->copy_stream));
+      MCNLA_HIP(hipEventRecord(s->chunk_ev[k], s->copy_stream));
+      if (!chunked) continue;
+      MCNLA_HIP(hipStreamWaitEvent(st, s->chunk_ev[k], 0));
+      GemmF64Args g;
+      g.N = nl;
+      g.a_f32 = (a_dtype == MCNLA_F32);
+      g.ldb = nl;
+      g.b_ncontig = true;
+      g.ldc = nl;
+      g.shape = GemmShape::kSquare;
+      if (cm) {  // Y += A(:, l0:l1) Omega(l0:l1, :)
+        g.M = p.nrow;
+        g.K = l1 - l0;
+        g.A = dA + l0 * line_bytes;
+        g.lda = len;
+        g.a_mcontig = true;
+        g.B = s->sketch.omega.as<double>() + l0 * nl;
+        g.C = s->Y.as<double>();
+        g.beta = k == 0 ? 0.0 : 1.0;
+      } else {  // Y(l0:l1, :) = A(l0:l1, :) Omega
+        g.M = l1 - l0;
+        g.K = p.ncol;
+        g.A = dA + l0 * line_bytes;
+        g.lda = len;
+        g.a_mcontig = false;
+        g.B = s->sketch.omega.p;
+        g.C = s->Y.as<double>() + l0 * nl;
+      }
+      gemm_f64(g, st);
+    }
+    if (chunked) {
+      cx.ldar = p.ncol;
+      cx.rotate = !is_combining(c.integrator);
+    } else {
+      MCNLA_HIP(hipStreamWaitEvent(st, s->chunk_ev[nch - 1], 0));
+      solve_sketch(s, dA, len, a_dtype, a_order, st, cx);
+    }
+    s->timer.mark(st);
+    solve_orthogonalize(s, st, cx);
+    s->timer.mark(st);
+    if (c.graph && !s->graphs) s->graphs = std::make_unique<IterationGraphs>();
+    s->iteration = run_integration(c.integrator, s->Bs.as<double>(), p.num_sketch, l, c.maxiter,
+                                   c.tol, s->coef.as<double>(), s->ws, st, &cx.conv, &cx.ok,
+                                   s->graphs.get());
+    s->converged = cx.conv;
+    solve_finish(s, dA, len, a_dtype, a_order, st, cx, true);
     int bad = 0;
     read_bad(s, &bad, st);
     MCNLA_HIP(hipStreamSynchronize(st));

@@ -282,16 +282,24 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
   // to the last block), overwrites stage kt - 1's slot, whose fragments every wave consumed in
   // step kt - 1, before this barrier.  The last step's look-ahead reads a slot that holds the
   // last block again and discards it.
+  // The MFMAs of this step depend only on fragments already in registers, so the step's memory
+  // instructions (the DMA of stage kt + 4, the fragment reads of stage kt + 1) are issued between
+  // rows of MFMAs, in an order fixed by compiler fences: the matrix pipe gets work right after
+  // the barrier instead of after ~80 address and memory instructions.
   auto step = [&](int64_t kt, const u32x4* caf, const u32x4* cbf, const uint16_t* snext,
                   u32x4* naf, u32x4* nbf, uint16_t* sn) {
     asm volatile("" ::: "memory");
     wait_vmcnt_lgkm0<2 * L>();  // and this step's fragments (read last step) are in registers
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
+#pragma unroll
+    for (int j = 0; j < 8; ++j) mfma_tied<FMT>(acc[0][j], caf[0], cbf[j]);
+    asm volatile("" ::: "memory");
     issue(kt + 4 < ktot ? kt + 4 : ktot - 1, sn);
     frags(snext, naf, nbf);
+    asm volatile("" ::: "memory");
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 1; i < 8; ++i)
 #pragma unroll
       for (int j = 0; j < 8; ++j) mfma_tied<FMT>(acc[i][j], caf[i], cbf[j]);
   };

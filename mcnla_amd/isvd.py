@@ -530,6 +530,25 @@ class Solver:
 
     __call__ = solve
 
+    def solve_host(self, matrix_a, chunk=0, stream=None, allow_not_converged=True):
+        """Solves a host numpy matrix (FP64 or FP32, C or F order), streamed to the device in
+        chunks of ``chunk`` columns (F order) or rows (C order) beside the sketch of the chunks
+        that have landed (``mcnla_solver_solve_host``)."""
+        a = np.asarray(matrix_a)
+        if a.dtype not in (np.float64, np.float32):
+            raise TypeError("A must be float64 or float32")
+        if a.flags.f_contiguous and not a.flags.c_contiguous:
+            order, ld = _lib.COL_MAJOR, a.shape[0]
+        else:
+            a = np.ascontiguousarray(a)
+            order, ld = _lib.ROW_MAJOR, a.shape[1]
+        dt = _lib.F64 if a.dtype == np.float64 else _lib.F32
+        allow = (_lib.ERR_NOT_CONVERGED,) if allow_not_converged else ()
+        st = _lib.check(_lib.lib().mcnla_solver_solve_host(self._h, a.ctypes.data, ld, dt, order,
+                                                           int(chunk), stream), allow=allow)
+        self.converged = st == _lib.OK
+        return self
+
     def iteration(self):
         it = ctypes.c_int64()
         _lib.check(_lib.lib().mcnla_solver_iteration(self._h, ctypes.byref(it)))
