@@ -221,13 +221,14 @@ __device__ __forceinline__ void wait_vmcnt() {
 // Y (m x nl) = sum over segments s of A_seg(s) * B_seg(s), from blocks
 // Ab[tm][a segment][kt] and Wb[tn][b segment][kt]: step kt of the K loop (0 <= kt < nseg * nkt)
 // reads k block kt - s*nkt of A segment amap[s] and B segment bmap[s], s = kt / nkt (2 bits per
-// segment in amap/bmap).  256 x 256 tiles, 4 waves of 128 x 128 (v_mfma_f32_16x16x32_bf16/_f16,
-// 256 accumulators per lane).  Blocks stream from HBM into LDS with global_load_lds_dwordx4
-// through a five-slot ring.  The ring buffers are distinct LDS arrays with static indices and the
-// loop body has no branches (the stage four ahead is issued every step, clamped to the last
-// block; the K step count is a multiple of 10), so the compiler keeps per-buffer wait tracking
-// and the only waits are the explicit ones.  rowmax (FP16): row i of Y is multiplied by 2^e_i.
-template <typename TY, int FMT>
+// segment in amap/bmap; ONESEG: a single segment, block kt).  256 x 256 tiles, 4 waves of
+// 128 x 128 (v_mfma_f32_16x16x32_bf16/_f16, 256 accumulators per lane).  Blocks stream from HBM
+// into LDS with global_load_lds_dwordx4 through a five-slot ring.  The ring buffers are distinct
+// LDS arrays with static indices and the loop body has no branches (the stage four ahead is
+// issued every step, clamped to the last block; the K step count is a multiple of 10), so the
+// compiler keeps per-buffer wait tracking and the only waits are the explicit ones.
+// rowmax (FP16): row i of Y is multiplied by 2^e_i.
+template <typename TY, int FMT, bool ONESEG>
 __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restrict__ Ab,
                                                         const uint16_t* __restrict__ Wb,
                                                         int64_t nkt, int nseg, int nseg_a,
@@ -248,18 +249,31 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
   const int64_t ktot = nseg * nkt;
   const uint16_t* abase = Ab + tm * nseg_a * nkt * kBlk;
   const uint16_t* bbase = Wb + tn * nseg_b * nkt * kBlk;
-  auto issue = [&](int64_t kt, uint16_t* sd) {
-    // segment of step kt without a division: at most three segments
-    const int s = static_cast<int>(kt >= nkt) + static_cast<int>(kt >= 2 * nkt);
-    const int64_t kk = kt - s * nkt;
-    const uint16_t* ak = abase + (((amap >> (2 * s)) & 3) * nkt + kk) * kBlk;
-    const uint16_t* bk = bbase + (((bmap >> (2 * s)) & 3) * nkt + kk) * kBlk;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int off = (q * 4 + wave) * 512;  // 1 KiB per wave instruction
-      __builtin_amdgcn_global_load_lds(ak + off + lane * 8, (lds_void*)&sd[off], 16, 0, 0);
-      __builtin_amdgcn_global_load_lds(bk + off + lane * 8, (lds_void*)&sd[kBlk + off], 16, 0, 0);
+  // the A and B blocks of step kt
+  auto blocks = [&](int64_t kt, const uint16_t*& ak, const uint16_t*& bk) {
+    if constexpr (ONESEG) {
+      ak = abase + kt * kBlk;
+      bk = bbase + kt * kBlk;
+    } else {  // segment of step kt without a division: at most three segments
+      const int s = static_cast<int>(kt >= nkt) + static_cast<int>(kt >= 2 * nkt);
+      const int64_t kk = kt - s * nkt;
+      ak = abase + (((amap >> (2 * s)) & 3) * nkt + kk) * kBlk;
+      bk = bbase + (((bmap >> (2 * s)) & 3) * nkt + kk) * kBlk;
     }
+  };
+  // DMA instruction i of a stage (i / 2 = 1 KiB chunk of the wave, i % 2 = A or B)
+  auto dma = [&](const uint16_t* ak, const uint16_t* bk, uint16_t* sd, int i) {
+    const int off = ((i >> 1) * 4 + wave) * 512;  // 1 KiB per wave instruction
+    if (i & 1)
+      __builtin_amdgcn_global_load_lds(bk + off + lane * 8, (lds_void*)&sd[kBlk + off], 16, 0, 0);
+    else
+      __builtin_amdgcn_global_load_lds(ak + off + lane * 8, (lds_void*)&sd[off], 16, 0, 0);
+  };
+  auto issue = [&](int64_t kt, uint16_t* sd) {
+    const uint16_t *ak, *bk;
+    blocks(kt, ak, bk);
+#pragma unroll
+    for (int i = 0; i < L; ++i) dma(ak, bk, sd, i);
   };
   f4 acc[8][8];
 #pragma unroll
@@ -281,27 +295,30 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
   // flight); the barrier makes that true in every wave.  The stage issued here, kt + 4 (clamped
   // to the last block), overwrites stage kt - 1's slot, whose fragments every wave consumed in
   // step kt - 1, before this barrier.  The last step's look-ahead reads a slot that holds the
-  // last block again and discards it.
-  // The MFMAs of this step depend only on fragments already in registers, so the step's memory
-  // instructions (the DMA of stage kt + 4, the fragment reads of stage kt + 1) are issued between
-  // rows of MFMAs, in an order fixed by compiler fences: the matrix pipe gets work right after
-  // the barrier instead of after ~80 address and memory instructions.
+  // last block again and discards it.  The step's MFMAs depend only on fragments already in
+  // registers, so its memory instructions go between the rows of MFMAs (after row i: DMA
+  // instruction i and stage kt + 1's fragments i of A and B), in an order fixed by compiler
+  // fences: issued in one burst, 16 fragment reads and 8 DMA instructions stalled the wave's
+  // MFMA issue behind the LDS queue (measured: the burst form took ~15 % longer).
   auto step = [&](int64_t kt, const u32x4* caf, const u32x4* cbf, const uint16_t* snext,
                   u32x4* naf, u32x4* nbf, uint16_t* sn) {
     asm volatile("" ::: "memory");
     wait_vmcnt_lgkm0<2 * L>();  // and this step's fragments (read last step) are in registers
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
+    const uint16_t *ak, *bk;
+    blocks(kt + 4 < ktot ? kt + 4 : ktot - 1, ak, bk);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) mfma_tied<FMT>(acc[0][j], caf[0], cbf[j]);
-    asm volatile("" ::: "memory");
-    issue(kt + 4 < ktot ? kt + 4 : ktot - 1, sn);
-    frags(snext, naf, nbf);
-    asm volatile("" ::: "memory");
-#pragma unroll
-    for (int i = 1; i < 8; ++i)
+    for (int i = 0; i < 8; ++i) {
 #pragma unroll
       for (int j = 0; j < 8; ++j) mfma_tied<FMT>(acc[i][j], caf[i], cbf[j]);
+      asm volatile("" ::: "memory");
+      dma(ak, bk, sn, i);
+      naf[i] = *reinterpret_cast<const u32x4*>(&snext[blk_off(wm0 + i * 16 + lane % 16, 8 * kg)]);
+      nbf[i] = *reinterpret_cast<const u32x4*>(
+          &snext[kBlk + blk_off(wn0 + i * 16 + lane % 16, 8 * kg)]);
+      asm volatile("" ::: "memory");
+    }
   };
   issue(0, S0);
   issue(1, S1);
@@ -477,9 +494,14 @@ void sketch_bf16(const SketchBf16Args& a, void* workspace, size_t workspace_byte
   auto launch = [&](auto* Y, auto fmt_tag) {
     using TY = std::remove_pointer_t<decltype(Y)>;
     constexpr int FMT = decltype(fmt_tag)::value;
-    hipLaunchKernelGGL((sketch_blk_kernel<TY, FMT>), grid, dim3(256), 0, st, w.Ab, w.Wb, d.nkt,
-                       t.nseg, t.nseg_a, t.nseg_b, t.amap, t.bmap,
-                       static_cast<const uint32_t*>(w.rowmax), a.m, a.nl, Y, a.ldy);
+    if (t.nseg == 1)
+      hipLaunchKernelGGL((sketch_blk_kernel<TY, FMT, true>), grid, dim3(256), 0, st, w.Ab, w.Wb,
+                         d.nkt, t.nseg, t.nseg_a, t.nseg_b, t.amap, t.bmap,
+                         static_cast<const uint32_t*>(w.rowmax), a.m, a.nl, Y, a.ldy);
+    else
+      hipLaunchKernelGGL((sketch_blk_kernel<TY, FMT, false>), grid, dim3(256), 0, st, w.Ab, w.Wb,
+                         d.nkt, t.nseg, t.nseg_a, t.nseg_b, t.amap, t.bmap,
+                         static_cast<const uint32_t*>(w.rowmax), a.m, a.nl, Y, a.ldy);
   };
   using Bf = std::integral_constant<int, kFmtBf16>;
   using F16 = std::integral_constant<int, kFmtF16>;
