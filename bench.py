@@ -329,6 +329,7 @@ def host_stream_bench(mc, P, A64, A32, s64, smx, args):
                                          shape=(m * n,))
             hA = flat.reshape((m, n), order="F")
             mc._lib.check(L.mcnla_memcpy_dtoh(hp.value, dA.ptr, dA.nbytes))
+            mc._lib.check(L.mcnla_memcpy_htod(dA.ptr, hp.value, dA.nbytes))  # pages mapped
             t0 = time.perf_counter()
             mc._lib.check(L.mcnla_memcpy_htod(dA.ptr, hp.value, dA.nbytes))
             h2d = time.perf_counter() - t0
@@ -373,7 +374,10 @@ def c4_bench(mc, args):
     P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
     kw = dict(sketcher=mc.RowBlockGaussianProjectionSketcher,
               integrator=mc.RowBlockWenYinIntegrator, former=mc.RowBlockGramianFormer, seed=2)
-    s = mc.Solver(P, precision=mc.PREC_MIXED_BF16, graph=args.graph, **kw)
+    # FP16 with per-row scaling as the mixed precision (on the Gram-of-A route it forms the basis
+    # Q = A X of the FP32 A on FP32 MFMA; there is no 16-bit sketch there), and the integrator's
+    # chunks of iterations replayed as HIP graphs recorded in the warm-up solve
+    s = mc.Solver(P, precision=mc.PREC_MIXED_F16, graph=args.c4_graph, **kw)
     tuning = tune(s, A, args, "c4")
     dt, kern, stages = timed_solves(mc, s, A, args.c4_steps, 1)
     S = s.vectorS()
@@ -386,7 +390,8 @@ def c4_bench(mc, args):
         % (dref * 1e3, ref.iteration()))
     out = {"workload": "iSVD 500000x2000 FP32 row-major, sigma_i=1/(1+i)^2, k=50 p=14 N=128 "
                        "(Nl=8192), row-block Wen-Yin tol 1e-3, row-block Gramian former",
-           "route": "gram_of_a (auto)", "solves_per_s": round(args.c4_steps / dt, 3),
+           "route": "gram_of_a (auto)", "precision": "MIXED_F16", "graph": bool(args.c4_graph),
+           "solves_per_s": round(args.c4_steps / dt, 3),
            "ms_per_solve": round(dt / args.c4_steps * 1e3, 3), "steps": args.c4_steps,
            "iterations": it, "stage_ms": [round(x * 1e3, 3) for x in stages.mean(axis=0)],
            "tuning": tuning,
@@ -493,6 +498,8 @@ def main():
     ap.add_argument("--c4-steps", type=int, default=3)
     ap.add_argument("--graph", type=int, default=0, help="replay the integrator iterations as "
                     "HIP graphs recorded in the warm-up solve")
+    ap.add_argument("--c4-graph", type=int, default=1, help="replay c4's integrator iterations "
+                    "as HIP graphs (recorded in its warm-up solve)")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--autotune", type=int, default=1, help="tune each solver's split-K plans "
                     "before its warm-up (untimed); 0 keeps the default plans")
