@@ -342,10 +342,11 @@ mcnla_status mcnla_solve_static(const mcnla_params* params, const mcnla_solver_c
 
 /* ---------------------------------------------------------------------------------------------
  * Autotuner.  No reference counterpart (the reference has no device plans to tune).  The FP64
- * GEMMs of a solve split K so that their grid holds about `target` workgroups; the defaults are
- * the values measured best at the 100000 x 10000 (Nl = 2048) and 500000 x 2000 (Nl = 8192)
- * shapes.  A split count changes only the order of the split-K sums, so a tuned plan gives the
- * same results to rounding, and the same plan always gives the same bits.
+ * GEMMs of a solve split K so that their grid holds about `target` workgroups, and each runs on
+ * one of two kernels with different tile shapes (`kernels`); the defaults are the values
+ * measured best at the 100000 x 10000 (Nl = 2048) and 500000 x 2000 (Nl = 8192) shapes.  A split
+ * count or tile shape changes only the order of the sums, so a tuned plan gives the same results
+ * to rounding, and the same plan always gives the same bits.
  *
  * Tuned plans are kept in a process-wide table keyed by (route, precision, m, n, l, N); a solver
  * created after an entry exists for its key uses it (servers too, per request shape). */
@@ -356,6 +357,11 @@ typedef struct {
   int64_t omega_p;  /* G = Omega'P (Gram-of-A route) */
   int64_t n_by_l;   /* X = Omega (T M) and Z = H X (Gram-of-A route) */
   int64_t qbar;     /* Qbar = Y (T M) (sketch route) */
+  int64_t kernels;  /* tile shape per GEMM, a bit each (set: the alternative kernel): 1 gram,
+                       2 former, 4 h_omega, 8 omega_p, 16 n_by_l, 32 qbar.  Square products:
+                       256 x 128 DMA tiles (clear) or register-staged 128 x 128 tiles (set);
+                       tall-skinny products: A through LDS slabs or straight into MFMA
+                       registers (the default is the faster per operand type). */
 } mcnla_tuning;
 
 /* The defaults (no table entry). */
@@ -364,7 +370,8 @@ void mcnla_tuning_default(mcnla_tuning* t);
 mcnla_status mcnla_solver_get_tuning(mcnla_solver s, mcnla_tuning* t);
 /* Times candidate targets for each GEMM of this solver's route on A (as mcnla_solver_solve takes
  * it; a few solves per candidate, `reps` >= 1 of them timed), one GEMM at a time with the others
- * fixed, keeps a candidate only if its stage is faster than the incumbent's by more than 1 %,
+ * fixed, then that GEMM's other kernel (tile shape) at the chosen target, and keeps a candidate
+ * only if its stage is faster than the incumbent's by more than 1 %,
  * applies the result to the solver and records it in the table.  The solver's results afterwards
  * are those of a solve with the tuned plan.  Synchronous.  MCNLA_ERR_STATE for a solver carved
  * from a reserved region (a server's), whose workspace cannot grow. */
@@ -378,7 +385,8 @@ mcnla_status mcnla_tune_get(const mcnla_params* params, const mcnla_solver_confi
                             mcnla_tuning* t, int* found);
 void mcnla_tune_clear(void);
 /* Text persistence, one entry a line: "route precision m n l N gram former h_omega omega_p
- * n_by_l qbar".  Load merges into the table; a malformed line is MCNLA_ERR_INVALID and loads nothing. */
+ * n_by_l qbar kernels" (a line without `kernels` reads as 0).  Load merges into the table; a
+ * malformed line is MCNLA_ERR_INVALID and loads nothing. */
 mcnla_status mcnla_tune_save(const char* path);
 mcnla_status mcnla_tune_load(const char* path);
 

@@ -61,9 +61,11 @@ class SolverConfig(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    """Split-K workgroup targets of a solve's FP64 GEMMs (mcnla_tuning)."""
+    """Split-K workgroup targets of a solve's FP64 GEMMs and their kernels' tile shapes (a bit
+    per GEMM in `kernels`: 1 gram, 2 former, 4 h_omega, 8 omega_p, 16 n_by_l, 32 qbar)."""
     _fields_ = [("gram", ctypes.c_int64), ("former", ctypes.c_int64), ("h_omega", ctypes.c_int64),
-                ("omega_p", ctypes.c_int64), ("n_by_l", ctypes.c_int64), ("qbar", ctypes.c_int64)]
+                ("omega_p", ctypes.c_int64), ("n_by_l", ctypes.c_int64), ("qbar", ctypes.c_int64),
+                ("kernels", ctypes.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -54,7 +54,11 @@ mcnla_status guard(F&& f) {
 }
 
 // ---- tuned GEMM plans: workgroup targets of the split-K FP64 GEMMs per solve shape ----
-constexpr mcnla_tuning kDefaultTuning = {2304, 4096, 2304, 1024, 512, 512};
+constexpr mcnla_tuning kDefaultTuning = {2304, 4096, 2304, 1024, 512, 512, 0};
+// bits of mcnla_tuning::kernels
+constexpr int64_t kKernGram = 1, kKernFormer = 2, kKernHOmega = 4, kKernOmegaP = 8,
+                  kKernNbyL = 16, kKernQbar = 32, kKernAll = 63;
+int kern_bit(const mcnla_tuning& t, int64_t bit) { return (t.kernels & bit) ? 1 : 0; }
 constexpr int64_t kMaxTarget = 1 << 20;
 
 struct TuneKey {
@@ -74,7 +78,7 @@ TuneKey tune_key(const mcnla_params& p, int route, int precision) {
 bool valid_tuning(const mcnla_tuning& t) {
   for (int64_t v : {t.gram, t.former, t.h_omega, t.omega_p, t.n_by_l, t.qbar})
     if (v < 1 || v > kMaxTarget) return false;
-  return true;
+  return t.kernels >= 0 && t.kernels <= kKernAll;
 }
 mcnla_tuning lookup_tuning(const TuneKey& k, bool* found) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -341,14 +345,15 @@ struct GramCore {
     MCNLA_HIP(hipEventElapsedTime(&ms, beg, end));
     return ms * 1e-3;
   }
-  // workgroups the split-K grid aims at (about 9 per CU by default)
+  // workgroups the split-K grid aims at (about 9 per CU by default), and the kernel (tile shape)
   int64_t target = kDefaultTuning.gram;
+  int kern = 0;
   // Arguments of G = Y'Y except the pointers.
   GemmF64Args plan(bool y_f32, int64_t m, int64_t nl) const {
     // Lower-triangle tiles: 256 x 128 on the FP64 fast path (tm (tm+1) tiles), else 128 x 128.
     // Split K so that the grid fills whole rounds of about `target` workgroups.
     const int64_t tm = ceil_div(nl, 256), t = ceil_div(nl, 128);
-    const int64_t tiles = (!y_f32 && nl >= 128) ? tm * (tm + 1) : t * (t + 1) / 2;
+    const int64_t tiles = (!y_f32 && nl >= 128 && kern == 0) ? tm * (tm + 1) : t * (t + 1) / 2;
     GemmF64Args g;
     g.M = nl;
     g.N = nl;
@@ -362,6 +367,7 @@ struct GramCore {
     g.ldc = nl;
     g.lower_only = true;
     g.shape = GemmShape::kSquare;
+    g.kernel = kern;
     g.splits = gemm_f64_splits_for(tiles, m, target);
     g.kchunk = ceil_div(m, g.splits);  // equal splits (the GEMM rounds them up where it must)
     g.splits = ceil_div(m, g.kchunk);
@@ -396,6 +402,7 @@ struct FormCore {
   // in place of Z'Z and forms no V.
   // Split-K of Z = A'Qbar (K = m over ceil(n/128) row tiles) and of Wg = Z'Z (K = n).
   int64_t z_target = kDefaultTuning.former;  // workgroups of Z = A'Qbar's split-K grid
+  int z_kern = 0;                            // and its kernel
   int64_t z_kchunk(int64_t m, int64_t n) const {
     return ceil_div(ceil_div(m, gemm_f64_splits_for(ceil_div(n, 128), m, z_target)), 16) * 16;
   }
@@ -443,6 +450,7 @@ struct FormCore {
     z.ldc = l;
     z.shape = tall_shape(l);
     z.f32_mfma = mixed;
+    z.kernel = z_kern;
     z.kchunk = z_kchunk(m, n);
     z.splits = ceil_div(m, z.kchunk);
     // Wg = Z' Z (symmetric former: Z' Qbar, Qbar having n = m rows)
@@ -810,8 +818,9 @@ int resolve_route(const mcnla_params& p, const mcnla_solver_config& c) {
 }
 
 // P = H Omega (n x Nl): H symmetric, so its rows serve as the M-contiguous operand.
-GemmF64Args plan_h_omega(int64_t n, int64_t nl, int64_t target) {
+GemmF64Args plan_h_omega(int64_t n, int64_t nl, int64_t target, int kern) {
   GemmF64Args g;
+  g.kernel = kern;
   g.M = n;
   g.N = nl;
   g.K = n;
@@ -821,7 +830,9 @@ GemmF64Args plan_h_omega(int64_t n, int64_t nl, int64_t target) {
   g.b_ncontig = true;
   g.ldc = nl;
   g.shape = GemmShape::kSquare;
-  g.splits = gemm_f64_splits_for(ceil_div(n, 256) * ceil_div(nl, 128), n, target);
+  g.splits = gemm_f64_splits_for(kern == 0 ? ceil_div(n, 256) * ceil_div(nl, 128)
+                                           : ceil_div(n, 128) * ceil_div(nl, 128),
+                                 n, target);
   g.kchunk = ceil_div(ceil_div(n, g.splits), 16) * 16;
   g.splits = ceil_div(n, g.kchunk);
   return g;
@@ -839,8 +850,9 @@ int64_t qbar_splits(int64_t m, int64_t nl, int64_t target) {
 }
 
 // G = Omega' P (Nl x Nl, symmetric): lower-triangle tiles.
-GemmF64Args plan_omega_p(int64_t n, int64_t nl, int64_t target) {
+GemmF64Args plan_omega_p(int64_t n, int64_t nl, int64_t target, int kern) {
   GemmF64Args g;
+  g.kernel = kern;
   g.M = nl;
   g.N = nl;
   g.K = n;
@@ -854,7 +866,8 @@ GemmF64Args plan_omega_p(int64_t n, int64_t nl, int64_t target) {
   const int64_t tm = ceil_div(nl, 256), t = ceil_div(nl, 128);
   // by default about 4 rounds of the grid without splitting K where the tiles allow it: a split
   // writes and re-reads Nl^2 doubles of partials (1.6 GB per 3 splits at Nl = 8192)
-  g.splits = gemm_f64_splits_for(nl >= 128 ? tm * (tm + 1) : t * (t + 1) / 2, n, target);
+  g.splits = gemm_f64_splits_for(nl >= 128 && kern == 0 ? tm * (tm + 1) : t * (t + 1) / 2, n,
+                                 target);
   g.kchunk = ceil_div(ceil_div(n, g.splits), 16) * 16;
   g.splits = ceil_div(n, g.kchunk);
   return g;
@@ -864,13 +877,15 @@ GemmF64Args plan_omega_p(int64_t n, int64_t nl, int64_t target) {
 void apply_tuning(mcnla_solver_s* s, const mcnla_tuning& t) {
   s->tune = t;
   s->gram.target = t.gram;
+  s->gram.kern = (t.kernels & kKernGram) ? 1 : 0;
   s->form.z_target = t.former;
+  s->form.z_kern = (t.kernels & kKernFormer) ? 1 : 0;
   const int64_t l = s->prm.rank + s->prm.over_rank, nl = l * s->prm.num_sketch;
   const int64_t m = s->prm.nrow, n = s->prm.ncol;
   const bool via_a = s->route == MCNLA_ROUTE_GRAM_OF_A;
   if (via_a)
-    s->gp.ensure(std::max({gemm_f64_partial_elems(n, nl, plan_h_omega(n, nl, t.h_omega).splits),
-                           gemm_f64_partial_elems(nl, nl, plan_omega_p(n, nl, t.omega_p).splits),
+    s->gp.ensure(std::max({gemm_f64_partial_elems(n, nl, plan_h_omega(n, nl, t.h_omega, kern_bit(t, kKernHOmega)).splits),
+                           gemm_f64_partial_elems(nl, nl, plan_omega_p(n, nl, t.omega_p, kern_bit(t, kKernOmegaP)).splits),
                            gemm_f64_partial_elems(n, l, n_by_l_splits(n, nl, t.n_by_l)),
                            gemm_f64_partial_elems(n, l, n_by_l_splits(n, n, t.n_by_l))}) *
                  sizeof(double));
@@ -1042,13 +1057,13 @@ void solve_orthogonalize(mcnla_solver_s* s, hipStream_t st, SolveCtx& cx) {
   if (!via_a) {
     s->gram.run(s->Y.p, false, m, nl, s->G.as<double>(), st);
   } else {  // G = Omega' (H Omega)
-    GemmF64Args hp = plan_h_omega(n, nl, s->tune.h_omega);
+    GemmF64Args hp = plan_h_omega(n, nl, s->tune.h_omega, kern_bit(s->tune, kKernHOmega));
     hp.A = s->H.p;
     hp.B = s->sketch.omega.p;
     hp.C = s->P.as<double>();
     hp.partial = s->gp.as<double>();
     gemm_f64(hp, st);
-    GemmF64Args pg = plan_omega_p(n, nl, s->tune.omega_p);
+    GemmF64Args pg = plan_omega_p(n, nl, s->tune.omega_p, kern_bit(s->tune, kKernOmegaP));
     pg.A = s->sketch.omega.p;
     pg.B = s->P.p;
     pg.C = s->G.as<double>();
@@ -1155,6 +1170,7 @@ void solve_finish(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, in
   q.ldc = l;
   q.shape = tall_shape(l);
   if (!via_a) {
+    q.kernel = kern_bit(s->tune, kKernQbar);
     q.splits = qbar_splits(m, nl, s->tune.qbar);
     if (q.splits > 1) {
       q.kchunk = ceil_div(ceil_div(nl, q.splits), 16) * 16;
@@ -1167,6 +1183,7 @@ void solve_finish(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, in
     x.A = s->sketch.omega.p;
     x.C = s->X.as<double>();
     x.splits = n_by_l_splits(n, nl, s->tune.n_by_l);
+    x.kernel = kern_bit(s->tune, kKernNbyL);
     x.partial = s->gp.as<double>();
     gemm_f64(x, st);
     q.K = n;
@@ -1200,6 +1217,7 @@ void solve_finish(mcnla_solver_s* s, const void* A, int64_t lda, int a_dtype, in
     z.ldc = l;
     z.shape = tall_shape(l);
     z.splits = n_by_l_splits(n, n, s->tune.n_by_l);
+    z.kernel = kern_bit(s->tune, kKernNbyL);
     z.partial = s->gp.as<double>();
     gemm_f64(z, st);
   }
@@ -2093,22 +2111,23 @@ mcnla_status mcnla_solver_autotune(mcnla_solver s, const void* A, int64_t lda, i
                   "cannot grow)");
     check_a(s->prm, A, lda, a_dtype, a_order);
     // One knob per GEMM of the route, with the solve stages ([sketch, orthogonalize, integrate,
-    // form]) that hold it.
+    // form]) that hold it and its bit in `kernels` (the alternative tile shape).
     struct Knob {
       int64_t mcnla_tuning::*field;
       std::vector<int64_t> candidates;
       unsigned stages;
+      int64_t kernel_bit;
     };
     std::vector<Knob> knobs;
     if (s->route == MCNLA_ROUTE_GRAM_OF_A) {
-      knobs = {{&mcnla_tuning::gram, {1152, 2304, 4608}, 1u},
-               {&mcnla_tuning::h_omega, {1152, 2304, 4608}, 2u},
-               {&mcnla_tuning::omega_p, {256, 512, 1024, 2048}, 2u},
-               {&mcnla_tuning::n_by_l, {128, 256, 512, 1024, 2048}, 12u}};
+      knobs = {{&mcnla_tuning::gram, {1152, 2304, 4608}, 1u, kKernGram},
+               {&mcnla_tuning::h_omega, {1152, 2304, 4608}, 2u, kKernHOmega},
+               {&mcnla_tuning::omega_p, {256, 512, 1024, 2048}, 2u, kKernOmegaP},
+               {&mcnla_tuning::n_by_l, {128, 256, 512, 1024, 2048}, 12u, kKernNbyL}};
     } else {
-      knobs = {{&mcnla_tuning::gram, {576, 1152, 2304, 4608, 9216}, 2u},
-               {&mcnla_tuning::former, {1024, 2048, 4096, 8192}, 8u},
-               {&mcnla_tuning::qbar, {512, 1024, 2048, 3072, 4096}, 4u}};
+      knobs = {{&mcnla_tuning::gram, {576, 1152, 2304, 4608, 9216}, 2u, kKernGram},
+               {&mcnla_tuning::former, {1024, 2048, 4096, 8192}, 8u, kKernFormer},
+               {&mcnla_tuning::qbar, {512, 1024, 2048, 3072, 4096}, 4u, kKernQbar}};
     }
     auto solve = [&] {
       const mcnla_status r = mcnla_solver_solve(s, A, lda, a_dtype, a_order, stream);
@@ -2144,6 +2163,12 @@ mcnla_status mcnla_solver_autotune(mcnla_solver s, const void* A, int64_t lda, i
         }
       }
       t.*k.field = best_v;
+      apply_tuning(s, t);
+      // then the GEMM's other kernel (tile shape) at the chosen target
+      mcnla_tuning c = t;
+      c.kernels ^= k.kernel_bit;
+      apply_tuning(s, c);
+      if (measure(k.stages) < 0.99 * best_t) t = c;
       apply_tuning(s, t);
     }
     solve();  // leave the results of the tuned plan
@@ -2192,11 +2217,11 @@ mcnla_status mcnla_tune_save(const char* path) {
         const mcnla_tuning& t = e.second;
         char line[256];
         std::snprintf(line, sizeof line,
-                      "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld\n",
+                      "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld\n",
                       (long long)k.v[0], (long long)k.v[1], (long long)k.v[2], (long long)k.v[3],
                       (long long)k.v[4], (long long)k.v[5], (long long)t.gram,
                       (long long)t.former, (long long)t.h_omega, (long long)t.omega_p,
-                      (long long)t.n_by_l, (long long)t.qbar);
+                      (long long)t.n_by_l, (long long)t.qbar, (long long)t.kernels);
         text += line;
       }
     }
@@ -2220,19 +2245,19 @@ mcnla_status mcnla_tune_load(const char* path) {
     std::string bad;
     while (bad.empty() && std::fgets(line, sizeof line, f)) {
       ++lineno;
-      long long v[12];
+      long long v[13] = {0};
       char extra;
-      const int got =
-          std::sscanf(line, "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %c", &v[0],
-                      &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11],
-                      &extra);
+      const int got = std::sscanf(line,
+                                  "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %lld %c",
+                                  &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8],
+                                  &v[9], &v[10], &v[11], &v[12], &extra);
       bool blank = true;
       for (const char* c = line; *c; ++c)
         if (*c != ' ' && *c != '\t' && *c != '\n' && *c != '\r') blank = false;
       if (blank) continue;
       TuneKey k{{v[0], v[1], v[2], v[3], v[4], v[5]}};
-      const mcnla_tuning t{v[6], v[7], v[8], v[9], v[10], v[11]};
-      const bool key_ok = got == 12 &&
+      const mcnla_tuning t{v[6], v[7], v[8], v[9], v[10], v[11], v[12]};
+      const bool key_ok = (got == 12 || got == 13) &&
                           (k.v[0] == MCNLA_ROUTE_SKETCH || k.v[0] == MCNLA_ROUTE_GRAM_OF_A) &&
                           k.v[2] >= 1 && k.v[3] >= 1 && k.v[4] >= 1 && k.v[5] >= 1;
       if (!key_ok || !valid_tuning(t)) bad = path + std::string(":") + std::to_string(lineno);

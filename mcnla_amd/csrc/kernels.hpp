@@ -41,6 +41,10 @@ struct GemmF64Args {
   int gate_t = 0;
   // Optional events recorded around the main GEMM launch (not the split-K reduction).
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  // 0: the default kernel for the shape; 1: the alternative one (square shapes: the
+  // register-staged 128 x 128 tiles instead of the 256 x 128 DMA tiles; tall shapes: the other
+  // tall-skinny kernel, LDS-staged A instead of A straight into registers or the reverse).
+  int kernel = 0;
 };
 
 void gemm_f64(GemmF64Args p, hipStream_t st);

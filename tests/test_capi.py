@@ -206,7 +206,7 @@ def test_tune_table_round_trip(tmp_path):
     P = _shape(100000, 10000, 20, 12, 64)
     d = mc.tuning_default()
     assert d == {"gram": 2304, "former": 4096, "h_omega": 2304, "omega_p": 1024, "n_by_l": 512,
-                 "qbar": 512}
+                 "qbar": 512, "kernels": 0}
     got, found = mc.tune_get(P, precision=_lib.PREC_MIXED_BF16)
     assert not found and got == d
     t = dict(d, gram=4608, former=2048)
@@ -217,7 +217,7 @@ def test_tune_table_round_trip(tmp_path):
     assert not mc.tune_get(P, precision=_lib.PREC_MIXED_BF16, route=_lib.ROUTE_GRAM_OF_A)[1]
     assert not mc.tune_get(P, precision=_lib.PREC_F64)[1]  # precision is part of the key
     Q = _shape(500000, 2000, 50, 14, 128)  # resolves to the Gram-of-A route
-    u = dict(d, omega_p=512, n_by_l=256)
+    u = dict(d, omega_p=512, n_by_l=256, kernels=4 | 16)  # and the other tile shapes of two GEMMs
     mc.tune_set(Q, u)
     assert mc.tune_get(Q, route=_lib.ROUTE_GRAM_OF_A) == (u, True)
     path = tmp_path / "plans.txt"
@@ -233,6 +233,14 @@ def test_tune_table_round_trip(tmp_path):
         mc.tune_set(P, dict(d, gram=0))
     with pytest.raises(mc.McnlaError):
         mc.tune_set(P, dict(d, n_by_l=(1 << 20) + 1))
+    with pytest.raises(mc.McnlaError):
+        mc.tune_set(P, dict(d, kernels=64))  # six GEMMs, six bits
+    # a line in the earlier 12-number form (no kernels column) loads with kernels = 0
+    old = tmp_path / "old.txt"
+    old.write_text("1 0 100 100 8 4 2304 4096 2304 1024 512 512\n")
+    mc.tune_clear()
+    mc.tune_load(old)
+    assert mc.tune_get(_shape(100, 100, 4, 4, 4), route=_lib.ROUTE_SKETCH)[0]["kernels"] == 0
     bad = tmp_path / "bad.txt"
     bad.write_text("1 0 100 100 8 4 2304 4096 2304 1024 512 512\n1 0 100 100 8 4 2304 4096\n")
     mc.tune_clear()

@@ -333,8 +333,12 @@ def test_solver_autotune(gpu, route):
     if route == mc._lib.ROUTE_GRAM_OF_A:
         tuned = {"gram": {1152, 2304, 4608}, "h_omega": {1152, 2304, 4608},
                  "omega_p": {256, 512, 1024, 2048}, "n_by_l": {128, 256, 512, 1024, 2048}}
+    allowed_bits = 1 | 2 | 32 if route == mc._lib.ROUTE_SKETCH else 1 | 4 | 8 | 16
     for key, v in t.items():
-        assert v in tuned[key] if key in tuned else v == d[key]
+        if key == "kernels":
+            assert v & ~allowed_bits == 0  # only this route's GEMMs change tile shape
+        else:
+            assert v in tuned[key] if key in tuned else v == d[key]
     assert s.tuning() == t
     assert mc.tune_get(P, route=route) == (t, True)
     S1 = s.vectorS()
@@ -345,6 +349,49 @@ def test_solver_autotune(gpu, route):
     s2.solve(dA)
     assert np.array_equal(s2.vectorS(), S1)  # same plan, same bits
     mc.tune_clear()
+
+
+@pytest.mark.parametrize("route,prec", [(mc._lib.ROUTE_SKETCH, mc.PREC_F64),
+                                        (mc._lib.ROUTE_SKETCH, mc.PREC_MIXED_BF16),
+                                        (mc._lib.ROUTE_GRAM_OF_A, mc.PREC_F64),
+                                        (mc._lib.ROUTE_GRAM_OF_A, mc.PREC_MIXED_F16)])
+def test_solver_alternative_tile_shapes(gpu, route, prec):
+    """Every GEMM of the route on its alternative kernel (`kernels` = all bits: register-staged
+    128 x 128 tiles for the square products, the other tall-skinny kernel for the tall ones, FP32
+    and FP64 A) gives the default plan's result up to the order of the sums: sigma 1e-10 (1e-7
+    where the former's Z = A'Qbar sums in FP32 on FP32 MFMA: mixed precision on the sketch
+    route), the projectors of U and Qbar 1e-8 (1e-5 for Qbar = A X on FP32 MFMA), equal
+    iterations."""
+    note("alternative tile shapes, route %d precision %d" % (route, prec))
+    mc.tune_clear()
+    m, n, k, p, N = 9000, 1300, 16, 8, 24
+    if route == mc._lib.ROUTE_GRAM_OF_A:
+        m, n, k, p, N = 20000, 300, 10, 6, 32
+    rng = np.random.default_rng(11)
+    r = min(n, 200)
+    U, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    A = (U * 0.9 ** np.arange(r)) @ V.T
+    A = A.astype(np.float64 if prec == mc.PREC_F64 else np.float32)
+    order = "C" if route == mc._lib.ROUTE_GRAM_OF_A else "F"
+    dA = mc.DeviceArray.from_numpy(np.asarray(A, order=order), order=order)
+    P = mc.Parameters().setSize(m, n).setRank(k).setOverRank(p).setNumSketch(N)
+    ref = mc.Solver(P, route=route, precision=prec, seed=4).solve(dA)
+    mc.tune_set(P, dict(mc.tuning_default(), kernels=63), precision=prec, route=route)
+    alt = mc.Solver(P, route=route, precision=prec, seed=4)
+    assert alt.tuning()["kernels"] == 63
+    alt.solve(dA)
+    mc.tune_clear()
+    proj = lambda X: X @ X.T
+    rel = np.max(np.abs(alt.vectorS() - ref.vectorS()) / ref.vectorS())
+    eu = np.abs(proj(alt.matrixU()) - proj(ref.matrixU())).max()
+    eq = np.abs(proj(alt.matrixQbar()) - proj(ref.matrixQbar())).max()
+    note("  sigma rel %.2e, U projector %.2e, Qbar projector %.2e, iterations %d/%d"
+         % (rel, eu, eq, alt.iteration(), ref.iteration()))
+    fp32_former = prec != mc.PREC_F64 and route == mc._lib.ROUTE_SKETCH
+    assert rel < (1e-7 if fp32_former else 1e-10) and eu < 1e-8
+    assert eq < (1e-5 if prec != mc.PREC_F64 and route == mc._lib.ROUTE_GRAM_OF_A else 1e-8)
+    assert alt.iteration() == ref.iteration()
 
 
 def test_gram_of_a_route_mixed_precision(gpu):
