@@ -11,6 +11,7 @@
 // Tiles are staged through LDS as [k][mn] (mn contiguous) so that every MFMA operand fragment is
 // one LDS row segment; v_mfma_f64_16x16x4f64 does the arithmetic.  Source elements may be FP32
 // (mixed precision collections); they are widened to FP64 on the way into LDS.
+#include <atomic>
 #include <type_traits>
 
 #include "common.hpp"
@@ -569,6 +570,178 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
       }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Tall FP32-A products on FP32 MFMA with both operands streamed from HBM into LDS by DMA
+// (global_load_lds_dwordx4) through an NS-stage ring, NS - 1 stages in flight: the mixed former's
+// Z = A'Qbar of a column-major FP32 A, and Qbar = A X of a row-major FP32 A.  Rows of A run along
+// K (element (i, k) at A[i*lda + k]); B is FP64 and row-major (element (k, j) at B[k*ldb + j]) and
+// is rounded to FP32 as its fragments are read.  The register-staged tall_kernel keeps one step of
+// A in flight per thread, which leaves this HBM-bound stream latency-bound (3.2 TB/s for the
+// former at 100000 x 10000, l = 32); with three stages and two workgroups per CU this kernel
+// streams it at 5.1 TB/s.  A itself reads at 6.1 TB/s with plain loads whatever the length of the
+// row pieces (128 bytes to 1 KiB), and loading A and B straight into registers (no LDS) stays at
+// 3.5 TB/s, MFMAs or not: 16 rows x 64 bytes per load instruction is slower to issue.
+//
+// A stage covers BK = 32 k: a 128 x 32 FP32 slab of A (16 KiB, 128 bytes of each row) and a
+// 32 x LP FP64 slab of B (8 or 16 KiB), in one LDS array per stage (distinct arrays addressed
+// with static indices keep the compiler's DMA wait tracking per stage).  A DMA wave instruction
+// writes 1 KiB of LDS contiguously, 16 bytes per lane; which 16-byte chunk of its rows a lane
+// fetches is permuted so that the fragment reads below meet each of the 64 banks once per
+// 16 lanes (b128) or 32 lanes (b64):
+//   A row r, chunk c (k = 4c .. 4c+3) at 16-byte unit 8 r + (c ^ ((r >> 1) & 7));
+//   B row k, chunk c (j = 2c, 2c+1)   at unit (LP / 2) k + (c ^ (((k >> 2) & 1) << 3)).
+// MFMA e (e < 4) of 16-k group q contracts lane L's k = 16 q + 4 (L / 16) + e, so one b128 read
+// gives a lane the 4 A values, and four b64 reads the 4 B values, of the same k.  Chunks past the
+// split's K range are fetched from its first chunk and zeroed in the A fragments; rows past M or
+// columns past N are fetched from row / column 0 and feed outputs that are never stored, so no
+// DMA leaves the operands.  Needs K, lda, kchunk multiples of 4, N and ldb even, 16-byte aligned
+// A and B.  FP32 sums within one split's K range, FP64 across splits.
+template <int LP, int NS>
+__global__ __launch_bounds__(256, 163840 / (NS * (128 * 32 * 4 + 32 * LP * 8))) void tall_stream_kernel(GemmF64Args p) {
+  constexpr int BM = 128, BK = 32;
+  constexpr int TJ = LP / 16;
+  constexpr int AU = BM * BK / 4;  // 16-byte units of a stage's A slab (1024)
+  constexpr int BU = BK * LP / 2;  // ... and of its B slab (512 or 1024)
+  constexpr int LA = AU / 256, LB = BU / 256;  // DMA instructions per wave and stage
+  constexpr int L = LA + LB;
+  constexpr int SW = (AU + BU) * 4;  // floats per stage
+  static_assert(NS >= 2 && NS <= 6, "stage count");
+  static_assert((NS - 2) * L < 64, "vmcnt range");
+  __shared__ __attribute__((aligned(16))) float S0[SW];
+  __shared__ __attribute__((aligned(16))) float S1[SW];
+  __shared__ __attribute__((aligned(16))) float S2[SW];
+  __shared__ __attribute__((aligned(16))) float S3[NS > 3 ? SW : 4];
+  __shared__ __attribute__((aligned(16))) float S4[NS > 4 ? SW : 4];
+  __shared__ __attribute__((aligned(16))) float S5[NS > 5 ? SW : 4];
+  if (p.gate != nullptr && *p.gate < p.gate_t) return;
+  const int64_t tiles_m = ceil_div(p.M, BM);
+  const int64_t tile = blockIdx.x % tiles_m, split = blockIdx.x / tiles_m;
+  const int64_t i0 = tile * BM;
+  const int64_t kbeg = split * p.kchunk;
+  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  const int lane = threadIdx.x % kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
+  const float* A = static_cast<const float*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+
+  // Per-lane DMA sources, relative to the stage's first k.
+  const float* asrc[LA];
+  int achunk[LA];
+#pragma unroll
+  for (int i = 0; i < LA; ++i) {
+    const int r = 8 * (i * 4 + wave) + lane / 8;
+    const int c = (lane % 8) ^ ((r >> 1) & 7);
+    const int64_t row = (i0 + r < p.M) ? i0 + r : i0;
+    asrc[i] = A + row * p.lda + 4 * c;
+    achunk[i] = 4 * c;
+  }
+  constexpr int BR = 64 * 16 / (LP * 8);  // B rows per DMA instruction (4 or 2)
+  const double* bsrc[LB];
+  int brow[LB];
+#pragma unroll
+  for (int i = 0; i < LB; ++i) {
+    const int k = BR * (i * 4 + wave) + lane / (LP / 2);
+    int c = (lane % (LP / 2)) ^ (((k >> 2) & 1) << 3);
+    if (2 * c >= p.N) c = 0;
+    bsrc[i] = B + 2 * c;
+    brow[i] = k;
+  }
+  const int64_t ntk = kbeg < kend ? ceil_div(kend - kbeg, BK) : 0;
+  auto issue = [&](int64_t kt, float* s) {
+    const int64_t k0 = kbeg + kt * BK;
+#pragma unroll
+    for (int i = 0; i < LA; ++i)  // a chunk past the split's end reads the split's first chunk
+      dma16(asrc[i] + (k0 + achunk[i] < kend ? k0 : kbeg - achunk[i]),
+            (lds_void*)&s[(i * 4 + wave) * 256]);
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int64_t k = (k0 + brow[i] < kend) ? k0 + brow[i] : kbeg;
+      dma16(bsrc[i] + k * p.ldb, (lds_void*)&s[AU * 4 + (i * 4 + wave) * 256]);
+    }
+  };
+
+  constexpr int TA2 = 2;  // 16-row groups per wave
+  f4 acc[TA2][TJ];
+#pragma unroll
+  for (int a = 0; a < TA2; ++a)
+#pragma unroll
+    for (int b = 0; b < TJ; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  const int g = lane / 16, fc = lane % 16;
+  const int bsw = (g & 1) << 3;
+  auto step = [&](int64_t kt, const float* cur, float* nxt) {
+    asm volatile("" ::: "memory");
+    wait_vmcnt<(NS - 2) * L>();    // this stage landed; the younger ones may still be in flight
+    __builtin_amdgcn_s_barrier();  // ... for every wave; and the oldest buffer is fully read
+    asm volatile("" ::: "memory");
+    issue(kt + NS - 1 < ntk ? kt + NS - 1 : ntk - 1, nxt);
+    asm volatile("" ::: "memory");
+    const int64_t k0 = kbeg + kt * BK;
+    const double* cb = reinterpret_cast<const double*>(cur + AU * 4);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = 4 * q + g;
+      const bool ok = k0 + 4 * c < kend;
+      f4 af[TA2];
+#pragma unroll
+      for (int a = 0; a < TA2; ++a) {
+        const int r = wave * 32 + a * 16 + fc;
+        const f4 v = *reinterpret_cast<const f4*>(&cur[(r * 8 + (c ^ ((r >> 1) & 7))) * 4]);
+        af[a] = ok ? v : f4{0.f, 0.f, 0.f, 0.f};
+      }
+      float bf[TJ][4];
+#pragma unroll
+      for (int b = 0; b < TJ; ++b) {
+        const int j = 16 * b + fc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int k = 16 * q + 4 * g + e;
+          bf[b][e] = static_cast<float>(cb[(k * (LP / 2) + ((j >> 1) ^ bsw)) * 2 + (j & 1)]);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int a = 0; a < TA2; ++a)
+#pragma unroll
+          for (int b = 0; b < TJ; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a][e], bf[b][e], acc[a][b], 0, 0, 0);
+    }
+  };
+  auto sbuf = [&](int s) -> float* {
+    return s == 0 ? S0 : s == 1 ? S1 : s == 2 ? S2 : s == 3 ? S3 : s == 4 ? S4 : S5;
+  };
+  if (ntk > 0) {
+#pragma unroll
+    for (int s = 0; s < NS - 1; ++s) issue(s < ntk ? s : ntk - 1, sbuf(s));
+    const int64_t nsteps = ceil_div(ntk, NS) * NS;
+    for (int64_t kt = 0; kt < nsteps; kt += NS) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) step(kt + s, sbuf(s), sbuf((s + NS - 1) % NS));
+    }
+    wait_vmcnt<0>();  // no DMA may still target LDS when the workgroup ends
+  }
+  // D element r of a 16 x 16 FP32 tile sits at row 4 (lane/16) + r, column lane%16
+  const bool to_partial = p.partial != nullptr;
+  double* P = to_partial ? p.partial + split * p.M * p.N : nullptr;
+#pragma unroll
+  for (int a = 0; a < TA2; ++a)
+#pragma unroll
+    for (int b = 0; b < TJ; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = i0 + wave * 32 + a * 16 + 4 * g + r;
+        const int64_t gj = b * 16 + fc;
+        if (gi >= p.M || gj >= p.N) continue;
+        const double v = static_cast<double>(acc[a][b][r]);
+        if (to_partial) {
+          P[gi * p.N + gj] = v;
+        } else {
+          double* c = p.C + gi * p.ldc + gj;
+          *c = (p.beta == 0.0) ? p.alpha * v : p.alpha * v + p.beta * *c;
+        }
+      }
+}
+
 // The tall_direct_tile product (tall_direct.hpp) as a GEMM launch.
 template <typename TA, typename TM, int LP, bool VEC>
 __global__ __launch_bounds__(256, LP == 32 ? 3 : 2) void tall_direct_kernel(GemmF64Args p) {
@@ -589,6 +762,62 @@ void launch_tall_direct(const GemmF64Args& p, hipStream_t st) {
   else
     hipLaunchKernelGGL((tall_direct_kernel<TA, TM, LP, false>),
                        dim3(static_cast<unsigned>(nblocks)), dim3(256), 0, st, p);
+  MCNLA_LAUNCHED();
+}
+
+// Compute units of the current device (cached per device).
+int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  MCNLA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) dev = 0;
+  int n = cache[dev].load(std::memory_order_relaxed);
+  if (n <= 0) {
+    MCNLA_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    if (n <= 0) n = 1;
+    cache[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
+
+// tall_stream_kernel's operand conditions (the others take the register-staged kernels).
+bool tall_stream_ok(const GemmF64Args& p) {
+  auto al16 = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
+  return p.a_f32 && p.f32_mfma && p.kernel == 0 && al16(p.A) && al16(p.B) && p.lda % 4 == 0 &&
+         p.K % 4 == 0 && p.K >= 4 && p.N % 2 == 0 && p.ldb % 2 == 0;
+}
+
+// Stages of the streaming kernel's ring (24 KiB each at N <= 32, 32 KiB at N <= 64), and the
+// workgroups a CU then holds (LDS-bound).
+constexpr int kStreamStages32 = 3, kStreamStages64 = 2;
+constexpr int stream_lds_bytes(int lp, int ns) { return ns * (128 * 32 * 4 + 32 * lp * 8); }
+constexpr int stream_per_cu(int lp) {
+  return 163840 / stream_lds_bytes(lp, lp <= 32 ? kStreamStages32 : kStreamStages64);
+}
+// A streaming kernel's grid is best a whole number of rounds of the workgroups the device runs
+// at once: among the split counts from half the requested one up to it, take the one whose grid
+// fills its last round best (the requested one on ties), with K ranges in whole 32-deep stages.
+void tall_stream_plan(GemmF64Args& p, int64_t per_cu) {
+  constexpr int64_t kmult = 32;
+  const int64_t tiles = ceil_div(p.M, 128);
+  const int64_t slots = static_cast<int64_t>(device_cus()) * per_cu;
+  int64_t best = p.splits;
+  double best_eff = 0.0;
+  for (int64_t s = p.splits; s >= 1 && 2 * s >= p.splits; --s) {
+    const int64_t w = tiles * s;
+    const double eff = static_cast<double>(w) / static_cast<double>(ceil_div(w, slots) * slots);
+    if (eff > best_eff + 1e-3) best_eff = eff, best = s;
+  }
+  p.kchunk = ceil_div(ceil_div(p.K, best), kmult) * kmult;
+  p.splits = ceil_div(p.K, p.kchunk);
+}
+
+template <int LP, int NS>
+void launch_tall_stream(const GemmF64Args& p, hipStream_t st) {
+  const int64_t nblocks = ceil_div(p.M, 128) * p.splits;
+  MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
+  hipLaunchKernelGGL((tall_stream_kernel<LP, NS>), dim3(static_cast<unsigned>(nblocks)),
+                     dim3(256), 0, st, p);
   MCNLA_LAUNCHED();
 }
 
@@ -710,9 +939,14 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
                    p.shape == GemmShape::kSquare && p.M >= 128 && p.N >= 128 &&
                    even16(p.A, p.lda, p.stride_a) &&
                    even16(p.B, p.ldb, p.stride_b);
-  // The register-staged kernels need split ranges that are whole K tiles (16); the DMA kernel
-  // masks a partial last tile, so its splits can be equal.
-  if (!dma) p.kchunk = ceil_div(p.kchunk, 16) * 16;
+  const bool tall = !p.a_mcontig && p.b_ncontig && !p.b_f32 && p.N <= 64 && p.batch == 1 &&
+                    !p.lower_only && p.M >= 128 &&
+                    (p.shape == GemmShape::kTallN32 || p.shape == GemmShape::kTallN64);
+  const bool stream = tall && (p.splits == 1 || p.partial != nullptr) && tall_stream_ok(p);
+  // The register-staged kernels need split ranges that are whole K tiles (16); the DMA kernels
+  // mask a partial last tile, so their splits can be equal.
+  if (stream) tall_stream_plan(p, p.N <= 32 ? stream_per_cu(32) : stream_per_cu(64));
+  else if (!dma) p.kchunk = ceil_div(p.kchunk, 16) * 16;
   if (p.K > 0) p.splits = ceil_div(p.K, p.kchunk);
   if (p.splits == 1 && !p.lower_only && p.C != nullptr) p.partial = nullptr;
   MCNLA_REQUIRE(!(p.splits > 1 && p.partial == nullptr), MCNLA_ERR_INVALID,
@@ -720,18 +954,18 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
   MCNLA_REQUIRE(!p.lower_only || p.M == p.N, MCNLA_ERR_INVALID, "lower_only needs M == N");
   MCNLA_REQUIRE(!p.lower_only || p.shape == GemmShape::kSquare || p.shape == GemmShape::kSmall,
                 MCNLA_ERR_INVALID, "lower_only needs square tiles");
-  const bool tall = !p.a_mcontig && p.b_ncontig && !p.b_f32 && p.N <= 64 && p.batch == 1 &&
-                    !p.lower_only && p.M >= 128 &&
-                    (p.shape == GemmShape::kTallN32 || p.shape == GemmShape::kTallN64);
   if (p.ev_begin) MCNLA_HIP(hipEventRecord(p.ev_begin, st));
   if (tall) {
-    // defaults: LDS-staged A for an FP32 A, A straight into registers for an FP64 A (the faster
-    // of the two measured at the benchmark shapes); p.kernel = 1 takes the other one
+    // defaults: both operands streamed into LDS by DMA for an FP32 A on FP32 MFMA (where its
+    // alignment conditions hold), LDS-staged A for other FP32 A, A straight into registers for
+    // an FP64 A (the fastest measured at the benchmark shapes); p.kernel = 1 takes the
+    // register-staged kernel for an FP32 A on FP32 MFMA, and the other one of the last two
+    // otherwise
     const bool direct = p.a_f32 ? p.kernel == 1 : p.kernel == 0;
     if (p.a_f32 && p.f32_mfma) {
-      if (direct) {
-        if (p.N <= 32) launch_tall_direct<float, float, 32>(p, st);
-        else launch_tall_direct<float, float, 64>(p, st);
+      if (stream) {
+        if (p.N <= 32) launch_tall_stream<32, kStreamStages32>(p, st);
+        else launch_tall_stream<64, kStreamStages64>(p, st);
       } else {
         if (p.N <= 32) launch_tall<float, 32, float>(p, st);
         else launch_tall<float, 64, float>(p, st);

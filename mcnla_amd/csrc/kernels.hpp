@@ -43,7 +43,9 @@ struct GemmF64Args {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   // 0: the default kernel for the shape; 1: the alternative one (square shapes: the
   // register-staged 128 x 128 tiles instead of the 256 x 128 DMA tiles; tall shapes: the other
-  // tall-skinny kernel, LDS-staged A instead of A straight into registers or the reverse).
+  // tall-skinny kernel -- for an FP32 A on FP32 MFMA the register-staged A instead of both
+  // operands streamed by DMA, else LDS-staged A instead of A straight into registers or the
+  // reverse).
   int kernel = 0;
 };
 
