@@ -360,8 +360,9 @@ typedef struct {
   int64_t kernels;  /* tile shape per GEMM, a bit each (set: the alternative kernel): 1 gram,
                        2 former, 4 h_omega, 8 omega_p, 16 n_by_l, 32 qbar.  Square products:
                        256 x 128 DMA tiles (clear) or register-staged 128 x 128 tiles (set);
-                       tall-skinny products: A through LDS slabs or straight into MFMA
-                       registers (the default is the faster per operand type). */
+                       tall-skinny products: A and B streamed into LDS by DMA, A staged through
+                       registers, or A straight into MFMA registers (the default is the faster
+                       per operand type and arithmetic, the bit takes another). */
 } mcnla_tuning;
 
 /* The defaults (no table entry). */

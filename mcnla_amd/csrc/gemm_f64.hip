@@ -369,14 +369,15 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
     const int64_t left = kend - (kbeg + kt * BK);
     const int kvalid = left < BK ? static_cast<int>(left) : BK;  // <= 0 past the last stage
     if (!live) return;
+    // A fragments of k-rows past the split's range are zeroed where an MFMA takes them, not as
+    // they are read: a select on the read result would wait for the LDS read at once.
     double af[2][FM], bf[2][FN];
+    bool aok[2];
     auto frags = [&](int kk, int b) {
       const int krow = kk + lane / 16;
 #pragma unroll
-      for (int i = 0; i < FM; ++i) {
-        const double v = as[krow * LDA + wm0 + i * 16 + lane % 16];
-        af[b][i] = krow < kvalid ? v : 0.0;
-      }
+      for (int i = 0; i < FM; ++i) af[b][i] = as[krow * LDA + wm0 + i * 16 + lane % 16];
+      aok[b] = krow < kvalid;
 #pragma unroll
       for (int j = 0; j < FN; ++j) bf[b][j] = bs[krow * LDB + wn0 + j * 16 + lane % 16];
     };
@@ -390,7 +391,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[b][i], bf[b][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aok[b] ? af[b][i] : 0.0, bf[b][j],
+                                                           acc[i][j], 0, 0, 0);
       if (kk + 4 < BK) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {  // one fragment read pair between every four MFMAs
@@ -571,45 +573,58 @@ __global__ __launch_bounds__(256) void tall_kernel(GemmF64Args p) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Tall FP32-A products on FP32 MFMA with both operands streamed from HBM into LDS by DMA
-// (global_load_lds_dwordx4) through an NS-stage ring, NS - 1 stages in flight: the mixed former's
-// Z = A'Qbar of a column-major FP32 A, and Qbar = A X of a row-major FP32 A.  Rows of A run along
-// K (element (i, k) at A[i*lda + k]); B is FP64 and row-major (element (k, j) at B[k*ldb + j]) and
-// is rounded to FP32 as its fragments are read.  The register-staged tall_kernel keeps one step of
+// Tall products with both operands streamed from HBM into LDS by DMA (global_load_lds_dwordx4)
+// through an NS-stage ring, NS - 1 stages in flight: an FP32 A on FP32 MFMA (the mixed former's
+// Z = A'Qbar of a column-major A, Qbar = A X of a row-major A) or an FP64 A on FP64 MFMA
+// (Qbar = Y (T M), the FP64 former's Z, the Wen-Yin integrators' Bs products).  Rows of A run
+// along K (element (i, k) at A[i*lda + k]); B is FP64 and row-major (element (k, j) at
+// B[k*ldb + j]), rounded to FP32 as its fragments are read when A is FP32.  The register-staged tall_kernel keeps one step of
 // A in flight per thread, which leaves this HBM-bound stream latency-bound (3.2 TB/s for the
 // former at 100000 x 10000, l = 32); with three stages and two workgroups per CU this kernel
 // streams it at 5.1 TB/s.  A itself reads at 6.1 TB/s with plain loads whatever the length of the
 // row pieces (128 bytes to 1 KiB), and loading A and B straight into registers (no LDS) stays at
 // 3.5 TB/s, MFMAs or not: 16 rows x 64 bytes per load instruction is slower to issue.
 //
-// A stage covers BK = 32 k: a 128 x 32 FP32 slab of A (16 KiB, 128 bytes of each row) and a
-// 32 x LP FP64 slab of B (8 or 16 KiB), in one LDS array per stage (distinct arrays addressed
-// with static indices keep the compiler's DMA wait tracking per stage).  A DMA wave instruction
-// writes 1 KiB of LDS contiguously, 16 bytes per lane; which 16-byte chunk of its rows a lane
-// fetches is permuted so that the fragment reads below meet each of the 64 banks once per
-// 16 lanes (b128) or 32 lanes (b64):
-//   A row r, chunk c (k = 4c .. 4c+3) at 16-byte unit 8 r + (c ^ ((r >> 1) & 7));
-//   B row k, chunk c (j = 2c, 2c+1)   at unit (LP / 2) k + (c ^ (((k >> 2) & 1) << 3)).
-// MFMA e (e < 4) of 16-k group q contracts lane L's k = 16 q + 4 (L / 16) + e, so one b128 read
-// gives a lane the 4 A values, and four b64 reads the 4 B values, of the same k.  Chunks past the
+// A stage covers the 8 16-byte chunks (128 bytes) of each A row that hold BK = 8 E k (E = 4 FP32
+// or 2 FP64 elements per chunk): a 128-row slab of A (16 KiB) and a BK x LP FP64 slab of B, in
+// one LDS array per stage (distinct arrays addressed with static indices keep the compiler's DMA
+// wait tracking per stage).  A DMA wave instruction writes 1 KiB of LDS contiguously, 16 bytes
+// per lane; which 16-byte chunk of its rows a lane fetches is permuted so that the fragment reads
+// below meet each of the 64 banks once per 16 lanes (b128) or 32 lanes (b64):
+//   A row r, chunk c (k = E c .. E c + E - 1) at 16-byte unit 8 r + (c ^ ((r >> 1) & 7));
+//   B row k, chunk c (j = 2c, 2c+1)           at unit (LP / 2) k + (c ^ (((k / E) & 1) << 3)).
+// MFMA e (e < E) of chunk group q contracts lane L's k = E (4 q + L / 16) + e, so one b128 read
+// gives a lane its E A values, and E b64 reads its E B values, of the same k.  Chunks past the
 // split's K range are fetched from its first chunk and zeroed in the A fragments; rows past M or
 // columns past N are fetched from row / column 0 and feed outputs that are never stored, so no
-// DMA leaves the operands.  Needs K, lda, kchunk multiples of 4, N and ldb even, 16-byte aligned
-// A and B.  FP32 sums within one split's K range, FP64 across splits.
-template <int LP, int NS>
-__global__ __launch_bounds__(256, 163840 / (NS * (128 * 32 * 4 + 32 * LP * 8))) void tall_stream_kernel(GemmF64Args p) {
-  constexpr int BM = 128, BK = 32;
+// DMA leaves the operands.  Needs K, lda multiples of E, N and ldb even, 16-byte aligned A and B.
+// FP32 A: FP32 sums within one split's K range, FP64 across splits.
+// LDS bytes of one stage: a 128-row A slab of 128 bytes per row, and its K x LP FP64 B slab.
+template <typename TA, int LP>
+constexpr int stream_stage_bytes() {
+  return 128 * 128 + (128 / static_cast<int>(sizeof(TA))) * LP * 8;
+}
+
+template <typename TA, int LP, int NS>
+__global__ __launch_bounds__(256, 163840 / (NS * stream_stage_bytes<TA, LP>()))
+void tall_stream_kernel(GemmF64Args p) {
+  constexpr int EPC = 16 / static_cast<int>(sizeof(TA));  // elements of A per 16-byte chunk
+  constexpr bool F32 = sizeof(TA) == 4;
+  constexpr int BM = 128, BK = 8 * EPC;  // a stage: 8 chunks (128 bytes) of each A row
   constexpr int TJ = LP / 16;
-  constexpr int AU = BM * BK / 4;  // 16-byte units of a stage's A slab (1024)
-  constexpr int BU = BK * LP / 2;  // ... and of its B slab (512 or 1024)
+  constexpr int AU = BM * 8;           // 16-byte units of a stage's A slab (1024)
+  constexpr int BU = BK * LP / 2;      // ... and of its B slab
   constexpr int LA = AU / 256, LB = BU / 256;  // DMA instructions per wave and stage
   constexpr int L = LA + LB;
   constexpr int SW = (AU + BU) * 4;  // floats per stage
+  static_assert(BU % 256 == 0, "B slab in whole DMA instructions per wave");
   static_assert(NS >= 2 && NS <= 6, "stage count");
   static_assert((NS - 2) * L < 64, "vmcnt range");
+  typedef typename std::conditional<F32, f4, d4>::type Acc;
+  typedef typename VecOf<TA, EPC>::type V;
   __shared__ __attribute__((aligned(16))) float S0[SW];
   __shared__ __attribute__((aligned(16))) float S1[SW];
-  __shared__ __attribute__((aligned(16))) float S2[SW];
+  __shared__ __attribute__((aligned(16))) float S2[NS > 2 ? SW : 4];
   __shared__ __attribute__((aligned(16))) float S3[NS > 3 ? SW : 4];
   __shared__ __attribute__((aligned(16))) float S4[NS > 4 ? SW : 4];
   __shared__ __attribute__((aligned(16))) float S5[NS > 5 ? SW : 4];
@@ -621,19 +636,19 @@ __global__ __launch_bounds__(256, 163840 / (NS * (128 * 32 * 4 + 32 * LP * 8))) 
   const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
-  const float* A = static_cast<const float*>(p.A);
+  const TA* A = static_cast<const TA*>(p.A);
   const double* B = static_cast<const double*>(p.B);
 
   // Per-lane DMA sources, relative to the stage's first k.
-  const float* asrc[LA];
+  const TA* asrc[LA];
   int achunk[LA];
 #pragma unroll
   for (int i = 0; i < LA; ++i) {
     const int r = 8 * (i * 4 + wave) + lane / 8;
     const int c = (lane % 8) ^ ((r >> 1) & 7);
     const int64_t row = (i0 + r < p.M) ? i0 + r : i0;
-    asrc[i] = A + row * p.lda + 4 * c;
-    achunk[i] = 4 * c;
+    asrc[i] = A + row * p.lda + EPC * c;
+    achunk[i] = EPC * c;
   }
   constexpr int BR = 64 * 16 / (LP * 8);  // B rows per DMA instruction (4 or 2)
   const double* bsrc[LB];
@@ -641,7 +656,7 @@ __global__ __launch_bounds__(256, 163840 / (NS * (128 * 32 * 4 + 32 * LP * 8))) 
 #pragma unroll
   for (int i = 0; i < LB; ++i) {
     const int k = BR * (i * 4 + wave) + lane / (LP / 2);
-    int c = (lane % (LP / 2)) ^ (((k >> 2) & 1) << 3);
+    int c = (lane % (LP / 2)) ^ (((k / EPC) & 1) << 3);
     if (2 * c >= p.N) c = 0;
     bsrc[i] = B + 2 * c;
     brow[i] = k;
@@ -661,11 +676,11 @@ __global__ __launch_bounds__(256, 163840 / (NS * (128 * 32 * 4 + 32 * LP * 8))) 
   };
 
   constexpr int TA2 = 2;  // 16-row groups per wave
-  f4 acc[TA2][TJ];
+  Acc acc[TA2][TJ];
 #pragma unroll
   for (int a = 0; a < TA2; ++a)
 #pragma unroll
-    for (int b = 0; b < TJ; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < TJ; ++b) acc[a][b] = Acc{0, 0, 0, 0};
   const int g = lane / 16, fc = lane % 16;
   const int bsw = (g & 1) << 3;
   auto step = [&](int64_t kt, const float* cur, float* nxt) {
@@ -680,31 +695,35 @@ __global__ __launch_bounds__(256, 163840 / (NS * (128 * 32 * 4 + 32 * LP * 8))) 
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int c = 4 * q + g;
-      const bool ok = k0 + 4 * c < kend;
-      f4 af[TA2];
+      const bool ok = k0 + EPC * c < kend;
+      V af[TA2];
 #pragma unroll
       for (int a = 0; a < TA2; ++a) {
         const int r = wave * 32 + a * 16 + fc;
-        const f4 v = *reinterpret_cast<const f4*>(&cur[(r * 8 + (c ^ ((r >> 1) & 7))) * 4]);
-        af[a] = ok ? v : f4{0.f, 0.f, 0.f, 0.f};
+        af[a] = *reinterpret_cast<const V*>(&cur[(r * 8 + (c ^ ((r >> 1) & 7))) * 4]);
       }
-      float bf[TJ][4];
+      TA bf[TJ][EPC];
 #pragma unroll
       for (int b = 0; b < TJ; ++b) {
         const int j = 16 * b + fc;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int k = 16 * q + 4 * g + e;
-          bf[b][e] = static_cast<float>(cb[(k * (LP / 2) + ((j >> 1) ^ bsw)) * 2 + (j & 1)]);
+        for (int e = 0; e < EPC; ++e) {
+          const int k = EPC * c + e;
+          bf[b][e] = static_cast<TA>(cb[(k * (LP / 2) + ((j >> 1) ^ bsw)) * 2 + (j & 1)]);
         }
       }
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
+      for (int e = 0; e < EPC; ++e)
 #pragma unroll
         for (int a = 0; a < TA2; ++a)
 #pragma unroll
-          for (int b = 0; b < TJ; ++b)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a][e], bf[b][e], acc[a][b], 0, 0, 0);
+          for (int b = 0; b < TJ; ++b) {
+            const TA x = ok ? af[a][e] : TA(0);
+            if constexpr (F32)
+              acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x, bf[b][e], acc[a][b], 0, 0, 0);
+            else
+              acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, bf[b][e], acc[a][b], 0, 0, 0);
+          }
     }
   };
   auto sbuf = [&](int s) -> float* {
@@ -720,7 +739,8 @@ __global__ __launch_bounds__(256, 163840 / (NS * (128 * 32 * 4 + 32 * LP * 8))) 
     }
     wait_vmcnt<0>();  // no DMA may still target LDS when the workgroup ends
   }
-  // D element r of a 16 x 16 FP32 tile sits at row 4 (lane/16) + r, column lane%16
+  // D element r of a 16 x 16 tile sits at row 4 (lane/16) + r (FP32) or lane/16 + 4r (FP64),
+  // column lane%16
   const bool to_partial = p.partial != nullptr;
   double* P = to_partial ? p.partial + split * p.M * p.N : nullptr;
 #pragma unroll
@@ -729,7 +749,7 @@ __global__ __launch_bounds__(256, 163840 / (NS * (128 * 32 * 4 + 32 * LP * 8))) 
     for (int b = 0; b < TJ; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gi = i0 + wave * 32 + a * 16 + 4 * g + r;
+        const int64_t gi = i0 + wave * 32 + a * 16 + (F32 ? 4 * g + r : g + 4 * r);
         const int64_t gj = b * 16 + fc;
         if (gi >= p.M || gj >= p.N) continue;
         const double v = static_cast<double>(acc[a][b][r]);
@@ -783,16 +803,22 @@ int device_cus() {
 // tall_stream_kernel's operand conditions (the others take the register-staged kernels).
 bool tall_stream_ok(const GemmF64Args& p) {
   auto al16 = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
-  return p.a_f32 && p.f32_mfma && p.kernel == 0 && al16(p.A) && al16(p.B) && p.lda % 4 == 0 &&
-         p.K % 4 == 0 && p.K >= 4 && p.N % 2 == 0 && p.ldb % 2 == 0;
+  const int64_t e = p.a_f32 ? 4 : 2;  // A elements per 16-byte chunk
+  // the default kernel for an FP32 A on FP32 MFMA, the alternative one for an FP64 A
+  const bool chosen = p.a_f32 ? (p.f32_mfma && p.kernel == 0) : p.kernel == 1;
+  return chosen && al16(p.A) && al16(p.B) && p.lda % e == 0 && p.K % e == 0 && p.K >= e &&
+         p.N % 2 == 0 && p.ldb % 2 == 0;
 }
 
-// Stages of the streaming kernel's ring (24 KiB each at N <= 32, 32 KiB at N <= 64), and the
-// workgroups a CU then holds (LDS-bound).
-constexpr int kStreamStages32 = 3, kStreamStages64 = 2;
-constexpr int stream_lds_bytes(int lp, int ns) { return ns * (128 * 32 * 4 + 32 * lp * 8); }
-constexpr int stream_per_cu(int lp) {
-  return 163840 / stream_lds_bytes(lp, lp <= 32 ? kStreamStages32 : kStreamStages64);
+// Stages of the streaming kernel's ring (FP32 A: 24 KiB each at N <= 32, 32 KiB at N <= 64;
+// FP64 A: 20 and 24 KiB), two workgroups per CU in each case.
+template <typename TA, int LP>
+constexpr int stream_stages() {
+  return sizeof(TA) == 4 ? (LP <= 32 ? 3 : 2) : (LP <= 32 ? 4 : 3);
+}
+template <typename TA, int LP>
+constexpr int stream_per_cu() {
+  return 163840 / (stream_stages<TA, LP>() * stream_stage_bytes<TA, LP>());
 }
 // A streaming kernel's grid is best a whole number of rounds of the workgroups the device runs
 // at once: among the split counts from half the requested one up to it, take the one whose grid
@@ -812,12 +838,12 @@ void tall_stream_plan(GemmF64Args& p, int64_t per_cu) {
   p.splits = ceil_div(p.K, p.kchunk);
 }
 
-template <int LP, int NS>
+template <typename TA, int LP>
 void launch_tall_stream(const GemmF64Args& p, hipStream_t st) {
   const int64_t nblocks = ceil_div(p.M, 128) * p.splits;
   MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
-  hipLaunchKernelGGL((tall_stream_kernel<LP, NS>), dim3(static_cast<unsigned>(nblocks)),
-                     dim3(256), 0, st, p);
+  hipLaunchKernelGGL((tall_stream_kernel<TA, LP, stream_stages<TA, LP>()>),
+                     dim3(static_cast<unsigned>(nblocks)), dim3(256), 0, st, p);
   MCNLA_LAUNCHED();
 }
 
@@ -858,8 +884,18 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t
     const int64_t i = ij / N, j = ij % N;
     if (!computed(i, j)) continue;
     const double* pb = partial + b * splits * per + ij;
+    // eight loads in flight per thread (one at a time leaves this stream latency-bound), summed
+    // in split order
     double s = 0.0;
-    for (int64_t q = 0; q < splits; ++q) s += pb[q * per];
+    int64_t q = 0;
+    for (; q + 8 <= splits; q += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = pb[(q + u) * per];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; q < splits; ++q) s += pb[q * per];
     double* c = C + b * stride_c + i * ldc + j;
     *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
     if (!computed(j, i)) {
@@ -945,8 +981,14 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
   const bool stream = tall && (p.splits == 1 || p.partial != nullptr) && tall_stream_ok(p);
   // The register-staged kernels need split ranges that are whole K tiles (16); the DMA kernels
   // mask a partial last tile, so their splits can be equal.
-  if (stream) tall_stream_plan(p, p.N <= 32 ? stream_per_cu(32) : stream_per_cu(64));
-  else if (!dma) p.kchunk = ceil_div(p.kchunk, 16) * 16;
+  if (stream) {
+    if (p.a_f32)
+      tall_stream_plan(p, p.N <= 32 ? stream_per_cu<float, 32>() : stream_per_cu<float, 64>());
+    else
+      tall_stream_plan(p, p.N <= 32 ? stream_per_cu<double, 32>() : stream_per_cu<double, 64>());
+  }
+  else if (!dma)
+    p.kchunk = ceil_div(p.kchunk, 16) * 16;
   if (p.K > 0) p.splits = ceil_div(p.K, p.kchunk);
   if (p.splits == 1 && !p.lower_only && p.C != nullptr) p.partial = nullptr;
   MCNLA_REQUIRE(!(p.splits > 1 && p.partial == nullptr), MCNLA_ERR_INVALID,
@@ -956,22 +998,23 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
                 MCNLA_ERR_INVALID, "lower_only needs square tiles");
   if (p.ev_begin) MCNLA_HIP(hipEventRecord(p.ev_begin, st));
   if (tall) {
-    // defaults: both operands streamed into LDS by DMA for an FP32 A on FP32 MFMA (where its
-    // alignment conditions hold), LDS-staged A for other FP32 A, A straight into registers for
-    // an FP64 A (the fastest measured at the benchmark shapes); p.kernel = 1 takes the
-    // register-staged kernel for an FP32 A on FP32 MFMA, and the other one of the last two
-    // otherwise
-    const bool direct = p.a_f32 ? p.kernel == 1 : p.kernel == 0;
+    // defaults (the fastest measured at the benchmark shapes): both operands streamed into LDS
+    // by DMA for an FP32 A on FP32 MFMA (where the alignment conditions hold, else A staged
+    // through registers), A staged through registers for an FP32 A in FP64 arithmetic, and A
+    // straight into registers for an FP64 A.  p.kernel = 1 takes the register-staged kernel for
+    // an FP32 A on FP32 MFMA, A straight into registers for an FP32 A in FP64 arithmetic, and
+    // the DMA stream for an FP64 A (faster for the 8 GB FP64 former Z = A'Qbar, slower for
+    // Qbar = Y (T M): 0.42 against 0.40 ms at 100000 x 2048).
     if (p.a_f32 && p.f32_mfma) {
       if (stream) {
-        if (p.N <= 32) launch_tall_stream<32, kStreamStages32>(p, st);
-        else launch_tall_stream<64, kStreamStages64>(p, st);
+        if (p.N <= 32) launch_tall_stream<float, 32>(p, st);
+        else launch_tall_stream<float, 64>(p, st);
       } else {
         if (p.N <= 32) launch_tall<float, 32, float>(p, st);
         else launch_tall<float, 64, float>(p, st);
       }
     } else if (p.a_f32) {
-      if (direct) {
+      if (p.kernel == 1) {
         if (p.N <= 32) launch_tall_direct<float, double, 32>(p, st);
         else launch_tall_direct<float, double, 64>(p, st);
       } else {
@@ -979,12 +1022,12 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
         else launch_tall<float, 64>(p, st);
       }
     } else {
-      if (direct) {
+      if (stream) {
+        if (p.N <= 32) launch_tall_stream<double, 32>(p, st);
+        else launch_tall_stream<double, 64>(p, st);
+      } else {
         if (p.N <= 32) launch_tall_direct<double, double, 32>(p, st);
         else launch_tall_direct<double, double, 64>(p, st);
-      } else {
-        if (p.N <= 32) launch_tall<double, 32>(p, st);
-        else launch_tall<double, 64>(p, st);
       }
     }
   } else if (dma) {

@@ -44,8 +44,8 @@ struct GemmF64Args {
   // 0: the default kernel for the shape; 1: the alternative one (square shapes: the
   // register-staged 128 x 128 tiles instead of the 256 x 128 DMA tiles; tall shapes: the other
   // tall-skinny kernel -- for an FP32 A on FP32 MFMA the register-staged A instead of both
-  // operands streamed by DMA, else LDS-staged A instead of A straight into registers or the
-  // reverse).
+  // operands streamed by DMA, for an FP64 A the DMA stream instead of A straight into registers,
+  // for an FP32 A in FP64 arithmetic A straight into registers instead of register-staged).
   int kernel = 0;
 };
 
