@@ -47,7 +47,7 @@ def make_factors(m, n, r, seed):
 
 def timed_solves(mc, solver, A, steps, warmup):
     """Times `steps` solves (after `warmup`); also returns, per solve, the HIP-event durations of
-    the sketch GEMM and Gram GEMM launches and of the four stages."""
+    the sketch GEMM, Gram GEMM and former Z = A'Qbar launches and of the four stages."""
     for _ in range(warmup):
         solver.solve(A)
     mc.synchronize()
@@ -56,7 +56,7 @@ def timed_solves(mc, solver, A, steps, warmup):
     for _ in range(steps):
         solver.solve(A)
         times, _ = solver.stage_times()
-        kern.append(solver.kernel_times())
+        kern.append(solver.kernel_times() + (solver.kernel_time(mc._lib.KERNEL_FORMER_Z),))
         stages.append(times)
     mc.synchronize()
     dt = time.perf_counter() - t0
@@ -609,6 +609,7 @@ def main():
     flops_mx = 2.0 * m * n * nl * terms
     flops_f64 = 2.0 * m * n * nl
     flops_gram = 1.0 * m * nl * (nl + 1)  # lower triangle with diagonal, 2 flops per product
+    bytes_z = 4.0 * m * n + 8.0 * m * l  # the former reads A (FP32) and Qbar (FP64) once
     ach_mx = flops_mx / gmx.mean()
     ach_64 = flops_f64 / g64.mean()
     traffic = load_traffic()
@@ -620,6 +621,14 @@ def main():
                         "unit": "TFLOP/s", "frac": round(ach_mx / MI355X_PEAK[fmt], 4),
                         "traffic": traffic.get("sketch_blk_kernel"),
                         "algorithmic": "2*m*n*N*l*products = %.4e FLOP per launch" % flops_mx},
+        "former_z": {"kernel": "tall_stream_kernel (Z = A'Qbar, A FP32 and Qbar streamed into "
+                               "LDS by DMA, FP32 MFMA)",
+                     "ms": round(kmx[:, 2].mean() * 1e3, 3), "bound": "hbm",
+                     "achieved": round(bytes_z / kmx[:, 2].mean() / 1e9, 1),
+                     "peak": MI355X_PEAK["hbm"] / 1e9, "unit": "GB/s",
+                     "frac": round(bytes_z / kmx[:, 2].mean() / MI355X_PEAK["hbm"], 4),
+                     "traffic": traffic.get("tall_stream_kernel_former"),
+                     "algorithmic": "4*m*n + 8*m*l = %.4e bytes per launch" % bytes_z},
         "gram_gemm": {"kernel": "gemm_f64_dma_kernel (G = Y'Y, FP64 MFMA, lower-triangle tiles)",
                       "ms": round(grmx.mean() * 1e3, 3), "bound": "mfma",
                       "achieved": round(flops_gram / grmx.mean() / 1e12, 2),
@@ -654,6 +663,8 @@ def main():
                  "stage_ms": [round(x * 1e3, 3) for x in st64.mean(axis=0)],
                  "sketch_gemm_ms": round(g64.mean() * 1e3, 3),
                  "gram_gemm_ms": round(k64[:, 1].mean() * 1e3, 3),
+                 "former_z_ms": round(k64[:, 2].mean() * 1e3, 3),
+                 "former_z_GBps": round((8.0 * m * n + 8.0 * m * l) / k64[:, 2].mean() / 1e9, 1),
                  "sketch_gemm_tflops": round(ach_64 / 1e12, 2),
                  "sketch_gemm_frac_of_f64_peak": round(ach_64 / MI355X_PEAK["f64"], 3)},
         "mixed": {"iterations": smx.iteration(), "tuning": tunemx,

@@ -320,6 +320,13 @@ mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* s
  * No reference counterpart (the reference times whole stages only, stage_wrapper.hpp:120-140);
  * this is for roofline accounting. */
 mcnla_status mcnla_solver_kernel_times(mcnla_solver s, double times[2]);
+/* The same for one kernel: MCNLA_KERNEL_SKETCH, MCNLA_KERNEL_GRAM, or MCNLA_KERNEL_FORMER_Z (the
+ * former's Z = A'Qbar, its one pass over A; 0 on the Gram-of-A route, which forms Z from H).
+ * No reference counterpart (roofline accounting, as above). */
+#define MCNLA_KERNEL_SKETCH 0
+#define MCNLA_KERNEL_GRAM 1
+#define MCNLA_KERNEL_FORMER_Z 2
+mcnla_status mcnla_solver_kernel_time(mcnla_solver s, int kernel, double* seconds);
 
 /* ---------------------------------------------------------------------------------------------
  * Solve without allocating.  No counterpart in the reference (its stages allocate their

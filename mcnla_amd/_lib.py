@@ -22,6 +22,7 @@ COL_MAJOR, ROW_MAJOR = 0, 1
  SKETCHER_COLUMN_SAMPLING, SKETCHER_ROW_BLOCK_COLUMN_SAMPLING) = range(5)
 OMEGA_PHILOX, OMEGA_REFERENCE, OMEGA_PHILOX_F16 = 0, 1, 2
 ROUTE_AUTO, ROUTE_SKETCH, ROUTE_GRAM_OF_A = 0, 1, 2
+KERNEL_SKETCH, KERNEL_GRAM, KERNEL_FORMER_Z = 0, 1, 2
 ORTH_GRAMIAN, ORTH_ROW_BLOCK_GRAMIAN = 0, 1
 (INT_KN, INT_ROW_BLOCK_KN, INT_ROW_BLOCK_GRAMIAN_KN, INT_ROW_BLOCK_WY, INT_ROW_BLOCK_GRAMIAN_WY,
  INT_ROW_BLOCK_REDUCTION, INT_ROW_BLOCK_EXTRINSIC_MEAN) = range(7)
@@ -171,6 +172,7 @@ SIGNATURES = {
     "mcnla_solver_iteration": (_st, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "mcnla_solver_stage_times": (_st, [_vp, _dp, _dp]),
     "mcnla_solver_kernel_times": (_st, [_vp, _dp]),
+    "mcnla_solver_kernel_time": (_st, [_vp, ctypes.c_int, _dp]),
     "mcnla_tuning_default": (None, [ctypes.POINTER(Tuning)]),
     "mcnla_solver_get_tuning": (_st, [_vp, ctypes.POINTER(Tuning)]),
     "mcnla_solver_autotune": (_st, [_vp, _vp, _i64, _i, _i, _i, _vp, ctypes.POINTER(Tuning)]),

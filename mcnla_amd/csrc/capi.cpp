@@ -398,6 +398,24 @@ struct GramCore {
 // Z = A' Qbar (n x l), Wg = Z' Z, eig, and the outputs of either former.
 struct FormCore {
   DevBuf Z, partial, Wg, eigw, eigv, Wk, S, U, V;
+  // events around the Z = A'Qbar launch (the former's stream over A)
+  hipEvent_t zbeg = nullptr, zend = nullptr;
+  bool own_ev = true;     // false: the events belong to a SolveRes
+  bool recorded = false;  // the events bracket a Z launch since the last reset
+  FormCore() = default;
+  FormCore(const FormCore&) = delete;
+  FormCore& operator=(const FormCore&) = delete;
+  ~FormCore() {
+    if (own_ev && zbeg) (void)hipEventDestroy(zbeg);
+    if (own_ev && zend) (void)hipEventDestroy(zend);
+  }
+  double z_seconds() {
+    if (!zbeg || !recorded) return 0.0;
+    MCNLA_HIP(hipEventSynchronize(zend));
+    float ms = 0.f;
+    MCNLA_HIP(hipEventElapsedTime(&ms, zbeg, zend));
+    return ms * 1e-3;
+  }
   // kind: MCNLA_FORM_*.  The symmetric former (square symmetric A) uses Wg = Z'Qbar = Qbar'A Qbar
   // in place of Z'Z and forms no V.
   // Split-K of Z = A'Qbar (K = m over ceil(n/128) row tiles) and of Wg = Z'Z (K = n).
@@ -470,7 +488,17 @@ struct FormCore {
     w.splits = ceil_div(n, w.kchunk);
     z.partial = partial.as<double>();
     w.partial = partial.as<double>();
-    if (!z_ready) gemm_f64(z, st);
+    recorded = false;
+    if (!z_ready) {
+      if (!zbeg) {
+        MCNLA_HIP(hipEventCreate(&zbeg));
+        MCNLA_HIP(hipEventCreate(&zend));
+      }
+      z.ev_begin = zbeg;
+      z.ev_end = zend;
+      gemm_f64(z, st);
+      recorded = true;
+    }
     gemm_f64(w, st);
     former_small(Wg.as<double>(), l, static_cast<int>(k), eigw.as<double>(), eigv.as<double>(),
                  Wk.as<double>(), S.as<double>(), st, sym);
@@ -796,6 +824,10 @@ struct mcnla_solver_s {
     gram.end = res->gram_end;
     gram.own_ev = false;
     gram.recorded = false;
+    form.zbeg = res->former_beg;
+    form.zend = res->former_end;
+    form.own_ev = false;
+    form.recorded = false;
   }
 };
 
@@ -983,7 +1015,7 @@ void SolveRes::create() {
     MCNLA_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     MCNLA_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     for (auto& e : marks) MCNLA_HIP(hipEventCreate(&e));
-    for (hipEvent_t* e : {&sketch_beg, &sketch_end, &gram_beg, &gram_end})
+    for (hipEvent_t* e : {&sketch_beg, &sketch_end, &gram_beg, &gram_end, &former_beg, &former_end})
       MCNLA_HIP(hipEventCreate(e));
   } catch (...) {
     destroy();
@@ -992,7 +1024,8 @@ void SolveRes::create() {
 }
 
 void SolveRes::destroy() {
-  for (hipEvent_t* e : {&fork, &join, &sketch_beg, &sketch_end, &gram_beg, &gram_end})
+  for (hipEvent_t* e :
+       {&fork, &join, &sketch_beg, &sketch_end, &gram_beg, &gram_end, &former_beg, &former_end})
     if (*e) (void)hipEventDestroy(*e), *e = nullptr;
   for (auto& e : marks)
     if (e) (void)hipEventDestroy(e), e = nullptr;
@@ -2075,6 +2108,19 @@ mcnla_status mcnla_solver_stage_times(mcnla_solver s, double times[4], double* s
     const auto v = s->timer.parts();
     for (int i = 0; i < 4; ++i) times[i] = i < static_cast<int>(v.size()) ? v[i] : 0.0;
     if (sketch_kernel_s) *sketch_kernel_s = s->sketch.gemm_seconds();
+    return MCNLA_OK;
+  });
+}
+
+mcnla_status mcnla_solver_kernel_time(mcnla_solver s, int kernel, double* seconds) {
+  return guard([&] {
+    MCNLA_REQUIRE(s && seconds, MCNLA_ERR_INVALID, "null argument");
+    MCNLA_REQUIRE(kernel >= MCNLA_KERNEL_SKETCH && kernel <= MCNLA_KERNEL_FORMER_Z,
+                  MCNLA_ERR_INVALID, "unknown kernel");
+    MCNLA_REQUIRE(s->computed, MCNLA_ERR_STATE, "solver has not run");
+    *seconds = kernel == MCNLA_KERNEL_SKETCH ? s->sketch.gemm_seconds()
+               : kernel == MCNLA_KERNEL_GRAM ? s->gram.gemm_seconds()
+                                             : s->form.z_seconds();
     return MCNLA_OK;
   });
 }

@@ -800,12 +800,18 @@ int device_cus() {
   return n;
 }
 
+// A sizes (bytes) from which an FP32 A on FP32 MFMA streams by DMA by default.
+constexpr double kStreamMinBytes = 1073741824.0;
+
 // tall_stream_kernel's operand conditions (the others take the register-staged kernels).
 bool tall_stream_ok(const GemmF64Args& p) {
   auto al16 = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
   const int64_t e = p.a_f32 ? 4 : 2;  // A elements per 16-byte chunk
-  // the default kernel for an FP32 A on FP32 MFMA, the alternative one for an FP64 A
-  const bool chosen = p.a_f32 ? (p.f32_mfma && p.kernel == 0) : p.kernel == 1;
+  // The default kernel for an FP32 A of at least kStreamMinBytes on FP32 MFMA (smaller ones run
+  // faster register-staged, alone and beside other solves), the alternative one otherwise.
+  const double abytes = static_cast<double>(p.M) * static_cast<double>(p.K) * (p.a_f32 ? 4 : 8);
+  const bool chosen = p.a_f32 ? p.f32_mfma && ((p.kernel == 0) == (abytes >= kStreamMinBytes))
+                              : p.kernel == 1;
   return chosen && al16(p.A) && al16(p.B) && p.lda % e == 0 && p.K % e == 0 && p.K >= e &&
          p.N % 2 == 0 && p.ldb % 2 == 0;
 }
@@ -999,12 +1005,13 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
   if (p.ev_begin) MCNLA_HIP(hipEventRecord(p.ev_begin, st));
   if (tall) {
     // defaults (the fastest measured at the benchmark shapes): both operands streamed into LDS
-    // by DMA for an FP32 A on FP32 MFMA (where the alignment conditions hold, else A staged
-    // through registers), A staged through registers for an FP32 A in FP64 arithmetic, and A
-    // straight into registers for an FP64 A.  p.kernel = 1 takes the register-staged kernel for
-    // an FP32 A on FP32 MFMA, A straight into registers for an FP32 A in FP64 arithmetic, and
-    // the DMA stream for an FP64 A (faster for the 8 GB FP64 former Z = A'Qbar, slower for
-    // Qbar = Y (T M): 0.42 against 0.40 ms at 100000 x 2048).
+    // by DMA for an FP32 A of 1 GiB or more on FP32 MFMA (where the alignment conditions hold;
+    // 0.79 against 1.25 ms for the 4 GB former Z = A'Qbar at 100000 x 10000), A staged through
+    // registers for smaller FP32 A (serving-size requests run faster so, alone and in the mix)
+    // and for an FP32 A in FP64 arithmetic, and A straight into registers for an FP64 A.
+    // p.kernel = 1 takes the other one for an FP32 A on FP32 MFMA, A straight into registers
+    // for an FP32 A in FP64 arithmetic, and the DMA stream for an FP64 A (faster for the 8 GB
+    // FP64 former Z = A'Qbar, slower for Qbar = Y (T M): 0.42 against 0.40 ms at 100000 x 2048).
     if (p.a_f32 && p.f32_mfma) {
       if (stream) {
         if (p.N <= 32) launch_tall_stream<float, 32>(p, st);

@@ -43,9 +43,10 @@ struct GemmF64Args {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   // 0: the default kernel for the shape; 1: the alternative one (square shapes: the
   // register-staged 128 x 128 tiles instead of the 256 x 128 DMA tiles; tall shapes: the other
-  // tall-skinny kernel -- for an FP32 A on FP32 MFMA the register-staged A instead of both
-  // operands streamed by DMA, for an FP64 A the DMA stream instead of A straight into registers,
-  // for an FP32 A in FP64 arithmetic A straight into registers instead of register-staged).
+  // tall-skinny kernel -- for an FP32 A on FP32 MFMA register-staged A and both operands
+  // streamed by DMA swap (the stream is the default from 1 GiB of A), for an FP64 A the DMA
+  // stream instead of A straight into registers, for an FP32 A in FP64 arithmetic A straight
+  // into registers instead of register-staged).
   int kernel = 0;
 };
 

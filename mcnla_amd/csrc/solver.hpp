@@ -52,6 +52,7 @@ struct SolveRes {
   hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t marks[kMarks] = {};
   hipEvent_t sketch_beg = nullptr, sketch_end = nullptr, gram_beg = nullptr, gram_end = nullptr;
+  hipEvent_t former_beg = nullptr, former_end = nullptr;
   bool ready() const { return side != nullptr; }
   void create();   // on the current device; throws Error (and holds nothing) on failure
   void destroy();  // idempotent, never throws

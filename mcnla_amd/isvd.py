@@ -588,6 +588,13 @@ class Solver:
         _lib.check(_lib.lib().mcnla_solver_kernel_times(self._h, t))
         return t[0], t[1]
 
+    def kernel_time(self, kernel):
+        """Seconds of one launch of the last solve: _lib.KERNEL_SKETCH, KERNEL_GRAM or
+        KERNEL_FORMER_Z (the former's Z = A'Qbar; 0 on the Gram-of-A route)."""
+        t = ctypes.c_double()
+        _lib.check(_lib.lib().mcnla_solver_kernel_time(self._h, int(kernel), ctypes.byref(t)))
+        return t.value
+
     def workspace_bytes(self):
         b = ctypes.c_size_t()
         _lib.check(_lib.lib().mcnla_solver_workspace_bytes(self._h, ctypes.byref(b)))

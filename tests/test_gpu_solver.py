@@ -272,6 +272,12 @@ def test_solver_gram_of_a_route(gpu, dtype, order, shape):
         assert all(t > 0 for t in times) and sketch_s >= 0
         kern = s.kernel_times()
         assert (kern[0] > 0) == (route == mc.ROUTE_SKETCH) and kern[1] > 0
+        assert s.kernel_time(mc._lib.KERNEL_SKETCH) == kern[0]
+        assert s.kernel_time(mc._lib.KERNEL_GRAM) == kern[1]
+        # the former's pass over A runs on the sketch route only (Z = H X on the other)
+        assert (s.kernel_time(mc._lib.KERNEL_FORMER_Z) > 0) == (route == mc.ROUTE_SKETCH)
+        with pytest.raises(mc.McnlaError):
+            s.kernel_time(3)
         outs[route] = (s.vectorS(), s.matrixU(), s.matrixV(), s.matrixQbar(), s.iteration())
     S, Uu, Vv, Qb, it = outs[mc.ROUTE_GRAM_OF_A]
     assert abs(it - it_o) <= 1
