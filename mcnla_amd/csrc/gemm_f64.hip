@@ -869,26 +869,18 @@ void launch_tall(const GemmF64Args& p, hipStream_t st) {
   MCNLA_LAUNCHED();
 }
 
-// Deterministic split-K reduction: C = alpha * sum_s partial[s] + beta * C.  With sym_bm > 0
-// (a symmetric output of which the kernels computed at least the lower triangle, in tiles of
-// sym_bm x sym_bn) each lower entry is summed where it lies (coalesced reads) and written to
-// both (i, j) and (j, i).
+// Deterministic split-K reduction: C = alpha * sum_s partial[s] + beta * C (sums in split order).
 __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t splits,
                                      int64_t batch, int64_t M, int64_t N, double alpha,
                                      double beta, double* __restrict__ C, int64_t ldc,
-                                     int64_t stride_c, int sym_bm, int sym_bn, const int* gate,
-                                     int gate_t) {
+                                     int64_t stride_c, const int* gate, int gate_t) {
   if (gate != nullptr && *gate < gate_t) return;
   const int64_t per = M * N;
   const int64_t total = per * batch;
-  // Symmetric outputs take every entry from the lower triangle (the kernels may leave the strictly
-  // upper part of their diagonal tiles uncomputed) and mirror it, so C is exactly symmetric.
-  auto computed = [&](int64_t i, int64_t j) { return sym_bm <= 0 || j <= i; };
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = e / per, ij = e % per;
     const int64_t i = ij / N, j = ij % N;
-    if (!computed(i, j)) continue;
     const double* pb = partial + b * splits * per + ij;
     // eight loads in flight per thread (one at a time leaves this stream latency-bound), summed
     // in split order
@@ -904,9 +896,61 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t
     for (; q < splits; ++q) s += pb[q * per];
     double* c = C + b * stride_c + i * ldc + j;
     *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
-    if (!computed(j, i)) {
-      double* cm = C + b * stride_c + j * ldc + i;
-      *cm = (beta == 0.0) ? alpha * s : alpha * s + beta * *cm;
+  }
+}
+
+// The reduction of a symmetric output (every entry with j <= i computed), by 64 x 64 tiles of the
+// lower triangle: a tile's sums (split order, as above) go through LDS, so that both it and its
+// mirror image above the diagonal are written in whole rows (a mirror written entry by entry
+// touches one cache line per entry).  Grid: (T (T + 1) / 2 tiles, batch), T = ceil(N / 64).
+__global__ __launch_bounds__(256) void splitk_reduce_sym_kernel(
+    const double* __restrict__ partial, int64_t splits, int64_t N, double alpha, double beta,
+    double* __restrict__ C, int64_t ldc, int64_t stride_c, const int* gate, int gate_t) {
+  __shared__ double t[64][65];
+  if (gate != nullptr && *gate < gate_t) return;
+  const int64_t x = blockIdx.x;
+  int64_t ti = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(x) + 1.0) - 1.0) / 2.0);
+  while (ti * (ti + 1) / 2 > x) --ti;
+  while ((ti + 1) * (ti + 2) / 2 <= x) ++ti;
+  const int64_t tj = x - ti * (ti + 1) / 2;
+  const int64_t per = N * N;
+  const double* pb = partial + static_cast<int64_t>(blockIdx.y) * splits * per;
+  double* Cb = C + static_cast<int64_t>(blockIdx.y) * stride_c;
+  const int c = threadIdx.x % 64, r0 = threadIdx.x / 64;
+  const bool diag = ti == tj;
+#pragma unroll 1
+  for (int r = r0; r < 64; r += 4) {
+    const int64_t i = ti * 64 + r, j = tj * 64 + c;
+    double s = 0.0;
+    if (i < N && j < N && (!diag || j <= i)) {
+      const double* q0 = pb + i * N + j;
+      int64_t q = 0;
+      for (; q + 8 <= splits; q += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = q0[(q + u) * per];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+      }
+      for (; q < splits; ++q) s += q0[q * per];
+    }
+    t[r][c] = s;
+  }
+  __syncthreads();
+  for (int r = r0; r < 64; r += 4) {
+    const int64_t i = ti * 64 + r, j = tj * 64 + c;
+    if (i < N && j < N) {
+      const double v = (!diag || j <= i) ? t[r][c] : t[c][r];
+      double* d = Cb + i * ldc + j;
+      *d = (beta == 0.0) ? alpha * v : alpha * v + beta * *d;
+    }
+    if (!diag) {
+      const int64_t i2 = tj * 64 + r, j2 = ti * 64 + c;  // the mirror tile, row by row
+      if (i2 < N && j2 < N) {
+        const double v = t[c][r];
+        double* d = Cb + i2 * ldc + j2;
+        *d = (beta == 0.0) ? alpha * v : alpha * v + beta * *d;
+      }
     }
   }
 }
@@ -1055,16 +1099,23 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
     else launch_types<double, double>(p, st);
   }
   if (p.ev_end) MCNLA_HIP(hipEventRecord(p.ev_end, st));
-  if (p.partial != nullptr && p.C != nullptr) {
-    const int sym_bm = !p.lower_only ? 0 : dma ? 256 : (p.shape == GemmShape::kSquare ? 128 : 32);
-    const int sym_bn = !p.lower_only ? 0 : dma ? 128 : sym_bm;
+  if (p.partial != nullptr && p.C != nullptr && p.lower_only) {
+    const int64_t T = ceil_div(p.N, 64);
+    MCNLA_REQUIRE(T * (T + 1) / 2 < (1LL << 31) && p.batch < 65536, MCNLA_ERR_INVALID,
+                  "GEMM grid too large");
+    hipLaunchKernelGGL(splitk_reduce_sym_kernel,
+                       dim3(static_cast<unsigned>(T * (T + 1) / 2), static_cast<unsigned>(p.batch)),
+                       dim3(256), 0, st, p.partial, p.splits, p.N, p.alpha, p.beta, p.C, p.ldc,
+                       p.stride_c, p.gate, p.gate_t);
+    MCNLA_LAUNCHED();
+  } else if (p.partial != nullptr && p.C != nullptr) {
     const int64_t total = p.M * p.N * p.batch;
     const int threads = 256;
     int64_t blocks = ceil_div(total, threads);
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0,
                        st, p.partial, p.splits, p.batch, p.M, p.N, p.alpha, p.beta, p.C, p.ldc,
-                       p.stride_c, sym_bm, sym_bn, p.gate, p.gate_t);
+                       p.stride_c, p.gate, p.gate_t);
     MCNLA_LAUNCHED();
   }
 }
