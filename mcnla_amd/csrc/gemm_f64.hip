@@ -902,8 +902,9 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ partial, int64_t
 // The reduction of a symmetric output (every entry with j <= i computed), by 64 x 64 tiles of the
 // lower triangle: a tile's sums (split order, as above) go through LDS, so that both it and its
 // mirror image above the diagonal are written in whole rows (a mirror written entry by entry
-// touches one cache line per entry).  Grid: (T (T + 1) / 2 tiles, batch), T = ceil(N / 64).
-__global__ __launch_bounds__(256) void splitk_reduce_sym_kernel(
+// touches one cache line per entry).  Grid: (T (T + 1) / 2 tiles, batch) of 1024 threads,
+// T = ceil(N / 64).
+__global__ __launch_bounds__(1024) void splitk_reduce_sym_kernel(
     const double* __restrict__ partial, int64_t splits, int64_t N, double alpha, double beta,
     double* __restrict__ C, int64_t ldc, int64_t stride_c, const int* gate, int gate_t) {
   __shared__ double t[64][65];
@@ -918,26 +919,39 @@ __global__ __launch_bounds__(256) void splitk_reduce_sym_kernel(
   double* Cb = C + static_cast<int64_t>(blockIdx.y) * stride_c;
   const int c = threadIdx.x % 64, r0 = threadIdx.x / 64;
   const bool diag = ti == tj;
-#pragma unroll 1
-  for (int r = r0; r < 64; r += 4) {
-    const int64_t i = ti * 64 + r, j = tj * 64 + c;
-    double s = 0.0;
-    if (i < N && j < N && (!diag || j <= i)) {
-      const double* q0 = pb + i * N + j;
-      int64_t q = 0;
-      for (; q + 8 <= splits; q += 8) {
-        double v[8];
+  // the thread's 4 entries (rows r0, r0 + 16, ...), summed split by split: 4 splits x 4 entries
+  // of loads in flight (and few registers, so that 8 waves share a SIMD), each entry's sum in
+  // split order
+  constexpr int RU = 4;
+  int64_t off[RU];
+  bool ok[RU];
+  double acc[RU];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = q0[(q + u) * per];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
-      }
-      for (; q < splits; ++q) s += q0[q * per];
-    }
-    t[r][c] = s;
+  for (int u = 0; u < RU; ++u) {
+    const int64_t i = ti * 64 + r0 + 16 * u, j = tj * 64 + c;
+    ok[u] = i < N && j < N && (!diag || j <= i);
+    off[u] = ok[u] ? i * N + j : 0;
+    acc[u] = 0.0;
   }
+  int64_t q = 0;
+  for (; q + 4 <= splits; q += 4) {
+    double v[4][RU];
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+      for (int u = 0; u < RU; ++u) v[w][u] = pb[(q + w) * per + off[u]];
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+      for (int u = 0; u < RU; ++u) acc[u] += v[w][u];
+  }
+  for (; q < splits; ++q)
+#pragma unroll
+    for (int u = 0; u < RU; ++u) acc[u] += pb[q * per + off[u]];
+#pragma unroll
+  for (int u = 0; u < RU; ++u) t[r0 + 16 * u][c] = ok[u] ? acc[u] : 0.0;
   __syncthreads();
-  for (int r = r0; r < 64; r += 4) {
+  for (int r = r0; r < 64; r += 16) {
     const int64_t i = ti * 64 + r, j = tj * 64 + c;
     if (i < N && j < N) {
       const double v = (!diag || j <= i) ? t[r][c] : t[c][r];
@@ -1105,7 +1119,7 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
                   "GEMM grid too large");
     hipLaunchKernelGGL(splitk_reduce_sym_kernel,
                        dim3(static_cast<unsigned>(T * (T + 1) / 2), static_cast<unsigned>(p.batch)),
-                       dim3(256), 0, st, p.partial, p.splits, p.N, p.alpha, p.beta, p.C, p.ldc,
+                       dim3(1024), 0, st, p.partial, p.splits, p.N, p.alpha, p.beta, p.C, p.ldc,
                        p.stride_c, p.gate, p.gate_t);
     MCNLA_LAUNCHED();
   } else if (p.partial != nullptr && p.C != nullptr) {
