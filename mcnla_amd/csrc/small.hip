@@ -9,17 +9,20 @@ namespace {
 
 constexpr int kSmallThreads = 256;
 constexpr int kLds = kMaxL * (kMaxL + 1);
+// A Jacobi round applies up to (l/2)^2 + (l/2) l = 3072 work items at l = 64: 1024 threads take
+// three each (256 threads took twelve, one after another, each waiting on its LDS reads).
+constexpr int kSyevThreads = 1024;
 
 // One workgroup per matrix: W[b] -> (w[b] descending, V[b]).
-__global__ __launch_bounds__(kSmallThreads) void batched_syev_kernel(const double* __restrict__ W,
-                                                                    double* __restrict__ w,
-                                                                    double* __restrict__ V,
-                                                                    int n) {
+__global__ __launch_bounds__(kSyevThreads) void batched_syev_kernel(const double* __restrict__ W,
+                                                                   double* __restrict__ w,
+                                                                   double* __restrict__ V,
+                                                                   int n) {
   __shared__ double a[kLds];
   __shared__ double v[kLds];
   __shared__ double cs[2 * kMaxL];
   __shared__ int pq[2 * kMaxL];
-  __shared__ double red[2 * (kSmallThreads / kWave)];
+  __shared__ double red[2 * (kSyevThreads / kWave)];
   __shared__ int ord[kMaxL];
   const int64_t b = blockIdx.x;
   const double* Wb = W + b * n * n;
@@ -302,7 +305,7 @@ unsigned grid_for(int64_t total, int threads = 256, int64_t cap = 16384) {
 void batched_syev(const double* W, double* w, double* V, int64_t batch, int n, hipStream_t st) {
   MCNLA_REQUIRE(n >= 1 && n <= kMaxL, MCNLA_ERR_INVALID, "eigen problem larger than 64");
   if (batch <= 0) return;
-  hipLaunchKernelGGL(batched_syev_kernel, dim3(static_cast<unsigned>(batch)), dim3(kSmallThreads),
+  hipLaunchKernelGGL(batched_syev_kernel, dim3(static_cast<unsigned>(batch)), dim3(kSyevThreads),
                      0, st, W, w, V, n);
   MCNLA_LAUNCHED();
 }

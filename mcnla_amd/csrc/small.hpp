@@ -17,8 +17,9 @@ __host__ __device__ constexpr int lds_ld(int n) { return n + 1; }
 // round-robin tournament ordering: n-1 rounds of n/2 disjoint rotations per sweep.  A pair is
 // rotated only while |a_pq| > 1e-15 sqrt(|a_pp a_qq|) (the threshold of one-sided/two-sided
 // Jacobi with high relative accuracy, Demmel & Veselic 1992), and the annihilated entry is set
-// to zero; the iteration stops after a sweep without rotations.  `cs` needs 2*64 doubles, `pq`
-// 2*64 ints, `red` 8 doubles of LDS.  All threads of the workgroup must call this.
+// to zero; the iteration stops after a sweep without rotations (a sweep that would rotate
+// nothing is recognised by testing every pair once, and not run).  `cs` needs 2*64 doubles,
+// `pq` 2*64 ints, `red` 8 doubles of LDS.  All threads of the workgroup must call this.
 __device__ inline int lds_jacobi(int n, double* a, double* v, bool init_v, double* cs, int* pq,
                                  double* red) {
   const int ld = lds_ld(n);
@@ -28,6 +29,7 @@ __device__ inline int lds_jacobi(int n, double* a, double* v, bool init_v, doubl
   const int np = (n + 1) & ~1;  // players, padded to even (index n is a bye when n is odd)
   const int npairs = np / 2;
   int* rotated = reinterpret_cast<int*>(red);
+  int* more = rotated + 1;
   int sweeps = 0;
   __syncthreads();
   for (int sweep = 0; sweep < 30; ++sweep) {
@@ -106,6 +108,26 @@ __device__ inline int lds_jacobi(int n, double* a, double* v, bool init_v, doubl
     const int any = *rotated;
     __syncthreads();
     if (!any) break;
+    // Would the next sweep rotate any pair?  (The same test as above, on the matrix it would
+    // start from: if no pair passes it, that sweep changes nothing and ends the iteration, so it
+    // is skipped -- the result is the same, bit for bit, one sweep sooner.)
+    if (tid == 0) *more = 0;
+    __syncthreads();
+    bool pass = false;
+    for (int e = tid; e < n * n; e += nt) {
+      const int p = e / n, q = e % n;
+      if (p >= q) continue;
+      const double apq = a[p * ld + q];
+      pass = pass ||
+             (fabs(apq) > 1e-15 * sqrt(fabs(a[p * ld + p] * a[q * ld + q])) && apq != 0.0);
+    }
+    if (pass) *more = 1;
+    __syncthreads();
+    const int again = *more;
+    if (!again) {
+      ++sweeps;
+      break;
+    }
   }
   return sweeps;
 }
