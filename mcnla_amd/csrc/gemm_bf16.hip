@@ -203,6 +203,74 @@ __global__ __launch_bounds__(256) void pack_blocks_kernel(const TS* __restrict__
   }
 }
 
+// pack_blocks_kernel for a one-term pack (no remainder segment) of a column-major FP32 source
+// with 16-byte aligned columns: each value is scaled and rounded as it is loaded, so the LDS
+// transpose tile holds 16-bit values (17 KiB instead of 33): twice the workgroups share a CU,
+// with twice the loads in flight.  Same block layout and the same values.
+template <int FMT>
+__global__ __launch_bounds__(256) void pack16_blocks_kernel(const float* __restrict__ src,
+                                                            int64_t ld, int64_t rows, int64_t cols,
+                                                            int64_t nkt, int nseg, int seg_hi,
+                                                            const uint32_t* __restrict__ rowmax,
+                                                            uint16_t* __restrict__ dst) {
+  constexpr int LDT = kBR + 8;  // 16-bit values per LDS row (rows stay 16-byte aligned)
+  __shared__ __attribute__((aligned(16))) uint16_t tile[kBKB][LDT];
+  const int64_t tr = blockIdx.y, kt = blockIdx.x;
+  const int64_t r0 = tr * kBR, k0 = kt * kBKB;
+  const int t = threadIdx.x;
+  const bool full = r0 + kBR <= rows;
+  const int r = 4 * (t % 64);
+  int e[4] = {0, 0, 0, 0};  // row scales (FP16)
+  if (rowmax) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) e[i] = (r0 + r + i < rows) ? row_exponent(rowmax[r0 + r + i]) : 0;
+  }
+  float4 v[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {  // a wave reads one 1 KiB column segment per instruction
+    const int64_t gk = k0 + j * 4 + t / 64;
+    v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (gk < cols) {
+      const float* col = src + gk * ld + r0 + r;
+      if (full) {
+        v[j] = *reinterpret_cast<const float4*>(col);
+      } else {
+        v[j].x = (r0 + r < rows) ? col[0] : 0.f;
+        v[j].y = (r0 + r + 1 < rows) ? col[1] : 0.f;
+        v[j].z = (r0 + r + 2 < rows) ? col[2] : 0.f;
+        v[j].w = (r0 + r + 3 < rows) ? col[3] : 0.f;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = j * 4 + t / 64;
+    const float x[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+    uint16_t h[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h[i] = to16<FMT>(e[i] ? ldexpf(x[i], -e[i]) : x[i]);
+    uint2 w;
+    w.x = static_cast<uint32_t>(h[0]) | (static_cast<uint32_t>(h[1]) << 16);
+    w.y = static_cast<uint32_t>(h[2]) | (static_cast<uint32_t>(h[3]) << 16);
+    *reinterpret_cast<uint2*>(&tile[k][r]) = w;
+  }
+  __syncthreads();
+  // every store instruction writes 1 KiB contiguous: chunk g = q*256 + t is (row g/4, slot g%4)
+  uint16_t* hi = dst + ((tr * nseg + seg_hi) * nkt + kt) * kBlk;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int g = q * 256 + t;
+    const int rr = g >> 2;
+    const int c = (g & 3) ^ ((rr >> 2) & 3);
+    u32x4 ph;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      ph[i] = static_cast<uint32_t>(tile[8 * c + 2 * i][rr]) |
+              (static_cast<uint32_t>(tile[8 * c + 2 * i + 1][rr]) << 16);
+    *reinterpret_cast<u32x4*>(hi + g * 8) = ph;
+  }
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
 
 // vmcnt <= VM and lgkmcnt == 0 (every LDS read issued so far has completed).  Being a builtin,
@@ -392,6 +460,12 @@ void launch_pack_fmt(const TS* src, int64_t ld, bool r_contig, int64_t rows, int
   const dim3 grid(static_cast<unsigned>(nkt), static_cast<unsigned>(tiles));
   const bool vec4 = r_contig && ld % 4 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0;
   if constexpr (sizeof(TS) == 4) {
+    if (vec4 && seg_lo < 0) {
+      hipLaunchKernelGGL((pack16_blocks_kernel<FMT>), grid, dim3(256), 0, st, src, ld, rows, cols,
+                         nkt, nseg, seg_hi, rowmax, dst);
+      MCNLA_LAUNCHED();
+      return;
+    }
     if (vec4) {
       hipLaunchKernelGGL((pack_blocks_kernel<TS, true, true, FMT>), grid, dim3(256), 0, st, src,
                          ld, rows, cols, nkt, nseg, seg_hi, seg_lo, rowmax, dst);
