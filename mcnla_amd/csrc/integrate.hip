@@ -33,7 +33,8 @@ struct IntegratorState {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRows = 16;   // Nl rows per workgroup in the WY step-size pass
+constexpr int kRows = 32;   // Nl rows per workgroup in the WY step-size pass
+constexpr int kURows = 32;  // Nl rows per workgroup in the WY update
 constexpr int kPRows = 64;  // Nl rows per workgroup in the partial dot products
 
 __device__ __forceinline__ bool skip(const IntegratorState* st, int t) {
@@ -739,40 +740,74 @@ __global__ __launch_bounds__(kThreads) void wy_update_kernel(
     double* __restrict__ Ep, const double* __restrict__ FFc, double* __restrict__ FFp,
     const double* __restrict__ Fc, const double* __restrict__ Fgc) {
   if (skip(st, t)) return;
+  // Workgroup b updates rows [kURows b, kURows b + kURows): their rows of Bc, Bg, Ec are staged
+  // in LDS (coalesced), lane j of wave w computes column j of rows w, w + 4, ... (F and G are
+  // read once per k for all of them); each sum runs over k in order.  The first
+  // ceil(l^2 / kThreads) workgroups also form FF+ = FFc F, one entry per thread.
   __shared__ double sF[kMaxL * kMaxL];
   __shared__ double sG[kMaxL * kMaxL];
-  for (int e = threadIdx.x; e < l * l; e += kThreads) {
+  __shared__ double sR[3][kURows][kMaxL];
+  constexpr int RW = kURows / (kThreads / kWave);  // rows per wave
+  const int tid = threadIdx.x;
+  for (int e = tid; e < l * l; e += kThreads) {
     sF[e] = Fc[e];
     sG[e] = Fgc[e];
   }
+  const int64_t r0 = blockIdx.x * static_cast<int64_t>(kURows);
+  for (int e = tid; e < kURows * l; e += kThreads) {
+    const int rr = e / l, j = e % l;
+    const int64_t r = r0 + rr;
+    const bool ok = r < nl;
+    sR[0][rr][j] = ok ? Bc[r * l + j] : 0.0;
+    sR[1][rr][j] = ok ? Bg[r * l + j] : 0.0;
+    sR[2][rr][j] = ok ? Ec[r * l + j] : 0.0;
+  }
   __syncthreads();
-  const int64_t total = nl * l;
-  for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * kThreads) {
-    const int64_t r = e / l;
-    const int j = e % l;
-    const double* bc = Bc + r * l;
-    const double* bg = Bg + r * l;
-    const double* ec = Ec + r * l;
-    double bcf = 0.0, bgg = 0.0, bcg = 0.0, ecf = 0.0;
+  const int j = tid % kWave, w = tid / kWave;
+  if (j < l && r0 < nl) {
+    double bcf[RW], bgg[RW], bcg[RW], ecf[RW];
+#pragma unroll
+    for (int u = 0; u < RW; ++u) bcf[u] = bgg[u] = bcg[u] = ecf[u] = 0.0;
     for (int q = 0; q < l; ++q) {
       const double f = sF[q * l + j], g = sG[q * l + j];
-      bcf += bc[q] * f;
-      bgg += bg[q] * g;
-      bcg += bc[q] * g;
-      ecf += ec[q] * f;
+#pragma unroll
+      for (int u = 0; u < RW; ++u) {
+        const int rr = w + 4 * u;
+        const double bc = sR[0][rr][q], bg = sR[1][rr][q], ec = sR[2][rr][q];
+        bcf[u] += bc * f;
+        bgg[u] += bg * g;
+        bcg[u] += bc * g;
+        ecf[u] += ec * f;
+      }
     }
-    Bp[e] = bcf + bgg;
-    U1[e] = invN * bcg;
-    BsU1[e] = bgg;
-    Ep[e] = invN * bcg + ecf;
+#pragma unroll
+    for (int u = 0; u < RW; ++u) {
+      const int64_t r = r0 + w + 4 * u;
+      if (r < nl) {
+        const int64_t e = r * l + j;
+        Bp[e] = bcf[u] + bgg[u];
+        U1[e] = invN * bcg[u];
+        BsU1[e] = bgg[u];
+        Ep[e] = invN * bcg[u] + ecf[u];
+      }
+    }
   }
-  if (blockIdx.x == 0) {
-    for (int e = threadIdx.x; e < l * l; e += kThreads) {
-      const int i = e / l, j = e % l;
+  // FF+: this workgroup's entries [kThreads b, kThreads (b + 1)) lie in rows
+  // [kThreads b / l, ...] of FFc: at most kThreads / l + 1 rows, staged like the others
+  const int64_t fe0 = blockIdx.x * static_cast<int64_t>(kThreads);
+  if (fe0 < static_cast<int64_t>(l) * l) {
+    __syncthreads();  // sR is reused
+    const int i0 = static_cast<int>(fe0 / l);
+    const int nr = std::min(l - i0, kThreads / l + 1);
+    double* sFF = &sR[0][0][0];
+    for (int e = tid; e < nr * l; e += kThreads) sFF[e] = FFc[i0 * l + e];
+    __syncthreads();
+    const int64_t fe = fe0 + tid;
+    if (fe < static_cast<int64_t>(l) * l) {
+      const int i = static_cast<int>(fe / l), jj = static_cast<int>(fe % l);
       double acc = 0.0;
-      for (int q = 0; q < l; ++q) acc += FFc[i * l + q] * sF[q * l + j];
-      FFp[e] = acc;
+      for (int q = 0; q < l; ++q) acc += sFF[(i - i0) * l + q] * sF[q * l + jj];
+      FFp[fe] = acc;
     }
   }
 }
@@ -815,34 +850,49 @@ __global__ __launch_bounds__(kThreads) void wy_terms_kernel(
     const double* __restrict__ Ec, const double* __restrict__ FFc, double* __restrict__ P) {
   if (skip(st, t) || st->converged_at == t) return;
   __shared__ double sD[kMaxL * kMaxL];
-  __shared__ double sBc[kRows][kMaxL];
-  __shared__ double sU1[kRows][kMaxL];
-  __shared__ double sU2[kRows][kMaxL];
+  __shared__ double sBc[kRows][kMaxL + 1];  // rows padded: the MFMA fragment reads below
+  __shared__ double sU1[kRows][kMaxL + 1];  // take two k-rows per 32 lanes
+  __shared__ double sU2[kRows][kMaxL + 1];
   __shared__ double sBsU[kRows][kMaxL];
-  __shared__ double sM[kRows][kMaxL];
+  __shared__ double sM[kRows][kMaxL + 1];
+  __shared__ double sBsU1[kRows][kMaxL];
   __shared__ double red[2][kThreads / kWave];
   for (int e = threadIdx.x; e < l * l; e += kThreads) sD[e] = Dp[e];
-  __syncthreads();
   const int64_t r0 = blockIdx.x * (int64_t)kRows;
+  // the block's rows of U1 and BsU1 go to LDS first (coalesced): the products with D+ below
+  // read them along the row
   for (int e = threadIdx.x; e < kRows * l; e += kThreads) {
     const int rr = e / l, j = e % l;
     const int64_t r = r0 + rr;
-    double bc = 0, u1 = 0, u2 = 0, bsu = 0, mc = 0;
+    sU1[rr][j] = (r < nl) ? U1[r * l + j] : 0.0;
+    sBsU1[rr][j] = (r < nl) ? BsU1[r * l + j] : 0.0;
+  }
+  // columns l .. pad16(l) of the MFMA operands are zero (those tiles' outputs are not stored)
+  for (int e = threadIdx.x; e < kRows * (kMaxL - l); e += kThreads) {
+    const int rr = e / (kMaxL - l), j = l + e % (kMaxL - l);
+    sBc[rr][j] = 0.0;
+    sU1[rr][j] = 0.0;
+    sU2[rr][j] = 0.0;
+    sM[rr][j] = 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kRows * l; e += kThreads) {
+    const int rr = e / l, j = e % l;
+    const int64_t r = r0 + rr;
+    double bc = 0, u2 = 0, bsu = 0, mc = 0;
     if (r < nl) {
       const int64_t o = r * l;
       bc = Bc[o + j];
-      u1 = U1[o + j];
       mc = Ec[o + j] + (r < l ? FFc[o + j] : 0.0);
       double u1d = 0.0, bsu1d = 0.0;
       for (int q = 0; q < l; ++q) {
-        u1d += U1[o + q] * sD[q * l + j];
-        bsu1d += BsU1[o + q] * sD[q * l + j];
+        u1d += sU1[rr][q] * sD[q * l + j];
+        bsu1d += sBsU1[rr][q] * sD[q * l + j];
       }
       u2 = invN * (Bp[o + j] - bc) - u1d;
-      bsu = odd ? BsU1[o + j] : (Bgp[o + j] - Bgc[o + j]) - bsu1d;
+      bsu = odd ? sBsU1[rr][j] : (Bgp[o + j] - Bgc[o + j]) - bsu1d;
     }
     sBc[rr][j] = bc;
-    sU1[rr][j] = u1;
     sU2[rr][j] = u2;
     sBsU[rr][j] = bsu;
     sM[rr][j] = mc;
@@ -850,14 +900,25 @@ __global__ __launch_bounds__(kThreads) void wy_terms_kernel(
   __syncthreads();
   const int count = 3 * l * l + 2;
   double* Pb = P + blockIdx.x * (int64_t)count;
-  for (int q = threadIdx.x; q < 3 * l * l; q += kThreads) {
-    const int part = q / (l * l), i = (q / l) % l, j = q % l;
-    double s = 0.0;
-    for (int rr = 0; rr < kRows; ++rr) {
-      const double y = part == 0 ? sU1[rr][j] : (part == 1 ? sU2[rr][j] : sM[rr][j]);
-      s += sBc[rr][i] * y;
+  {  // Bc'U1, Bc'U2, Bc'Mc over the block's rows on f64 MFMA tiles (16 x 16, k = row)
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int nt1 = pad16(l) / 16, tiles = nt1 * nt1;
+    for (int tile = wave; tile < 3 * tiles; tile += kThreads / kWave) {
+      const int part = tile / tiles, i0 = ((tile % tiles) / nt1) * 16, j0 = (tile % nt1) * 16;
+      const double(*Y)[kMaxL + 1] = part == 0 ? sU1 : (part == 1 ? sU2 : sM);
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+      for (int k0 = 0; k0 < kRows; k0 += 4) {
+        const int k = k0 + lane / 16;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sBc[k][i0 + lane % 16], Y[k][j0 + lane % 16],
+                                                   acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + lane / 16 + 4 * r, j = j0 + lane % 16;
+        if (i < l && j < l) Pb[part * l * l + i * l + j] = acc[r];
+      }
     }
-    Pb[q] = s;
   }
   double d1 = 0.0, d2 = 0.0;
   for (int e = threadIdx.x; e < kRows * l; e += kThreads) {
@@ -1081,6 +1142,12 @@ __global__ void coef_kernel(const double* __restrict__ E, const double* __restri
        e += (int64_t)gridDim.x * blockDim.x) {
     M[e] = E[e] + (e < (int64_t)l * l ? FF[e] : 0.0);
   }
+}
+
+// Workgroups of wy_update_kernel: the row blocks, and at least those FF+ = FFc F needs.
+unsigned wy_update_blocks(int64_t nl, int l) {
+  const int64_t b = std::max(ceil_div(nl, kURows), ceil_div(static_cast<int64_t>(l) * l, kThreads));
+  return static_cast<unsigned>(b);
 }
 
 unsigned grid_elems(int64_t total) {
@@ -1558,7 +1625,7 @@ int64_t integrate_gram(const GramIntegrator& g, double* coef, hipStream_t st, bo
         hipLaunchKernelGGL(wy_search_kernel, dim3(1), dim3(kWyThreads), wy_lds, ls, ti, L.st, l,
                            L.D3[c], L.F, L.C);
         MCNLA_LAUNCHED();
-        hipLaunchKernelGGL(wy_update_kernel, dim3(grid_elems(nl * l)), dim3(kThreads), 0, ls, ti,
+        hipLaunchKernelGGL(wy_update_kernel, dim3(wy_update_blocks(nl, l)), dim3(kThreads), 0, ls, ti,
                            L.st, nl, l, invN, L.B[c], L.Bg[c], L.B[p], L.U1, L.BsU1, L.E[c],
                            L.E[p], L.FF[c], L.FF[p], L.F, L.C);
         MCNLA_LAUNCHED();
