@@ -394,10 +394,13 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aok[b] ? af[b][i] : 0.0, bf[b][j],
                                                            acc[i][j], 0, 0, 0);
       if (kk + 4 < BK) {
+        // the next group's fragment reads (paired into (FM + FN) / 2 instructions), spread
+        // evenly between this group's FM FN MFMAs
+        constexpr int NQ = (FM + FN) / 2;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {  // one fragment read pair between every four MFMAs
+        for (int q = 0; q < NQ; ++q) {
           __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, FM * FN / NQ, 0);
         }
       }
     }
