@@ -258,6 +258,14 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((VM & 0xF) | (0x7 << 4) | (0xF << 8) | (((VM >> 4) & 3) << 14));
 }
 
+// A 64-bit value that is the same in every lane, moved into scalar registers.
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+  const uint64_t u = static_cast<uint64_t>(x);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u >> 32));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 template <int BM, int BN, int BK, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kernel(GemmF64Args p) {
   constexpr int NW = (BM / WM) * (BN / WN);
@@ -316,14 +324,17 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
     tm = t / tiles_n;
     tn = t % tiles_n;
   }
-  const int64_t m0 = tm * BM, n0 = tn * BN;
-  const int64_t kbeg = split * p.kchunk;
-  const int64_t kend = (kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K;
+  // The tile's coordinates are uniform by construction; moved into scalar registers (the
+  // schedule's square root and 64-bit divisions leave them in vector registers), every DMA
+  // address below is scalar arithmetic.
+  const int64_t m0 = uniform64(tm * BM), n0 = uniform64(tn * BN);
+  const int64_t kbeg = uniform64(split * p.kchunk);
+  const int64_t kend = uniform64((kbeg + p.kchunk < p.K) ? kbeg + p.kchunk : p.K);
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
   const int wm0 = (wave / (BN / WN)) * WM, wn0 = (wave % (BN / WN)) * WN;
-  const double* __restrict__ A = static_cast<const double*>(p.A) + bat * p.stride_a;
-  const double* __restrict__ B = static_cast<const double*>(p.B) + bat * p.stride_b;
+  const double* __restrict__ A = static_cast<const double*>(p.A) + uniform64(bat * p.stride_a);
+  const double* __restrict__ B = static_cast<const double*>(p.B) + uniform64(bat * p.stride_b);
   // A wave whose WM x WN block lies strictly above the diagonal of a symmetric output only moves
   // tiles for the others; the reduction never reads its entries.
   const bool live = !p.lower_only || m0 + wm0 + WM - 1 >= n0 + wn0;
@@ -334,41 +345,78 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  const int64_t ntk = kbeg < kend ? ceil_div(kend - kbeg, BK) : 0;
-  auto issue = [&](int64_t kt, double* Ad, double* Bd) {
-    const int64_t k0 = kbeg + kt * BK;
+  // k-rows within the split are counted from kbeg in 32 bits (SALU has no signed 64-bit compare,
+  // so 64-bit bounds checks would run on the VALU)
+  const int krem = static_cast<int>(kend - kbeg);  // k-rows of this split
+  const int ntk = krem > 0 ? (krem + BK - 1) / BK : 0;
+  // DMA instruction q of a stage moves 128 doubles of k-row kk(q) of a tile, from column chunk
+  // ch(q).  Its source is a uniform k-row address (the stage's first k-row plus kk(q) rows, or the
+  // split's first k-row past the split's end) plus the lane's byte offset in the chunk, which is
+  // fixed (columns past M / N read the tile's first column).
+  const char* a0 = reinterpret_cast<const char*>(A + m0);
+  const char* b0 = reinterpret_cast<const char*>(B + n0);
+  const int64_t lda8 = p.lda * 8, ldb8 = p.ldb * 8;
+  uint32_t aoff[LA], boff[LB];
+  int64_t akoff[LA], bkoff[LB];
 #pragma unroll
-    for (int q = 0; q < LA; ++q) {
-      const int idx = q * NW + wave;
-      const int kk = idx / RA, ch = idx % RA;
-      const int64_t gk = (k0 + kk < kend) ? k0 + kk : kbeg;
-      int64_t gm = m0 + ch * 128 + lane * 2;
-      if (gm >= p.M) gm = m0;
-      dma16(A + gm + gk * p.lda, (lds_void*)&Ad[kk * LDA + ch * 128]);
-    }
+  for (int q = 0; q < LA; ++q) {
+    const int idx = q * NW + wave;
+    const int64_t gm = m0 + (idx % RA) * 128 + lane * 2;
+    aoff[q] = static_cast<uint32_t>(((gm < p.M) ? gm - m0 : 0) * 8);
+    akoff[q] = (idx / RA) * lda8;
+  }
 #pragma unroll
-    for (int q = 0; q < LB; ++q) {
-      const int idx = q * NW + wave;
-      const int kk = idx / RB, ch = idx % RB;
-      const int64_t gk = (k0 + kk < kend) ? k0 + kk : kbeg;
-      int64_t gn = n0 + ch * 128 + lane * 2;
-      if (gn >= p.N) gn = n0;
-      dma16(B + gn + gk * p.ldb, (lds_void*)&Bd[kk * LDB + ch * 128]);
+  for (int q = 0; q < LB; ++q) {
+    const int idx = q * NW + wave;
+    const int64_t gn = n0 + (idx % RB) * 128 + lane * 2;
+    boff[q] = static_cast<uint32_t>(((gn < p.N) ? gn - n0 : 0) * 8);
+    bkoff[q] = (idx / RB) * ldb8;
+  }
+  // DMA instruction i (A for i < LA, then B) of the stage starting at k-row k0
+  const char* a_first = a0 + kbeg * lda8;  // the split's first k-row
+  const char* b_first = b0 + kbeg * ldb8;
+  // DMA instruction i (A for i < LA, then B) of the stage starting at k-row k0, whose k-row
+  // addresses start at arow / brow
+  auto dma = [&](int i, int kofs, const char* arow, const char* brow, double* Ad, double* Bd) {
+    if (i < LA) {
+      const int q = i, kk = (q * NW + wave) / RA, ch = (q * NW + wave) % RA;
+      const char* row = (kofs + kk < krem) ? arow + akoff[q] : a_first;
+      dma16(row + aoff[q], (lds_void*)&Ad[kk * LDA + ch * 128]);
+    } else {
+      const int q = i - LA, kk = (q * NW + wave) / RB, ch = (q * NW + wave) % RB;
+      const char* row = (kofs + kk < krem) ? brow + bkoff[q] : b_first;
+      dma16(row + boff[q], (lds_void*)&Bd[kk * LDB + ch * 128]);
     }
+  };
+  auto issue = [&](int kt, double* Ad, double* Bd) {
+    const int kofs = kt * BK;
+    const char* arow = a_first + kofs * lda8;
+    const char* brow = b_first + kofs * ldb8;
+#pragma unroll
+    for (int i = 0; i < L; ++i) dma(i, kofs, arow, brow, Ad, Bd);
   };
   // Branch-free pipeline: the stage two ahead is issued every step (clamped to the last stage,
   // whose rows are then read again and never used), and the step count is padded to a multiple
   // of three with steps whose A fragments are all zero.  Without branches in the loop the
   // compiler keeps its per-buffer wait tracking, so the only DMA waits are the explicit ones.
-  auto step = [&](int64_t kt, const double* as, const double* bs, double* An, double* Bn) {
+  // MASK: the step may hold k-rows past the split's range (its last stage, and the padding
+  // steps); the steps before it run without the masks.
+  auto step = [&](auto mask, int kt, const double* as, const double* bs, double* An, double* Bn) {
+    constexpr bool MASK = decltype(mask)::value;
     asm volatile("" ::: "memory");
     wait_vmcnt<L>();               // this stage landed; the next may still be in flight
     __builtin_amdgcn_s_barrier();  // ... for every wave; and stage kt-1 is fully read
     asm volatile("" ::: "memory");
-    issue(kt + 2 < ntk ? kt + 2 : ntk - 1, An, Bn);
-    const int64_t left = kend - (kbeg + kt * BK);
-    const int kvalid = left < BK ? static_cast<int>(left) : BK;  // <= 0 past the last stage
-    if (!live) return;
+    const int kn = kt + 2 < ntk ? kt + 2 : ntk - 1;
+    const int left = krem - kt * BK;
+    const int kvalid = left < BK ? left : BK;  // <= 0 past the last stage
+    // The stage two ahead is issued DPG instructions at a time between the MFMAs of the first
+    // k groups (its buffer was last read in step kt - 1, before the barrier): issued as one burst
+    // after the barrier, the DMA held every wave's first MFMAs back.
+    constexpr int NG = BK / 4, DPG = (L + NG - 2) / (NG - 1);
+    const int kofs_n = kn * BK;
+    const char* arow_n = a_first + kofs_n * lda8;
+    const char* brow_n = b_first + kofs_n * ldb8;
     // A fragments of k-rows past the split's range are zeroed where an MFMA takes them, not as
     // they are read: a select on the read result would wait for the LDS read at once.
     double af[2][FM], bf[2][FN];
@@ -391,29 +439,65 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aok[b] ? af[b][i] : 0.0, bf[b][j],
-                                                           acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              (!MASK || aok[b]) ? af[b][i] : 0.0, bf[b][j], acc[i][j], 0, 0, 0);
       if (kk + 4 < BK) {
-        // the next group's fragment reads (paired into (FM + FN) / 2 instructions), spread
-        // evenly between this group's FM FN MFMAs
+        const int g = kk / 4;
+#pragma unroll
+        for (int i = g * DPG; i < (g + 1) * DPG && i < L; ++i) dma(i, kofs_n, arow_n, brow_n, An, Bn);
+        // the next group's fragment reads (paired into (FM + FN) / 2 instructions) and this
+        // group's DMA instructions, spread evenly between its FM FN MFMAs, MFMAs first: LDS DMA
+        // shares the LDS counter, so each wait for fragments waits for every LDS read issued
+        // before it, and a read issued ahead of the group's first MFMA would be waited for there
         constexpr int NQ = (FM + FN) / 2;
+        const int nd = (g + 1) * DPG <= L ? DPG : (L > g * DPG ? L - g * DPG : 0);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
+          __builtin_amdgcn_sched_group_barrier(0x008, FM * FN / NQ / 2, 0);
           __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x008, FM * FN / NQ, 0);
+          if (q < nd) __builtin_amdgcn_sched_group_barrier(0x010, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, FM * FN / NQ - FM * FN / NQ / 2, 0);
         }
       }
     }
     __builtin_amdgcn_s_setprio(0);
   };
+  // A wave that computes nothing runs the same barriers and DMA in a loop of its own (a branch
+  // inside the loop would merge two DMA schedules, after which the compiler waits for every
+  // DMA in flight before the next fragment read).
+  auto move = [&](int kt, double* An, double* Bn) {
+    asm volatile("" ::: "memory");
+    wait_vmcnt<L>();
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    issue(kt + 2 < ntk ? kt + 2 : ntk - 1, An, Bn);
+  };
   if (ntk > 0) {
     issue(0, A0, B0);
     issue(ntk > 1 ? 1 : 0, A1, B1);
-    const int64_t nsteps = ceil_div(ntk, 3) * 3;
-    for (int64_t kt = 0; kt < nsteps; kt += 3) {
-      step(kt, A0, B0, A2, B2);
-      step(kt + 1, A1, B1, A0, B0);
-      step(kt + 2, A2, B2, A1, B1);
+    const int nsteps = (ntk + 2) / 3 * 3;
+    if (live) {
+      // whole stages of the split's range, three at a time, without masks; then the rest
+      const int nfull = (krem / BK) / 3 * 3;
+      const std::integral_constant<bool, false> unmasked;
+      const std::integral_constant<bool, true> masked;
+      int kt = 0;
+      for (; kt < nfull; kt += 3) {
+        step(unmasked, kt, A0, B0, A2, B2);
+        step(unmasked, kt + 1, A1, B1, A0, B0);
+        step(unmasked, kt + 2, A2, B2, A1, B1);
+      }
+      for (; kt < nsteps; kt += 3) {
+        step(masked, kt, A0, B0, A2, B2);
+        step(masked, kt + 1, A1, B1, A0, B0);
+        step(masked, kt + 2, A2, B2, A1, B1);
+      }
+    } else {
+      for (int kt = 0; kt < nsteps; kt += 3) {
+        move(kt, A2, B2);
+        move(kt + 1, A0, B0);
+        move(kt + 2, A1, B1);
+      }
     }
     wait_vmcnt<0>();  // no DMA may still target LDS when the workgroup ends
   }
