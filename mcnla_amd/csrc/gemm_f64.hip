@@ -748,18 +748,25 @@ void tall_stream_kernel(GemmF64Args p) {
     bsrc[i] = B + 2 * c;
     brow[i] = k;
   }
-  const int64_t ntk = kbeg < kend ? ceil_div(kend - kbeg, BK) : 0;
-  auto issue = [&](int64_t kt, float* s) {
-    const int64_t k0 = kbeg + kt * BK;
-#pragma unroll
-    for (int i = 0; i < LA; ++i)  // a chunk past the split's end reads the split's first chunk
-      dma16(asrc[i] + (k0 + achunk[i] < kend ? k0 : kbeg - achunk[i]),
+  // k within the split counted from kbeg in 32 bits (no 64-bit compares in the loop)
+  const int krem = kbeg < kend ? static_cast<int>(kend - kbeg) : 0;
+  const int ntk = (krem + BK - 1) / BK;
+  // DMA instruction i (A for i < LA, then B) of stage kt; a chunk past the split's end reads
+  // the split's first chunk
+  auto dma = [&](int i, int kt, float* s) {
+    const int kofs = kt * BK;
+    if (i < LA) {
+      dma16(asrc[i] + (kofs + achunk[i] < krem ? kbeg + kofs : kbeg - achunk[i]),
             (lds_void*)&s[(i * 4 + wave) * 256]);
-#pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      const int64_t k = (k0 + brow[i] < kend) ? k0 + brow[i] : kbeg;
-      dma16(bsrc[i] + k * p.ldb, (lds_void*)&s[AU * 4 + (i * 4 + wave) * 256]);
+    } else {
+      const int q = i - LA;
+      const int64_t k = kbeg + ((kofs + brow[q] < krem) ? kofs + brow[q] : 0);
+      dma16(bsrc[q] + k * p.ldb, (lds_void*)&s[AU * 4 + (q * 4 + wave) * 256]);
     }
+  };
+  auto issue = [&](int kt, float* s) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) dma(i, kt, s);
   };
 
   constexpr int TA2 = 2;  // 16-row groups per wave
@@ -770,19 +777,23 @@ void tall_stream_kernel(GemmF64Args p) {
     for (int b = 0; b < TJ; ++b) acc[a][b] = Acc{0, 0, 0, 0};
   const int g = lane / 16, fc = lane % 16;
   const int bsw = (g & 1) << 3;
-  auto step = [&](int64_t kt, const float* cur, float* nxt) {
+  // MASK: the stage may hold chunks past the split's range (its last stage and the padding
+  // steps); whole stages run without the masks.  The stage NS - 1 ahead is issued half after each
+  // chunk group's fragment reads, under the MFMAs (issued as one burst after the barrier, the DMA
+  // held every wave's first MFMAs back).
+  auto step = [&](auto mask, int kt, const float* cur, float* nxt) {
+    constexpr bool MASK = decltype(mask)::value;
     asm volatile("" ::: "memory");
     wait_vmcnt<(NS - 2) * L>();    // this stage landed; the younger ones may still be in flight
     __builtin_amdgcn_s_barrier();  // ... for every wave; and the oldest buffer is fully read
     asm volatile("" ::: "memory");
-    issue(kt + NS - 1 < ntk ? kt + NS - 1 : ntk - 1, nxt);
-    asm volatile("" ::: "memory");
-    const int64_t k0 = kbeg + kt * BK;
+    const int kn = kt + NS - 1 < ntk ? kt + NS - 1 : ntk - 1;
+    const int kofs = kt * BK;
     const double* cb = reinterpret_cast<const double*>(cur + AU * 4);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int c = 4 * q + g;
-      const bool ok = k0 + EPC * c < kend;
+      const bool ok = kofs + EPC * c < krem;
       V af[TA2];
 #pragma unroll
       for (int a = 0; a < TA2; ++a) {
@@ -800,12 +811,14 @@ void tall_stream_kernel(GemmF64Args p) {
         }
       }
 #pragma unroll
+      for (int i = q * L / 2; i < (q + 1) * L / 2; ++i) dma(i, kn, nxt);
+#pragma unroll
       for (int e = 0; e < EPC; ++e)
 #pragma unroll
         for (int a = 0; a < TA2; ++a)
 #pragma unroll
           for (int b = 0; b < TJ; ++b) {
-            const TA x = ok ? af[a][e] : TA(0);
+            const TA x = (!MASK || ok) ? af[a][e] : TA(0);
             if constexpr (F32)
               acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x, bf[b][e], acc[a][b], 0, 0, 0);
             else
@@ -819,10 +832,18 @@ void tall_stream_kernel(GemmF64Args p) {
   if (ntk > 0) {
 #pragma unroll
     for (int s = 0; s < NS - 1; ++s) issue(s < ntk ? s : ntk - 1, sbuf(s));
-    const int64_t nsteps = ceil_div(ntk, NS) * NS;
-    for (int64_t kt = 0; kt < nsteps; kt += NS) {
+    const int nsteps = (ntk + NS - 1) / NS * NS;
+    const int nfull = (krem / BK) / NS * NS;  // whole stages, NS at a time
+    const std::integral_constant<bool, false> unmasked;
+    const std::integral_constant<bool, true> masked;
+    int kt = 0;
+    for (; kt < nfull; kt += NS) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) step(kt + s, sbuf(s), sbuf((s + NS - 1) % NS));
+      for (int s = 0; s < NS; ++s) step(unmasked, kt + s, sbuf(s), sbuf((s + NS - 1) % NS));
+    }
+    for (; kt < nsteps; kt += NS) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) step(masked, kt + s, sbuf(s), sbuf((s + NS - 1) % NS));
     }
     wait_vmcnt<0>();  // no DMA may still target LDS when the workgroup ends
   }
@@ -1155,8 +1176,9 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
     // registers for smaller FP32 A (serving-size requests run faster so, alone and in the mix)
     // and for an FP32 A in FP64 arithmetic, and A straight into registers for an FP64 A.
     // p.kernel = 1 takes the other one for an FP32 A on FP32 MFMA, A straight into registers
-    // for an FP32 A in FP64 arithmetic, and the DMA stream for an FP64 A (faster for the 8 GB
-    // FP64 former Z = A'Qbar, slower for Qbar = Y (T M): 0.42 against 0.40 ms at 100000 x 2048).
+    // for an FP32 A in FP64 arithmetic, and the DMA stream for an FP64 A (the integrators' Bs
+    // products ask for it: 174 against 196 us for B_s B at 8192 x 8192, l = 64; slower for
+    // Qbar = Y (T M): 0.42 against 0.40 ms at 100000 x 2048).
     if (p.a_f32 && p.f32_mfma) {
       if (stream) {
         if (p.N <= 32) launch_tall_stream<float, 32>(p, st);

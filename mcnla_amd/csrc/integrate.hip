@@ -1257,6 +1257,9 @@ void bs_product(const GramIntegrator& g, const Layout& L, const double* Bin, dou
   p.partial = L.Pbg;
   p.gate = &L.st->converged_at;
   p.gate_t = t;
+  // Bs streamed into LDS by DMA (tall_stream_kernel; the register-staged kernel where its
+  // operand conditions fail): 174 us against 196 us for the 8192 x 8192 Bs at l = 64
+  p.kernel = 1;
   gemm_f64(p, st);
 }
 
