@@ -238,6 +238,7 @@ GemmShape tall_shape(int64_t ncols) {
 // Y = A * Omega with Omega generated into `omega` (n x Nl FP64).
 struct SketchCore {
   DevBuf omega, ws;
+  DevBuf tail;  // split-K partials of the FP64 sketch's last rows (allocated on first use)
   std::vector<double> host;
   hipEvent_t gemm_beg = nullptr, gemm_end = nullptr;
   bool own_ev = true;     // false: the events belong to a SolveRes
@@ -298,8 +299,31 @@ struct SketchCore {
       g.C = static_cast<double*>(Y);
       g.ldc = nl;
       g.shape = GemmShape::kSquare;
+      // A column-major FP64 A takes the DMA-tiled GEMM, one workgroup per CU: the rows of whole
+      // rounds of the grid go as one product, the rest split K (their last round would otherwise
+      // run a few full-length workgroups on a mostly idle device: 6256 tiles at 100000 x 10000,
+      // Nl = 2048, are 24 rounds and 112 tiles).  Not for a solver carved from a reserved
+      // region, which may not allocate the partials.
+      int64_t ts = 1;
+      const int64_t main = (g.a_mcontig && !g.a_f32 && !omega.borrowed)
+                               ? gemm_f64_dma_tail(g.M, g.N, g.K, &ts)
+                               : g.M;
       MCNLA_HIP(hipEventRecord(gemm_beg, st));
-      gemm_f64(g, st);
+      if (ts > 1) {
+        GemmF64Args t = g;
+        t.M = p.nrow - main;
+        t.A = static_cast<const double*>(A) + main;
+        t.C = static_cast<double*>(Y) + main * nl;
+        t.kchunk = ceil_div(ceil_div(g.K, ts), 16) * 16;
+        t.splits = ceil_div(g.K, t.kchunk);
+        tail.ensure(gemm_f64_partial_elems(t.M, t.N, t.splits) * sizeof(double));
+        t.partial = tail.as<double>();
+        g.M = main;
+        gemm_f64(g, st);
+        gemm_f64(t, st);
+      } else {
+        gemm_f64(g, st);
+      }
       MCNLA_HIP(hipEventRecord(gemm_end, st));
       recorded = true;
     } else {
@@ -1923,7 +1947,7 @@ mcnla_status mcnla_solver_workspace_bytes(mcnla_solver s, size_t* bytes) {
     size_t b = 0;
     for (const DevBuf* d : {&s->Y, &s->A64, &s->H, &s->P, &s->X, &s->gp, &s->L0, &s->T0, &s->W0,
                             &s->G, &s->W, &s->w, &s->V, &s->T, &s->Bs, &s->tmp, &s->ws,
-                            &s->coef, &s->M2, &s->Qbar, &s->bad, &s->sketch.omega, &s->sketch.ws,
+                            &s->coef, &s->M2, &s->Qbar, &s->bad, &s->sketch.omega, &s->sketch.ws, &s->sketch.tail,
                             &s->gram.partial, &s->form.Z, &s->form.partial, &s->form.Wg,
                             &s->form.eigw, &s->form.eigv, &s->form.Wk, &s->form.S, &s->form.U,
                             &s->form.V})

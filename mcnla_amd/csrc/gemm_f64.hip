@@ -1248,6 +1248,29 @@ size_t gemm_f64_partial_elems(int64_t M, int64_t N, int64_t splits, int64_t batc
          static_cast<size_t>(batch);
 }
 
+int64_t gemm_f64_dma_tail(int64_t M, int64_t N, int64_t K, int64_t* tail_splits) {
+  // 256 x 128 tiles, one workgroup per CU.  The rows of whole rounds run as they are; the rest
+  // (less than half a round) split K so that their workgroups end sooner.
+  *tail_splits = 1;
+  const int64_t cus = device_cus(), tn = ceil_div(N, 128), tm = ceil_div(M, 256);
+  int64_t g = cus, b = tn;  // row tiles per whole round: cus / gcd(cus, tn)
+  while (b != 0) {
+    const int64_t r = g % b;
+    g = b;
+    b = r;
+  }
+  const int64_t per_round = cus / g;
+  const int64_t main_tm = tm / per_round * per_round;
+  const int64_t tail = (tm - main_tm) * tn;  // workgroups of the last, partial round
+  if (main_tm == 0 || tail == 0 || 2 * tail > cus || (M - main_tm * 256) < 128) return M;
+  int64_t s = cus / tail;
+  if (s > 4) s = 4;
+  if (K / s < 1024) s = K / 1024;  // keep each split long
+  if (s < 2) return M;
+  *tail_splits = s;
+  return main_tm * 256;
+}
+
 int64_t gemm_f64_splits_for(int64_t tiles, int64_t K, int64_t target_blocks) {
   int64_t s = ceil_div(target_blocks, tiles > 0 ? tiles : 1);
   const int64_t cap = ceil_div(K, 256);  // keep at least ~256 k per split

@@ -54,6 +54,11 @@ void gemm_f64(GemmF64Args p, hipStream_t st);
 size_t gemm_f64_partial_elems(int64_t M, int64_t N, int64_t splits, int64_t batch = 1);
 // Split-K factor giving about target_blocks workgroups for `tiles` output tiles.
 int64_t gemm_f64_splits_for(int64_t tiles, int64_t K, int64_t target_blocks);
+// For a DMA-tiled product (M x N, K deep) without split-K: the rows that fill whole rounds of
+// the device's workgroups (returned; M when there is nothing to gain) and the split-K factor for
+// the remaining rows (*tail_splits, 1 when none), whose last round would otherwise run a few
+// full-length workgroups on a mostly idle device.
+int64_t gemm_f64_dma_tail(int64_t M, int64_t N, int64_t K, int64_t* tail_splits);
 
 // ------------------------------------------------------------------------------------------------
 // Mixed-precision sketch (gemm_bf16.hip): Y (FP32, row-major m x nl) = A * Omega with A and

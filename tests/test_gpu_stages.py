@@ -51,6 +51,29 @@ def test_sketch_f64(gpu, shape, order):
     assert np.array_equal(sk.omega(), om)
 
 
+def test_sketch_f64_tail_split(gpu):
+    """A column-major FP64 A whose row tiles do not fill whole rounds of the grid: 20 row tiles of
+    256 x 16 column tiles of 128, of which 16 fill one round of a 256-CU device and the last 4
+    (1000 rows, 64 workgroups) run as a second product with K split 4."""
+    m, n, k, p, N = 5096, 4096, 20, 12, 64
+    note("sketch f64 tail split %s" % ((m, n, k, p, N),))
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((m, n))
+    P = params(m, n, k, p, N)
+    sk = mc.ColBlockGaussianProjectionSketcher(P, seed=7)
+    sk.initialize()
+    dA = mc.DeviceArray.from_numpy(np.asfortranarray(A))
+    Y = P.createCollectionQ()
+    sk(dA, Y)
+    ref = A @ sk.omega()
+    got = Y.numpy()
+    scale = np.abs(ref).max()
+    assert np.abs(got[:4096] - ref[:4096]).max() < 1e-12 * scale
+    assert np.abs(got[4096:] - ref[4096:]).max() < 1e-12 * scale
+    sk(dA, Y)  # the same again, bit for bit (the partials are summed in a fixed order)
+    assert np.array_equal(Y.numpy(), got)
+
+
 @pytest.mark.parametrize("kind,cb", [(mc.ColBlockGaussianProjectionSketcher, True),
                                      (mc.RowBlockGaussianProjectionSketcher, False)])
 def test_sketch_reference_omega(gpu, kind, cb):
