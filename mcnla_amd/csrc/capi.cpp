@@ -238,7 +238,7 @@ GemmShape tall_shape(int64_t ncols) {
 // Y = A * Omega with Omega generated into `omega` (n x Nl FP64).
 struct SketchCore {
   DevBuf omega, ws;
-  DevBuf tail;  // split-K partials of the FP64 sketch's last rows (allocated on first use)
+  DevBuf tail;  // split-K partials of the sketch's last rows (allocated on first use)
   std::vector<double> host;
   hipEvent_t gemm_beg = nullptr, gemm_end = nullptr;
   bool own_ev = true;     // false: the events belong to a SolveRes
@@ -339,6 +339,16 @@ struct SketchCore {
         s.Y = static_cast<float*>(Y);
       s.ldy = nl;
       ws.ensure(sketch_bf16_workspace(s));
+      // the rows past the grid's whole rounds split K (not in a solver carved from a reserved
+      // region, which may not allocate the partials)
+      if (!omega.borrowed) {
+        const size_t tb = sketch_bf16_tail_bytes(s);
+        if (tb) {
+          tail.ensure(tb);
+          s.tail = tail.as<float>();
+          s.tail_bytes = tail.bytes;
+        }
+      }
       sketch_bf16(s, ws.p, ws.bytes, st, gemm_beg, gemm_end);
       recorded = true;
     }

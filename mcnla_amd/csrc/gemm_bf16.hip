@@ -296,6 +296,9 @@ __device__ __forceinline__ void wait_vmcnt() {
 // issued every step, clamped to the last block; the K step count is a multiple of 10), so the
 // compiler keeps per-buffer wait tracking and the only waits are the explicit ones.
 // rowmax (FP16): row i of Y is multiplied by 2^e_i.
+// The grid covers row tiles [tm0, tm0 + tcount) (x: tcount x tiles_n tiles) and the K steps in
+// ranges of kper (y: the range; kper a multiple of kUnroll).  P null: the sums go to Y; else to
+// FP32 partials P[range][row - 256 tm0][column] (unscaled), summed by sketch_tail_reduce_kernel.
 template <typename TY, int FMT, bool ONESEG>
 __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restrict__ Ab,
                                                         const uint16_t* __restrict__ Wb,
@@ -303,18 +306,21 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
                                                         int nseg_b, uint32_t amap, uint32_t bmap,
                                                         const uint32_t* __restrict__ rowmax,
                                                         int64_t m, int64_t nl,
-                                                        TY* __restrict__ Y, int64_t ldy) {
+                                                        TY* __restrict__ Y, int64_t ldy,
+                                                        int64_t tm0, int64_t tcount, int64_t kper,
+                                                        float* __restrict__ P) {
   constexpr int L = 8;  // DMA instructions per wave and stage (4 for A, 4 for B)
   static_assert(kStages == 5 && kUnroll == 10, "the loop body below is written for 5 slots");
   __shared__ __attribute__((aligned(16))) uint16_t S0[2 * kBlk], S1[2 * kBlk], S2[2 * kBlk],
       S3[2 * kBlk], S4[2 * kBlk];
-  const int64_t tiles_n = ceil_div(nl, kBR), tiles_m = ceil_div(m, kBR);
-  const int64_t logical = xcd_swizzle(blockIdx.x, tiles_m * tiles_n);
-  const int64_t tm = logical / tiles_n, tn = logical % tiles_n;
+  const int64_t tiles_n = ceil_div(nl, kBR);
+  const int64_t logical = xcd_swizzle(blockIdx.x, tcount * tiles_n);
+  const int64_t tm = tm0 + logical / tiles_n, tn = logical % tiles_n;
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const int wm0 = (wave / 2) * 128, wn0 = (wave % 2) * 128;
   const int kg = lane / 16;
-  const int64_t ktot = nseg * nkt;
+  const int64_t kbeg = static_cast<int64_t>(blockIdx.y) * kper;
+  const int64_t ktot = (kbeg + kper < nseg * nkt) ? kbeg + kper : nseg * nkt;  // this range's end
   const uint16_t* abase = Ab + tm * nseg_a * nkt * kBlk;
   const uint16_t* bbase = Wb + tn * nseg_b * nkt * kBlk;
   // the A and B blocks of step kt
@@ -388,17 +394,17 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
       asm volatile("" ::: "memory");
     }
   };
-  issue(0, S0);
-  issue(1, S1);
-  issue(2, S2);
-  issue(3, S3);
+  issue(kbeg, S0);
+  issue(kbeg + 1, S1);
+  issue(kbeg + 2, S2);
+  issue(kbeg + 3, S3);
   asm volatile("" ::: "memory");
   wait_vmcnt<3 * L>();  // stage 0 landed in every wave
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
   frags(S0, fa0, fb0);
   asm volatile("s_nop 7");  // zeroed accumulators (VALU writes) before their first MFMA read
-  for (int64_t kt = 0; kt < ktot; kt += kUnroll) {  // stage s sits in slot s % 5
+  for (int64_t kt = kbeg; kt < ktot; kt += kUnroll) {  // stage s sits in slot s % 5 (kbeg % 10 == 0)
     step(kt, fa0, fb0, S1, fa1, fb1, S4);
     step(kt + 1, fa1, fb1, S2, fa0, fb0, S0);
     step(kt + 2, fa0, fb0, S3, fa1, fb1, S1);
@@ -413,6 +419,22 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
   asm volatile("s_nop 7\n s_nop 7\n s_nop 7\n s_nop 7");  // last MFMAs drain before VALU reads
   wait_vmcnt<0>();  // no DMA may still target LDS when the workgroup ends
   const int64_t m0 = tm * kBR, n0 = tn * kBR;
+  if (P != nullptr) {
+    float* Pr = P + (static_cast<int64_t>(blockIdx.y) * tcount * kBR - tm0 * kBR) * nl;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = m0 + wm0 + i * 16 + 4 * (lane / 16) + r;
+        if (gi >= m) continue;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int64_t gj = n0 + wn0 + j * 16 + lane % 16;
+          if (gj < nl) Pr[gi * nl + gj] = acc[i][j][r];
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -431,6 +453,27 @@ __global__ __launch_bounds__(256) void sketch_blk_kernel(const uint16_t* __restr
         }
       }
     }
+}
+
+// Y rows [row0, row0 + rows) from the FP32 partials of sketch_blk_kernel's K ranges: their sum
+// in range order (FP32, as the kernel accumulates), times 2^e_row for FP16.
+template <typename TY>
+__global__ __launch_bounds__(256) void sketch_tail_reduce_kernel(
+    const float* __restrict__ P, int64_t ranges, int64_t prows, int64_t row0, int64_t rows,
+    int64_t nl, const uint32_t* __restrict__ rowmax, TY* __restrict__ Y, int64_t ldy) {
+  const int64_t total = rows * nl;
+  for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = e / nl, j = e % nl;
+    float s = 0.f;
+    for (int64_t q = 0; q < ranges; ++q) s += P[(q * prows + i) * nl + j];
+    const int64_t gi = row0 + i;
+    const int ex = rowmax ? row_exponent(rowmax[gi]) : 0;
+    if constexpr (sizeof(TY) == 8)
+      Y[gi * ldy + j] = ldexp(static_cast<double>(s), ex);
+    else
+      Y[gi * ldy + j] = ldexpf(s, ex);
+  }
 }
 
 // Segment layout of a term count: A segments, B segments, and the (A, B) segment of each step.
@@ -531,9 +574,43 @@ Workspace carve(const SketchBf16Args& a, void* base) {
                    rb ? reinterpret_cast<uint32_t*>(p + ab + wb) : nullptr, ab + wb + rb};
 }
 
+// One workgroup per CU (160 KiB of LDS each): the row tiles that fill whole rounds of the grid
+// run in full-length workgroups, the rest (at most half a round) with K split into `ranges`
+// ranges of kper steps, so that their last round ends sooner.
+struct TailPlan {
+  int64_t main_tm, tail_tm, ranges, kper;
+};
+TailPlan tail_plan(const BlkDims& d, const Terms& t) {
+  const int64_t ktot = t.nseg * d.nkt;
+  TailPlan p{d.tm, 0, 1, ktot};
+  const int64_t cus = compute_units();
+  int64_t g = cus, b = d.tn;
+  while (b != 0) {
+    const int64_t r = g % b;
+    g = b;
+    b = r;
+  }
+  const int64_t per_round = cus / g;  // row tiles whose tiles fill whole rounds
+  const int64_t main_tm = d.tm / per_round * per_round, tail = (d.tm - main_tm) * d.tn;
+  if (main_tm == 0 || tail == 0 || 2 * tail > cus) return p;
+  const int64_t units = ktot / kUnroll;  // K step groups
+  int64_t ranges = cus / tail;
+  if (ranges > 4) ranges = 4;
+  if (ranges > units / 4) ranges = units / 4;  // each range at least 4 groups
+  if (ranges < 2) return p;
+  const int64_t kper = ceil_div(units, ranges) * kUnroll;
+  return TailPlan{main_tm, d.tm - main_tm, ceil_div(ktot, kper), kper};
+}
+
 }  // namespace
 
 size_t sketch_bf16_workspace(const SketchBf16Args& a) { return carve(a, nullptr).bytes; }
+
+size_t sketch_bf16_tail_bytes(const SketchBf16Args& a) {
+  if (a.m == 0 || a.nl == 0) return 0;
+  const TailPlan p = tail_plan(blk_dims(a), terms_of(a.terms));
+  return p.tail_tm ? static_cast<size_t>(p.ranges * p.tail_tm * kBR * a.nl) * sizeof(float) : 0;
+}
 
 void sketch_bf16(const SketchBf16Args& a, void* workspace, size_t workspace_bytes, hipStream_t st,
                  hipEvent_t gemm_begin, hipEvent_t gemm_end) {
@@ -563,19 +640,40 @@ void sketch_bf16(const SketchBf16Args& a, void* workspace, size_t workspace_byte
   // no scaling in either format.
   launch_pack(a.fmt, a.omega, a.nl, true, a.nl, a.n, d.tn, d.nkt, t.nseg_b, 0,
               t.nseg_b == 2 ? 1 : -1, static_cast<const uint32_t*>(nullptr), w.Wb, st);
-  const dim3 grid(static_cast<unsigned>(d.tm * d.tn));
+  TailPlan tp{d.tm, 0, 1, t.nseg * d.nkt};
+  if (a.tail != nullptr) {
+    const TailPlan q = tail_plan(d, t);
+    if (q.tail_tm && a.tail_bytes >= static_cast<size_t>(q.ranges * q.tail_tm * kBR * a.nl) *
+                                          sizeof(float))
+      tp = q;
+  }
   if (gemm_begin) MCNLA_HIP(hipEventRecord(gemm_begin, st));
   auto launch = [&](auto* Y, auto fmt_tag) {
     using TY = std::remove_pointer_t<decltype(Y)>;
     constexpr int FMT = decltype(fmt_tag)::value;
-    if (t.nseg == 1)
-      hipLaunchKernelGGL((sketch_blk_kernel<TY, FMT, true>), grid, dim3(256), 0, st, w.Ab, w.Wb,
-                         d.nkt, t.nseg, t.nseg_a, t.nseg_b, t.amap, t.bmap,
-                         static_cast<const uint32_t*>(w.rowmax), a.m, a.nl, Y, a.ldy);
-    else
-      hipLaunchKernelGGL((sketch_blk_kernel<TY, FMT, false>), grid, dim3(256), 0, st, w.Ab, w.Wb,
-                         d.nkt, t.nseg, t.nseg_a, t.nseg_b, t.amap, t.bmap,
-                         static_cast<const uint32_t*>(w.rowmax), a.m, a.nl, Y, a.ldy);
+    const uint32_t* rm = w.rowmax;
+    // tm0, tcount, kper, partials: the rows of whole rounds, then the rest in K ranges
+    auto gemm = [&](int64_t tm0, int64_t tcount, int64_t ranges, int64_t kper, float* P) {
+      const dim3 grid(static_cast<unsigned>(tcount * d.tn), static_cast<unsigned>(ranges));
+      if (t.nseg == 1)
+        hipLaunchKernelGGL((sketch_blk_kernel<TY, FMT, true>), grid, dim3(256), 0, st, w.Ab, w.Wb,
+                           d.nkt, t.nseg, t.nseg_a, t.nseg_b, t.amap, t.bmap, rm, a.m, a.nl, Y,
+                           a.ldy, tm0, tcount, kper, P);
+      else
+        hipLaunchKernelGGL((sketch_blk_kernel<TY, FMT, false>), grid, dim3(256), 0, st, w.Ab,
+                           w.Wb, d.nkt, t.nseg, t.nseg_a, t.nseg_b, t.amap, t.bmap, rm, a.m, a.nl,
+                           Y, a.ldy, tm0, tcount, kper, P);
+      MCNLA_LAUNCHED();
+    };
+    gemm(0, tp.main_tm, 1, t.nseg * d.nkt, nullptr);
+    if (tp.tail_tm) {
+      gemm(tp.main_tm, tp.tail_tm, tp.ranges, tp.kper, a.tail);
+      const int64_t row0 = tp.main_tm * kBR, rows = a.m - row0;
+      const int64_t blocks = std::min<int64_t>(ceil_div(rows * a.nl, 256), 4096);
+      hipLaunchKernelGGL(sketch_tail_reduce_kernel<TY>, dim3(static_cast<unsigned>(blocks)),
+                         dim3(256), 0, st, a.tail, tp.ranges, tp.tail_tm * kBR, row0, rows, a.nl,
+                         rm, Y, a.ldy);
+    }
   };
   using Bf = std::integral_constant<int, kFmtBf16>;
   using F16 = std::integral_constant<int, kFmtF16>;

@@ -1248,6 +1248,8 @@ size_t gemm_f64_partial_elems(int64_t M, int64_t N, int64_t splits, int64_t batc
          static_cast<size_t>(batch);
 }
 
+int compute_units() { return device_cus(); }
+
 int64_t gemm_f64_dma_tail(int64_t M, int64_t N, int64_t K, int64_t* tail_splits) {
   // 256 x 128 tiles, one workgroup per CU.  The rows of whole rounds run as they are; the rest
   // (less than half a round) split K so that their workgroups end sooner.

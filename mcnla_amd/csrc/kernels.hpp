@@ -59,6 +59,8 @@ int64_t gemm_f64_splits_for(int64_t tiles, int64_t K, int64_t target_blocks);
 // the remaining rows (*tail_splits, 1 when none), whose last round would otherwise run a few
 // full-length workgroups on a mostly idle device.
 int64_t gemm_f64_dma_tail(int64_t M, int64_t N, int64_t K, int64_t* tail_splits);
+// Compute units of the current device (cached per device).
+int compute_units();
 
 // ------------------------------------------------------------------------------------------------
 // Mixed-precision sketch (gemm_bf16.hip): Y (FP32, row-major m x nl) = A * Omega with A and
@@ -76,11 +78,17 @@ struct SketchBf16Args {
   int64_t ldy = 0;
   int terms = 1;  // 1, 2 or 3
   int fmt = 0;    // 0: BF16, 1: FP16
+  // Optional FP32 scratch for the K-split partials of the rows past the grid's whole rounds
+  // (sketch_bf16_tail_bytes); without it those rows run in full-length workgroups.
+  float* tail = nullptr;
+  size_t tail_bytes = 0;
 };
 // Packs A and Omega, then runs the GEMM; the optional events bracket the GEMM launch alone.
 void sketch_bf16(const SketchBf16Args& a, void* workspace, size_t workspace_bytes, hipStream_t st,
                  hipEvent_t gemm_begin = nullptr, hipEvent_t gemm_end = nullptr);
 size_t sketch_bf16_workspace(const SketchBf16Args& a);
+// Bytes of SketchBf16Args::tail the split of the last rows takes on this device (0: no split).
+size_t sketch_bf16_tail_bytes(const SketchBf16Args& a);
 
 // ------------------------------------------------------------------------------------------------
 // Random numbers and synthetic data (rng.hip).

@@ -181,6 +181,39 @@ def test_sketch_mixed_f16(gpu, order):
     assert rowrel(Y3, A.astype(np.float64) @ om3) < 2e-6
 
 
+def test_sketch_mixed_tail_split(gpu):
+    """16-bit sketches whose 256 x 256 tiles do not fill whole rounds of the grid: 36 row tiles
+    of 8 column tiles, of which 32 fill one round of a 256-CU device and the last 4 (1000 rows)
+    run with K split in 4 ranges whose FP32 partials are summed by a second kernel (FP16: then
+    scaled by the row's power of two).  Every row equals the product of the rounded operands up
+    to FP32 accumulation, and a second run is bit-identical."""
+    m, n, k, p, N = 9192, 5000, 20, 12, 64
+    note("sketch mixed tail split %s" % ((m, n, k, p, N),))
+    rng = np.random.default_rng(5)
+    A = (rng.standard_normal((m, n)) * 10.0 ** rng.uniform(-3, 3, (m, 1))).astype(np.float32)
+    P = params(m, n, k, p, N)
+    dA = mc.DeviceArray.from_numpy(np.asfortranarray(A))
+    e, As = f16_rows(A)
+    for prec in (mc.PREC_MIXED_BF16, mc.PREC_MIXED_F16):
+        sk = mc.ColBlockGaussianProjectionSketcher(P, seed=11, precision=prec)
+        Y = P.createCollectionQ(np.float32)
+        sk(dA, Y)
+        got = Y.numpy().astype(np.float64)
+        om = sk.omega()
+        if prec == mc.PREC_MIXED_BF16:
+            pinned = bf16_round(A) @ bf16_round(om)
+        else:
+            hi = As.astype(np.float16).astype(np.float64)
+            pinned = np.ldexp(hi @ om.astype(np.float32).astype(np.float16).astype(np.float64),
+                              e[:, None])
+        scale = np.abs(pinned).max(axis=1)
+        rel = np.abs(got - pinned).max(axis=1) / scale
+        assert rel[:8192].max() < 2e-5 and rel[8192:].max() < 2e-5, (prec, rel[:8192].max(),
+                                                                      rel[8192:].max())
+        sk(dA, Y)
+        assert np.array_equal(Y.numpy().astype(np.float64), got)
+
+
 @pytest.mark.parametrize("kind,cb", [(mc.ColumnSamplingSketcher, True),
                                      (mc.RowBlockColumnSamplingSketcher, False)])
 @pytest.mark.parametrize("dtype,order", [(np.float64, "F"), (np.float32, "C")])
