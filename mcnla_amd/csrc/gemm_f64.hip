@@ -270,13 +270,17 @@ template <int BM, int BN, int BK, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kernel(GemmF64Args p) {
   constexpr int NW = (BM / WM) * (BN / WN);
   // LDS row strides of 128 bytes modulo 256: a fragment read (lanes 0-15 on k-row r, 16-31 on
-  // r + 1, 8 bytes each) then covers the 64 banks once per 32 lanes.
-  constexpr int LDA = BM + 16, LDB = BN + 16;
-  constexpr int RA = BM / 128, RB = BN / 128;          // wave instructions per tile row
-  constexpr int LA = BK * RA / NW, LB = BK * RB / NW;  // per wave and stage
+  // r + 1, 8 bytes each) then covers the 64 banks once per 32 lanes.  NARROW (BN = 64): B rows
+  // are 512 bytes, unpadded, two k-rows per DMA instruction, and element (k, j) sits at
+  // k BN + (j ^ 16 (k & 1)): odd rows have their 128-byte halves swapped, to the same effect.
+  constexpr bool NARROW = BN == 64;
+  constexpr int LDA = BM + 16, LDB = NARROW ? BN : BN + 16;
+  constexpr int RA = BM / 128, RB = NARROW ? 1 : BN / 128;  // wave instructions per tile row
+  constexpr int LA = BK * RA / NW, LB = (NARROW ? BK / 2 : BK * RB) / NW;  // per wave and stage
   constexpr int L = LA + LB;
   constexpr int FM = WM / 16, FN = WN / 16;
-  static_assert(BM % 128 == 0 && BN % 128 == 0 && (BK * RA) % NW == 0 && (BK * RB) % NW == 0,
+  static_assert(BM % 128 == 0 && (NARROW || BN % 128 == 0) && (BK * RA) % NW == 0 &&
+                    (NARROW ? (BK / 2) % NW == 0 : (BK * RB) % NW == 0) && LB >= 1,
                 "DMA tile shape");
   __shared__ __attribute__((aligned(16))) double A0[BK * LDA];
   __shared__ __attribute__((aligned(16))) double A1[BK * LDA];
@@ -286,8 +290,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
   __shared__ __attribute__((aligned(16))) double B2[BK * LDB];
 
   if (p.gate != nullptr && *p.gate < p.gate_t) return;
-  constexpr int64_t R = BM / BN;
-  static_assert(R == 2, "the lower-triangle schedule assumes BM = 2 BN");
+  constexpr int64_t R = BM / BN;  // the lower-triangle schedule needs R = 2 (host-checked)
   const int64_t tiles_m = ceil_div(p.M, BM), tiles_n = ceil_div(p.N, BN);
   int64_t bat, split, tm, tn;
   if (p.lower_only) {
@@ -368,9 +371,16 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
 #pragma unroll
   for (int q = 0; q < LB; ++q) {
     const int idx = q * NW + wave;
-    const int64_t gn = n0 + (idx % RB) * 128 + lane * 2;
-    boff[q] = static_cast<uint32_t>(((gn < p.N) ? gn - n0 : 0) * 8);
-    bkoff[q] = (idx / RB) * ldb8;
+    if constexpr (NARROW) {  // k-rows 2 idx (lanes 0-31) and 2 idx + 1 (lanes 32-63)
+      const int h = lane / 32;
+      const int64_t gn = n0 + ((2 * (lane % 32)) ^ (16 * h));
+      boff[q] = static_cast<uint32_t>(h * ldb8 + ((gn < p.N) ? gn - n0 : 0) * 8);
+      bkoff[q] = 2 * idx * ldb8;
+    } else {
+      const int64_t gn = n0 + (idx % RB) * 128 + lane * 2;
+      boff[q] = static_cast<uint32_t>(((gn < p.N) ? gn - n0 : 0) * 8);
+      bkoff[q] = (idx / RB) * ldb8;
+    }
   }
   // DMA instruction i (A for i < LA, then B) of the stage starting at k-row k0
   const char* a_first = a0 + kbeg * lda8;  // the split's first k-row
@@ -382,6 +392,10 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
       const int q = i, kk = (q * NW + wave) / RA, ch = (q * NW + wave) % RA;
       const char* row = (kofs + kk < krem) ? arow + akoff[q] : a_first;
       dma16(row + aoff[q], (lds_void*)&Ad[kk * LDA + ch * 128]);
+    } else if constexpr (NARROW) {  // K even: a pair of k-rows is in range or out together
+      const int q = i - LA, kk = 2 * (q * NW + wave);
+      const char* row = (kofs + kk < krem) ? brow + bkoff[q] : b_first;
+      dma16(row + boff[q], (lds_void*)&Bd[kk * LDB]);
     } else {
       const int q = i - LA, kk = (q * NW + wave) / RB, ch = (q * NW + wave) % RB;
       const char* row = (kofs + kk < krem) ? brow + bkoff[q] : b_first;
@@ -427,7 +441,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
       for (int i = 0; i < FM; ++i) af[b][i] = as[krow * LDA + wm0 + i * 16 + lane % 16];
       aok[b] = krow < kvalid;
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bf[b][j] = bs[krow * LDB + wn0 + j * 16 + lane % 16];
+      for (int j = 0; j < FN; ++j)
+        bf[b][j] = bs[krow * LDB + ((wn0 + j * 16 + lane % 16) ^ (NARROW ? 16 * (krow & 1) : 0))];
     };
     frags(0, 0);
     __builtin_amdgcn_s_setprio(1);
@@ -1143,8 +1158,11 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
   auto even16 = [](const void* ptr, int64_t ld, int64_t stride) {
     return reinterpret_cast<uintptr_t>(ptr) % 16 == 0 && ld % 2 == 0 && stride % 2 == 0;
   };
+  // 256 x 128 DMA tiles from N = 128, 256 x 64 ones for 32 < N <= 64 (an even K: B's k-rows go
+  // in pairs; no lower-triangle schedule)
+  const bool narrow = p.N > 32 && p.N <= 64 && !p.lower_only && p.K % 2 == 0;
   const bool dma = p.kernel == 0 && !p.a_f32 && !p.b_f32 && p.a_mcontig && p.b_ncontig &&
-                   p.shape == GemmShape::kSquare && p.M >= 128 && p.N >= 128 &&
+                   p.shape == GemmShape::kSquare && p.M >= 128 && (p.N >= 128 || narrow) &&
                    even16(p.A, p.lda, p.stride_a) &&
                    even16(p.B, p.ldb, p.stride_b);
   const bool tall = !p.a_mcontig && p.b_ncontig && !p.b_f32 && p.N <= 64 && p.batch == 1 &&
@@ -1208,11 +1226,18 @@ void gemm_f64(GemmF64Args p, hipStream_t st) {
     // 256 x 128 tiles of 8 waves (64 x 64 each); a lower-triangle schedule takes the tiles that
     // touch the diagonal (2 (tm + 1) of them in tile row tm).
     const int64_t tm = ceil_div(p.M, 256);
-    const int64_t tiles = p.lower_only ? tm * tm + tm : tm * ceil_div(p.N, 128);
-    const int64_t nblocks = tiles * p.splits * p.batch;
-    MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
-    hipLaunchKernelGGL((gemm_f64_dma_kernel<256, 128, 16, 64, 64>),
-                       dim3(static_cast<unsigned>(nblocks)), dim3(512), 0, st, p);
+    if (p.N <= 64) {  // 256 x 64 tiles of 8 waves (64 x 32 each)
+      const int64_t nblocks = tm * p.splits * p.batch;
+      MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
+      hipLaunchKernelGGL((gemm_f64_dma_kernel<256, 64, 16, 64, 32>),
+                         dim3(static_cast<unsigned>(nblocks)), dim3(512), 0, st, p);
+    } else {
+      const int64_t tiles = p.lower_only ? tm * tm + tm : tm * ceil_div(p.N, 128);
+      const int64_t nblocks = tiles * p.splits * p.batch;
+      MCNLA_REQUIRE(nblocks < (1LL << 31), MCNLA_ERR_INVALID, "GEMM grid too large");
+      hipLaunchKernelGGL((gemm_f64_dma_kernel<256, 128, 16, 64, 64>),
+                         dim3(static_cast<unsigned>(nblocks)), dim3(512), 0, st, p);
+    }
     MCNLA_LAUNCHED();
   } else if (p.a_f32) {
     if (p.b_f32) launch_types<float, float>(p, st);

@@ -1257,9 +1257,18 @@ void bs_product(const GramIntegrator& g, const Layout& L, const double* Bin, dou
   p.partial = L.Pbg;
   p.gate = &L.st->converged_at;
   p.gate_t = t;
-  // Bs streamed into LDS by DMA (tall_stream_kernel; the register-staged kernel where its
-  // operand conditions fail): 174 us against 196 us for the 8192 x 8192 Bs at l = 64
-  p.kernel = 1;
+  if (g.l > 32 && nl % 2 == 0) {
+    // 256 x 64 DMA tiles of Bs read as M-contiguous (Bs is stored exactly symmetric, so its
+    // rows stand in for its columns): 157 us for the 8192 x 8192 Bs at l = 64, against 175 us
+    // streamed and 196 us register-staged
+    p.a_mcontig = true;
+    p.b_ncontig = true;
+    p.shape = GemmShape::kSquare;
+  } else {
+    // Bs streamed into LDS by DMA (tall_stream_kernel; the register-staged kernel where its
+    // operand conditions fail)
+    p.kernel = 1;
+  }
   gemm_f64(p, st);
 }
 
