@@ -28,7 +28,8 @@ def params(m, n, k, p, N):
 
 # ------------------------------------------------------------------------------------------------
 @pytest.mark.parametrize("shape", [(101, 1001, 11, 0, 12), (2000, 1000, 10, 10, 4), (257, 129, 3, 8, 3),
-                                   (998, 333, 13, 4, 10), (1024, 4096, 20, 12, 16)])
+                                   (998, 333, 13, 4, 10), (1024, 4096, 20, 12, 16),
+                                   (3001, 1000, 20, 12, 2), (1500, 600, 10, 10, 2)])
 @pytest.mark.parametrize("order", ["F", "C"])
 def test_sketch_f64(gpu, shape, order):
     m, n, k, p, N = shape
